@@ -1,0 +1,215 @@
+"""Benchmark of the HyperFlow V3 hot path on one MI355X.
+
+Headline workload (one "step" = one pass of the hot path over one batch):
+batched chemistry, 2^20 independent H2/air cells (H2_Li mechanism, FP64
+state), seeded per cell: T uniform [1100, 2400] K, p log-uniform [0.5, 50]
+atm, phi uniform [0.5, 2], progress fraction f uniform [0, 1] toward TP
+equilibrium, dt log-uniform [1e-8, 1e-6] s. Each step restores the initial
+cells (a device-to-device copy inside the timed region) and advances every
+cell over its dt. metric value = cells x steps / wall time of the timed steps
+(cell-steps/s) for the batched (persistent, stiffness-binned) form; the plain
+one-cell-per-lane form runs the same workload beside it.
+
+Also reported: the 2-D flux sweep (one Euler stage over a 4096 x 1024 box,
+Mcell-updates/s, HBM roofline at 192 algorithmic bytes per cell-update),
+chemistry parity against the CPU oracle on a sample, and the CPU oracle's
+throughput on a bounded sample of the same workload (all host cores).
+
+    python bench.py [--gpus 1] [--steps 3] [--warmup 1]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def chem_run(torch, batch, Ud, Ud0, dtd, steps, warmup):
+    for _ in range(warmup):
+        Ud.copy_(Ud0)
+        batch.advance(Ud, dtd)
+    torch.cuda.synchronize()
+    kernel_ms, stats = [], None
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        Ud.copy_(Ud0)
+        stats = batch.advance(Ud, dtd)
+        kernel_ms.append(stats["kernel_ms"])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return wall, kernel_ms, stats
+
+
+def flux_run(torch, hf, mech, nx, ny, steps, warmup):
+    from hyperflow_amd.physics import FluxSweep, box_for_arrays
+    nv = mech.nv
+    U, _ = mech.synth_cells(0, 3, 4096, with_dt=False)
+    rng = np.random.default_rng(3)
+    pick = rng.integers(0, 4096, size=(ny + 4, nx + 4))
+    # smooth-ish field: blocks of 8x8 cells share a synthetic state, velocities added
+    blk = pick[::8, ::8]
+    idx = np.repeat(np.repeat(blk, 8, 0), 8, 1)[: ny + 4, : nx + 4]
+    G = U[:, idx].copy()
+    rho = G[:mech.ns].sum(0)
+    u = 200.0 * np.sin(np.linspace(0, 6.28, nx + 4))[None, :] * np.ones((ny + 4, 1))
+    G[mech.ns] = rho * u
+    G[mech.ns + 2] += 0.5 * rho * u * u
+    Gd = torch.from_numpy(G).to("cuda")
+    out = torch.zeros((nv, ny, nx), dtype=torch.float64, device="cuda")
+    sweep = FluxSweep(mech)
+    box = [box_for_arrays(G, out, 1e-4, 1e-4)]
+    for _ in range(warmup):
+        sweep.stage(box, Gd, out)
+    torch.cuda.synchronize()
+    ms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st = sweep.stage(box, Gd, out)
+        ms.append(st["kernel_ms"])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return wall, ms, st
+
+
+def cpu_baseline(U, dt, target_s=15.0, rtol=1e-8, atol=1e-15):
+    from oracle import Oracle
+    from hyperflow_amd import H2_LI
+    o = Oracle(H2_LI)
+    cores = len(os.sched_getaffinity(0))
+    n0 = 512
+    Us = U[:, :n0].copy()
+    t = time.perf_counter()
+    o.chem_advance(Us, dt[:n0], rtol=rtol, atol=atol, threads=cores)
+    t = time.perf_counter() - t
+    n = int(min(U.shape[1], max(n0, n0 * target_s / max(t, 1e-3))))
+    Us = U[:, :n].copy()
+    t = time.perf_counter()
+    r, st, ff, msg = o.chem_advance(Us, dt[:n], rtol=rtol, atol=atol, threads=cores)
+    t = time.perf_counter() - t
+    return dict(value=n / t, unit="cell-steps/s", cores=cores, kind="port",
+                sample=f"first {n} cells of the same seeded 2^20-cell H2 workload, oracle BDF at "
+                       f"rtol {rtol:g}/atol {atol:g}, OpenMP dynamic,8, {t:.1f} s")
+
+
+def parity(U0, dt, out, n=256):
+    """Max errors of the GPU result against the oracle at rtol 1e-10 / atol 1e-14 on n cells."""
+    from oracle import Oracle
+    from hyperflow_amd import H2_LI
+    o = Oracle(H2_LI)
+    ref = U0[:, :n].copy()
+    o.chem_advance(ref, dt[:n], rtol=1e-10, atol=1e-14, threads=len(os.sched_getaffinity(0)))
+    ns = o.ns
+    Yg, Yr = out[:ns, :n] / out[:ns, :n].sum(0), ref[:ns] / ref[:ns].sum(0)
+    _, _, Tg, _ = o.properties(out[:, :n].T.copy())
+    _, _, Tr, _ = o.properties(ref.T.copy())
+    return dict(cells=n, T_rel=float(np.max(np.abs(Tg - Tr) / Tr)),
+                Y_rel_floor_1e-6=float(np.max(np.abs(Yg - Yr) / np.maximum(np.abs(Yr), 1e-6))),
+                Y_rel_floor_1e-10=float(np.max(np.abs(Yg - Yr) / np.maximum(np.abs(Yr), 1e-10))))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cells", type=int, default=1 << 20)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and parity legs")
+    ap.add_argument("--skip-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (this build has no multi-GPU path); --gpus {args.gpus} "
+              "is not supported", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import hyperflow_amd as hf
+    from hyperflow_amd import flops as F
+    from hyperflow_amd.build import GEN
+
+    mech = hf.Mechanism(hf.H2_LI)
+    log(f"synthesising {args.cells} cells")
+    U0, dt = mech.synth_cells(0, 2, args.cells)
+    Ud0 = torch.from_numpy(U0).to("cuda")
+    Ud = Ud0.clone()
+    dtd = torch.from_numpy(dt).to("cuda")
+    ns, reactions = F.topology_summary(GEN / "h2_li.inc")
+
+    res = {}
+    for name, mode in (("plain", hf.HF_CHEM_PLAIN), ("batched", hf.HF_CHEM_BATCHED)):
+        if name == "plain" and args.skip_plain:
+            continue
+        log(f"chemistry {name}: warmup {args.warmup}, steps {args.steps}")
+        wall, kms, st = chem_run(torch, hf.ChemistryBatch(mech, mode=mode), Ud, Ud0, dtd, args.steps, args.warmup)
+        flop = F.cell_flops(st, ns, reactions)
+        res[name] = dict(wall=wall, kernel_ms=kms, stats=st, flops=flop,
+                         value=args.cells * args.steps / wall)
+        log(f"chemistry {name}: {res[name]['value']:.4g} cell-steps/s, kernel {np.mean(kms):.2f} ms, "
+            f"steps/cell {st['steps'] / args.cells:.1f}")
+    out = Ud.cpu().numpy()
+
+    log("flux sweep 4096 x 1024")
+    fwall, fms, fst = flux_run(torch, hf, mech, 4096, 1024, max(args.steps, 5), args.warmup)
+    fcells = fst["cells"]
+    f_upd = fcells * max(args.steps, 5) / fwall
+    f_kernel_gbs = 192.0 * fcells / (np.mean(fms) * 1e-3) / 1e9
+
+    head = res["batched"]
+    kern_s = np.mean(head["kernel_ms"]) * 1e-3
+    achieved = head["flops"] / kern_s / 1e12
+    line = {
+        "metric": "chem cell-steps/s (H2, DME) + 2D Mcell-updates/s vs plain FP64 at 1e-6 rtol",
+        "value": head["value"],
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["wall"] / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "batched chemistry, 2^20 independent H2/air cells (H2_Li), seed 2, "
+                               "dt log-uniform [1e-8,1e-6] s, rtol 1e-8 atol 1e-15",
+                   "cells": args.cells, "form": "batched"},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "note": "FP64 algorithmic operations of the chemistry kernel (hyperflow_amd/flops.py) "
+                             "over its event-timed duration; peak is the FP64 rate"},
+    }
+    if "plain" in res:
+        line["plain"] = {"value": res["plain"]["value"], "kernel_ms": float(np.mean(res["plain"]["kernel_ms"])),
+                         "steps_per_cell": res["plain"]["stats"]["steps"] / args.cells}
+        line["batched_over_plain"] = head["value"] / res["plain"]["value"]
+    line["batched_kernel_ms"] = float(np.mean(head["kernel_ms"]))
+    line["batched_stats"] = {k: head["stats"][k] for k in ("steps", "rhs", "jac", "lu", "newton",
+                                                             "conv_failures", "err_failures", "max_cell_steps")}
+    line["flux_sweep"] = {"workload": "one Euler stage, 4096 x 1024 box, H2_Li", "Mcell_updates_per_s": f_upd / 1e6,
+                          "kernel_ms": float(np.mean(fms)),
+                          "roofline": {"bound": "hbm", "achieved": f_kernel_gbs, "peak": HBM_PEAK_GBS,
+                                       "unit": "GB/s", "frac": f_kernel_gbs / HBM_PEAK_GBS}}
+    if not args.no_cpu:
+        log("parity sample vs oracle (rtol 1e-10, atol 1e-14)")
+        line["parity_vs_oracle"] = parity(U0, dt, out)
+        log("CPU baseline")
+        line["cpu_baseline"] = cpu_baseline(U0, dt)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

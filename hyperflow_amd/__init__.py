@@ -1,0 +1,20 @@
+"""MI355X-native hot path of HyperFlow V3: stiff per-cell chemistry and
+NND/HLLC flux sweeps as HIP kernels for gfx950, behind the C ABI of
+include/hyperflow.h. This package is the Python mirror of the reference's
+operator interface for that path; the arithmetic runs in libhyperflow_hip.so.
+"""
+from __future__ import annotations
+
+from pathlib import Path
+
+from ._lib import (HF_CHEM_BATCHED, HF_CHEM_PLAIN, HF_PREC_FP64, HF_PREC_MIXED, HyperflowError, LIB_PATH,
+                   load)
+from .chemistry import ChemistryBatch, Mechanism
+from .physics import FluxSweep
+
+MECHANISM_DIR = Path(__file__).resolve().parent / "mechanisms"
+H2_LI = MECHANISM_DIR / "H2_Li.yaml"
+DME_SK39 = MECHANISM_DIR / "dme_sk39.yaml"
+
+__all__ = ["ChemistryBatch", "Mechanism", "FluxSweep", "HyperflowError", "load", "LIB_PATH", "H2_LI", "DME_SK39",
+           "HF_CHEM_PLAIN", "HF_CHEM_BATCHED", "HF_PREC_FP64", "HF_PREC_MIXED"]

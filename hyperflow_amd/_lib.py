@@ -1,0 +1,110 @@
+"""ctypes binding of libhyperflow_hip.so (the C ABI in include/hyperflow.h).
+
+The library is loaded from this package directory only. There is no fallback:
+if the HIP library is missing or cannot be loaded, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+from ctypes import (POINTER, Structure, byref, c_char_p, c_double, c_int, c_int32, c_int64, c_uint64,
+                    c_void_p)
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libhyperflow_hip.so"
+
+HF_OK, HF_EINVAL, HF_EDOMAIN, HF_ECONV, HF_ENOMEM, HF_EDEVICE, HF_EBUSY = range(7)
+HF_CHEM_PLAIN, HF_CHEM_BATCHED = 0, 1
+HF_PREC_FP64, HF_PREC_MIXED = 0, 1
+
+ERROR_NAMES = {0: "HF_OK", 1: "HF_EINVAL", 2: "HF_EDOMAIN", 3: "HF_ECONV", 4: "HF_ENOMEM", 5: "HF_EDEVICE",
+               6: "HF_EBUSY"}
+
+
+class hf_options(Structure):
+    _fields_ = [("rtol", c_double), ("atol", c_double), ("max_steps", c_int), ("precision", c_int),
+                ("device", c_int), ("waves_per_cu", c_int), ("chunk", c_int), ("nbins", c_int)]
+
+
+class hf_chem_stats(Structure):
+    _fields_ = [("cells", c_int64), ("steps", c_int64), ("rhs", c_int64), ("jac", c_int64), ("lu", c_int64),
+                ("newton", c_int64), ("conv_failures", c_int64), ("err_failures", c_int64),
+                ("max_cell_steps", c_int64), ("team", c_int), ("kernel_ms", c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class hf_box(Structure):
+    _fields_ = [("nx", c_int32), ("ny", c_int32), ("walls", c_int32), ("wedge_i", c_int32),
+                ("in_offset", c_int64), ("in_pitch", c_int64), ("in_plane", c_int64),
+                ("out_offset", c_int64), ("out_pitch", c_int64), ("out_plane", c_int64),
+                ("dtdx", c_double), ("dtdy", c_double)]
+
+
+class hf_flux_stats(Structure):
+    _fields_ = [("cells", c_int64), ("faces", c_int64), ("first_order_sides", c_int64), ("kernel_ms", c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_dp = POINTER(c_double)
+_SIGNATURES = {
+    "hf_create": (c_int, [c_char_p, POINTER(hf_options), POINTER(c_void_p)]),
+    "hf_destroy": (None, [c_void_p]),
+    "hf_last_error": (c_char_p, [c_void_p]),
+    "hf_create_error": (c_char_p, []),
+    "hf_num_species": (c_int, [c_void_p]),
+    "hf_num_reactions": (c_int, [c_void_p]),
+    "hf_species_name": (c_char_p, [c_void_p, c_int]),
+    "hf_molecular_weights": (c_int, [c_void_p, _dp]),
+    "hf_set_options": (c_int, [c_void_p, POINTER(hf_options)]),
+    "hf_get_options": (c_int, [c_void_p, POINTER(hf_options)]),
+    "hf_chem_advance": (c_int, [c_void_p, c_void_p, c_int64, c_int64, c_void_p, c_int, c_int,
+                                POINTER(hf_chem_stats), POINTER(c_int64)]),
+    "hf_flux_stage": (c_int, [c_void_p, POINTER(hf_box), c_int, c_void_p, c_void_p, c_int,
+                              POINTER(hf_flux_stats), POINTER(c_int64)]),
+    "hf_properties": (c_int, [c_void_p, c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_void_p,
+                              POINTER(c_int64)]),
+    "hf_stable_dt": (c_int, [c_void_p, c_void_p, c_int64, c_int64, c_double, c_double, c_double, _dp]),
+    "hf_chem_adjoint": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_double, c_int, c_int,
+                                c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "hf_chem_tangent": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_double, c_int, c_void_p,
+                                c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "hf_synth_cells": (c_int, [c_void_p, c_int, c_uint64, c_int64, c_void_p, c_void_p]),
+    "hf_equilibrate_tp": (c_int, [c_void_p, c_double, c_double, _dp, _dp]),
+    "hf_state_from_TPY": (c_int, [c_void_p, c_double, c_double, _dp, c_double, c_double, _dp]),
+}
+EXPORTED = tuple(_SIGNATURES)
+
+_lib = None
+
+
+def load():
+    """Load the HIP library; raises if it is absent or lacks an entry point."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is not built: run `python -m hyperflow_amd.build` "
+                               "(there is no CPU fallback)")
+        L = ctypes.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+class HyperflowError(RuntimeError):
+    def __init__(self, code, message, first_failed=-1):
+        super().__init__(f"{ERROR_NAMES.get(code, code)}: {message}")
+        self.code = code
+        self.message = message
+        self.first_failed = first_failed
+
+
+def check(ctx, code, first_failed=-1):
+    if code != HF_OK:
+        msg = load().hf_last_error(ctx).decode() if ctx else load().hf_create_error().decode()
+        raise HyperflowError(code, msg, first_failed)

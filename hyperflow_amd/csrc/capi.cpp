@@ -1,0 +1,553 @@
+// The extern "C" boundary (include/hyperflow.h): argument checks, host/device
+// staging, all-or-nothing commit, lowest-failing-index reporting and the
+// dispatch to the kernels compiled for the mechanism's topology.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "host_thermo.hpp"
+#include "hyperflow.h"
+#include "mechanism.hpp"
+#include "topo_ops.hpp"
+
+using namespace hf;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t n = 0;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    bool ensure(size_t bytes) {
+        if (bytes <= n) return true;
+        release();
+        if (hipMalloc(&p, bytes) != hipSuccess) {
+            p = nullptr;
+            return false;
+        }
+        n = bytes;
+        return true;
+    }
+    template <class T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+bool is_device_ptr(const void* ptr) {
+    if (!ptr) return false;
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, ptr) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
+}
+
+}  // namespace
+
+struct hf_ctx {
+    Mechanism mech;
+    const TopoOps* ops = nullptr;
+    hf_options opt{};
+    int num_cus = 256;
+    hipStream_t stream = nullptr;
+    void* d_mech = nullptr;
+    std::string err;
+    std::atomic<bool> busy{false};
+    DevBuf U, out, dt, status, scal, work, boxes, tiles, aux[6];
+};
+
+namespace {
+
+struct BusyGuard {
+    hf_ctx* c;
+    bool ok;
+    explicit BusyGuard(hf_ctx* ctx) : c(ctx) {
+        bool expected = false;
+        ok = c->busy.compare_exchange_strong(expected, true);
+    }
+    ~BusyGuard() {
+        if (ok) c->busy.store(false);
+    }
+};
+
+int fail(hf_ctx* c, int code, const std::string& msg) {
+    c->err = msg;
+    return code;
+}
+
+// Bind the context's device: stream, CU count and the mechanism tables.
+int ensure_device(hf_ctx* c) {
+    if (c->d_mech) return hipSetDevice(c->opt.device) == hipSuccess ? HF_OK : fail(c, HF_EDEVICE, "hipSetDevice failed");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= c->opt.device) {
+        (void)hipGetLastError();
+        return fail(c, HF_EDEVICE, "no HIP device " + std::to_string(c->opt.device));
+    }
+    if (hipSetDevice(c->opt.device) != hipSuccess) return fail(c, HF_EDEVICE, "hipSetDevice failed");
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, c->opt.device) == hipSuccess) c->num_cus = prop.multiProcessorCount;
+    if (!c->stream && hipStreamCreate(&c->stream) != hipSuccess) return fail(c, HF_EDEVICE, "hipStreamCreate failed");
+    std::vector<char> host(c->ops->mech_bytes);
+    c->ops->fill(c->mech, host.data());
+    void* d = nullptr;
+    if (hipMalloc(&d, host.size()) != hipSuccess) return fail(c, HF_ENOMEM, "cannot allocate mechanism tables");
+    if (hipMemcpy(d, host.data(), host.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(d);
+        return fail(c, HF_EDEVICE, "cannot upload mechanism tables");
+    }
+    c->d_mech = d;
+    return HF_OK;
+}
+
+#define HF_DEVICE(c)                       \
+    do {                                   \
+        const int r_ = ensure_device(c);   \
+        if (r_ != HF_OK) return r_;        \
+    } while (0)
+
+hf_options defaults() {
+    hf_options o{};
+    o.rtol = 1e-8;
+    o.atol = 1e-15;
+    o.max_steps = 5000;
+    o.precision = HF_PREC_FP64;
+    o.device = 0;
+    return o;
+}
+
+int hip_fail(hf_ctx* c, hipError_t e, const char* what) {
+    return fail(c, HF_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HF_HIP(c, call)                                   \
+    do {                                                  \
+        hipError_t e_ = (call);                           \
+        if (e_ != hipSuccess) return hip_fail(c, e_, #call); \
+    } while (0)
+
+}  // namespace
+
+extern "C" {
+
+const char* hf_create_error(void) { return g_create_error.c_str(); }
+
+int hf_create(const char* mechanism_yaml, const hf_options* opt, hf_ctx** out) {
+    if (!out || !mechanism_yaml) {
+        g_create_error = "null argument";
+        return HF_EINVAL;
+    }
+    *out = nullptr;
+    auto c = std::make_unique<hf_ctx>();
+    c->opt = opt ? *opt : defaults();
+    if (!(c->opt.rtol > 0) || !(c->opt.atol > 0) || c->opt.max_steps < 1) {
+        g_create_error = "invalid tolerances or step limit";
+        return HF_EINVAL;
+    }
+    try {
+        c->mech = load_mechanism(mechanism_yaml);
+    } catch (const std::exception& e) {
+        g_create_error = e.what();
+        return HF_EINVAL;
+    }
+    for (auto& r : c->mech.reactions)
+        if (!(r.A > 0) || (r.kind == Reaction::FALLOFF && !(r.A0 > 0))) {
+            g_create_error = "reaction '" + r.equation + "' has a non-positive pre-exponential factor";
+            return HF_EINVAL;
+        }
+    const std::string topo = c->mech.topology();
+    for (const TopoOps* const* t = compiled_topologies(); *t; ++t)
+        if (topo == (*t)->topology) c->ops = *t;
+    if (!c->ops) {
+        g_create_error = "mechanism '" + std::string(mechanism_yaml) +
+                         "' has no compiled kernels (its species/reaction topology matches no built mechanism)";
+        return HF_EINVAL;
+    }
+    if (c->opt.device < 0) {
+        g_create_error = "invalid device ordinal";
+        return HF_EINVAL;
+    }
+    // The device is bound on the first call that needs it (ensure_device), so
+    // host-only set-up helpers work on a machine without a GPU.
+    *out = c.release();
+    return HF_OK;
+}
+
+void hf_destroy(hf_ctx* c) {
+    if (!c) return;
+    if (c->d_mech || c->stream) {
+        (void)hipSetDevice(c->opt.device);
+        if (c->stream) (void)hipStreamSynchronize(c->stream);
+        if (c->d_mech) (void)hipFree(c->d_mech);
+        if (c->stream) (void)hipStreamDestroy(c->stream);
+    }
+    delete c;
+}
+
+const char* hf_last_error(const hf_ctx* c) { return c ? c->err.c_str() : "null context"; }
+int hf_num_species(const hf_ctx* c) { return c ? c->mech.ns() : 0; }
+int hf_num_reactions(const hf_ctx* c) { return c ? c->mech.nr() : 0; }
+const char* hf_species_name(const hf_ctx* c, int k) {
+    return (c && k >= 0 && k < c->mech.ns()) ? c->mech.species[k].c_str() : nullptr;
+}
+int hf_molecular_weights(const hf_ctx* c, double* W) {
+    if (!c || !W) return HF_EINVAL;
+    for (int k = 0; k < c->mech.ns(); ++k) W[k] = c->mech.W[k];
+    return HF_OK;
+}
+int hf_set_options(hf_ctx* c, const hf_options* o) {
+    if (!c || !o) return HF_EINVAL;
+    if (!(o->rtol > 0) || !(o->atol > 0) || o->max_steps < 1) return fail(c, HF_EINVAL, "invalid tolerances or step limit");
+    if (o->device != c->opt.device) return fail(c, HF_EINVAL, "the device of a context cannot change");
+    c->opt = *o;
+    return HF_OK;
+}
+int hf_get_options(const hf_ctx* c, hf_options* o) {
+    if (!c || !o) return HF_EINVAL;
+    *o = c->opt;
+    return HF_OK;
+}
+
+int hf_chem_advance(hf_ctx* c, double* U, int64_t ncell, int64_t ld, const double* dt, int dt_per_cell, int mode,
+                    hf_chem_stats* stats, int64_t* first_failed) {
+    if (!c) return HF_EINVAL;
+    BusyGuard guard(c);
+    if (!guard.ok) return fail(c, HF_EBUSY, "nested or concurrent chemistry batch is unsupported");
+    if (first_failed) *first_failed = -1;
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    if (ncell < 0 || ld < ncell || !dt || (ncell > 0 && !U) || (mode != HF_CHEM_PLAIN && mode != HF_CHEM_BATCHED))
+        return fail(c, HF_EINVAL, "invalid chemistry batch extent or interval");
+    const bool dev_dt = is_device_ptr(dt);
+    if (!dt_per_cell) {
+        double h = 0;
+        if (dev_dt) HF_HIP(c, hipMemcpy(&h, dt, sizeof(double), hipMemcpyDeviceToHost));
+        else h = dt[0];
+        if (!(h > 0) || !std::isfinite(h)) return fail(c, HF_EINVAL, "invalid chemistry batch extent or interval");
+    }
+    if (ncell == 0) return HF_OK;
+    HF_DEVICE(c);
+    const int ns = c->mech.ns(), nv = ns + 3;
+    const bool dev_U = is_device_ptr(U);
+    const double* dU = U;
+    int64_t dld = ld;
+    if (!dev_U) {
+        if (!c->U.ensure(sizeof(double) * nv * ncell)) return fail(c, HF_ENOMEM, "device staging");
+        HF_HIP(c, hipMemcpy2DAsync(c->U.p, ncell * sizeof(double), U, ld * sizeof(double), ncell * sizeof(double), nv,
+                                   hipMemcpyHostToDevice, c->stream));
+        dU = c->U.as<double>();
+        dld = ncell;
+    }
+    const double* ddt = dt;
+    const int64_t ndt = dt_per_cell ? ncell : 1;
+    if (!dev_dt) {
+        if (!c->dt.ensure(sizeof(double) * ndt)) return fail(c, HF_ENOMEM, "device staging");
+        HF_HIP(c, hipMemcpyAsync(c->dt.p, dt, sizeof(double) * ndt, hipMemcpyHostToDevice, c->stream));
+        ddt = c->dt.as<double>();
+    }
+    if (!c->out.ensure(sizeof(double) * ns * ncell) || !c->status.ensure(ncell) ||
+        !c->scal.ensure(sizeof(unsigned long long) * 16))
+        return fail(c, HF_ENOMEM, "device scratch");
+    unsigned long long* scal = c->scal.as<unsigned long long>();
+    // scal[0] = first failed (init ~0), scal[1..8] = counters
+    std::vector<unsigned long long> init(16, 0);
+    init[0] = ~0ull;
+    HF_HIP(c, hipMemcpyAsync(scal, init.data(), sizeof(unsigned long long) * 16, hipMemcpyHostToDevice, c->stream));
+    ChemLaunch L{};
+    L.mech = c->d_mech;
+    L.U = dU;
+    L.ncell = ncell;
+    L.ld = dld;
+    L.dt = ddt;
+    L.dt_per_cell = dt_per_cell;
+    L.out = c->out.as<double>();
+    L.ldo = ncell;
+    L.status = c->status.as<int8_t>();
+    L.first_failed = scal;
+    L.counters = scal + 1;
+    L.rtol = c->opt.rtol;
+    L.atol = c->opt.atol;
+    L.maxsteps = c->opt.max_steps;
+    L.mode = mode;
+    L.precision = c->opt.precision;
+    L.num_cus = c->num_cus;
+    L.waves_per_cu = c->opt.waves_per_cu;
+    L.chunk = c->opt.chunk;
+    L.nbins = c->opt.nbins;
+    const size_t wbytes = c->ops->chem_workspace(ncell, c->opt.nbins, c->num_cus, c->opt.waves_per_cu);
+    if (wbytes) {
+        if (!c->work.ensure(wbytes)) return fail(c, HF_ENOMEM, "device workspace");
+        L.workspace = c->work.p;
+        L.workspace_bytes = wbytes;
+    }
+    L.stream = c->stream;
+    float ms = 0;
+    L.kernel_ms = &ms;
+    const int r = c->ops->chem(L);
+    if (r != HF_OK) return fail(c, r, "chemistry kernel launch failed");
+    unsigned long long host_scal[9];
+    HF_HIP(c, hipMemcpyAsync(host_scal, scal, sizeof(host_scal), hipMemcpyDeviceToHost, c->stream));
+    HF_HIP(c, hipStreamSynchronize(c->stream));
+    if (stats) {
+        stats->cells = ncell;
+        stats->steps = (int64_t)host_scal[1];
+        stats->rhs = (int64_t)host_scal[2];
+        stats->jac = (int64_t)host_scal[3];
+        stats->lu = (int64_t)host_scal[4];
+        stats->newton = (int64_t)host_scal[5];
+        stats->conv_failures = (int64_t)host_scal[6];
+        stats->err_failures = (int64_t)host_scal[7];
+        stats->max_cell_steps = (int64_t)host_scal[8];
+        stats->team = (int)std::min<int64_t>(ncell, INT32_MAX);
+        stats->kernel_ms = ms;
+    }
+    if (host_scal[0] != ~0ull) {
+        const int64_t k = (int64_t)host_scal[0];
+        int8_t code = 0;
+        HF_HIP(c, hipMemcpy(&code, c->status.as<int8_t>() + k, 1, hipMemcpyDeviceToHost));
+        if (first_failed) *first_failed = k;
+        const char* what = code == HF_EINVAL ? "invalid chemistry interval"
+                           : code == HF_EDOMAIN ? "invalid accepted species or inadmissible state"
+                                                : "integrator failed (CVode-equivalent error)";
+        return fail(c, code ? code : HF_ECONV, "chemistry cell " + std::to_string(k) + ": " + what);
+    }
+    // commit the species planes
+    if (dev_U) {
+        HF_HIP(c, hipMemcpy2DAsync(U, ld * sizeof(double), c->out.p, ncell * sizeof(double), ncell * sizeof(double), ns,
+                                   hipMemcpyDeviceToDevice, c->stream));
+    } else {
+        HF_HIP(c, hipMemcpy2DAsync(U, ld * sizeof(double), c->out.p, ncell * sizeof(double), ncell * sizeof(double), ns,
+                                   hipMemcpyDeviceToHost, c->stream));
+    }
+    HF_HIP(c, hipStreamSynchronize(c->stream));
+    return HF_OK;
+}
+
+int hf_flux_stage(hf_ctx* c, const hf_box* boxes, int nbox, const double* Uin, double* Uout, int first_order,
+                  hf_flux_stats* stats, int64_t* first_failed) {
+    if (!c) return HF_EINVAL;
+    BusyGuard guard(c);
+    if (!guard.ok) return fail(c, HF_EBUSY, "concurrent flux stage is unsupported");
+    if (first_failed) *first_failed = -1;
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    if (nbox < 0 || (nbox > 0 && (!boxes || !Uin || !Uout))) return fail(c, HF_EINVAL, "invalid flux-stage arguments");
+    if (nbox == 0) return HF_OK;
+    HF_DEVICE(c);
+    const int nv = c->mech.ns() + 3;
+    // all boxes 1-D (x only) or all 2-D
+    bool one_d = true;
+    int64_t in_extent = 0, out_extent = 0, cells = 0;
+    for (int b = 0; b < nbox; ++b) {
+        const hf_box& B = boxes[b];
+        if (B.nx < 1 || B.ny < 1 || B.in_pitch < B.nx + 4 || B.out_pitch < B.nx || B.in_plane < B.in_pitch * (B.ny + 4) ||
+            B.out_plane < B.out_pitch * B.ny || B.in_offset < 0 || B.out_offset < 0 || !std::isfinite(B.dtdx) ||
+            !std::isfinite(B.dtdy))
+            return fail(c, HF_EINVAL, "box " + std::to_string(b) + " has an invalid extent or stride");
+        if (!(B.ny == 1 && B.dtdy == 0)) one_d = false;
+        in_extent = std::max<int64_t>(in_extent, B.in_offset + (nv - 1) * B.in_plane + (B.ny + 3) * B.in_pitch + B.nx + 4);
+        out_extent = std::max<int64_t>(out_extent, B.out_offset + (nv - 1) * B.out_plane + (B.ny - 1) * B.out_pitch + B.nx);
+        cells += (int64_t)B.nx * B.ny;
+    }
+    const int TX = one_d ? 255 : 15, TY = one_d ? 1 : 15;
+    std::vector<int32_t> tiles;
+    for (int b = 0; b < nbox; ++b)
+        for (int tj = 0; tj < (boxes[b].ny + TY - 1) / TY; ++tj)
+            for (int ti = 0; ti < (boxes[b].nx + TX - 1) / TX; ++ti) {
+                tiles.push_back(b);
+                tiles.push_back(ti);
+                tiles.push_back(tj);
+            }
+    const int ntiles = (int)(tiles.size() / 3);
+    if (!c->boxes.ensure(sizeof(hf_box) * nbox) || !c->tiles.ensure(sizeof(int32_t) * tiles.size()) ||
+        !c->scal.ensure(sizeof(unsigned long long) * 16))
+        return fail(c, HF_ENOMEM, "device scratch");
+    HF_HIP(c, hipMemcpyAsync(c->boxes.p, boxes, sizeof(hf_box) * nbox, hipMemcpyHostToDevice, c->stream));
+    HF_HIP(c, hipMemcpyAsync(c->tiles.p, tiles.data(), sizeof(int32_t) * tiles.size(), hipMemcpyHostToDevice, c->stream));
+    const bool dev_in = is_device_ptr(Uin), dev_out = is_device_ptr(Uout);
+    const double* din = Uin;
+    double* dout = Uout;
+    if (!dev_in) {
+        if (!c->U.ensure(sizeof(double) * in_extent)) return fail(c, HF_ENOMEM, "device staging");
+        HF_HIP(c, hipMemcpyAsync(c->U.p, Uin, sizeof(double) * in_extent, hipMemcpyHostToDevice, c->stream));
+        din = c->U.as<double>();
+    }
+    if (!dev_out) {
+        if (!c->out.ensure(sizeof(double) * out_extent)) return fail(c, HF_ENOMEM, "device staging");
+        HF_HIP(c, hipMemcpyAsync(c->out.p, Uout, sizeof(double) * out_extent, hipMemcpyHostToDevice, c->stream));
+        dout = c->out.as<double>();
+    }
+    unsigned long long* scal = c->scal.as<unsigned long long>();
+    unsigned long long init[3] = {~0ull, 0, 0};
+    HF_HIP(c, hipMemcpyAsync(scal, init, sizeof(init), hipMemcpyHostToDevice, c->stream));
+    FluxLaunch L{};
+    L.mech = c->d_mech;
+    L.boxes = c->boxes.as<hf_box>();
+    L.nbox = nbox;
+    L.tiles = c->tiles.as<int32_t>();
+    L.ntiles = ntiles;
+    L.Uin = din;
+    L.Uout = dout;
+    L.first_order = first_order ? 1 : 0;
+    L.one_d = one_d ? 1 : 0;
+    L.first_failed = scal;
+    L.counters = scal + 1;
+    L.stream = c->stream;
+    float ms = 0;
+    L.kernel_ms = &ms;
+    const int r = c->ops->flux(L);
+    if (r != HF_OK) return fail(c, r, "flux kernel launch failed");
+    unsigned long long hs[3];
+    HF_HIP(c, hipMemcpyAsync(hs, scal, sizeof(hs), hipMemcpyDeviceToHost, c->stream));
+    if (!dev_out)
+        HF_HIP(c, hipMemcpyAsync(Uout, dout, sizeof(double) * out_extent, hipMemcpyDeviceToHost, c->stream));
+    HF_HIP(c, hipStreamSynchronize(c->stream));
+    if (stats) {
+        stats->cells = cells;
+        stats->faces = (int64_t)hs[1];
+        stats->first_order_sides = (int64_t)hs[2];
+        stats->kernel_ms = ms;
+    }
+    if (hs[0] != ~0ull) {
+        if (first_failed) *first_failed = (int64_t)hs[0];
+        return fail(c, HF_EDOMAIN, "inadmissible state after flux stage at box " + std::to_string(hs[0] >> 32) +
+                                       " cell " + std::to_string(hs[0] & 0xffffffffull));
+    }
+    return HF_OK;
+}
+
+static int props_common(hf_ctx* c, const double* U, int64_t ncell, int64_t ld, double* p, double* a, double* T,
+                        int64_t* first_failed, double dx, double dy, double* rate) {
+    if (ncell < 0 || ld < ncell || (ncell > 0 && !U)) return fail(c, HF_EINVAL, "invalid extent");
+    if (first_failed) *first_failed = -1;
+    if (ncell == 0) return HF_OK;
+    HF_DEVICE(c);
+    const int nv = c->mech.ns() + 3;
+    const double* dU = U;
+    int64_t dld = ld;
+    if (!is_device_ptr(U)) {
+        if (!c->U.ensure(sizeof(double) * nv * ncell)) return fail(c, HF_ENOMEM, "device staging");
+        HF_HIP(c, hipMemcpy2DAsync(c->U.p, ncell * sizeof(double), U, ld * sizeof(double), ncell * sizeof(double), nv,
+                                   hipMemcpyHostToDevice, c->stream));
+        dU = c->U.as<double>();
+        dld = ncell;
+    }
+    double* outs[3] = {p, a, T};
+    double* douts[3] = {nullptr, nullptr, nullptr};
+    bool host_out[3] = {false, false, false};
+    for (int i = 0; i < 3; ++i) {
+        if (!outs[i]) continue;
+        if (is_device_ptr(outs[i])) {
+            douts[i] = outs[i];
+        } else {
+            if (!c->aux[i].ensure(sizeof(double) * ncell)) return fail(c, HF_ENOMEM, "device staging");
+            douts[i] = c->aux[i].as<double>();
+            host_out[i] = true;
+        }
+    }
+    if (!c->scal.ensure(sizeof(unsigned long long) * 16)) return fail(c, HF_ENOMEM, "device scratch");
+    unsigned long long* scal = c->scal.as<unsigned long long>();
+    unsigned long long init[2] = {~0ull, 0};
+    HF_HIP(c, hipMemcpyAsync(scal, init, sizeof(init), hipMemcpyHostToDevice, c->stream));
+    PropsLaunch L{};
+    L.mech = c->d_mech;
+    L.U = dU;
+    L.ncell = ncell;
+    L.ld = dld;
+    L.p = douts[0];
+    L.a = douts[1];
+    L.T = douts[2];
+    L.first_failed = scal;
+    L.dx = dx;
+    L.dy = dy;
+    L.rate_max = rate ? reinterpret_cast<double*>(scal + 1) : nullptr;
+    L.stream = c->stream;
+    const int r = c->ops->props(L);
+    if (r != HF_OK) return fail(c, r, "properties kernel launch failed");
+    unsigned long long hs[2];
+    HF_HIP(c, hipMemcpyAsync(hs, scal, sizeof(hs), hipMemcpyDeviceToHost, c->stream));
+    for (int i = 0; i < 3; ++i)
+        if (host_out[i])
+            HF_HIP(c, hipMemcpyAsync(outs[i], douts[i], sizeof(double) * ncell, hipMemcpyDeviceToHost, c->stream));
+    HF_HIP(c, hipStreamSynchronize(c->stream));
+    if (rate) std::memcpy(rate, &hs[1], sizeof(double));
+    if (hs[0] != ~0ull) {
+        if (first_failed) *first_failed = (int64_t)hs[0];
+        return fail(c, HF_EDOMAIN, "inadmissible conserved state at cell " + std::to_string(hs[0]));
+    }
+    return HF_OK;
+}
+
+int hf_properties(hf_ctx* c, const double* U, int64_t ncell, int64_t ld, double* p, double* a, double* T,
+                  int64_t* first_failed) {
+    if (!c) return HF_EINVAL;
+    BusyGuard guard(c);
+    if (!guard.ok) return fail(c, HF_EBUSY, "concurrent call is unsupported");
+    return props_common(c, U, ncell, ld, p, a, T, first_failed, 0, 0, nullptr);
+}
+
+int hf_stable_dt(hf_ctx* c, const double* U, int64_t ncell, int64_t ld, double dx, double dy, double cfl, double* dt) {
+    if (!c) return HF_EINVAL;
+    BusyGuard guard(c);
+    if (!guard.ok) return fail(c, HF_EBUSY, "concurrent call is unsupported");
+    if (!dt || !(dx > 0) || !(dy >= 0) || !(cfl > 0)) return fail(c, HF_EINVAL, "invalid CFL arguments");
+    double rate = 0;
+    int64_t ff;
+    const int r = props_common(c, U, ncell, ld, nullptr, nullptr, nullptr, &ff, dx, dy, &rate);
+    if (r != HF_OK) return r;
+    *dt = rate > 0 ? cfl / rate : INFINITY;
+    return HF_OK;
+}
+
+int hf_chem_adjoint(hf_ctx* c, int64_t, const double*, const double*, const double*, double, int, int, const double*,
+                    double*, double*, double*, double*) {
+    if (!c) return HF_EINVAL;
+    return fail(c, HF_EINVAL, "adjoint not built");
+}
+
+int hf_chem_tangent(hf_ctx* c, int64_t, const double*, const double*, const double*, double, int, const double*,
+                    const double*, const double*, const double*, double*, double*) {
+    if (!c) return HF_EINVAL;
+    return fail(c, HF_EINVAL, "tangent not built");
+}
+
+int hf_synth_cells(hf_ctx* c, int which, uint64_t seed, int64_t ncell, double* U, double* dt) {
+    if (!c) return HF_EINVAL;
+    if (ncell < 0 || (ncell > 0 && !U)) return fail(c, HF_EINVAL, "invalid extent");
+    try {
+        synth_cells(c->mech, which, seed, ncell, U, dt);
+    } catch (const std::exception& e) {
+        return fail(c, HF_EINVAL, e.what());
+    }
+    return HF_OK;
+}
+
+int hf_equilibrate_tp(hf_ctx* c, double T, double p, const double* X, double* Y) {
+    if (!c) return HF_EINVAL;
+    if (!X || !Y || !(T > 0) || !(p > 0)) return fail(c, HF_EINVAL, "invalid equilibrium arguments");
+    try {
+        equilibrate_tp(c->mech, T, p, X, Y);
+    } catch (const std::exception& e) {
+        return fail(c, HF_ECONV, e.what());
+    }
+    return HF_OK;
+}
+
+int hf_state_from_TPY(hf_ctx* c, double T, double p, const double* Y, double u, double v, double* q) {
+    if (!c) return HF_EINVAL;
+    if (!Y || !q || !(T > 0) || !(p > 0)) return fail(c, HF_EINVAL, "invalid state arguments");
+    state_from_TPY(c->mech, T, p, Y, u, v, q);
+    return HF_OK;
+}
+
+}  // extern "C"

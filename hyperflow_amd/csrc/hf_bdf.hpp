@@ -1,0 +1,569 @@
+// Per-lane variable-order (1-5) BDF integrator in fixed-leading-coefficient
+// Nordsieck form, written for gfx950: the Nordsieck history lives in LDS
+// (one [order][unknown][lane] slab per wave, conflict-free), everything else
+// has a compile-time size (N unknowns, order <= 5) with loops over the current
+// order unrolled and predicated, so the Newton matrix, weights and correction
+// stay in VGPRs.
+//
+// This is the integrator that stands in for the reference's CVODE call
+// (HyperFlow_V3/src/chemistry.cpp:10-26,64-68: CV_BDF, dense Newton, stop time
+// = dt, 5000 max steps). Step/order control, the WRMS error test with
+// w_i = 1/(rtol |y_i| + atol), the Newton convergence test and the exact stop
+// at tout follow CVODE's published algorithm (Hindmarsh et al., ACM TOMS
+// 31(3), 2005). GPU-specific choices: the Newton matrix I - gamma J is rebuilt
+// from an analytic Jacobian at every setup (no saved J), and it is factored
+// without pivoting (a zero or non-finite pivot is a recoverable failure that
+// reduces the step, where I - gamma J tends to I).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace hf {
+
+struct BdfCounters {
+    int steps = 0, rhs = 0, jac = 0, lu = 0, newton = 0, ncf = 0, netf = 0;
+};
+
+enum { BDF_OK = 0, BDF_FAIL = 3 };
+
+template <int N, class Sys, int ZSTRIDE = 64>
+struct Bdf {
+    static constexpr int QMAX = 5;
+    static constexpr double UROUND = 2.220446049250313e-16;
+
+    // Nordsieck history: element (j, i) at zs[(j*N + i) * ZSTRIDE]; the caller
+    // points zs at this lane's slot of an LDS (or global) array laid out
+    // [j][i][lane], so consecutive lanes touch consecutive doubles.
+    double* zs;
+    double ewt[N], acor[N];
+    double M[N * N];
+    double tau[QMAX + 2];
+    double l[QMAX + 1];
+    double tq1, tq2, tq3, tq4, tq5;
+    double tn, h, hprime, hscale, eta, etamax, gamma, gammap, gamrat, rl1, crate, acnrm, saved_tq5;
+    double rtol, atol;
+    int q, qprime, L, qwait, nst, nstlp;
+    bool force_setup, jfresh;
+    BdfCounters c;
+
+    // ------------------------------------------------------------ helpers
+    template <int K>
+    __device__ __forceinline__ static double sel(const double (&a)[K], int idx) {
+        double r = a[0];
+        #pragma unroll
+        for (int j = 1; j < K; ++j) r = (j == idx) ? a[j] : r;
+        return r;
+    }
+    __device__ __forceinline__ double& Z(int j, int i) const { return zs[(j * N + i) * ZSTRIDE]; }
+    __device__ __forceinline__ double zrow(int j, int i) const { return Z(j, i); }
+    __device__ __forceinline__ double wrms(const double* v) const {
+        double s = 0;
+        #pragma unroll
+        for (int i = 0; i < N; ++i) {
+            const double x = v[i] * ewt[i];
+            s = fma(x, x, s);
+        }
+        return sqrt(s * (1.0 / N));
+    }
+    __device__ __forceinline__ void set_ewt() {
+        #pragma unroll
+        for (int i = 0; i < N; ++i) ewt[i] = 1.0 / fma(rtol, fabs(Z(0, i)), atol);
+    }
+
+    __device__ __forceinline__ void rescale() {
+        double factor = eta;
+        #pragma unroll
+        for (int j = 1; j <= QMAX; ++j) {
+            if (j <= q) {
+                #pragma unroll
+                for (int i = 0; i < N; ++i) Z(j, i) *= factor;
+            }
+            factor *= eta;
+        }
+        h = hscale * eta;
+        hprime = h;
+        hscale = h;
+    }
+    __device__ __forceinline__ void predict() {
+        tn += h;
+        #pragma unroll
+        for (int k = 1; k <= QMAX; ++k)
+            #pragma unroll
+            for (int j = QMAX; j >= k; --j)
+                if (k <= q && j <= q) {
+                    #pragma unroll
+                    for (int i = 0; i < N; ++i) Z(j - 1, i) += Z(j, i);
+                }
+    }
+    __device__ __forceinline__ void restore(double saved_t) {
+        tn = saved_t;
+        #pragma unroll
+        for (int k = 1; k <= QMAX; ++k)
+            #pragma unroll
+            for (int j = QMAX; j >= k; --j)
+                if (k <= q && j <= q) {
+                    #pragma unroll
+                    for (int i = 0; i < N; ++i) Z(j - 1, i) -= Z(j, i);
+                }
+    }
+
+    // Order q -> q-1: subtract the top column's contribution.
+    __device__ __forceinline__ void decrease_order() {
+        if (q <= 2) return;
+        double lc[QMAX + 1];
+        #pragma unroll
+        for (int j = 0; j <= QMAX; ++j) lc[j] = 0;
+        lc[2] = 1;
+        double hsum = 0;
+        #pragma unroll
+        for (int j = 1; j <= QMAX - 2; ++j) {
+            if (j <= q - 2) {
+                hsum += tau[j];
+                const double xi = hsum / hscale;
+                #pragma unroll
+                for (int i = j + 2; i >= 2; --i) lc[i] = lc[i] * xi + lc[i - 1];
+            }
+        }
+        #pragma unroll
+        for (int i = 0; i < N; ++i) {
+            const double top = zrow(q, i);
+            #pragma unroll
+            for (int j = 2; j < QMAX; ++j)
+                if (j < q) Z(j, i) -= top * lc[j];
+        }
+    }
+    // Order q -> q+1: new top column from the saved correction in zn[QMAX].
+    __device__ __forceinline__ void increase_order() {
+        double lc[QMAX + 1];
+        #pragma unroll
+        for (int j = 0; j <= QMAX; ++j) lc[j] = 0;
+        lc[2] = 1;
+        double alpha1 = 1, prod = 1, xiold = 1, alpha0 = -1, hsum = hscale;
+        #pragma unroll
+        for (int j = 1; j < QMAX; ++j) {
+            if (j < q) {
+                hsum += tau[j + 1];
+                const double xi = hsum / hscale;
+                prod *= xi;
+                alpha0 -= 1.0 / (j + 1);
+                alpha1 += 1.0 / xi;
+                #pragma unroll
+                for (int i = j + 2; i >= 2; --i) lc[i] = lc[i] * xiold + lc[i - 1];
+                xiold = xi;
+            }
+        }
+        const double A1 = (-alpha0 - alpha1) / prod;
+        #pragma unroll
+        for (int i = 0; i < N; ++i) {
+            const double nz = A1 * Z(QMAX, i);
+            #pragma unroll
+            for (int j = 2; j <= QMAX; ++j) {
+                if (j <= q) Z(j, i) += lc[j] * nz;
+                if (j == q + 1) Z(j, i) = nz;
+            }
+        }
+    }
+    __device__ __forceinline__ void adjust_order(int dq) {
+        if (q == 2 && dq != 1) return;
+        if (dq == 1) increase_order();
+        else if (dq == -1) decrease_order();
+    }
+
+    __device__ __forceinline__ void set_bdf() {
+        #pragma unroll
+        for (int i = 0; i <= QMAX; ++i) l[i] = 0;
+        l[0] = l[1] = 1;
+        double xi_inv = 1, xistar_inv = 1, alpha0 = -1, alpha0_hat = -1, hsum = h;
+        if (q > 1) {
+            #pragma unroll
+            for (int j = 2; j < QMAX; ++j) {
+                if (j < q) {
+                    hsum += tau[j - 1];
+                    xi_inv = h / hsum;
+                    alpha0 -= 1.0 / j;
+                    #pragma unroll
+                    for (int i = j; i >= 1; --i) l[i] += l[i - 1] * xi_inv;
+                }
+            }
+            alpha0 -= 1.0 / q;
+            xistar_inv = -l[1] - alpha0;
+            hsum += sel(tau, q - 1);
+            xi_inv = h / hsum;
+            alpha0_hat = -l[1] - xi_inv;
+            #pragma unroll
+            for (int i = QMAX; i >= 1; --i)
+                if (i <= q) l[i] += l[i - 1] * xistar_inv;
+        }
+        const double lq = sel(l, q);
+        const double A1 = 1.0 - alpha0_hat + alpha0;
+        const double A2 = 1.0 + q * A1;
+        tq2 = fabs(A1 / (alpha0 * A2));
+        tq5 = fabs(A2 * xistar_inv / (lq * xi_inv));
+        if (qwait == 1) {
+            if (q > 1) {
+                const double C = xistar_inv / lq;
+                const double A3 = alpha0 + 1.0 / q;
+                const double A4 = alpha0_hat + xi_inv;
+                tq1 = fabs(C * (1.0 - A4 + A3) / A3);
+            } else {
+                tq1 = 1.0;
+            }
+            hsum += sel(tau, q);
+            xi_inv = h / hsum;
+            const double A5 = alpha0 - 1.0 / (q + 1);
+            const double A6 = alpha0_hat - xi_inv;
+            const double Cppinv = (1.0 - A6 + A5) / A2;
+            tq3 = fabs(Cppinv / (xi_inv * (q + 2) * A5));
+        }
+        tq4 = 0.1 / tq2;
+    }
+
+    // ------------------------------------------------------------ linear algebra
+    __device__ __forceinline__ bool lu_factor() {
+        #pragma unroll
+        for (int k = 0; k < N; ++k) {
+            const double p = M[k * N + k];
+            if (!(fabs(p) > 1e-300) || !__builtin_isfinite(p)) return false;
+            const double inv = 1.0 / p;
+            #pragma unroll
+            for (int i = k + 1; i < N; ++i) {
+                const double li = M[i * N + k] * inv;
+                M[i * N + k] = li;
+                #pragma unroll
+                for (int j = k + 1; j < N; ++j) M[i * N + j] = fma(-li, M[k * N + j], M[i * N + j]);
+            }
+        }
+        return true;
+    }
+    __device__ __forceinline__ void lu_solve(double* b) const {
+        #pragma unroll
+        for (int k = 0; k < N; ++k)
+            #pragma unroll
+            for (int i = k + 1; i < N; ++i) b[i] = fma(-M[i * N + k], b[k], b[i]);
+        #pragma unroll
+        for (int i = N - 1; i >= 0; --i) {
+            double s = b[i];
+            #pragma unroll
+            for (int j = i + 1; j < N; ++j) s = fma(-M[i * N + j], b[j], s);
+            b[i] = s / M[i * N + i];
+        }
+    }
+
+    __device__ __forceinline__ bool lsetup(Sys& sys, const double* y) {
+        double f[N];
+        ++c.jac;
+        ++c.lu;
+        if (!sys.jac(y, M, f)) return false;
+        #pragma unroll
+        for (int i = 0; i < N * N; ++i) M[i] *= -gamma;
+        #pragma unroll
+        for (int i = 0; i < N; ++i) M[i * N + i] += 1.0;
+        jfresh = true;
+        gammap = gamma;
+        gamrat = 1;
+        crate = 1;
+        nstlp = nst;
+        return lu_factor();
+    }
+
+    // 0 converged, 1 retry with a fresh matrix, 2 reduce the step, 3 rhs/setup failure
+    __device__ __forceinline__ int newton(Sys& sys, int& convfail) {
+        double y[N], f[N], delta[N];
+        #pragma unroll
+        for (int i = 0; i < N; ++i) y[i] = Z(0, i);
+        ++c.rhs;
+        if (!sys.rhs(y, f)) return 3;
+        const bool callsetup = nst == 0 || nst >= nstlp + 20 || fabs(gamrat - 1.0) > 0.3 || convfail != 0 || force_setup;
+        force_setup = false;
+        jfresh = false;
+        if (callsetup && !lsetup(sys, y)) return 3;
+        #pragma unroll
+        for (int i = 0; i < N; ++i) acor[i] = 0;
+        double delp = 0;
+        for (int m = 0; m < 3; ++m) {
+            ++c.newton;
+            #pragma unroll
+            for (int i = 0; i < N; ++i) delta[i] = -(fma(rl1, Z(1, i), acor[i]) - gamma * f[i]);
+            lu_solve(delta);
+            if (gamrat != 1.0) {
+                const double s = 2.0 / (1.0 + gamrat);
+                #pragma unroll
+                for (int i = 0; i < N; ++i) delta[i] *= s;
+            }
+            #pragma unroll
+            for (int i = 0; i < N; ++i) {
+                acor[i] += delta[i];
+                y[i] = Z(0, i) + acor[i];
+            }
+            const double del = wrms(delta);
+            if (m > 0) crate = fmax(0.3 * crate, del / delp);
+            const double dcon = del * fmin(1.0, crate) / tq4;
+            if (dcon <= 1.0) {
+                acnrm = (m == 0) ? del : wrms(acor);
+                return 0;
+            }
+            if (m == 2 || (m >= 1 && del > 2.0 * delp)) break;
+            delp = del;
+            ++c.rhs;
+            if (!sys.rhs(y, f)) return 3;
+        }
+        if (!jfresh) {
+            convfail = 1;
+            return 1;
+        }
+        return 2;
+    }
+
+    __device__ __forceinline__ int step(Sys& sys, double tout) {
+        const double saved_t = tn;
+        int ncf = 0, nef = 0, convfail = 0;
+        double dsm = 0;
+        for (;;) {
+            predict();
+            if ((tn - tout) * h > 0) tn = tout;
+            set_bdf();
+            rl1 = 1.0 / l[1];
+            gamma = h * rl1;
+            if (nst == 0) gammap = gamma;
+            gamrat = (nst > 0) ? gamma / gammap : 1.0;
+            int nf = newton(sys, convfail);
+            if (nf == 1) nf = newton(sys, convfail);
+            if (nf != 0) {
+                restore(saved_t);
+                ++ncf;
+                ++c.ncf;
+                etamax = 1;
+                if (ncf >= 10) return BDF_FAIL;
+                eta = 0.25;
+                convfail = 2;
+                rescale();
+                continue;
+            }
+            convfail = 0;
+            dsm = acnrm * tq2;
+            if (dsm <= 1.0) break;
+            ++nef;
+            ++c.netf;
+            restore(saved_t);
+            force_setup = true;
+            if (nef >= 7) return BDF_FAIL;
+            etamax = 1;
+            if (nef <= 3) {
+                eta = 1.0 / (pow(6.0 * dsm, 1.0 / L) + 1e-6);
+                eta = fmax(0.1, eta);
+                if (nef >= 2) eta = fmin(eta, 0.2);
+                rescale();
+                continue;
+            }
+            if (q > 1) {
+                eta = 0.1;
+                adjust_order(-1);
+                L = q;
+                --q;
+                qwait = L;
+                rescale();
+                continue;
+            }
+            eta = 0.1;
+            h *= eta;
+            hprime = h;
+            hscale = h;
+            qwait = 10;
+            double f[N], y0[N];
+            #pragma unroll
+            for (int i = 0; i < N; ++i) y0[i] = Z(0, i);
+            ++c.rhs;
+            if (!sys.rhs(y0, f)) return BDF_FAIL;
+            #pragma unroll
+            for (int i = 0; i < N; ++i) Z(1, i) = h * f[i];
+        }
+        // accept
+        ++nst;
+        ++c.steps;
+        #pragma unroll
+        for (int i = QMAX; i >= 2; --i)
+            if (i <= q) tau[i] = tau[i - 1];
+        if (q == 1 && nst > 1) tau[2] = tau[1];
+        tau[1] = h;
+        #pragma unroll
+        for (int j = 0; j <= QMAX; ++j)
+            if (j <= q) {
+                #pragma unroll
+                for (int i = 0; i < N; ++i) Z(j, i) = fma(l[j], acor[i], Z(j, i));
+            }
+        --qwait;
+        if (qwait == 1 && q != QMAX) {
+            #pragma unroll
+            for (int i = 0; i < N; ++i) Z(QMAX, i) = acor[i];
+            saved_tq5 = tq5;
+        }
+        prepare_next(dsm);
+        return BDF_OK;
+    }
+
+    __device__ __forceinline__ void prepare_next(double dsm) {
+        if (etamax == 1.0) {
+            qwait = max(qwait, 2);
+            qprime = q;
+            hprime = h;
+            eta = 1.0;
+        } else {
+            const double etaq = 1.0 / (pow(6.0 * dsm, 1.0 / L) + 1e-6);
+            if (qwait != 0) {
+                eta = etaq;
+                qprime = q;
+            } else {
+                qwait = 2;
+                double etaqm1 = 0, etaqp1 = 0;
+                if (q > 1) {
+                    double tmp[N];
+                    #pragma unroll
+                    for (int i = 0; i < N; ++i) tmp[i] = zrow(q, i);
+                    const double ddn = wrms(tmp) * tq1;
+                    etaqm1 = 1.0 / (pow(6.0 * ddn, 1.0 / q) + 1e-6);
+                }
+                if (q != QMAX && saved_tq5 != 0) {
+                    const double cquot = (tq5 / saved_tq5) * pow(h / tau[2], (double)L);
+                    double tmp[N];
+                    #pragma unroll
+                    for (int i = 0; i < N; ++i) tmp[i] = acor[i] - cquot * Z(QMAX, i);
+                    const double dup = wrms(tmp) * tq3;
+                    etaqp1 = 1.0 / (pow(10.0 * dup, 1.0 / (L + 1)) + 1e-6);
+                }
+                const double etam = fmax(etaqm1, fmax(etaq, etaqp1));
+                if (etam < 1.5) {
+                    eta = 1.0;
+                    qprime = q;
+                } else if (etam == etaq) {
+                    eta = etaq;
+                    qprime = q;
+                } else if (etam == etaqm1) {
+                    eta = etaqm1;
+                    qprime = q - 1;
+                } else {
+                    eta = etaqp1;
+                    qprime = q + 1;
+                    #pragma unroll
+                    for (int i = 0; i < N; ++i) Z(QMAX, i) = acor[i];
+                }
+            }
+            if (eta < 1.5) {
+                eta = 1.0;
+                hprime = h;
+            } else {
+                eta = fmin(eta, etamax);
+                hprime = h * eta;
+            }
+        }
+        etamax = 10.0;
+    }
+
+    __device__ __forceinline__ double initial_step(Sys& sys, const double* f0, double tout) {
+        const double tdist = fabs(tout);
+        const double tround = UROUND * tdist;
+        if (tdist < 2.0 * tround) return -1;
+        const double hlb = 100.0 * tround;
+        double hub_inv = 0;
+        #pragma unroll
+        for (int i = 0; i < N; ++i) hub_inv = fmax(hub_inv, fabs(f0[i]) / fma(0.1, fabs(Z(0, i)), atol));
+        double hub = 0.1 * tdist;
+        if (hub * hub_inv > 1.0) hub = 1.0 / hub_inv;
+        double hg = sqrt(hlb * hub);
+        if (hub < hlb) return hg;
+        double hnew = hg;
+        for (int count = 1;; ++count) {
+            bool ok = false;
+            double yddnrm = 0;
+            for (int tries = 0; tries < 4 && !ok; ++tries) {
+                double y[N], f1[N];
+                #pragma unroll
+                for (int i = 0; i < N; ++i) y[i] = fma(hg, f0[i], Z(0, i));
+                ++c.rhs;
+                if (sys.rhs(y, f1)) {
+                    #pragma unroll
+                    for (int i = 0; i < N; ++i) f1[i] = (f1[i] - f0[i]) / hg;
+                    yddnrm = wrms(f1);
+                    ok = __builtin_isfinite(yddnrm);
+                }
+                if (!ok) hg *= 0.2;
+            }
+            if (!ok) return -1;
+            hnew = (yddnrm * hub * hub > 2.0) ? sqrt(2.0 / yddnrm) : sqrt(hg * hub);
+            if (count >= 4) break;
+            const double hrat = hnew / hg;
+            if (hrat > 0.5 && hrat < 2.0) break;
+            if (count > 1 && hrat > 2.0) {
+                hnew = hg;
+                break;
+            }
+            hg = hnew;
+        }
+        return fmin(fmax(0.5 * hnew, hlb), hub);
+    }
+
+    // Integrate y from t = 0 to tout in place. Returns BDF_OK or BDF_FAIL.
+    __device__ __forceinline__ int integrate(Sys& sys, double* y, double tout, int maxsteps) {
+        #pragma unroll
+        for (int i = 0; i < N; ++i) Z(0, i) = y[i];
+        tn = 0;
+        double f0[N];
+        ++c.rhs;
+        if (!sys.rhs(y, f0)) return BDF_FAIL;
+        set_ewt();
+        const double h0 = initial_step(sys, f0, tout);
+        if (!(h0 > 0)) return BDF_FAIL;
+        h = h0;
+        #pragma unroll
+        for (int i = 0; i < N; ++i) Z(1, i) = h * f0[i];
+        q = qprime = 1;
+        L = 2;
+        qwait = 2;
+        hscale = hprime = h;
+        eta = 1;
+        etamax = 10000.0;
+        #pragma unroll
+        for (int i = 0; i < QMAX + 2; ++i) tau[i] = 0;
+        tq1 = tq2 = tq3 = tq4 = tq5 = 0;
+        nst = nstlp = 0;
+        gammap = 0;
+        gamrat = 1;
+        crate = 1;
+        saved_tq5 = 0;
+        force_setup = false;
+        jfresh = false;
+        bool first = true;
+        for (;;) {
+            if (nst >= maxsteps) return BDF_FAIL;
+            if (!first) set_ewt();
+            if ((tn + hprime - tout) * hprime > 0) {
+                hprime = (tout - tn) * (1.0 - 4.0 * UROUND);
+                eta = hprime / h;
+            }
+            if (hprime != h) {
+                if (!first && qprime != q) {
+                    adjust_order(qprime - q);
+                    q = qprime;
+                    L = q + 1;
+                    qwait = L;
+                }
+                rescale();
+            }
+            first = false;
+            if (step(sys, tout) != BDF_OK) return BDF_FAIL;
+            const double troundoff = 100.0 * UROUND * (fabs(tn) + fabs(h));
+            if (fabs(tn - tout) <= troundoff || (tn - tout) * h >= 0) {
+                const double s = (tout - tn) / h;
+                #pragma unroll
+                for (int i = 0; i < N; ++i) {
+                    double v = zrow(q, i);
+                    #pragma unroll
+                    for (int j = QMAX - 1; j >= 0; --j)
+                        if (j < q) v = fma(s, v, Z(j, i));
+                    y[i] = v;
+                }
+                return BDF_OK;
+            }
+        }
+    }
+};
+
+}  // namespace hf

@@ -1,0 +1,130 @@
+// Chemistry system of the V3 reaction step and the per-cell advance that
+// replaces Chemistry::advance (HyperFlow_V3/src/chemistry.cpp:55-76).
+#pragma once
+#include "hf_bdf.hpp"
+#include "hf_device.hpp"
+#include "hyperflow.h"
+
+namespace hf {
+
+// Constant-volume adiabatic cell: unknowns y = Y[0..NS-2] at fixed rho and
+// specific internal energy e; the last species closes the mass,
+// Y[NS-1] = 1 - sum(y) (chemistry.cpp:36-53). T is recovered from e at every
+// evaluation, warm-started from the previous evaluation's T.
+template <class T>
+struct ChemSys {
+    static constexpr int NS = T::NS, N = NS - 1;
+    const MechData<T>& d;
+    double rho, e, t;
+    const double* shift;   // optional per-reaction ln A shift (sensitivities)
+
+    __device__ __forceinline__ bool temp(const double* y, double* Y) {
+        double s = 0;
+        #pragma unroll
+        for (int k = 0; k < N; ++k) { Y[k] = y[k]; s += y[k]; }
+        Y[N] = 1.0 - s;
+        double tt = t;
+        if (!temperature_from_e(d, Y, e, tt)) return false;
+        t = tt;
+        return true;
+    }
+    __device__ __forceinline__ bool rhs(const double* y, double* f) {
+        double Y[NS], C[NS], w[NS];
+        if (!temp(y, Y)) return false;
+        #pragma unroll
+        for (int k = 0; k < NS; ++k) C[k] = rho * Y[k] * d.invW[k];
+        production_rates(d, t, C, w, shift);
+        bool ok = true;
+        const double rr = 1.0 / rho;
+        #pragma unroll
+        for (int k = 0; k < N; ++k) {
+            f[k] = w[k] * d.W[k] * rr;
+            ok = ok && finite(f[k]);
+        }
+        return ok;
+    }
+    __device__ __forceinline__ bool jac(const double* y, double* J, double* f) {
+        double Y[NS];
+        if (!temp(y, Y)) return false;
+        chem_jacobian(d, t, rho, Y, J, f, shift);
+        bool ok = true;
+        #pragma unroll
+        for (int i = 0; i < N * N; ++i) ok = ok && finite(J[i]);
+        return ok;
+    }
+};
+
+struct ChemParams {
+    double rtol, atol;
+    int maxsteps;
+};
+
+// One cell: U holds nv = NS+3 component planes U[n*ld + i]; the advanced
+// species planes go to out[k*ldo + i]. Momenta and energy are not changed.
+// Returns HF_OK, HF_EINVAL (bad dt), HF_EDOMAIN (inadmissible state before or
+// after, or accepted species below -1e-13) or HF_ECONV (integrator failure).
+// zs: this lane's slot of the wave's Nordsieck slab ([order][unknown][64 lanes]).
+template <class T>
+__device__ __forceinline__ int advance_cell(const MechData<T>& d, const double* __restrict__ U, long i, long ld,
+                                            double dt, double* __restrict__ out, long ldo, const ChemParams& p,
+                                            BdfCounters& c, double* zs) {
+    constexpr int NS = T::NS, NV = NS + 3, N = NS - 1;
+    if (!(dt > 0) || !finite(dt)) return HF_EINVAL;
+    double u[NV];
+    #pragma unroll
+    for (int n = 0; n < NV; ++n) u[n] = U[n * ld + i];
+    Props pr;
+    if (!properties(d, u, pr, 1000.0)) return HF_EDOMAIN;
+    double rho = 0;
+    #pragma unroll
+    for (int k = 0; k < NS; ++k) rho += u[k];
+    const double ux = u[NS] / rho, vy = u[NS + 1] / rho;
+    const double e = u[NS + 2] / rho - 0.5 * (ux * ux + vy * vy);
+    double y[N];
+    #pragma unroll
+    for (int k = 0; k < N; ++k) y[k] = u[k] / rho;
+    ChemSys<T> sys{d, rho, e, pr.T, nullptr};
+    Bdf<N, ChemSys<T>> bdf;
+    bdf.zs = zs;
+    bdf.rtol = p.rtol;
+    bdf.atol = p.atol;
+    const int flag = bdf.integrate(sys, y, dt, p.maxsteps);
+    c = bdf.c;
+    if (flag != BDF_OK) return HF_ECONV;
+    double sum = 0;
+    bool ok = true;
+    #pragma unroll
+    for (int k = 0; k < N; ++k) {
+        ok = ok && finite(y[k]) && y[k] >= -1e-13;
+        u[k] = rho * fmax(0.0, y[k]);
+        sum += u[k];
+    }
+    if (!ok) return HF_EDOMAIN;
+    u[N] = rho - sum;
+    if (!properties(d, u, pr, sys.t)) return HF_EDOMAIN;
+    #pragma unroll
+    for (int k = 0; k < NS; ++k) out[k * ldo + i] = u[k];
+    return HF_OK;
+}
+
+// counters: [steps, rhs, jac, lu, newton, conv fails, err fails, max steps of a cell]
+__device__ __forceinline__ void reduce_counters(const BdfCounters& c, unsigned long long* counters) {
+    unsigned long long v[7] = {(unsigned long long)c.steps, (unsigned long long)c.rhs, (unsigned long long)c.jac,
+                               (unsigned long long)c.lu, (unsigned long long)c.newton, (unsigned long long)c.ncf,
+                               (unsigned long long)c.netf};
+    int mx = c.steps;
+    #pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        #pragma unroll
+        for (int k = 0; k < 7; ++k) v[k] += __shfl_xor(v[k], off, 64);
+        mx = max(mx, __shfl_xor(mx, off, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        #pragma unroll
+        for (int k = 0; k < 7; ++k)
+            if (v[k]) atomicAdd(&counters[k], v[k]);
+        atomicMax(&counters[7], (unsigned long long)mx);
+    }
+}
+
+}  // namespace hf

@@ -1,0 +1,506 @@
+// Device-side thermodynamics and kinetics for gfx950, specialised on a
+// mechanism topology T (generated by mechgen: species order, reactant/product
+// lists, reaction kinds, efficiency species). Every species index is a
+// compile-time constant, so per-cell arrays (concentrations, production rates,
+// Jacobian rows) live in registers. Numeric coefficients come from MechData<T>
+// in device memory and are read with uniform (scalar) loads.
+//
+// Physics restated from the reference's call sites:
+//   properties   HyperFlow_V3/src/physics.cpp:4-15 (Y = rhoY/rho, not renormalised;
+//                T from e by Newton; p = rho R T / Wbar; a = sqrt(cp/cv R T / Wbar))
+//   minmod       HyperFlow_V3/src/physics.cpp:40-44
+//   hllc         HyperFlow_V3/src/physics.cpp:17-38 (Davis wave speeds)
+//   kinetics     Cantera GasKinetics net production rates, as called at
+//                HyperFlow_V3/src/chemistry.cpp:45
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <utility>
+
+namespace hf {
+
+constexpr double kR = 8314.462618;
+constexpr double kP0 = 101325.0;
+constexpr double kSmall = 1e-300;
+constexpr double kLn10 = 2.302585092994045684;
+
+template <class T>
+struct MechData {
+    static constexpr int NS = T::NS, NR = T::NR, NEFF = T::NEFF;
+    double W[NS], invW[NS], Tmid[NS];
+    // NASA7 low/high coefficient sets a1..a7 (cp/R = a1 + a2 T + ... + a5 T^4)
+    double lo[NS][7], hi[NS][7];
+    // reaction coefficients, SI kmol units; Ea/R in K
+    double lnA[NR], b[NR], EaR[NR];
+    double lnA0[NR], b0[NR], Ea0R[NR];
+    double troeA[NR], troeRT3[NR], troeRT1[NR], troeT2[NR];
+    double defeff[NR];          // default third-body efficiency
+    double effd[NR][NEFF];      // listed efficiency minus the default
+    int common_tmid;            // every species switches range at the same T
+    double tmid;
+};
+
+// ----------------------------------------------------------------- utilities
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(f);
+    }
+}
+
+__device__ __forceinline__ bool finite(double x) { return __builtin_isfinite(x); }
+
+// ----------------------------------------------------------------- thermo
+// Mixture energy polynomial for fixed mass fractions: with yw_k = Y_k / W_k,
+// e/R = sum_k yw_k (h_k/RT - 1) T and cv/R = sum_k yw_k (cp_k/R - 1).
+// When every species shares Tmid the sums collapse to one polynomial per range.
+template <class T>
+struct EnergyPoly {
+    double lo[6], hi[6];   // e/R = c0 T + c1 T^2/2 + c2 T^3/3 + c3 T^4/4 + c4 T^5/5 + c5
+    double tmid;
+
+    __device__ __forceinline__ void build(const MechData<T>& d, const double* Y) {
+        #pragma unroll
+        for (int i = 0; i < 6; ++i) { lo[i] = 0; hi[i] = 0; }
+        double s = 0;
+        #pragma unroll
+        for (int k = 0; k < T::NS; ++k) {
+            const double yw = Y[k] * d.invW[k];
+            s += yw;
+            #pragma unroll
+            for (int i = 0; i < 5; ++i) {
+                lo[i] = fma(yw, d.lo[k][i], lo[i]);
+                hi[i] = fma(yw, d.hi[k][i], hi[i]);
+            }
+            lo[5] = fma(yw, d.lo[k][5], lo[5]);
+            hi[5] = fma(yw, d.hi[k][5], hi[5]);
+        }
+        lo[0] -= s;
+        hi[0] -= s;
+        tmid = d.tmid;
+    }
+    // e (J/kg) and cv (J/kg/K) at temperature t
+    __device__ __forceinline__ void eval(double t, double& e, double& cv) const {
+        const double* c = t <= tmid ? lo : hi;
+        cv = kR * fma(t, fma(t, fma(t, fma(t, c[4], c[3]), c[2]), c[1]), c[0]);
+        const double p = fma(t, fma(t, fma(t, fma(t, c[4] * 0.2, c[3] * 0.25), c[2] * (1.0 / 3.0)), c[1] * 0.5), c[0]);
+        e = kR * fma(t, p, c[5]);
+    }
+};
+
+// Per-species evaluation (used when species switch range at different T).
+template <class T>
+__device__ __forceinline__ void energy_species(const MechData<T>& d, const double* Y, double t, double& e,
+                                               double& cv) {
+    e = 0;
+    cv = 0;
+    #pragma unroll
+    for (int k = 0; k < T::NS; ++k) {
+        const double* a = t <= d.Tmid[k] ? d.lo[k] : d.hi[k];
+        const double yw = Y[k] * d.invW[k];
+        const double cpr = fma(t, fma(t, fma(t, fma(t, a[4], a[3]), a[2]), a[1]), a[0]);
+        const double hrt = fma(t, fma(t, fma(t, fma(t, a[4] * 0.2, a[3] * 0.25), a[2] * (1.0 / 3.0)), a[1] * 0.5), a[0]) + a[5] / t;
+        cv = fma(yw, cpr - 1.0, cv);
+        e = fma(yw, (hrt - 1.0) * t, e);
+    }
+    e *= kR;
+    cv *= kR;
+}
+
+// Energy -> temperature Newton. Converged when the applied step is below
+// 1e-9 T: Newton's quadratic convergence leaves the result at round-off level.
+// Returns false when the iteration does not converge or leaves T <= 0.
+template <class T>
+__device__ __forceinline__ bool temperature_from_e(const MechData<T>& d, const double* Y, double e_target,
+                                                   double& t, int max_iter = 60) {
+    if (!(t > 0) || !finite(t)) t = 1000.0;
+    if (d.common_tmid) {
+        EnergyPoly<T> P;
+        P.build(d, Y);
+        for (int it = 0; it < max_iter; ++it) {
+            double e, cv;
+            P.eval(t, e, cv);
+            const double dT = (e - e_target) / cv;
+            double tn = t - dT;
+            tn = tn > 0 ? tn : 0.5 * t;
+            const bool done = fabs(dT) <= 1e-9 * t;
+            t = tn;
+            if (done) return finite(t) && cv > 0;
+        }
+        return false;
+    }
+    for (int it = 0; it < max_iter; ++it) {
+        double e, cv;
+        energy_species(d, Y, t, e, cv);
+        const double dT = (e - e_target) / cv;
+        double tn = t - dT;
+        tn = tn > 0 ? tn : 0.5 * t;
+        const bool done = fabs(dT) <= 1e-9 * t;
+        t = tn;
+        if (done) return finite(t) && cv > 0;
+    }
+    return false;
+}
+
+// Frozen cp, cv and 1/Wbar at (t, Y).
+template <class T>
+__device__ __forceinline__ void cp_cv_rw(const MechData<T>& d, const double* Y, double t, double& cp, double& cv,
+                                          double& rw) {
+    cp = 0;
+    rw = 0;
+    #pragma unroll
+    for (int k = 0; k < T::NS; ++k) {
+        const double* a = t <= d.Tmid[k] ? d.lo[k] : d.hi[k];
+        const double yw = Y[k] * d.invW[k];
+        rw += yw;
+        cp = fma(yw, fma(t, fma(t, fma(t, fma(t, a[4], a[3]), a[2]), a[1]), a[0]), cp);
+    }
+    cv = kR * (cp - rw);
+    cp *= kR;
+}
+
+struct Props {
+    double p, a, T;
+};
+
+// properties(u) of physics.cpp:4-15 on nv = NS+3 conserved values.
+// Returns false for an inadmissible state (the reference throws domain_error).
+template <class T>
+__device__ __forceinline__ bool properties(const MechData<T>& d, const double* u, Props& out, double t_guess) {
+    constexpr int NS = T::NS;
+    double rho = 0;
+    bool ok = true;
+    #pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        rho += u[k];
+        ok = ok && finite(u[k]) && u[k] >= 0;
+    }
+    ok = ok && finite(u[NS]) && finite(u[NS + 1]) && finite(u[NS + 2]) && rho > 0;
+    if (!ok) return false;
+    double Y[NS];
+    #pragma unroll
+    for (int k = 0; k < NS; ++k) Y[k] = u[k] / rho;
+    const double ux = u[NS] / rho, vy = u[NS + 1] / rho;
+    const double e = u[NS + 2] / rho - 0.5 * (ux * ux + vy * vy);
+    double t = t_guess;
+    if (!temperature_from_e(d, Y, e, t)) return false;
+    double cp, cv, rw;
+    cp_cv_rw(d, Y, t, cp, cv, rw);
+    out.T = t;
+    out.p = rho * kR * t * rw;
+    out.a = sqrt(cp / cv * kR * t * rw);
+    return out.T > 0 && out.p > 0 && out.a > 0 && finite(out.a);
+}
+
+__device__ __forceinline__ double minmod(double x, double y) {
+    return x * y > 0 ? copysign(fmin(fabs(x), fabs(y)), x) : 0.0;
+}
+
+// hllc(l, r, n, pl, pr) of physics.cpp:17-38; N = normal momentum component.
+template <int NS>
+__device__ __forceinline__ void hllc(const double* l, const double* r, int N, const Props& pl, const Props& pr,
+                                     double* f) {
+    constexpr int NV = NS + 3, E = NS + 2;
+    double rl = 0, rr = 0;
+    #pragma unroll
+    for (int k = 0; k < NS; ++k) { rl += l[k]; rr += r[k]; }
+    const double ul = (N == NS ? l[NS] : l[NS + 1]) / rl;
+    const double ur = (N == NS ? r[NS] : r[NS + 1]) / rr;
+    const double sl = fmin(ul - pl.a, ur - pr.a), sr = fmax(ul + pl.a, ur + pr.a);
+    if (sl >= 0 || sr <= 0) {
+        const bool useL = sl >= 0;
+        const double v = useL ? ul : ur, p = useL ? pl.p : pr.p;
+        #pragma unroll
+        for (int c = 0; c < NV; ++c) f[c] = v * (useL ? l[c] : r[c]);
+        if (N == NS) f[NS] += p; else f[NS + 1] += p;
+        f[E] += p * v;
+        return;
+    }
+    const double dl = rl * (sl - ul), dr = rr * (sr - ur);
+    const double sm = (pr.p - pl.p + dl * ul - dr * ur) / (dl - dr);
+    const bool left = sm >= 0;
+    const double v = left ? ul : ur, s = left ? sl : sr, p = left ? pl.p : pr.p, rq = left ? rl : rr;
+    const double scale = (s - v) / (s - sm);
+    double rs = 0;
+    double q[NV];
+    #pragma unroll
+    for (int c = 0; c < NV; ++c) q[c] = left ? l[c] : r[c];
+    #pragma unroll
+    for (int k = 0; k < NS; ++k) rs += q[k] * scale;
+    #pragma unroll
+    for (int c = 0; c < NV; ++c) {
+        double star = q[c] * scale;
+        if (c == N) star = rs * sm;
+        if (c == E) star += rs * (sm - v) * (sm + p / (rq * (s - v)));
+        double fc = v * q[c];
+        if (c == N) fc += p;
+        if (c == E) fc += p * v;
+        f[c] = fc + s * (star - q[c]);
+    }
+}
+
+// ----------------------------------------------------------------- kinetics
+// Per-species h/RT and g/RT = h/RT - s/R at temperature t.
+template <class T>
+__device__ __forceinline__ void species_h_g(const MechData<T>& d, double t, double logt, double* hrt, double* grt) {
+    const double it = 1.0 / t;
+    #pragma unroll
+    for (int k = 0; k < T::NS; ++k) {
+        const double* a = t <= d.Tmid[k] ? d.lo[k] : d.hi[k];
+        const double h = fma(t, fma(t, fma(t, fma(t, a[4] * 0.2, a[3] * 0.25), a[2] * (1.0 / 3.0)), a[1] * 0.5), a[0]) + a[5] * it;
+        const double s = fma(a[0], logt, fma(t, fma(t, fma(t, fma(t, a[4] * 0.25, a[3] * (1.0 / 3.0)), a[2] * 0.5), a[1]), a[6]));
+        hrt[k] = h;
+        grt[k] = h - s;
+    }
+}
+
+// Rate constants of reaction I: forward kf (including [M] for three-body and
+// the falloff blend), and 1/Kc. Optional derivatives for the Jacobian:
+// dlnkf_dT (at fixed [M]), dkf_dM, dln_rKc_dT.
+template <class T, int I, bool DERIV>
+struct RateI {
+    double kf, rkc, dlnkf_dT, dkf_dM, dlnrkc_dT;
+
+    __device__ __forceinline__ RateI(const MechData<T>& d, double t, double logt, double it, double logc0,
+                                     const double* hrt, const double* grt, double Mc, double lnA_shift) {
+        constexpr int kind = T::kind[I];
+        double kinf = exp(d.lnA[I] + lnA_shift + d.b[I] * logt - d.EaR[I] * it);
+        dlnkf_dT = 0;
+        dkf_dM = 0;
+        if constexpr (DERIV) dlnkf_dT = (d.b[I] + d.EaR[I] * it) * it;
+        if constexpr (kind == 0) {
+            kf = kinf;
+        } else if constexpr (kind == 1) {
+            kf = kinf * Mc;
+            if constexpr (DERIV) dkf_dM = kinf;
+        } else {
+            const double k0 = exp(d.lnA0[I] + lnA_shift + d.b0[I] * logt - d.Ea0R[I] * it);
+            const double pr = Mc * k0 / (kinf + kSmall);
+            double F = 1.0, g_lpr = 0.0, g_lc = 0.0, dlc_dT = 0.0;
+            if constexpr (T::troe[I] > 0) {
+                const double A = d.troeA[I];
+                const double e3 = exp(-t * d.troeRT3[I]), e1 = exp(-t * d.troeRT1[I]);
+                double fc = (1.0 - A) * e3 + A * e1;
+                double dfc = -(1.0 - A) * d.troeRT3[I] * e3 - A * d.troeRT1[I] * e1;
+                if constexpr (T::troe[I] == 4) {
+                    const double e2 = exp(-d.troeT2[I] * it);
+                    fc += e2;
+                    dfc += d.troeT2[I] * it * it * e2;
+                }
+                const double fcc = fmax(fc, kSmall);
+                const double lc = log10(fcc);
+                const double lpr = log10(fmax(pr, kSmall));
+                const double c = -0.4 - 0.67 * lc, n = 0.75 - 1.27 * lc;
+                const double u = lpr + c;
+                const double den = n - 0.14 * u;
+                const double f1 = u / den;
+                const double q = 1.0 / (1.0 + f1 * f1);
+                const double lgf = lc * q;
+                F = exp(kLn10 * lgf);
+                if constexpr (DERIV) {
+                    const double df1_du = n / (den * den), df1_dn = -u / (den * den);
+                    const double dlgf_df1 = -2.0 * lc * f1 * q * q;
+                    g_lpr = pr > kSmall ? dlgf_df1 * df1_du : 0.0;
+                    g_lc = q + dlgf_df1 * (df1_du * (-0.67) + df1_dn * (-1.27));
+                    dlc_dT = fc > kSmall ? dfc / (fcc * kLn10) : 0.0;
+                }
+            }
+            const double opr = 1.0 / (1.0 + pr);
+            kf = kinf * pr * opr * F;
+            if constexpr (DERIV) {
+                const double dlnk0 = (d.b0[I] + d.Ea0R[I] * it) * it;
+                const double dlnpr_dT = dlnk0 - dlnkf_dT;
+                // d ln F / dT = ln10 (dlog10F/dlPr dlPr/dT + dlog10F/dLc dLc/dT)
+                const double dlnF_dT = g_lpr * dlnpr_dT + kLn10 * g_lc * dlc_dT;
+                dlnkf_dT = dlnkf_dT + opr * dlnpr_dT + dlnF_dT;
+                dkf_dM = k0 * F * opr * (opr + g_lpr);
+            }
+        }
+        // 1/Kc = exp(sum nu g/RT - dn ln(p0/RT))
+        if constexpr (T::rev[I]) {
+            double dg = 0, dh = 0;
+            int dn = 0;
+            static_for<0, T::NRMAX>([&](auto j) {
+                constexpr int r = T::reac[I][j], p = T::prod[I][j];
+                if constexpr (r >= 0) { dg -= grt[r]; dn -= 1; if constexpr (DERIV) dh -= hrt[r]; }
+                if constexpr (p >= 0) { dg += grt[p]; dn += 1; if constexpr (DERIV) dh += hrt[p]; }
+            });
+            rkc = exp(dg - dn * logc0);
+            if constexpr (DERIV) dlnrkc_dT = (dn - dh) * it;
+        } else {
+            rkc = 0;
+            dlnrkc_dT = 0;
+        }
+    }
+};
+
+template <class T, int I>
+__device__ __forceinline__ double third_body(const MechData<T>& d, const double* C, double Ctot) {
+    double M = d.defeff[I] * Ctot;
+    static_for<0, T::NEFF>([&](auto j) {
+        constexpr int k = T::effsp[I][j];
+        if constexpr (k >= 0) M = fma(d.effd[I][j], C[k], M);
+    });
+    return M;
+}
+
+// Net production rates wdot (kmol/m^3/s) at (t, C). lnA_shift (optional, per
+// reaction) multiplies both k and k0 of reaction i by exp(lnA_shift[i]).
+template <class T>
+__device__ __forceinline__ void production_rates(const MechData<T>& d, double t, const double* C, double* wdot,
+                                                 const double* lnA_shift = nullptr, double* qout = nullptr) {
+    constexpr int NS = T::NS;
+    const double logt = log(t), it = 1.0 / t;
+    const double logc0 = log(kP0 / (kR * t));
+    double hrt[NS], grt[NS];
+    species_h_g(d, t, logt, hrt, grt);
+    double Ctot = 0;
+    #pragma unroll
+    for (int k = 0; k < NS; ++k) { Ctot += C[k]; wdot[k] = 0; }
+    static_for<0, T::NR>([&](auto ic) {
+        constexpr int I = decltype(ic)::value;
+        double Mc = 0;
+        if constexpr (T::kind[I] != 0) Mc = third_body<T, I>(d, C, Ctot);
+        const RateI<T, I, false> k(d, t, logt, it, logc0, hrt, grt, Mc, lnA_shift ? lnA_shift[I] : 0.0);
+        double fwd = k.kf, bwd = k.kf * k.rkc;
+        static_for<0, T::NRMAX>([&](auto j) {
+            constexpr int r = T::reac[I][j], p = T::prod[I][j];
+            if constexpr (r >= 0) fwd *= C[r];
+            if constexpr (p >= 0) bwd *= C[p];
+        });
+        const double q = fwd - bwd;
+        if (qout) qout[I] = q;
+        static_for<0, T::NRMAX>([&](auto j) {
+            constexpr int r = T::reac[I][j], p = T::prod[I][j];
+            if constexpr (r >= 0) wdot[r] -= q;
+            if constexpr (p >= 0) wdot[p] += q;
+        });
+    });
+}
+
+// Analytic Jacobian of the V3 chemistry right-hand side
+//   f_k(y) = wdot_k(T(y), C(y)) W_k / rho,   k < NS-1,
+// with y = Y[0..NS-2], Y[NS-1] = 1 - sum(y), C_k = rho Y_k / W_k and T(y)
+// defined by e(T, Y) = e at fixed e (HyperFlow_V3/src/chemistry.cpp:36-53):
+//   df_k/dy_j = W_k [dwdot_k/dC_j / W_j - dwdot_k/dC_last / W_last]
+//             + W_k/rho * dwdot_k/dT * dT/dy_j,  dT/dy_j = -(e_j - e_last)/cv.
+// Writes J (row-major N x N, N = NS-1) and f.
+template <class T>
+__device__ __forceinline__ void chem_jacobian(const MechData<T>& d, double t, double rho, const double* Y,
+                                              double* J, double* f, const double* lnA_shift = nullptr) {
+    constexpr int NS = T::NS, N = NS - 1, L = NS - 1;
+    const double logt = log(t), it = 1.0 / t;
+    const double logc0 = log(kP0 / (kR * t));
+    double hrt[NS], grt[NS], C[NS], wd[NS], dwdT[NS], dwdCL[N];
+    species_h_g(d, t, logt, hrt, grt);
+    double Ctot = 0;
+    #pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        C[k] = rho * Y[k] * d.invW[k];
+        Ctot += C[k];
+        wd[k] = 0;
+        dwdT[k] = 0;
+    }
+    #pragma unroll
+    for (int k = 0; k < N; ++k) {
+        dwdCL[k] = 0;
+        #pragma unroll
+        for (int j = 0; j < N; ++j) J[k * N + j] = 0;
+    }
+    // J accumulates dwdot_k/dC_j (k, j < N) in this pass.
+    static_for<0, T::NR>([&](auto ic) {
+        constexpr int I = decltype(ic)::value;
+        double Mc = 0;
+        if constexpr (T::kind[I] != 0) Mc = third_body<T, I>(d, C, Ctot);
+        const RateI<T, I, true> k(d, t, logt, it, logc0, hrt, grt, Mc, lnA_shift ? lnA_shift[I] : 0.0);
+        const double kr = k.kf * k.rkc;
+        double pf = 1.0, pb = 1.0;
+        static_for<0, T::NRMAX>([&](auto j) {
+            constexpr int r = T::reac[I][j], p = T::prod[I][j];
+            if constexpr (r >= 0) pf *= C[r];
+            if constexpr (p >= 0) pb *= C[p];
+        });
+        const double q = k.kf * pf - kr * pb;
+        const double dqdT = k.kf * k.dlnkf_dT * pf - kr * (k.dlnkf_dT + k.dlnrkc_dT) * pb;
+        // d q / d C_s for the species in the reaction's concentration products
+        // (the kf/kr factors that carry [M] are handled separately below).
+        auto add = [&](auto sc, double dq) {
+            // species touched by reaction I: accumulate nu_k * dq into column s
+            constexpr int s = decltype(sc)::value;
+            static_for<0, T::NRMAX>([&](auto jj) {
+                constexpr int r = T::reac[I][jj], p = T::prod[I][jj];
+                if constexpr (r >= 0 && r < N) {
+                    if constexpr (s < N) J[r * N + s] -= dq; else dwdCL[r] -= dq;
+                }
+                if constexpr (p >= 0 && p < N) {
+                    if constexpr (s < N) J[p * N + s] += dq; else dwdCL[p] += dq;
+                }
+            });
+        };
+        // reactant concentrations: d(prod C^nu')/dC_s = product with one C_s removed
+        static_for<0, T::NRMAX>([&](auto j) {
+            constexpr int s = T::reac[I][j];
+            if constexpr (s >= 0) {
+                double pr_ = k.kf;
+                static_for<0, T::NRMAX>([&](auto m) {
+                    constexpr int r2 = T::reac[I][m];
+                    if constexpr (r2 >= 0 && m != j) pr_ *= C[r2];
+                });
+                add(std::integral_constant<int, s>{}, pr_);
+            }
+            constexpr int p = T::prod[I][j];
+            if constexpr (p >= 0) {
+                double pp_ = kr;
+                static_for<0, T::NRMAX>([&](auto m) {
+                    constexpr int p2 = T::prod[I][m];
+                    if constexpr (p2 >= 0 && m != j) pp_ *= C[p2];
+                });
+                add(std::integral_constant<int, p>{}, -pp_);
+            }
+        });
+        // third-body concentration dependence: dq/dC_s += eff_s * dkf/d[M] * (pf - rkc pb)
+        if constexpr (T::kind[I] != 0) {
+            const double dqdM = k.dkf_dM * (pf - k.rkc * pb);
+            // default efficiency on every species, plus the listed corrections
+            static_for<0, NS>([&](auto s) {
+                double eff = d.defeff[I];
+                static_for<0, T::NEFF>([&](auto j) {
+                    if constexpr (T::effsp[I][j] == decltype(s)::value) eff += d.effd[I][j];
+                });
+                add(s, eff * dqdM);
+            });
+        }
+        static_for<0, T::NRMAX>([&](auto j) {
+            constexpr int r = T::reac[I][j], p = T::prod[I][j];
+            if constexpr (r >= 0) { wd[r] -= q; dwdT[r] -= dqdT; }
+            if constexpr (p >= 0) { wd[p] += q; dwdT[p] += dqdT; }
+        });
+    });
+    // dT/dy_j = -(e_j - e_last)/cv with e_k = (h_k/RT - 1) R T / W_k
+    double cv = 0, ek[NS];
+    #pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        const double* a = t <= d.Tmid[k] ? d.lo[k] : d.hi[k];
+        const double cpr = fma(t, fma(t, fma(t, fma(t, a[4], a[3]), a[2]), a[1]), a[0]);
+        cv = fma(Y[k] * d.invW[k], cpr - 1.0, cv);
+        ek[k] = (hrt[k] - 1.0) * kR * t * d.invW[k];
+    }
+    cv *= kR;
+    const double rcv = 1.0 / cv, rrho = 1.0 / rho;
+    #pragma unroll
+    for (int k = 0; k < N; ++k) {
+        const double Wk = d.W[k];
+        f[k] = wd[k] * Wk * rrho;
+        const double cl = dwdCL[k] * d.invW[L];
+        const double tk = Wk * rrho * dwdT[k];
+        #pragma unroll
+        for (int j = 0; j < N; ++j) {
+            const double dTdy = -(ek[j] - ek[L]) * rcv;
+            J[k * N + j] = Wk * (J[k * N + j] * d.invW[j] - cl) + tk * dTdy;
+        }
+    }
+}
+
+}  // namespace hf

@@ -1,0 +1,124 @@
+// Interface between the host C-ABI (capi.cpp) and the kernels compiled for
+// one mechanism topology (topo_instance.hip, built once per mechanism).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "hyperflow.h"
+#include "mechanism.hpp"
+
+namespace hf {
+
+struct ChemLaunch {
+    const void* mech;               // device MechData<T>
+    const double* U;                // device, nv planes, stride ld
+    int64_t ncell, ld;
+    const double* dt;               // device: 1 or ncell values
+    int dt_per_cell;
+    double* out;                    // device, ns planes, stride ldo
+    int64_t ldo;
+    int8_t* status;                 // device, ncell
+    unsigned long long* first_failed;  // device scalar (atomicMin)
+    unsigned long long* counters;   // device, 8 values
+    double rtol, atol;
+    int maxsteps;
+    int mode;                       // HF_CHEM_PLAIN / HF_CHEM_BATCHED
+    int precision;
+    // batched form
+    int num_cus, waves_per_cu, chunk, nbins;
+    void* workspace;                // device scratch for the batched form
+    size_t workspace_bytes;
+    hipStream_t stream;
+    float* kernel_ms;               // optional: measured with events around the integration kernel
+};
+
+struct FluxLaunch {
+    const void* mech;
+    const hf_box* boxes;            // device copy of the box table
+    int nbox;
+    const int32_t* tiles;           // device tile table: (box, tile_i, tile_j) triples
+    int ntiles;
+    const double* Uin;
+    double* Uout;
+    int first_order;
+    int one_d;                      // x-only stage: tiles are TX1 x 1 (see flux_tile_shape)
+    unsigned long long* first_failed;  // device scalar
+    unsigned long long* counters;   // device: [faces, first-order sides]
+    hipStream_t stream;
+    float* kernel_ms;
+};
+
+struct PropsLaunch {
+    const void* mech;
+    const double* U;
+    int64_t ncell, ld;
+    double *p, *a, *T;
+    unsigned long long* first_failed;
+    double dx, dy;                  // for the CFL rate (dx > 0)
+    double* rate_max;               // device scalar (max of the CFL rate), optional
+    hipStream_t stream;
+};
+
+struct AdjointLaunch {
+    const void* mech;
+    int64_t ncell;
+    const double *rho, *T0, *Y0;    // device
+    double h;
+    int nsteps, ckpt_every;
+    const double* lnA_shift;        // device, nr (or null)
+    double* J;
+    double *grad_lnA, *grad_T0, *grad_Y0;
+    const double *v_lnA, *v_T0, *v_Y0;  // tangent directions (tangent mode)
+    double* dJ;
+    double* workspace;              // checkpoints
+    size_t workspace_bytes;
+    unsigned long long* first_failed;
+    hipStream_t stream;
+};
+
+struct TopoOps {
+    const char* name;
+    const char* topology;
+    int ns, nr;
+    size_t mech_bytes;
+    void (*fill)(const Mechanism& m, void* host);   // writes a MechData<T>
+    int (*chem)(const ChemLaunch& L);
+    size_t (*chem_workspace)(int64_t ncell, int nbins, int num_cus, int waves_per_cu);
+    int (*flux)(const FluxLaunch& L);
+    int (*props)(const PropsLaunch& L);
+    int (*adjoint)(const AdjointLaunch& L);
+    int (*tangent)(const AdjointLaunch& L);
+};
+
+// All compiled topologies (null-terminated).
+const TopoOps* const* compiled_topologies();
+
+// Device time of the work enqueued between construction and stop() on one
+// stream, measured with HIP events on that stream. A null target disables it.
+struct StreamTimer {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    float* target;
+    hipStream_t s;
+    StreamTimer(float* t, hipStream_t stream) : target(t), s(stream) {
+        if (target && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess ||
+                       hipEventRecord(e0, s) != hipSuccess))
+            target = nullptr;
+    }
+    void stop() {
+        if (!target) return;
+        float ms = 0;
+        if (hipEventRecord(e1, s) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
+            hipEventElapsedTime(&ms, e0, e1) == hipSuccess)
+            *target = ms;
+        target = nullptr;
+    }
+    ~StreamTimer() {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    }
+};
+
+}  // namespace hf

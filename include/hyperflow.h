@@ -1,0 +1,198 @@
+/* hyperflow.h — C ABI of the MI355X-native HyperFlow V3 hot path.
+ *
+ * The reference's two hot entry points are C++ member functions:
+ *   ChemistryBatch::advance(Field&, double dt, size_t count)
+ *       HyperFlow_V3/src/chemistry_batch.hpp:8-17, chemistry_batch.cpp:21-50,
+ *       called from BlockAMR::react (HyperFlow_V3/src/BlockAMR.cpp:282);
+ *   the per-box face loop + update of BlockAMR::euler
+ *       HyperFlow_V3/src/BlockAMR.cpp:210-243, calling properties / minmod /
+ *       admissible / hllc (HyperFlow_V3/src/solver.hpp:27-30, physics.cpp:4-48).
+ * Behind this ABI they run as HIP kernels for gfx950. Signatures carry plain
+ * pointers and sizes only.
+ *
+ * Conventions (each mirrors the reference):
+ *  - State layout: species-first conserved vector [rhoY_0..rhoY_{ns-1}, rho u,
+ *    rho v, rho E] (solver.hpp:17-18), stored SoA: component n of cell i at
+ *    U[n*ld + i].
+ *  - All-or-nothing: on any failed cell the caller's field is unchanged and
+ *    the LOWEST failing cell index is reported (chemistry_batch.cpp:26,40-49).
+ *  - Inadmissible states return HF_EDOMAIN (std::domain_error in physics.cpp:7,13,
+ *    chemistry.cpp:71); integrator failure returns HF_ECONV (CVODE flag < 0 /
+ *    incomplete interval, chemistry.cpp:7,67-68); bad arguments HF_EINVAL
+ *    (chemistry.cpp:56, chemistry_batch.cpp:23-24).
+ *  - One caller at a time per context: a concurrent or nested call returns
+ *    HF_EBUSY (chemistry_batch.hpp:6-7, chemistry_batch.cpp:22).
+ *  - Pointers may be device (HBM) or host memory; host buffers are staged
+ *    through device memory for the call. The library keeps no pointer after
+ *    a call returns.
+ */
+#ifndef HYPERFLOW_H
+#define HYPERFLOW_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    HF_OK = 0,
+    HF_EINVAL = 1,
+    HF_EDOMAIN = 2,
+    HF_ECONV = 3,
+    HF_ENOMEM = 4,
+    HF_EDEVICE = 5,
+    HF_EBUSY = 6
+};
+
+/* Chemistry forms. PLAIN: one cell per lane, FP64 (the drop-in).
+ * BATCHED: persistent kernel over a device work queue of stiffness-binned
+ * cells, lanes refilled as their cells finish. */
+enum { HF_CHEM_PLAIN = 0, HF_CHEM_BATCHED = 1 };
+
+/* Newton-matrix precision: FP64 throughout, or the Jacobian and its LU in
+ * FP32 with FP64 residuals, state and error control. */
+enum { HF_PREC_FP64 = 0, HF_PREC_MIXED = 1 };
+
+typedef struct hf_ctx hf_ctx;
+
+typedef struct hf_options {
+    double rtol;        /* relative tolerance; reference: 1e-8 (chemistry.cpp:114) */
+    double atol;        /* absolute tolerance; reference: 1e-15 (chemistry.cpp:114) */
+    int max_steps;      /* per cell; reference: 5000 (chemistry.cpp:120) */
+    int precision;      /* HF_PREC_FP64 or HF_PREC_MIXED */
+    int device;         /* HIP device ordinal */
+    /* batched-form tuning; 0 selects the autotuned / default value */
+    int waves_per_cu;   /* resident waves per CU of the persistent kernel */
+    int chunk;          /* cells a wave claims from the queue at once */
+    int nbins;          /* stiffness bins */
+} hf_options;
+
+typedef struct hf_chem_stats {
+    int64_t cells;          /* cells advanced (BlockAMR.cpp:282 chemistry_cell_calls) */
+    int64_t steps, rhs, jac, lu, newton, conv_failures, err_failures;
+    int64_t max_cell_steps; /* largest step count of any cell */
+    int team;               /* lanes of work (ChemistryBatch::lastTeamSize analogue) */
+    double kernel_ms;       /* device time of the integration kernel(s) */
+} hf_chem_stats;
+
+/* One ghosted SoA box for a flux stage. Element (component n, cell i, j) of
+ * the input, with -2 <= i < nx+2 and -2 <= j < ny+2 (two ghost layers,
+ * BlockAMR.cpp:206), is Uin[in_offset + n*in_plane + (j+2)*in_pitch + (i+2)];
+ * the updated valid cell (0 <= i < nx, 0 <= j < ny) goes to
+ * Uout[out_offset + n*out_plane + j*out_pitch + i]. */
+typedef struct hf_box {
+    int32_t nx, ny;
+    int32_t walls;      /* bit0 x-lo, bit1 x-hi, bit2 y-lo, bit3 y-hi: the box face lies on a
+                           reflecting wall (species, tangential momentum and energy flux zeroed,
+                           BlockAMR.cpp:224-229) */
+    int32_t wedge_i;    /* the y-lo wall applies only to faces with i >= wedge_i (0: all) */
+    int64_t in_offset, in_pitch, in_plane;
+    int64_t out_offset, out_pitch, out_plane;
+    double dtdx, dtdy;  /* h/dx, h/dy; dtdy == 0 gives an x-only (1-D) stage */
+} hf_box;
+
+typedef struct hf_flux_stats {
+    int64_t cells, faces;
+    int64_t first_order_sides;  /* face sides that fell back to first order (BlockAMR.cpp:221-222) */
+    double kernel_ms;
+} hf_flux_stats;
+
+/* Create a context: load the Cantera-YAML mechanism (replaces
+ * Cantera::newSolution(mechanism, "", "none"), chemistry_batch.cpp:17-18) and
+ * upload its tables. opt may be NULL for the reference defaults.
+ * Returns HF_EINVAL for an unreadable mechanism or one whose topology has no
+ * compiled kernels (hf_last_error names it). */
+int hf_create(const char *mechanism_yaml, const hf_options *opt, hf_ctx **out);
+void hf_destroy(hf_ctx *ctx);
+/* Message of the last failure on this context (e.g. "chemistry cell 17: invalid accepted species"). */
+const char *hf_last_error(const hf_ctx *ctx);
+/* Message of the last hf_create failure (no context exists then). */
+const char *hf_create_error(void);
+
+int hf_num_species(const hf_ctx *ctx);
+int hf_num_reactions(const hf_ctx *ctx);
+const char *hf_species_name(const hf_ctx *ctx, int k);
+/* Molecular weights (kg/kmol), ns values. */
+int hf_molecular_weights(const hf_ctx *ctx, double *W);
+int hf_set_options(hf_ctx *ctx, const hf_options *opt);
+int hf_get_options(const hf_ctx *ctx, hf_options *opt);
+
+/* ChemistryBatch::advance (chemistry_batch.cpp:21-50) over SoA planes U[n*ld + i],
+ * n < ns+3, i < ncell. dt: one value, or one per cell when dt_per_cell != 0.
+ * Each cell is advanced over its dt at constant density and internal energy
+ * (Chemistry::advance, chemistry.cpp:55-76); only the species planes change.
+ * mode: HF_CHEM_PLAIN or HF_CHEM_BATCHED. first_failed (optional) receives the
+ * lowest failing cell index, -1 on success. */
+int hf_chem_advance(hf_ctx *ctx, double *U, int64_t ncell, int64_t ld, const double *dt, int dt_per_cell,
+                    int mode, hf_chem_stats *stats, int64_t *first_failed);
+
+/* One Euler stage of the flux sweep (BlockAMR.cpp:210-243) over nbox boxes:
+ * NND/minmod reconstruction of the conserved variables, per-side first-order
+ * fallback when a reconstructed state is inadmissible, HLLC flux, wall-face
+ * zeroing, conservative update Uout = U - dtdx dFx - dtdy dFy. first_order != 0
+ * forces first-order face states (BlockAMR.cpp:220). Returns HF_EDOMAIN with
+ * first_failed = (box index << 32) | (j*nx + i) when an updated cell is
+ * inadmissible, so the driver can redo the stage first-order
+ * (BlockAMR.cpp:257-263). boxes is a host array. */
+int hf_flux_stage(hf_ctx *ctx, const hf_box *boxes, int nbox, const double *Uin, double *Uout, int first_order,
+                  hf_flux_stats *stats, int64_t *first_failed);
+
+/* properties(u) (physics.cpp:4-15) on SoA planes: p, a, T per cell; status
+ * per cell (HF_OK / HF_EDOMAIN). Returns HF_EDOMAIN when any cell fails
+ * (BlockAMR::validate, BlockAMR.cpp:286-298), first_failed = lowest index. */
+int hf_properties(hf_ctx *ctx, const double *U, int64_t ncell, int64_t ld, double *p, double *a, double *T,
+                  int64_t *first_failed);
+
+/* CFL time step of BlockAMR::stableDt (BlockAMR.cpp:335-346) over SoA cells:
+ * returns cfl / max((|u|+a)/dx + (|v|+a)/dy) in *dt. */
+int hf_stable_dt(hf_ctx *ctx, const double *U, int64_t ncell, int64_t ld, double dx, double dy, double cfl,
+                 double *dt);
+
+/* Discrete adjoint of a fixed-step backward-Euler chemistry integration (no
+ * reference counterpart; rate-parameter sensitivities). Per cell c: density
+ * rho[c], initial temperature T0[c] and mass fractions Y0[c*ns + k]; e is held
+ * at e(T0, Y0). nsteps steps of size h; checkpoints every ckpt_every steps are
+ * recomputed in reverse. Objective J = T(t_end) + sum_n h qdot(y_{n+1}),
+ * qdot = -sum_k u_k wdot_k (u_k molar internal energy), in J[c].
+ * Gradients: grad_lnA[c*nr + i] = dJ/d ln A_i (A_i scales k and k0 of
+ * reaction i), grad_T0[c], grad_Y0[c*ns + k]. lnA_shift (optional, nr values)
+ * perturbs every cell's rate constants. All arrays host or device. */
+int hf_chem_adjoint(hf_ctx *ctx, int64_t ncell, const double *rho, const double *T0, const double *Y0, double h,
+                    int nsteps, int ckpt_every, const double *lnA_shift, double *J, double *grad_lnA,
+                    double *grad_T0, double *grad_Y0);
+
+/* Forward tangent of the same integration along one direction per cell:
+ * dJ = dJ/dlnA . v_lnA + dJ/dT0 v_T0 + dJ/dY0 . v_Y0 (v arrays shaped like the
+ * gradients of hf_chem_adjoint). */
+int hf_chem_tangent(hf_ctx *ctx, int64_t ncell, const double *rho, const double *T0, const double *Y0, double h,
+                    int nsteps, const double *lnA_shift, const double *v_lnA, const double *v_T0,
+                    const double *v_Y0, double *J, double *dJ);
+
+/* Synthetic inputs (no reference counterpart). Seeded splitmix64/xoshiro256**
+ * cells for the batched-chemistry workloads. Per cell, in draw order:
+ * T, ln p, phi, f, ln dt (uniform doubles u = (x >> 11) 2^-53).
+ * Y = (1-f) Y_fresh(phi) + f Y_eq(T, p, phi), rho = p Wbar / (R T), e = e(T, Y),
+ * u = v = 0. which: 0 = H2/air (T in [1100,2400] K, p log-uniform
+ * [0.5,50] atm, phi in [0.5,2], progress f in [0,1] toward TP equilibrium,
+ * dt log-uniform [1e-8,1e-6] s); 1 = DME/air (T in [700,1800] K, p log-uniform
+ * [1,40] atm, phi in [0.6,1.5]); 2 = H2/air adjoint cells (T in [1000,1500] K,
+ * p in [1,10] atm, phi in [0.5,2], fresh mixture). Writes host SoA planes
+ * U[n*ncell + i] (ns+3 planes) and dt[i] (which 0/1). */
+int hf_synth_cells(hf_ctx *ctx, int which, uint64_t seed, int64_t ncell, double *U, double *dt);
+
+/* TP equilibrium mass fractions for element amounts given by a reactant
+ * composition (mole fractions X, ns values) at (T, p), by element-potential
+ * Newton on the ideal-gas Gibbs minimum. Y out (ns values). Host arrays. */
+int hf_equilibrate_tp(hf_ctx *ctx, double T, double p, const double *X, double *Y);
+
+/* Conserved state (SoA, nv values) of a gas at (T, p, Y, u, v): replaces the
+ * reference's primitive() initial-condition helper (BlockAMR.cpp:20-24). */
+int hf_state_from_TPY(hf_ctx *ctx, double T, double p, const double *Y, double u, double v, double *q);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HYPERFLOW_H */

@@ -1,0 +1,116 @@
+"""The C ABI library: it loads, exports every declared entry point, and its
+host-side logic (mechanism loading, argument checks, set-up helpers) behaves
+without a GPU. No kernel is launched here."""
+import re
+
+import numpy as np
+import pytest
+
+import hyperflow_amd as hf
+from hyperflow_amd import _lib
+from tests.conftest import DME, GOLDEN, H2, ROOT
+
+
+def declared_functions():
+    text = (ROOT / "include" / "hyperflow.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(hf_[a-z_A-Z0-9]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.load()
+    names = declared_functions()
+    assert len(names) >= 18
+    for n in names:
+        assert hasattr(L, n), n
+    assert sorted(_lib.EXPORTED) == names
+
+
+def test_mechanism_tables_match_independent_loader(oracle_h2, oracle_dme):
+    for path, o in ((H2, oracle_h2), (DME, oracle_dme)):
+        m = hf.Mechanism(path)
+        assert m.species == o.m["species"]
+        assert m.nr == o.m["nr"]
+        assert np.array_equal(m.W, o.m["W"])
+
+
+def test_unknown_topology_and_bad_file_are_rejected(tmp_path):
+    text = H2.read_text()
+    # drop the last reaction: a valid mechanism whose topology has no kernels
+    cut = text.rstrip().rfind("- equation:")
+    p = tmp_path / "h2_minus_one.yaml"
+    p.write_text(text[:cut])
+    with pytest.raises(hf.HyperflowError) as e:
+        hf.Mechanism(p)
+    assert e.value.code == _lib.HF_EINVAL and "compiled kernels" in e.value.message
+    with pytest.raises(hf.HyperflowError) as e:
+        hf.Mechanism(tmp_path / "missing.yaml")
+    assert e.value.code == _lib.HF_EINVAL
+
+
+def test_argument_validation_before_any_device_work(mech_h2):
+    batch = hf.ChemistryBatch(mech_h2)
+    U = np.zeros((12, 4))
+    with pytest.raises(hf.HyperflowError) as e:
+        batch.advance(U, -1.0)
+    assert e.value.code == _lib.HF_EINVAL
+    with pytest.raises(ValueError):
+        batch.advance(U, 1e-7, count=5)
+    # count 0 is a successful no-op (chemistry_batch.cpp:25)
+    batch.advance(U, 1e-7, count=0)
+    assert batch.lastTeamSize() == 0
+
+
+def test_synthetic_cells_are_deterministic(mech_h2):
+    U1, dt1 = mech_h2.synth_cells(0, 2, 64)
+    g = np.load(GOLDEN / "synth_h2_seed2.npz")
+    assert np.array_equal(U1, g["U"]) and np.array_equal(dt1, g["dt"])
+    U2, _ = mech_h2.synth_cells(0, 3, 64)
+    assert not np.array_equal(U1, U2)
+    assert (dt1 >= 1e-8).all() and (dt1 <= 1e-6).all()
+
+
+def test_synthetic_cells_sample_the_stated_ranges(oracle_h2, mech_h2):
+    U, dt = mech_h2.synth_cells(0, 11, 2000)
+    p, a, T, st = oracle_h2.properties(U.T.copy())
+    assert (st == 0).all()
+    assert T.min() >= 1100 - 1e-6 and T.max() <= 2400 + 1e-6
+    assert p.min() >= 0.5 * 101325 * (1 - 1e-9) and p.max() <= 50 * 101325 * (1 + 1e-9)
+    # H/O element ratio is 2 phi: phi in [0.5, 2]
+    m = oracle_h2.m
+    el = (U[:9] / m["W"][:, None]).T @ m["comp"]
+    phi = el[:, 0] / el[:, 1] / 2 * 1.0
+    assert phi.min() > 0.5 - 1e-9 and phi.max() < 2.0 + 1e-9
+
+
+def test_equilibrium_conserves_elements(oracle_h2, mech_h2):
+    m = oracle_h2.m
+    X = np.zeros(9)
+    X[0], X[1], X[8] = 3.0, 1.0, 3.76
+    X /= X.sum()
+    Y = mech_h2.equilibrate_tp(2400.0, 2 * 101325.0, X)
+    b_in = X @ m["comp"]
+    Xe = (Y / m["W"]) / np.sum(Y / m["W"])
+    b_out = Xe @ m["comp"]
+    assert np.allclose(b_in / b_in.sum(), b_out / b_out.sum(), rtol=1e-12)
+
+
+def test_state_from_TPY_round_trip(oracle_h2, mech_h2):
+    Y = np.array([0.02, 0.2, 0.05, 1e-5, 1e-4, 1e-3, 1e-6, 1e-7, 0.0])
+    Y[8] = 1 - Y.sum()
+    q = mech_h2.state_from_TPY(1234.5, 3.3e5, Y, 120.0, -40.0)
+    p, a, T, st = oracle_h2.properties(q)
+    assert st[0] == 0 and abs(T[0] - 1234.5) < 1e-9 and abs(p[0] - 3.3e5) / 3.3e5 < 1e-12
+
+
+def test_no_gpu_means_device_error(mech_h2):
+    try:
+        import torch
+        if torch.cuda.is_available():
+            pytest.skip("a GPU is present")
+    except ImportError:
+        pass
+    U, dt = mech_h2.synth_cells(0, 2, 8)
+    with pytest.raises(hf.HyperflowError) as e:
+        hf.ChemistryBatch(mech_h2).advance(U, dt)
+    assert e.value.code == _lib.HF_EDEVICE

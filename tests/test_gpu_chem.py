@@ -1,0 +1,119 @@
+"""GPU parity of the chemistry step (HIP kernels through the C ABI) against
+the CPU oracle run at the reference CVODE tolerances of rtol 1e-10, atol 1e-14.
+
+Tolerances (stated with the result): T within 1e-6 relative; Y_k within
+1e-6 * max(|Y_k|, Y_floor) with Y_floor = 1e-6; element masses within 1e-12
+relative. The product runs at its defaults (the reference's rtol 1e-8,
+atol 1e-15, 5000 steps) unless a test says otherwise.
+"""
+import numpy as np
+import pytest
+
+import hyperflow_amd as hf
+from hyperflow_amd import _lib
+from tests.conftest import GOLDEN, H2
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+Y_FLOOR = 1e-6
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+
+
+def compare(o, U_gpu, U_ref, W, comp):
+    ns = len(W)
+    rg, rr = U_gpu[:ns].sum(0), U_ref[:ns].sum(0)
+    Yg, Yr = U_gpu[:ns] / rg, U_ref[:ns] / rr
+    _, _, Tg, sg = o.properties(U_gpu.T.copy())
+    _, _, Tr, sr = o.properties(U_ref.T.copy())
+    assert (sg == 0).all() and (sr == 0).all()
+    eT = np.max(np.abs(Tg - Tr) / Tr)
+    eY = np.max(np.abs(Yg - Yr) / np.maximum(np.abs(Yr), Y_FLOOR))
+    return eT, eY
+
+
+@pytest.mark.parametrize("mode", [hf.HF_CHEM_PLAIN, hf.HF_CHEM_BATCHED])
+def test_chemistry_matches_oracle(oracle_h2, mode):
+    print(f"[gpu] chemistry parity mode={mode} n=512 seed=2", flush=True)
+    mech = hf.Mechanism(H2)
+    U0, dt = mech.synth_cells(0, 2, 512)
+    ref = U0.copy()
+    r, st, ff, msg = oracle_h2.chem_advance(ref, dt, rtol=1e-10, atol=1e-14)
+    assert r == 0, msg
+    Ud = dev(U0)
+    batch = hf.ChemistryBatch(mech, mode=mode)
+    stats = batch.advance(Ud, dev(dt))
+    out = Ud.cpu().numpy()
+    eT, eY = compare(oracle_h2, out, ref, mech.W, oracle_h2.m["comp"])
+    print(f"[gpu] mode={mode} T rel err {eT:.3e}  Y rel err (floor {Y_FLOOR:g}) {eY:.3e}  stats {stats}", flush=True)
+    assert eT < 1e-6 and eY < 1e-6
+    # element masses and untouched momentum/energy planes
+    m = oracle_h2.m
+    el0 = (U0[:9] / m["W"][:, None]).T @ m["comp"]
+    el1 = (out[:9] / m["W"][:, None]).T @ m["comp"]
+    assert np.all(np.abs(el1 - el0) <= 1e-12 * np.abs(el0))
+    assert np.array_equal(out[9:], U0[9:])
+
+
+def test_zero_dimensional_validation_scenario(oracle_h2):
+    """The reference's 0-D check (validate.py:140-161): uniform H2/air 1500 K,
+    5 atm, 2e-7 s; T within 1e-7 relative and Y within 1e-8 absolute of the
+    Radau golden solution."""
+    g = np.load(GOLDEN / "chem_radau.npz")
+    mech = hf.Mechanism(H2)
+    Ud = dev(g["U0"][:, :1])
+    hf.ChemistryBatch(mech).advance(Ud, 2e-7)
+    out = Ud.cpu().numpy()
+    _, _, T, st = oracle_h2.properties(out.T.copy())
+    Y = out[:9, 0] / out[:9, 0].sum()
+    assert abs(T[0] - g["Tend"][0]) / g["Tend"][0] < 1e-7
+    assert np.abs(Y - g["Yend"][:, 0]).max() < 1e-8
+
+
+def test_all_or_nothing_and_lowest_index():
+    mech = hf.Mechanism(H2)
+    U0, dt = mech.synth_cells(0, 5, 300)
+    U0[0, 211] = np.nan
+    U0[2, 97] = -1.0
+    Ud = dev(U0)
+    with pytest.raises(hf.HyperflowError) as e:
+        hf.ChemistryBatch(mech).advance(Ud, dev(dt))
+    assert e.value.code == _lib.HF_EDOMAIN and e.value.first_failed == 97
+    assert "chemistry cell 97" in e.value.message
+    assert np.array_equal(Ud.cpu().numpy(), U0, equal_nan=True)
+
+
+def test_max_steps_failure_is_econv():
+    mech = hf.Mechanism(H2, max_steps=3)
+    U0, dt = mech.synth_cells(0, 6, 64)
+    Ud = dev(U0)
+    with pytest.raises(hf.HyperflowError) as e:
+        hf.ChemistryBatch(mech).advance(Ud, dev(dt))
+    assert e.value.code == _lib.HF_ECONV and e.value.first_failed >= 0
+    assert np.array_equal(Ud.cpu().numpy(), U0)
+
+
+def test_host_and_device_buffers_agree_and_ld_stride():
+    mech = hf.Mechanism(H2)
+    U0, dt = mech.synth_cells(0, 7, 200)
+    host = U0.copy()
+    hf.ChemistryBatch(mech).advance(host, dt)
+    wide = np.zeros((12, 256))
+    wide[:, :200] = U0
+    Ud = dev(wide)
+    hf.ChemistryBatch(mech).advance(Ud, dev(dt), count=200)
+    out = Ud.cpu().numpy()
+    assert np.array_equal(out[:, :200], host)
+    assert np.array_equal(out[:, 200:], 0 * out[:, 200:])
+
+
+def test_scalar_dt_matches_per_cell_dt():
+    mech = hf.Mechanism(H2)
+    U0, _ = mech.synth_cells(0, 8, 128)
+    a, b = dev(U0), dev(U0)
+    hf.ChemistryBatch(mech).advance(a, 3e-7)
+    hf.ChemistryBatch(mech).advance(b, dev(np.full(128, 3e-7)))
+    assert torch.equal(a, b)
