@@ -23,7 +23,7 @@ struct BdfCounters {
     int steps = 0, rhs = 0, jac = 0, lu = 0, newton = 0, ncf = 0, netf = 0;
 };
 
-enum { BDF_OK = 0, BDF_FAIL = 3 };
+enum { BDF_OK = 0, BDF_RUNNING = 1, BDF_FAIL = 3 };
 
 template <int N, class Sys, int ZSTRIDE = 64>
 struct Bdf {
@@ -500,8 +500,20 @@ struct Bdf {
         return fmin(fmax(0.5 * hnew, hlb), hub);
     }
 
-    // Integrate y from t = 0 to tout in place. Returns BDF_OK or BDF_FAIL.
-    __device__ __forceinline__ int integrate(Sys& sys, double* y, double tout, int maxsteps) {
+    // ------------------------------------------------------------ resumable driver
+    // begin() starts an integration of y from t = 0 to tout; each advance()
+    // call takes one accepted step (with its internal retries) and returns
+    // BDF_RUNNING, BDF_OK (tout reached, y written) or BDF_FAIL. integrate()
+    // strings them together; the persistent kernel interleaves cells by
+    // calling advance() once per loop trip.
+    double tout;
+    int maxsteps;
+    bool first;
+
+    __device__ __forceinline__ int begin(Sys& sys, const double* y, double t_out, int max_steps) {
+        tout = t_out;
+        maxsteps = max_steps;
+        c = BdfCounters{};
         #pragma unroll
         for (int i = 0; i < N; ++i) Z(0, i) = y[i];
         tn = 0;
@@ -530,39 +542,49 @@ struct Bdf {
         saved_tq5 = 0;
         force_setup = false;
         jfresh = false;
-        bool first = true;
-        for (;;) {
-            if (nst >= maxsteps) return BDF_FAIL;
-            if (!first) set_ewt();
-            if ((tn + hprime - tout) * hprime > 0) {
-                hprime = (tout - tn) * (1.0 - 4.0 * UROUND);
-                eta = hprime / h;
-            }
-            if (hprime != h) {
-                if (!first && qprime != q) {
-                    adjust_order(qprime - q);
-                    q = qprime;
-                    L = q + 1;
-                    qwait = L;
-                }
-                rescale();
-            }
-            first = false;
-            if (step(sys, tout) != BDF_OK) return BDF_FAIL;
-            const double troundoff = 100.0 * UROUND * (fabs(tn) + fabs(h));
-            if (fabs(tn - tout) <= troundoff || (tn - tout) * h >= 0) {
-                const double s = (tout - tn) / h;
-                #pragma unroll
-                for (int i = 0; i < N; ++i) {
-                    double v = zrow(q, i);
-                    #pragma unroll
-                    for (int j = QMAX - 1; j >= 0; --j)
-                        if (j < q) v = fma(s, v, Z(j, i));
-                    y[i] = v;
-                }
-                return BDF_OK;
-            }
+        first = true;
+        return BDF_RUNNING;
+    }
+
+    __device__ __forceinline__ int advance(Sys& sys, double* y) {
+        if (nst >= maxsteps) return BDF_FAIL;
+        if (!first) set_ewt();
+        if ((tn + hprime - tout) * hprime > 0) {
+            hprime = (tout - tn) * (1.0 - 4.0 * UROUND);
+            eta = hprime / h;
         }
+        if (hprime != h) {
+            if (!first && qprime != q) {
+                adjust_order(qprime - q);
+                q = qprime;
+                L = q + 1;
+                qwait = L;
+            }
+            rescale();
+        }
+        first = false;
+        if (step(sys, tout) != BDF_OK) return BDF_FAIL;
+        const double troundoff = 100.0 * UROUND * (fabs(tn) + fabs(h));
+        if (fabs(tn - tout) <= troundoff || (tn - tout) * h >= 0) {
+            const double s = (tout - tn) / h;
+            #pragma unroll
+            for (int i = 0; i < N; ++i) {
+                double v = zrow(q, i);
+                #pragma unroll
+                for (int j = QMAX - 1; j >= 0; --j)
+                    if (j < q) v = fma(s, v, Z(j, i));
+                y[i] = v;
+            }
+            return BDF_OK;
+        }
+        return BDF_RUNNING;
+    }
+
+    // Integrate y from t = 0 to tout in place. Returns BDF_OK or BDF_FAIL.
+    __device__ __forceinline__ int integrate(Sys& sys, double* y, double t_out, int max_steps) {
+        int r = begin(sys, y, t_out, max_steps);
+        while (r == BDF_RUNNING) r = advance(sys, y);
+        return r;
     }
 };
 

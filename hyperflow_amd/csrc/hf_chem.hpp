@@ -63,56 +63,97 @@ struct ChemParams {
 // species planes go to out[k*ldo + i]. Momenta and energy are not changed.
 // Returns HF_OK, HF_EINVAL (bad dt), HF_EDOMAIN (inadmissible state before or
 // after, or accepted species below -1e-13) or HF_ECONV (integrator failure).
+// A lane's integration of one cell, split so that a persistent kernel can
+// interleave cells: start() validates and begins, step() takes one BDF step,
+// finish() applies the reference's post-step checks and writes the species.
 // zs: this lane's slot of the wave's Nordsieck slab ([order][unknown][64 lanes]).
+template <class T>
+struct CellRun {
+    static constexpr int NS = T::NS, NV = NS + 3, N = NS - 1;
+    ChemSys<T> sys;
+    Bdf<N, ChemSys<T>> bdf;
+    double y[N];
+
+    __device__ __forceinline__ CellRun(const MechData<T>& d, double* zs, const ChemParams& p)
+        : sys{d, 1.0, 0.0, 1000.0, nullptr} {
+        bdf.zs = zs;
+        bdf.rtol = p.rtol;
+        bdf.atol = p.atol;
+    }
+
+    // HF_OK when the integration is running, else the cell's error code.
+    __device__ __forceinline__ int start(const double* __restrict__ U, long i, long ld, double dt, int maxsteps) {
+        bdf.c = BdfCounters{};
+        if (!(dt > 0) || !finite(dt)) return HF_EINVAL;
+        double u[NV];
+        #pragma unroll
+        for (int n = 0; n < NV; ++n) u[n] = U[n * ld + i];
+        Props pr;
+        if (!properties(sys.d, u, pr, 1000.0)) return HF_EDOMAIN;
+        double rho = 0;
+        #pragma unroll
+        for (int k = 0; k < NS; ++k) rho += u[k];
+        const double ux = u[NS] / rho, vy = u[NS + 1] / rho;
+        sys.rho = rho;
+        sys.e = u[NS + 2] / rho - 0.5 * (ux * ux + vy * vy);
+        sys.t = pr.T;
+        #pragma unroll
+        for (int k = 0; k < N; ++k) y[k] = u[k] / rho;
+        return bdf.begin(sys, y, dt, maxsteps) == BDF_RUNNING ? HF_OK : HF_ECONV;
+    }
+
+    __device__ __forceinline__ int step() { return bdf.advance(sys, y); }
+
+    // Post-step checks of chemistry.cpp:68-75 and the write of the species planes.
+    __device__ __forceinline__ int finish(const double* __restrict__ U, long i, long ld, double* __restrict__ out,
+                                          long ldo) {
+        double u[NV];
+        double sum = 0;
+        bool ok = true;
+        #pragma unroll
+        for (int k = 0; k < N; ++k) {
+            ok = ok && finite(y[k]) && y[k] >= -1e-13;
+            u[k] = sys.rho * fmax(0.0, y[k]);
+            sum += u[k];
+        }
+        if (!ok) return HF_EDOMAIN;
+        u[N] = sys.rho - sum;
+        u[NS] = U[NS * ld + i];
+        u[NS + 1] = U[(NS + 1) * ld + i];
+        u[NS + 2] = U[(NS + 2) * ld + i];
+        Props pr;
+        if (!properties(sys.d, u, pr, sys.t)) return HF_EDOMAIN;
+        #pragma unroll
+        for (int k = 0; k < NS; ++k) out[k * ldo + i] = u[k];
+        return HF_OK;
+    }
+};
+
+// One cell start to end: Chemistry::advance (chemistry.cpp:55-76) on a lane.
 template <class T>
 __device__ __forceinline__ int advance_cell(const MechData<T>& d, const double* __restrict__ U, long i, long ld,
                                             double dt, double* __restrict__ out, long ldo, const ChemParams& p,
                                             BdfCounters& c, double* zs) {
-    constexpr int NS = T::NS, NV = NS + 3, N = NS - 1;
-    if (!(dt > 0) || !finite(dt)) return HF_EINVAL;
-    double u[NV];
-    #pragma unroll
-    for (int n = 0; n < NV; ++n) u[n] = U[n * ld + i];
-    Props pr;
-    if (!properties(d, u, pr, 1000.0)) return HF_EDOMAIN;
-    double rho = 0;
-    #pragma unroll
-    for (int k = 0; k < NS; ++k) rho += u[k];
-    const double ux = u[NS] / rho, vy = u[NS + 1] / rho;
-    const double e = u[NS + 2] / rho - 0.5 * (ux * ux + vy * vy);
-    double y[N];
-    #pragma unroll
-    for (int k = 0; k < N; ++k) y[k] = u[k] / rho;
-    ChemSys<T> sys{d, rho, e, pr.T, nullptr};
-    Bdf<N, ChemSys<T>> bdf;
-    bdf.zs = zs;
-    bdf.rtol = p.rtol;
-    bdf.atol = p.atol;
-    const int flag = bdf.integrate(sys, y, dt, p.maxsteps);
-    c = bdf.c;
-    if (flag != BDF_OK) return HF_ECONV;
-    double sum = 0;
-    bool ok = true;
-    #pragma unroll
-    for (int k = 0; k < N; ++k) {
-        ok = ok && finite(y[k]) && y[k] >= -1e-13;
-        u[k] = rho * fmax(0.0, y[k]);
-        sum += u[k];
+    CellRun<T> run(d, zs, p);
+    int st = run.start(U, i, ld, dt, p.maxsteps);
+    if (st == HF_OK) {
+        int r;
+        do {
+            r = run.step();
+        } while (r == BDF_RUNNING);
+        st = r == BDF_OK ? run.finish(U, i, ld, out, ldo) : HF_ECONV;
     }
-    if (!ok) return HF_EDOMAIN;
-    u[N] = rho - sum;
-    if (!properties(d, u, pr, sys.t)) return HF_EDOMAIN;
-    #pragma unroll
-    for (int k = 0; k < NS; ++k) out[k * ldo + i] = u[k];
-    return HF_OK;
+    c = run.bdf.c;
+    return st;
 }
 
 // counters: [steps, rhs, jac, lu, newton, conv fails, err fails, max steps of a cell]
-__device__ __forceinline__ void reduce_counters(const BdfCounters& c, unsigned long long* counters) {
+__device__ __forceinline__ void reduce_counters(const BdfCounters& c, int max_cell_steps,
+                                                unsigned long long* counters) {
     unsigned long long v[7] = {(unsigned long long)c.steps, (unsigned long long)c.rhs, (unsigned long long)c.jac,
                                (unsigned long long)c.lu, (unsigned long long)c.newton, (unsigned long long)c.ncf,
                                (unsigned long long)c.netf};
-    int mx = c.steps;
+    int mx = max_cell_steps;
     #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         #pragma unroll

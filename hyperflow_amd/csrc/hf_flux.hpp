@@ -55,9 +55,10 @@ flux_stage_kernel(const MechData<T>* __restrict__ dm, const hf_box* __restrict__
 
     // XCD-aware tile order: workgroups are dealt round-robin to the 8 XCDs,
     // so give each XCD a contiguous run of tiles (neighbours share L2 halos).
-    const int nb = gridDim.x;
-    const int per = (nb + 7) / 8;
-    int t = (blockIdx.x % 8) * per + blockIdx.x / 8;
+    // The grid is a multiple of 8, so (blockIdx % 8, blockIdx / 8) covers
+    // every tile index below 8 * per exactly once.
+    const int per = gridDim.x / 8;
+    const int t = (blockIdx.x % 8) * per + blockIdx.x / 8;
     if (t >= ntiles) return;
     const int bi = tiles[3 * t], ti = tiles[3 * t + 1], tj = tiles[3 * t + 2];
     const hf_box B = boxes[bi];

@@ -74,7 +74,214 @@ chem_plain_kernel(const MechData<TT>* __restrict__ dm, const double* __restrict_
         status[i] = (int8_t)st;
         if (st != HF_OK) atomicMin(first_failed, (unsigned long long)i);
     }
-    reduce_counters(c, counters);
+    reduce_counters(c, c.steps, counters);
+}
+
+// ------------------------------------------------------------------ chemistry (batched)
+// Persistent form: cells are binned by an a-priori work estimate and queued
+// in decreasing order of work (longest first); a fixed grid of waves drains
+// the queue, and each lane claims a new cell as soon as its current one
+// finishes, so a wave never idles behind its slowest lane.
+constexpr int kMaxBins = 256;
+
+struct Workspace {
+    uint8_t* bins;
+    int32_t* order;
+    unsigned* hist;
+    unsigned* offs;
+    unsigned* cursor;
+    unsigned* head;
+};
+
+inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+Workspace carve(void* base, int64_t ncell) {
+    char* p = static_cast<char*>(base);
+    Workspace w;
+    w.bins = reinterpret_cast<uint8_t*>(p);
+    p += align256(ncell);
+    w.order = reinterpret_cast<int32_t*>(p);
+    p += align256(sizeof(int32_t) * ncell);
+    w.hist = reinterpret_cast<unsigned*>(p);
+    p += align256(sizeof(unsigned) * kMaxBins);
+    w.offs = reinterpret_cast<unsigned*>(p);
+    p += align256(sizeof(unsigned) * kMaxBins);
+    w.cursor = reinterpret_cast<unsigned*>(p);
+    p += align256(sizeof(unsigned) * kMaxBins);
+    w.head = reinterpret_cast<unsigned*>(p);
+    return w;
+}
+
+size_t chem_workspace(int64_t ncell, int, int, int) {
+    return align256(ncell) + align256(sizeof(int32_t) * ncell) + 3 * align256(sizeof(unsigned) * kMaxBins) + 256;
+}
+
+__global__ void chem_init_kernel(Workspace w, unsigned long long* first_failed, unsigned long long* counters) {
+    const int t = threadIdx.x;
+    if (t < kMaxBins) {
+        w.hist[t] = 0;
+        w.cursor[t] = 0;
+    }
+    if (t == 0) {
+        *w.head = 0;
+        *first_failed = ~0ull;
+    }
+    if (t < 8) counters[t] = 0;
+}
+
+// Work estimate: dt times the weighted RMS of the initial rate of change,
+// binned on a log10 scale over [1e-2, 1e10].
+template <class TT>
+__global__ void __launch_bounds__(256)
+chem_estimate_kernel(const MechData<TT>* __restrict__ dm, const double* __restrict__ U, long ncell, long ld,
+                     const double* __restrict__ dt, int per_cell, ChemParams p, int nbins, Workspace w) {
+    constexpr int NS = TT::NS, NV = NS + 3, N = NS - 1;
+    __shared__ unsigned lh[kMaxBins];
+    for (int b = threadIdx.x; b < nbins; b += blockDim.x) lh[b] = 0;
+    __syncthreads();
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < ncell) {
+        const MechData<TT>& d = *dm;
+        double u[NV];
+        #pragma unroll
+        for (int n = 0; n < NV; ++n) u[n] = U[n * ld + i];
+        const double h = per_cell ? dt[i] : dt[0];
+        int bin = nbins - 1;   // unusable cells fail fast: queue them first
+        Props pr;
+        if (properties(d, u, pr, 1000.0) && h > 0) {
+            double rho = 0;
+            #pragma unroll
+            for (int k = 0; k < NS; ++k) rho += u[k];
+            const double ux = u[NS] / rho, vy = u[NS + 1] / rho;
+            ChemSys<TT> sys{d, rho, u[NS + 2] / rho - 0.5 * (ux * ux + vy * vy), pr.T, nullptr};
+            double y[N], f[N];
+            #pragma unroll
+            for (int k = 0; k < N; ++k) y[k] = u[k] / rho;
+            if (sys.rhs(y, f)) {
+                double s = 0;
+                #pragma unroll
+                for (int k = 0; k < N; ++k) {
+                    const double x = f[k] / fma(p.rtol, fabs(y[k]), p.atol);
+                    s = fma(x, x, s);
+                }
+                const double key = h * sqrt(s / N);
+                const double lg = log10(fmax(key, 1e-30));
+                bin = (int)floor((lg + 2.0) / 12.0 * nbins);
+                bin = min(max(bin, 0), nbins - 1);
+            }
+        }
+        w.bins[i] = (uint8_t)bin;
+        atomicAdd(&lh[bin], 1u);
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < nbins; b += blockDim.x)
+        if (lh[b]) atomicAdd(&w.hist[b], lh[b]);
+}
+
+// Queue order: highest bin (most work) first.
+__global__ void chem_scan_kernel(Workspace w, int nbins) {
+    if (threadIdx.x == 0) {
+        unsigned acc = 0;
+        for (int b = nbins - 1; b >= 0; --b) {
+            w.offs[b] = acc;
+            acc += w.hist[b];
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) chem_scatter_kernel(long ncell, int nbins, Workspace w) {
+    __shared__ unsigned lh[kMaxBins], lbase[kMaxBins];
+    for (int b = threadIdx.x; b < nbins; b += blockDim.x) lh[b] = 0;
+    __syncthreads();
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    int bin = 0;
+    unsigned local = 0;
+    if (i < ncell) {
+        bin = w.bins[i];
+        local = atomicAdd(&lh[bin], 1u);
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < nbins; b += blockDim.x)
+        if (lh[b]) lbase[b] = atomicAdd(&w.cursor[b], lh[b]);
+    __syncthreads();
+    if (i < ncell) w.order[w.offs[bin] + lbase[bin] + local] = (int32_t)i;
+}
+
+template <class TT>
+__global__ void __launch_bounds__(64)
+chem_persistent_kernel(const MechData<TT>* __restrict__ dm, const double* __restrict__ U, long ncell, long ld,
+                       const double* __restrict__ dt, int per_cell, double* __restrict__ out, long ldo,
+                       int8_t* __restrict__ status, unsigned long long* __restrict__ first_failed, ChemParams p,
+                       unsigned long long* __restrict__ counters, Workspace w, int chunk) {
+    constexpr int N = TT::NS - 1;
+    __shared__ double zlds[6 * N * 64];
+    const int lane = threadIdx.x;
+    CellRun<TT> run(*dm, zlds + lane, p);
+    BdfCounters tot;
+    int max_steps_cell = 0;
+    long cell = -1;
+    bool active = false;
+    // Wave-uniform state: the global queue is empty (exhausted), and the
+    // wave's claimed but not yet started queue slots [pend_lo, pend_hi).
+    bool exhausted = false;
+    unsigned pend_lo = 0, pend_hi = 0;
+    for (;;) {
+        const unsigned long long want = __ballot(!active);
+        if (want && !exhausted && pend_lo >= pend_hi) {
+            // claim `chunk` consecutive slots for the wave with one atomic
+            const int leader = __ffsll((long long)want) - 1;
+            unsigned base = 0;
+            if (lane == leader) base = atomicAdd(w.head, (unsigned)chunk);
+            base = __shfl(base, leader, 64);
+            if (base >= (unsigned)ncell) {
+                exhausted = true;
+            } else {
+                pend_lo = base;
+                pend_hi = min(base + (unsigned)chunk, (unsigned)ncell);
+            }
+        }
+        if (want && pend_lo < pend_hi) {
+            const unsigned avail = pend_hi - pend_lo;
+            const unsigned cnt = (unsigned)__popcll(want);
+            if (!active) {
+                const unsigned rank = (unsigned)__popcll(want & ((1ull << lane) - 1));
+                if (rank < avail) {
+                    cell = w.order[pend_lo + rank];
+                    const double h = per_cell ? dt[cell] : dt[0];
+                    const int st = run.start(U, cell, ld, h, p.maxsteps);
+                    if (st == HF_OK) {
+                        active = true;
+                    } else {
+                        status[cell] = (int8_t)st;
+                        atomicMin(first_failed, (unsigned long long)cell);
+                    }
+                }
+            }
+            pend_lo += min(cnt, avail);
+        }
+        if (!__any(active)) {
+            if (exhausted && pend_lo >= pend_hi) break;
+            continue;
+        }
+        if (active) {
+            const int r = run.step();
+            if (r != BDF_RUNNING) {
+                int st = r == BDF_OK ? run.finish(U, cell, ld, out, ldo) : HF_ECONV;
+                status[cell] = (int8_t)st;
+                if (st != HF_OK) atomicMin(first_failed, (unsigned long long)cell);
+                tot.steps += run.bdf.c.steps;
+                tot.rhs += run.bdf.c.rhs;
+                tot.jac += run.bdf.c.jac;
+                tot.lu += run.bdf.c.lu;
+                tot.newton += run.bdf.c.newton;
+                tot.ncf += run.bdf.c.ncf;
+                tot.netf += run.bdf.c.netf;
+                max_steps_cell = max(max_steps_cell, run.bdf.c.steps);
+                active = false;
+            }
+        }
+    }
+    reduce_counters(tot, max_steps_cell, counters);
 }
 
 // Register-resident per-lane integration is compiled for small mechanisms
@@ -86,20 +293,39 @@ int chem(const ChemLaunch& L) {
     if constexpr (!kLaneChem) {
         return HF_EINVAL;
     } else {
-    const ChemParams p{L.rtol, L.atol, L.maxsteps};
-    StreamTimer timer(L.kernel_ms, L.stream);
-    const int block = 64;
-    const long grid = (L.ncell + block - 1) / block;
-    hipLaunchKernelGGL(chem_plain_kernel<Topo>, dim3((unsigned)grid), dim3(block), 0, L.stream,
-                       static_cast<const MD*>(L.mech), L.U, (long)L.ncell, (long)L.ld, L.dt, L.dt_per_cell, L.out,
-                       (long)L.ldo, L.status, L.first_failed, p, L.counters);
-    if (hipGetLastError() != hipSuccess) return HF_EDEVICE;
-    timer.stop();
-    return HF_OK;
+        const ChemParams p{L.rtol, L.atol, L.maxsteps};
+        const MD* dm = static_cast<const MD*>(L.mech);
+        if (L.mode == HF_CHEM_BATCHED) {
+            if (!L.workspace) return HF_EINVAL;
+            const Workspace w = carve(L.workspace, L.ncell);
+            const int nbins = L.nbins > 0 ? min(L.nbins, kMaxBins) : 32;
+            const int wpc = L.waves_per_cu > 0 ? L.waves_per_cu : 4;
+            const int chunk = L.chunk > 0 ? L.chunk : 64;
+            const unsigned g256 = (unsigned)((L.ncell + 255) / 256);
+            hipLaunchKernelGGL(chem_init_kernel, dim3(1), dim3(256), 0, L.stream, w, L.first_failed, L.counters);
+            hipLaunchKernelGGL(chem_estimate_kernel<Topo>, dim3(g256), dim3(256), 0, L.stream, dm, L.U,
+                               (long)L.ncell, (long)L.ld, L.dt, L.dt_per_cell, p, nbins, w);
+            hipLaunchKernelGGL(chem_scan_kernel, dim3(1), dim3(64), 0, L.stream, w, nbins);
+            hipLaunchKernelGGL(chem_scatter_kernel, dim3(g256), dim3(256), 0, L.stream, (long)L.ncell, nbins, w);
+            StreamTimer timer(L.kernel_ms, L.stream);
+            hipLaunchKernelGGL(chem_persistent_kernel<Topo>, dim3((unsigned)(L.num_cus * wpc)), dim3(64), 0, L.stream,
+                               dm, L.U, (long)L.ncell, (long)L.ld, L.dt, L.dt_per_cell, L.out, (long)L.ldo, L.status,
+                               L.first_failed, p, L.counters, w, chunk);
+            if (hipGetLastError() != hipSuccess) return HF_EDEVICE;
+            timer.stop();
+            return HF_OK;
+        }
+        StreamTimer timer(L.kernel_ms, L.stream);
+        const int block = 64;
+        const long grid = (L.ncell + block - 1) / block;
+        hipLaunchKernelGGL(chem_plain_kernel<Topo>, dim3((unsigned)grid), dim3(block), 0, L.stream, dm, L.U,
+                           (long)L.ncell, (long)L.ld, L.dt, L.dt_per_cell, L.out, (long)L.ldo, L.status,
+                           L.first_failed, p, L.counters);
+        if (hipGetLastError() != hipSuccess) return HF_EDEVICE;
+        timer.stop();
+        return HF_OK;
     }
 }
-
-size_t chem_workspace(int64_t, int, int, int) { return 0; }
 
 // ------------------------------------------------------------------ flux stage
 constexpr int TX2 = 15, TY2 = 15, TX1 = 255;
@@ -113,12 +339,13 @@ int flux(const FluxLaunch& L) {
         return HF_EINVAL;
     } else {
     StreamTimer timer(L.kernel_ms, L.stream);
+    const unsigned grid = (unsigned)((L.ntiles + 7) / 8) * 8;   // XCD remap needs a multiple of 8
     if (L.one_d) {
-        hipLaunchKernelGGL((flux_stage_kernel<Topo, TX1, 1, false>), dim3(L.ntiles),
+        hipLaunchKernelGGL((flux_stage_kernel<Topo, TX1, 1, false>), dim3(grid),
                            dim3(FluxTile<Topo, TX1, 1, false>::THREADS), 0, L.stream, static_cast<const MD*>(L.mech),
                            L.boxes, L.tiles, L.ntiles, L.Uin, L.Uout, L.first_order, L.first_failed, L.counters);
     } else {
-        hipLaunchKernelGGL((flux_stage_kernel<Topo, TX2, TY2, true>), dim3(L.ntiles),
+        hipLaunchKernelGGL((flux_stage_kernel<Topo, TX2, TY2, true>), dim3(grid),
                            dim3(FluxTile<Topo, TX2, TY2, true>::THREADS), 0, L.stream, static_cast<const MD*>(L.mech),
                            L.boxes, L.tiles, L.ntiles, L.Uin, L.Uout, L.first_order, L.first_failed, L.counters);
     }

@@ -1,6 +1,10 @@
 """GPU parity of the chemistry step (HIP kernels through the C ABI) against
-the CPU oracle run at the reference CVODE tolerances of rtol 1e-10, atol 1e-14.
+the CPU oracle.
 
+The oracle runs converged (rtol 1e-12, atol 1e-20: within 2e-11 of a Radau
+IIA solution at rtol 1e-12 on the golden cells). Its run at the reference
+setting rtol 1e-10 / atol 1e-14 is itself 1.7e-6 away from the converged
+solution on HO2 in this workload, so it is reported, not asserted against.
 Tolerances (stated with the result): T within 1e-6 relative; Y_k within
 1e-6 * max(|Y_k|, Y_floor) with Y_floor = 1e-6; element masses within 1e-12
 relative. The product runs at its defaults (the reference's rtol 1e-8,
@@ -17,6 +21,7 @@ torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
 
 Y_FLOOR = 1e-6
+ORACLE_RTOL, ORACLE_ATOL = 1e-12, 1e-20
 
 
 def dev(a):
@@ -41,14 +46,18 @@ def test_chemistry_matches_oracle(oracle_h2, mode):
     mech = hf.Mechanism(H2)
     U0, dt = mech.synth_cells(0, 2, 512)
     ref = U0.copy()
-    r, st, ff, msg = oracle_h2.chem_advance(ref, dt, rtol=1e-10, atol=1e-14)
+    r, st, ff, msg = oracle_h2.chem_advance(ref, dt, rtol=ORACLE_RTOL, atol=ORACLE_ATOL)
     assert r == 0, msg
+    ref_cvode = U0.copy()
+    assert oracle_h2.chem_advance(ref_cvode, dt, rtol=1e-10, atol=1e-14)[0] == 0
     Ud = dev(U0)
     batch = hf.ChemistryBatch(mech, mode=mode)
     stats = batch.advance(Ud, dev(dt))
     out = Ud.cpu().numpy()
     eT, eY = compare(oracle_h2, out, ref, mech.W, oracle_h2.m["comp"])
-    print(f"[gpu] mode={mode} T rel err {eT:.3e}  Y rel err (floor {Y_FLOOR:g}) {eY:.3e}  stats {stats}", flush=True)
+    eT2, eY2 = compare(oracle_h2, out, ref_cvode, mech.W, oracle_h2.m["comp"])
+    print(f"[gpu] mode={mode} vs converged oracle: T {eT:.3e} Y(floor {Y_FLOOR:g}) {eY:.3e}; "
+          f"vs oracle at rtol 1e-10/atol 1e-14: T {eT2:.3e} Y {eY2:.3e}; stats {stats}", flush=True)
     assert eT < 1e-6 and eY < 1e-6
     # element masses and untouched momentum/energy planes
     m = oracle_h2.m
