@@ -25,19 +25,19 @@ struct BdfCounters {
 
 enum { BDF_OK = 0, BDF_RUNNING = 1, BDF_FAIL = 3 };
 
-template <int N, class Sys, int ZSTRIDE = 64>
+template <int N, class Sys, class MT = double, int ZSTRIDE = 64>
 struct Bdf {
     static constexpr int QMAX = 5;
     static constexpr double UROUND = 2.220446049250313e-16;
+    // doubles per lane in the LDS slab: Nordsieck history, tau, l
+    static constexpr int SLAB = (QMAX + 1) * N + (QMAX + 2) + (QMAX + 1);
 
     // Nordsieck history: element (j, i) at zs[(j*N + i) * ZSTRIDE]; the caller
     // points zs at this lane's slot of an LDS (or global) array laid out
     // [j][i][lane], so consecutive lanes touch consecutive doubles.
     double* zs;
     double ewt[N], acor[N];
-    double M[N * N];
-    double tau[QMAX + 2];
-    double l[QMAX + 1];
+    MT M[N * N];
     double tq1, tq2, tq3, tq4, tq5;
     double tn, h, hprime, hscale, eta, etamax, gamma, gammap, gamrat, rl1, crate, acnrm, saved_tq5;
     double rtol, atol;
@@ -46,15 +46,21 @@ struct Bdf {
     BdfCounters c;
 
     // ------------------------------------------------------------ helpers
-    template <int K>
-    __device__ __forceinline__ static double sel(const double (&a)[K], int idx) {
-        double r = a[0];
-        #pragma unroll
-        for (int j = 1; j < K; ++j) r = (j == idx) ? a[j] : r;
-        return r;
+    // zs points into LDS: address it as LDS so the accesses are ds_read/ds_write
+    // rather than flat loads through a generic pointer.
+    __device__ __forceinline__ double& Z(int j, int i) const {
+        typedef __attribute__((address_space(3))) double lds_double;
+        return *(double*)((lds_double*)zs + (j * N + i) * ZSTRIDE);
     }
-    __device__ __forceinline__ double& Z(int j, int i) const { return zs[(j * N + i) * ZSTRIDE]; }
     __device__ __forceinline__ double zrow(int j, int i) const { return Z(j, i); }
+    // Step-size history tau and BDF coefficients l share the lane's LDS slab,
+    // after the Nordsieck array: they are indexed by the current order.
+    __device__ __forceinline__ double& slab(int k) const {
+        typedef __attribute__((address_space(3))) double lds_double;
+        return *(double*)((lds_double*)zs + k * ZSTRIDE);
+    }
+    __device__ __forceinline__ double& TAU(int i) const { return slab((QMAX + 1) * N + i); }
+    __device__ __forceinline__ double& LC(int i) const { return slab((QMAX + 1) * N + QMAX + 2 + i); }
     __device__ __forceinline__ double wrms(const double* v) const {
         double s = 0;
         #pragma unroll
@@ -117,7 +123,7 @@ struct Bdf {
         #pragma unroll
         for (int j = 1; j <= QMAX - 2; ++j) {
             if (j <= q - 2) {
-                hsum += tau[j];
+                hsum += TAU(j);
                 const double xi = hsum / hscale;
                 #pragma unroll
                 for (int i = j + 2; i >= 2; --i) lc[i] = lc[i] * xi + lc[i - 1];
@@ -141,7 +147,7 @@ struct Bdf {
         #pragma unroll
         for (int j = 1; j < QMAX; ++j) {
             if (j < q) {
-                hsum += tau[j + 1];
+                hsum += TAU(j + 1);
                 const double xi = hsum / hscale;
                 prod *= xi;
                 alpha0 -= 1.0 / (j + 1);
@@ -170,30 +176,30 @@ struct Bdf {
 
     __device__ __forceinline__ void set_bdf() {
         #pragma unroll
-        for (int i = 0; i <= QMAX; ++i) l[i] = 0;
-        l[0] = l[1] = 1;
+        for (int i = 0; i <= QMAX; ++i) LC(i) = 0;
+        LC(0) = LC(1) = 1;
         double xi_inv = 1, xistar_inv = 1, alpha0 = -1, alpha0_hat = -1, hsum = h;
         if (q > 1) {
             #pragma unroll
             for (int j = 2; j < QMAX; ++j) {
                 if (j < q) {
-                    hsum += tau[j - 1];
+                    hsum += TAU(j - 1);
                     xi_inv = h / hsum;
                     alpha0 -= 1.0 / j;
                     #pragma unroll
-                    for (int i = j; i >= 1; --i) l[i] += l[i - 1] * xi_inv;
+                    for (int i = j; i >= 1; --i) LC(i) += LC(i - 1) * xi_inv;
                 }
             }
             alpha0 -= 1.0 / q;
-            xistar_inv = -l[1] - alpha0;
-            hsum += sel(tau, q - 1);
+            xistar_inv = -LC(1) - alpha0;
+            hsum += TAU(q - 1);
             xi_inv = h / hsum;
-            alpha0_hat = -l[1] - xi_inv;
+            alpha0_hat = -LC(1) - xi_inv;
             #pragma unroll
             for (int i = QMAX; i >= 1; --i)
-                if (i <= q) l[i] += l[i - 1] * xistar_inv;
+                if (i <= q) LC(i) += LC(i - 1) * xistar_inv;
         }
-        const double lq = sel(l, q);
+        const double lq = LC(q);
         const double A1 = 1.0 - alpha0_hat + alpha0;
         const double A2 = 1.0 + q * A1;
         tq2 = fabs(A1 / (alpha0 * A2));
@@ -207,7 +213,7 @@ struct Bdf {
             } else {
                 tq1 = 1.0;
             }
-            hsum += sel(tau, q);
+            hsum += TAU(q);
             xi_inv = h / hsum;
             const double A5 = alpha0 - 1.0 / (q + 1);
             const double A6 = alpha0_hat - xi_inv;
@@ -218,15 +224,19 @@ struct Bdf {
     }
 
     // ------------------------------------------------------------ linear algebra
+    // In-place LU of M without pivoting; the diagonal keeps the reciprocal
+    // pivots. MT is the Newton-matrix precision (double, or float for the
+    // mixed form, where the residuals and the corrections stay double).
     __device__ __forceinline__ bool lu_factor() {
         #pragma unroll
         for (int k = 0; k < N; ++k) {
-            const double p = M[k * N + k];
-            if (!(fabs(p) > 1e-300) || !__builtin_isfinite(p)) return false;
-            const double inv = 1.0 / p;
+            const MT p = M[k * N + k];
+            if (!(fabs(p) > __FLT_MIN__) || !__builtin_isfinite(p)) return false;
+            const MT inv = MT(1) / p;
+            M[k * N + k] = inv;
             #pragma unroll
             for (int i = k + 1; i < N; ++i) {
-                const double li = M[i * N + k] * inv;
+                const MT li = M[i * N + k] * inv;
                 M[i * N + k] = li;
                 #pragma unroll
                 for (int j = k + 1; j < N; ++j) M[i * N + j] = fma(-li, M[k * N + j], M[i * N + j]);
@@ -234,18 +244,23 @@ struct Bdf {
         }
         return true;
     }
-    __device__ __forceinline__ void lu_solve(double* b) const {
+    __device__ __forceinline__ void lu_solve(double* bd) const {
+        MT b[N];
+        #pragma unroll
+        for (int i = 0; i < N; ++i) b[i] = (MT)bd[i];
         #pragma unroll
         for (int k = 0; k < N; ++k)
             #pragma unroll
             for (int i = k + 1; i < N; ++i) b[i] = fma(-M[i * N + k], b[k], b[i]);
         #pragma unroll
         for (int i = N - 1; i >= 0; --i) {
-            double s = b[i];
+            MT s = b[i];
             #pragma unroll
             for (int j = i + 1; j < N; ++j) s = fma(-M[i * N + j], b[j], s);
-            b[i] = s / M[i * N + i];
+            b[i] = s * M[i * N + i];
         }
+        #pragma unroll
+        for (int i = 0; i < N; ++i) bd[i] = (double)b[i];
     }
 
     __device__ __forceinline__ bool lsetup(Sys& sys, const double* y) {
@@ -253,10 +268,11 @@ struct Bdf {
         ++c.jac;
         ++c.lu;
         if (!sys.jac(y, M, f)) return false;
+        const MT g = (MT)gamma;
         #pragma unroll
-        for (int i = 0; i < N * N; ++i) M[i] *= -gamma;
+        for (int i = 0; i < N * N; ++i) M[i] *= -g;
         #pragma unroll
-        for (int i = 0; i < N; ++i) M[i * N + i] += 1.0;
+        for (int i = 0; i < N; ++i) M[i * N + i] += MT(1);
         jfresh = true;
         gammap = gamma;
         gamrat = 1;
@@ -321,7 +337,7 @@ struct Bdf {
             predict();
             if ((tn - tout) * h > 0) tn = tout;
             set_bdf();
-            rl1 = 1.0 / l[1];
+            rl1 = 1.0 / LC(1);
             gamma = h * rl1;
             if (nst == 0) gammap = gamma;
             gamrat = (nst > 0) ? gamma / gammap : 1.0;
@@ -381,14 +397,14 @@ struct Bdf {
         ++c.steps;
         #pragma unroll
         for (int i = QMAX; i >= 2; --i)
-            if (i <= q) tau[i] = tau[i - 1];
-        if (q == 1 && nst > 1) tau[2] = tau[1];
-        tau[1] = h;
+            if (i <= q) TAU(i) = TAU(i - 1);
+        if (q == 1 && nst > 1) TAU(2) = TAU(1);
+        TAU(1) = h;
         #pragma unroll
         for (int j = 0; j <= QMAX; ++j)
             if (j <= q) {
                 #pragma unroll
-                for (int i = 0; i < N; ++i) Z(j, i) = fma(l[j], acor[i], Z(j, i));
+                for (int i = 0; i < N; ++i) Z(j, i) = fma(LC(j), acor[i], Z(j, i));
             }
         --qwait;
         if (qwait == 1 && q != QMAX) {
@@ -422,7 +438,7 @@ struct Bdf {
                     etaqm1 = 1.0 / (pow(6.0 * ddn, 1.0 / q) + 1e-6);
                 }
                 if (q != QMAX && saved_tq5 != 0) {
-                    const double cquot = (tq5 / saved_tq5) * pow(h / tau[2], (double)L);
+                    const double cquot = (tq5 / saved_tq5) * pow(h / TAU(2), (double)L);
                     double tmp[N];
                     #pragma unroll
                     for (int i = 0; i < N; ++i) tmp[i] = acor[i] - cquot * Z(QMAX, i);
@@ -533,7 +549,7 @@ struct Bdf {
         eta = 1;
         etamax = 10000.0;
         #pragma unroll
-        for (int i = 0; i < QMAX + 2; ++i) tau[i] = 0;
+        for (int i = 0; i < QMAX + 2; ++i) TAU(i) = 0;
         tq1 = tq2 = tq3 = tq4 = tq5 = 0;
         nst = nstlp = 0;
         gammap = 0;

@@ -18,35 +18,39 @@ struct ChemSys {
     double rho, e, t;
     const double* shift;   // optional per-reaction ln A shift (sensitivities)
 
-    __device__ __forceinline__ bool temp(const double* y, double* Y) {
+    __device__ __forceinline__ const MechData<T>& tables() const { return d; }
+    __device__ __forceinline__ bool temp(const MechData<T>& dd, const double* y, double* Y) {
         double s = 0;
         #pragma unroll
         for (int k = 0; k < N; ++k) { Y[k] = y[k]; s += y[k]; }
         Y[N] = 1.0 - s;
         double tt = t;
-        if (!temperature_from_e(d, Y, e, tt)) return false;
+        if (!temperature_from_e(dd, Y, e, tt)) return false;
         t = tt;
         return true;
     }
     __device__ __forceinline__ bool rhs(const double* y, double* f) {
+        const MechData<T>& dd = tables();
         double Y[NS], C[NS], w[NS];
-        if (!temp(y, Y)) return false;
+        if (!temp(dd, y, Y)) return false;
         #pragma unroll
-        for (int k = 0; k < NS; ++k) C[k] = rho * Y[k] * d.invW[k];
-        production_rates(d, t, C, w, shift);
+        for (int k = 0; k < NS; ++k) C[k] = rho * Y[k] * dd.invW[k];
+        production_rates(dd, t, C, w, shift);
         bool ok = true;
         const double rr = 1.0 / rho;
         #pragma unroll
         for (int k = 0; k < N; ++k) {
-            f[k] = w[k] * d.W[k] * rr;
+            f[k] = w[k] * dd.W[k] * rr;
             ok = ok && finite(f[k]);
         }
         return ok;
     }
-    __device__ __forceinline__ bool jac(const double* y, double* J, double* f) {
+    template <class MT>
+    __device__ __forceinline__ bool jac(const double* y, MT* J, double* f) {
+        const MechData<T>& dd = tables();
         double Y[NS];
-        if (!temp(y, Y)) return false;
-        chem_jacobian(d, t, rho, Y, J, f, shift);
+        if (!temp(dd, y, Y)) return false;
+        chem_jacobian(dd, t, rho, Y, J, f, shift);
         bool ok = true;
         #pragma unroll
         for (int i = 0; i < N * N; ++i) ok = ok && finite(J[i]);
@@ -67,11 +71,11 @@ struct ChemParams {
 // interleave cells: start() validates and begins, step() takes one BDF step,
 // finish() applies the reference's post-step checks and writes the species.
 // zs: this lane's slot of the wave's Nordsieck slab ([order][unknown][64 lanes]).
-template <class T>
+template <class T, class MT = double>
 struct CellRun {
     static constexpr int NS = T::NS, NV = NS + 3, N = NS - 1;
     ChemSys<T> sys;
-    Bdf<N, ChemSys<T>> bdf;
+    Bdf<N, ChemSys<T>, MT> bdf;
     double y[N];
 
     __device__ __forceinline__ CellRun(const MechData<T>& d, double* zs, const ChemParams& p)

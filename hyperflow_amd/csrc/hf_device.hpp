@@ -18,6 +18,11 @@
 #include <cstdint>
 #include <utility>
 
+// Lambdas that carry per-reaction or per-face code must be inlined: a lambda
+// left as a call would take the per-lane arrays it captures by reference into
+// scratch memory.
+#define HF_AI __attribute__((always_inline))
+
 namespace hf {
 
 constexpr double kR = 8314.462618;
@@ -52,6 +57,21 @@ __device__ __forceinline__ void static_for(F&& f) {
 
 __device__ __forceinline__ bool finite(double x) { return __builtin_isfinite(x); }
 
+// NASA7 coefficients of species k for temperature t (low range for t <= Tmid).
+// Both sets are read with uniform loads and the per-lane choice is a value
+// select, so the coefficients never go through a per-lane pointer.
+struct Nasa7 {
+    double a[7];
+};
+template <class D>
+__device__ __forceinline__ Nasa7 nasa_select(const D& d, int k, double t) {
+    Nasa7 r;
+    const bool low = t <= d.Tmid[k];
+    #pragma unroll
+    for (int i = 0; i < 7; ++i) r.a[i] = low ? d.lo[k][i] : d.hi[k][i];
+    return r;
+}
+
 // ----------------------------------------------------------------- thermo
 // Mixture energy polynomial for fixed mass fractions: with yw_k = Y_k / W_k,
 // e/R = sum_k yw_k (h_k/RT - 1) T and cv/R = sum_k yw_k (cp_k/R - 1).
@@ -83,7 +103,11 @@ struct EnergyPoly {
     }
     // e (J/kg) and cv (J/kg/K) at temperature t
     __device__ __forceinline__ void eval(double t, double& e, double& cv) const {
-        const double* c = t <= tmid ? lo : hi;
+        // select values, not arrays: a pointer select would force lo/hi to scratch
+        const bool low = t <= tmid;
+        double c[6];
+        #pragma unroll
+        for (int i = 0; i < 6; ++i) c[i] = low ? lo[i] : hi[i];
         cv = kR * fma(t, fma(t, fma(t, fma(t, c[4], c[3]), c[2]), c[1]), c[0]);
         const double p = fma(t, fma(t, fma(t, fma(t, c[4] * 0.2, c[3] * 0.25), c[2] * (1.0 / 3.0)), c[1] * 0.5), c[0]);
         e = kR * fma(t, p, c[5]);
@@ -98,7 +122,8 @@ __device__ __forceinline__ void energy_species(const MechData<T>& d, const doubl
     cv = 0;
     #pragma unroll
     for (int k = 0; k < T::NS; ++k) {
-        const double* a = t <= d.Tmid[k] ? d.lo[k] : d.hi[k];
+        const Nasa7 na = nasa_select(d, k, t);
+        const double* a = na.a;
         const double yw = Y[k] * d.invW[k];
         const double cpr = fma(t, fma(t, fma(t, fma(t, a[4], a[3]), a[2]), a[1]), a[0]);
         const double hrt = fma(t, fma(t, fma(t, fma(t, a[4] * 0.2, a[3] * 0.25), a[2] * (1.0 / 3.0)), a[1] * 0.5), a[0]) + a[5] / t;
@@ -152,7 +177,8 @@ __device__ __forceinline__ void cp_cv_rw(const MechData<T>& d, const double* Y, 
     rw = 0;
     #pragma unroll
     for (int k = 0; k < T::NS; ++k) {
-        const double* a = t <= d.Tmid[k] ? d.lo[k] : d.hi[k];
+        const Nasa7 na = nasa_select(d, k, t);
+        const double* a = na.a;
         const double yw = Y[k] * d.invW[k];
         rw += yw;
         cp = fma(yw, fma(t, fma(t, fma(t, fma(t, a[4], a[3]), a[2]), a[1]), a[0]), cp);
@@ -248,7 +274,8 @@ __device__ __forceinline__ void species_h_g(const MechData<T>& d, double t, doub
     const double it = 1.0 / t;
     #pragma unroll
     for (int k = 0; k < T::NS; ++k) {
-        const double* a = t <= d.Tmid[k] ? d.lo[k] : d.hi[k];
+        const Nasa7 na = nasa_select(d, k, t);
+        const double* a = na.a;
         const double h = fma(t, fma(t, fma(t, fma(t, a[4] * 0.2, a[3] * 0.25), a[2] * (1.0 / 3.0)), a[1] * 0.5), a[0]) + a[5] * it;
         const double s = fma(a[0], logt, fma(t, fma(t, fma(t, fma(t, a[4] * 0.25, a[3] * (1.0 / 3.0)), a[2] * 0.5), a[1]), a[6]));
         hrt[k] = h;
@@ -322,7 +349,7 @@ struct RateI {
         if constexpr (T::rev[I]) {
             double dg = 0, dh = 0;
             int dn = 0;
-            static_for<0, T::NRMAX>([&](auto j) {
+            static_for<0, T::NRMAX>([&](auto j) HF_AI {
                 constexpr int r = T::reac[I][j], p = T::prod[I][j];
                 if constexpr (r >= 0) { dg -= grt[r]; dn -= 1; if constexpr (DERIV) dh -= hrt[r]; }
                 if constexpr (p >= 0) { dg += grt[p]; dn += 1; if constexpr (DERIV) dh += hrt[p]; }
@@ -339,7 +366,7 @@ struct RateI {
 template <class T, int I>
 __device__ __forceinline__ double third_body(const MechData<T>& d, const double* C, double Ctot) {
     double M = d.defeff[I] * Ctot;
-    static_for<0, T::NEFF>([&](auto j) {
+    static_for<0, T::NEFF>([&](auto j) HF_AI {
         constexpr int k = T::effsp[I][j];
         if constexpr (k >= 0) M = fma(d.effd[I][j], C[k], M);
     });
@@ -359,20 +386,20 @@ __device__ __forceinline__ void production_rates(const MechData<T>& d, double t,
     double Ctot = 0;
     #pragma unroll
     for (int k = 0; k < NS; ++k) { Ctot += C[k]; wdot[k] = 0; }
-    static_for<0, T::NR>([&](auto ic) {
+    static_for<0, T::NR>([&](auto ic) HF_AI {
         constexpr int I = decltype(ic)::value;
         double Mc = 0;
         if constexpr (T::kind[I] != 0) Mc = third_body<T, I>(d, C, Ctot);
         const RateI<T, I, false> k(d, t, logt, it, logc0, hrt, grt, Mc, lnA_shift ? lnA_shift[I] : 0.0);
         double fwd = k.kf, bwd = k.kf * k.rkc;
-        static_for<0, T::NRMAX>([&](auto j) {
+        static_for<0, T::NRMAX>([&](auto j) HF_AI {
             constexpr int r = T::reac[I][j], p = T::prod[I][j];
             if constexpr (r >= 0) fwd *= C[r];
             if constexpr (p >= 0) bwd *= C[p];
         });
         const double q = fwd - bwd;
         if (qout) qout[I] = q;
-        static_for<0, T::NRMAX>([&](auto j) {
+        static_for<0, T::NRMAX>([&](auto j) HF_AI {
             constexpr int r = T::reac[I][j], p = T::prod[I][j];
             if constexpr (r >= 0) wdot[r] -= q;
             if constexpr (p >= 0) wdot[p] += q;
@@ -386,10 +413,12 @@ __device__ __forceinline__ void production_rates(const MechData<T>& d, double t,
 // defined by e(T, Y) = e at fixed e (HyperFlow_V3/src/chemistry.cpp:36-53):
 //   df_k/dy_j = W_k [dwdot_k/dC_j / W_j - dwdot_k/dC_last / W_last]
 //             + W_k/rho * dwdot_k/dT * dT/dy_j,  dT/dy_j = -(e_j - e_last)/cv.
-// Writes J (row-major N x N, N = NS-1) and f.
-template <class T>
+// Writes J (row-major N x N, N = NS-1) and f. Rate constants, their
+// derivatives and f are FP64; J is accumulated in MT (double, or float for
+// the mixed-precision Newton matrix).
+template <class T, class MT>
 __device__ __forceinline__ void chem_jacobian(const MechData<T>& d, double t, double rho, const double* Y,
-                                              double* J, double* f, const double* lnA_shift = nullptr) {
+                                              MT* J, double* f, const double* lnA_shift = nullptr) {
     constexpr int NS = T::NS, N = NS - 1, L = NS - 1;
     const double logt = log(t), it = 1.0 / t;
     const double logc0 = log(kP0 / (kR * t));
@@ -407,17 +436,17 @@ __device__ __forceinline__ void chem_jacobian(const MechData<T>& d, double t, do
     for (int k = 0; k < N; ++k) {
         dwdCL[k] = 0;
         #pragma unroll
-        for (int j = 0; j < N; ++j) J[k * N + j] = 0;
+        for (int j = 0; j < N; ++j) J[k * N + j] = MT(0);
     }
     // J accumulates dwdot_k/dC_j (k, j < N) in this pass.
-    static_for<0, T::NR>([&](auto ic) {
+    static_for<0, T::NR>([&](auto ic) HF_AI {
         constexpr int I = decltype(ic)::value;
         double Mc = 0;
         if constexpr (T::kind[I] != 0) Mc = third_body<T, I>(d, C, Ctot);
         const RateI<T, I, true> k(d, t, logt, it, logc0, hrt, grt, Mc, lnA_shift ? lnA_shift[I] : 0.0);
         const double kr = k.kf * k.rkc;
         double pf = 1.0, pb = 1.0;
-        static_for<0, T::NRMAX>([&](auto j) {
+        static_for<0, T::NRMAX>([&](auto j) HF_AI {
             constexpr int r = T::reac[I][j], p = T::prod[I][j];
             if constexpr (r >= 0) pf *= C[r];
             if constexpr (p >= 0) pb *= C[p];
@@ -426,25 +455,25 @@ __device__ __forceinline__ void chem_jacobian(const MechData<T>& d, double t, do
         const double dqdT = k.kf * k.dlnkf_dT * pf - kr * (k.dlnkf_dT + k.dlnrkc_dT) * pb;
         // d q / d C_s for the species in the reaction's concentration products
         // (the kf/kr factors that carry [M] are handled separately below).
-        auto add = [&](auto sc, double dq) {
+        auto add = [&](auto sc, double dq) HF_AI {
             // species touched by reaction I: accumulate nu_k * dq into column s
             constexpr int s = decltype(sc)::value;
-            static_for<0, T::NRMAX>([&](auto jj) {
+            static_for<0, T::NRMAX>([&](auto jj) HF_AI {
                 constexpr int r = T::reac[I][jj], p = T::prod[I][jj];
                 if constexpr (r >= 0 && r < N) {
-                    if constexpr (s < N) J[r * N + s] -= dq; else dwdCL[r] -= dq;
+                    if constexpr (s < N) J[r * N + s] -= (MT)dq; else dwdCL[r] -= dq;
                 }
                 if constexpr (p >= 0 && p < N) {
-                    if constexpr (s < N) J[p * N + s] += dq; else dwdCL[p] += dq;
+                    if constexpr (s < N) J[p * N + s] += (MT)dq; else dwdCL[p] += dq;
                 }
             });
         };
         // reactant concentrations: d(prod C^nu')/dC_s = product with one C_s removed
-        static_for<0, T::NRMAX>([&](auto j) {
+        static_for<0, T::NRMAX>([&](auto j) HF_AI {
             constexpr int s = T::reac[I][j];
             if constexpr (s >= 0) {
                 double pr_ = k.kf;
-                static_for<0, T::NRMAX>([&](auto m) {
+                static_for<0, T::NRMAX>([&](auto m) HF_AI {
                     constexpr int r2 = T::reac[I][m];
                     if constexpr (r2 >= 0 && m != j) pr_ *= C[r2];
                 });
@@ -453,7 +482,7 @@ __device__ __forceinline__ void chem_jacobian(const MechData<T>& d, double t, do
             constexpr int p = T::prod[I][j];
             if constexpr (p >= 0) {
                 double pp_ = kr;
-                static_for<0, T::NRMAX>([&](auto m) {
+                static_for<0, T::NRMAX>([&](auto m) HF_AI {
                     constexpr int p2 = T::prod[I][m];
                     if constexpr (p2 >= 0 && m != j) pp_ *= C[p2];
                 });
@@ -464,15 +493,15 @@ __device__ __forceinline__ void chem_jacobian(const MechData<T>& d, double t, do
         if constexpr (T::kind[I] != 0) {
             const double dqdM = k.dkf_dM * (pf - k.rkc * pb);
             // default efficiency on every species, plus the listed corrections
-            static_for<0, NS>([&](auto s) {
+            static_for<0, NS>([&](auto s) HF_AI {
                 double eff = d.defeff[I];
-                static_for<0, T::NEFF>([&](auto j) {
+                static_for<0, T::NEFF>([&](auto j) HF_AI {
                     if constexpr (T::effsp[I][j] == decltype(s)::value) eff += d.effd[I][j];
                 });
                 add(s, eff * dqdM);
             });
         }
-        static_for<0, T::NRMAX>([&](auto j) {
+        static_for<0, T::NRMAX>([&](auto j) HF_AI {
             constexpr int r = T::reac[I][j], p = T::prod[I][j];
             if constexpr (r >= 0) { wd[r] -= q; dwdT[r] -= dqdT; }
             if constexpr (p >= 0) { wd[p] += q; dwdT[p] += dqdT; }
@@ -482,7 +511,8 @@ __device__ __forceinline__ void chem_jacobian(const MechData<T>& d, double t, do
     double cv = 0, ek[NS];
     #pragma unroll
     for (int k = 0; k < NS; ++k) {
-        const double* a = t <= d.Tmid[k] ? d.lo[k] : d.hi[k];
+        const Nasa7 na = nasa_select(d, k, t);
+        const double* a = na.a;
         const double cpr = fma(t, fma(t, fma(t, fma(t, a[4], a[3]), a[2]), a[1]), a[0]);
         cv = fma(Y[k] * d.invW[k], cpr - 1.0, cv);
         ek[k] = (hrt[k] - 1.0) * kR * t * d.invW[k];
@@ -493,12 +523,13 @@ __device__ __forceinline__ void chem_jacobian(const MechData<T>& d, double t, do
     for (int k = 0; k < N; ++k) {
         const double Wk = d.W[k];
         f[k] = wd[k] * Wk * rrho;
-        const double cl = dwdCL[k] * d.invW[L];
-        const double tk = Wk * rrho * dwdT[k];
+        const MT cl = (MT)(dwdCL[k] * d.invW[L]);
+        const MT tk = (MT)(Wk * rrho * dwdT[k]);
+        const MT wk = (MT)Wk;
         #pragma unroll
         for (int j = 0; j < N; ++j) {
-            const double dTdy = -(ek[j] - ek[L]) * rcv;
-            J[k * N + j] = Wk * (J[k * N + j] * d.invW[j] - cl) + tk * dTdy;
+            const MT dTdy = (MT)(-(ek[j] - ek[L]) * rcv);
+            J[k * N + j] = wk * (J[k * N + j] * (MT)d.invW[j] - cl) + tk * dTdy;
         }
     }
 }

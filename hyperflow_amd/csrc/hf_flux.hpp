@@ -80,7 +80,7 @@ flux_stage_kernel(const MechData<T>* __restrict__ dm, const hf_box* __restrict__
     }
     __syncthreads();
 
-    auto cellv = [&](int gi, int gj, int n) -> double { return s_in[n * GX * GY + gj * GX + gi]; };
+    auto cellv = [&](int gi, int gj, int n) HF_AI -> double { return s_in[n * GX * GY + gj * GX + gi]; };
     int fallbacks = 0, faces = 0;
     bool fail = false;
     unsigned long long fail_idx = ~0ull;
@@ -90,7 +90,7 @@ flux_stage_kernel(const MechData<T>* __restrict__ dm, const hf_box* __restrict__
     const int gyc = TWO_D ? ly + 2 : ly;               // staged row of this lane's cell
 
     // ---- one face: stencil cells (a, b | c, d) along direction DIR
-    auto face = [&](auto dirc, int fi, int fj, bool active) {
+    auto face = [&](auto dirc, int fi, int fj, bool active) HF_AI {
         constexpr int DIR = decltype(dirc)::value;
         constexpr int NRM = DIR == 0 ? NS : NS + 1, TAN = DIR == 0 ? NS + 1 : NS;
         double f[NV];

@@ -65,7 +65,7 @@ chem_plain_kernel(const MechData<TT>* __restrict__ dm, const double* __restrict_
                   int8_t* __restrict__ status, unsigned long long* __restrict__ first_failed, ChemParams p,
                   unsigned long long* __restrict__ counters) {
     constexpr int N = TT::NS - 1;
-    __shared__ double zlds[6 * N * 64];
+    __shared__ double zlds[(6 * N + 13) * 64];   // per-lane BDF slab: Nordsieck history, tau, l
     const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
     BdfCounters c;
     if (i < ncell) {
@@ -207,46 +207,38 @@ __global__ void __launch_bounds__(256) chem_scatter_kernel(long ncell, int nbins
     if (i < ncell) w.order[w.offs[bin] + lbase[bin] + local] = (int32_t)i;
 }
 
-template <class TT>
+template <class TT, class MT>
 __global__ void __launch_bounds__(64)
 chem_persistent_kernel(const MechData<TT>* __restrict__ dm, const double* __restrict__ U, long ncell, long ld,
                        const double* __restrict__ dt, int per_cell, double* __restrict__ out, long ldo,
                        int8_t* __restrict__ status, unsigned long long* __restrict__ first_failed, ChemParams p,
-                       unsigned long long* __restrict__ counters, Workspace w, int chunk) {
+                       unsigned long long* __restrict__ counters, Workspace w, int refill) {
     constexpr int N = TT::NS - 1;
-    __shared__ double zlds[6 * N * 64];
+    __shared__ double zlds[(6 * N + 13) * 64];   // per-lane BDF slab: Nordsieck history, tau, l
     const int lane = threadIdx.x;
-    CellRun<TT> run(*dm, zlds + lane, p);
+    CellRun<TT, MT> run(*dm, zlds + lane, p);
     BdfCounters tot;
     int max_steps_cell = 0;
     long cell = -1;
     bool active = false;
-    // Wave-uniform state: the global queue is empty (exhausted), and the
-    // wave's claimed but not yet started queue slots [pend_lo, pend_hi).
+    // Wave-uniform: the global queue is empty. Idle lanes are refilled
+    // together once at least `refill` of them wait (or when the whole wave is
+    // idle), so the cost of starting a cell (initial rate and step-size
+    // estimate) is paid by many lanes in the same trip, not one at a time.
     bool exhausted = false;
-    unsigned pend_lo = 0, pend_hi = 0;
     for (;;) {
         const unsigned long long want = __ballot(!active);
-        if (want && !exhausted && pend_lo >= pend_hi) {
-            // claim `chunk` consecutive slots for the wave with one atomic
+        const int idle = __popcll(want);
+        if (!exhausted && idle > 0 && (idle >= refill || idle == 64)) {
             const int leader = __ffsll((long long)want) - 1;
             unsigned base = 0;
-            if (lane == leader) base = atomicAdd(w.head, (unsigned)chunk);
+            if (lane == leader) base = atomicAdd(w.head, (unsigned)idle);
             base = __shfl(base, leader, 64);
-            if (base >= (unsigned)ncell) {
-                exhausted = true;
-            } else {
-                pend_lo = base;
-                pend_hi = min(base + (unsigned)chunk, (unsigned)ncell);
-            }
-        }
-        if (want && pend_lo < pend_hi) {
-            const unsigned avail = pend_hi - pend_lo;
-            const unsigned cnt = (unsigned)__popcll(want);
+            if (base + (unsigned)idle >= (unsigned)ncell) exhausted = true;
             if (!active) {
-                const unsigned rank = (unsigned)__popcll(want & ((1ull << lane) - 1));
-                if (rank < avail) {
-                    cell = w.order[pend_lo + rank];
+                const unsigned slot = base + (unsigned)__popcll(want & ((1ull << lane) - 1));
+                if (slot < (unsigned)ncell) {
+                    cell = w.order[slot];
                     const double h = per_cell ? dt[cell] : dt[0];
                     const int st = run.start(U, cell, ld, h, p.maxsteps);
                     if (st == HF_OK) {
@@ -257,10 +249,9 @@ chem_persistent_kernel(const MechData<TT>* __restrict__ dm, const double* __rest
                     }
                 }
             }
-            pend_lo += min(cnt, avail);
         }
         if (!__any(active)) {
-            if (exhausted && pend_lo >= pend_hi) break;
+            if (exhausted) break;
             continue;
         }
         if (active) {
@@ -300,7 +291,8 @@ int chem(const ChemLaunch& L) {
             const Workspace w = carve(L.workspace, L.ncell);
             const int nbins = L.nbins > 0 ? min(L.nbins, kMaxBins) : 32;
             const int wpc = L.waves_per_cu > 0 ? L.waves_per_cu : 4;
-            const int chunk = L.chunk > 0 ? L.chunk : 64;
+            // queue chunking: idle lanes refilled together
+            const int chunk = L.chunk > 0 ? min(L.chunk, 64) : 16;
             const unsigned g256 = (unsigned)((L.ncell + 255) / 256);
             hipLaunchKernelGGL(chem_init_kernel, dim3(1), dim3(256), 0, L.stream, w, L.first_failed, L.counters);
             hipLaunchKernelGGL(chem_estimate_kernel<Topo>, dim3(g256), dim3(256), 0, L.stream, dm, L.U,
@@ -308,9 +300,14 @@ int chem(const ChemLaunch& L) {
             hipLaunchKernelGGL(chem_scan_kernel, dim3(1), dim3(64), 0, L.stream, w, nbins);
             hipLaunchKernelGGL(chem_scatter_kernel, dim3(g256), dim3(256), 0, L.stream, (long)L.ncell, nbins, w);
             StreamTimer timer(L.kernel_ms, L.stream);
-            hipLaunchKernelGGL(chem_persistent_kernel<Topo>, dim3((unsigned)(L.num_cus * wpc)), dim3(64), 0, L.stream,
-                               dm, L.U, (long)L.ncell, (long)L.ld, L.dt, L.dt_per_cell, L.out, (long)L.ldo, L.status,
-                               L.first_failed, p, L.counters, w, chunk);
+            if (L.precision == HF_PREC_MIXED)
+                hipLaunchKernelGGL((chem_persistent_kernel<Topo, float>), dim3((unsigned)(L.num_cus * wpc)), dim3(64),
+                                   0, L.stream, dm, L.U, (long)L.ncell, (long)L.ld, L.dt, L.dt_per_cell, L.out,
+                                   (long)L.ldo, L.status, L.first_failed, p, L.counters, w, chunk);
+            else
+                hipLaunchKernelGGL((chem_persistent_kernel<Topo, double>), dim3((unsigned)(L.num_cus * wpc)), dim3(64),
+                                   0, L.stream, dm, L.U, (long)L.ncell, (long)L.ld, L.dt, L.dt_per_cell, L.out,
+                                   (long)L.ldo, L.status, L.first_failed, p, L.counters, w, chunk);
             if (hipGetLastError() != hipSuccess) return HF_EDEVICE;
             timer.stop();
             return HF_OK;
