@@ -36,10 +36,6 @@ HIP_FLAGS = [
     "-O3", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC",
     # packed FP32 instructions are disabled for all device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    # Keep SimplifyCFG from merging stores to different per-lane fields into
-    # one store through a phi'd address: that blocks scalar replacement of the
-    # integrator state and sends it to scratch memory.
-    "-mllvm", "-simplifycfg-sink-common=false",
     "-Wall", "-Wno-unused-function", "-Wno-unused-parameter",
 ]
 

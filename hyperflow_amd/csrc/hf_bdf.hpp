@@ -416,19 +416,21 @@ struct Bdf {
         return BDF_OK;
     }
 
+    // Every branch computes into locals and the state is written once at the
+    // end, so no path-dependent store addresses appear (they would keep the
+    // lane's integrator state out of registers).
     __device__ __forceinline__ void prepare_next(double dsm) {
+        double n_eta, n_hprime;
+        int n_qprime = q, n_qwait = qwait;
         if (etamax == 1.0) {
-            qwait = max(qwait, 2);
-            qprime = q;
-            hprime = h;
-            eta = 1.0;
+            n_qwait = max(qwait, 2);
+            n_hprime = h;
+            n_eta = 1.0;
         } else {
             const double etaq = 1.0 / (pow(6.0 * dsm, 1.0 / L) + 1e-6);
-            if (qwait != 0) {
-                eta = etaq;
-                qprime = q;
-            } else {
-                qwait = 2;
+            double e = etaq;
+            if (qwait == 0) {
+                n_qwait = 2;
                 double etaqm1 = 0, etaqp1 = 0;
                 if (q > 1) {
                     double tmp[N];
@@ -447,29 +449,26 @@ struct Bdf {
                 }
                 const double etam = fmax(etaqm1, fmax(etaq, etaqp1));
                 if (etam < 1.5) {
-                    eta = 1.0;
-                    qprime = q;
+                    e = 1.0;
                 } else if (etam == etaq) {
-                    eta = etaq;
-                    qprime = q;
+                    e = etaq;
                 } else if (etam == etaqm1) {
-                    eta = etaqm1;
-                    qprime = q - 1;
+                    e = etaqm1;
+                    n_qprime = q - 1;
                 } else {
-                    eta = etaqp1;
-                    qprime = q + 1;
+                    e = etaqp1;
+                    n_qprime = q + 1;
                     #pragma unroll
                     for (int i = 0; i < N; ++i) Z(QMAX, i) = acor[i];
                 }
             }
-            if (eta < 1.5) {
-                eta = 1.0;
-                hprime = h;
-            } else {
-                eta = fmin(eta, etamax);
-                hprime = h * eta;
-            }
+            n_eta = e < 1.5 ? 1.0 : fmin(e, etamax);
+            n_hprime = e < 1.5 ? h : h * n_eta;
         }
+        eta = n_eta;
+        hprime = n_hprime;
+        qprime = n_qprime;
+        qwait = n_qwait;
         etamax = 10.0;
     }
 

@@ -57,24 +57,12 @@ void fill(const Mechanism& m, void* host) {
     }
 }
 
-// ------------------------------------------------------------------ chemistry (plain)
-template <class TT>
-__global__ void __launch_bounds__(64)
-chem_plain_kernel(const MechData<TT>* __restrict__ dm, const double* __restrict__ U, long ncell, long ld,
-                  const double* __restrict__ dt, int per_cell, double* __restrict__ out, long ldo,
-                  int8_t* __restrict__ status, unsigned long long* __restrict__ first_failed, ChemParams p,
-                  unsigned long long* __restrict__ counters) {
-    constexpr int N = TT::NS - 1;
-    __shared__ double zlds[(6 * N + 13) * 64];   // per-lane BDF slab: Nordsieck history, tau, l
-    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    BdfCounters c;
-    if (i < ncell) {
-        const double h = per_cell ? dt[i] : dt[0];
-        const int st = advance_cell<TT>(*dm, U, i, ld, h, out, ldo, p, c, zlds + threadIdx.x);
-        status[i] = (int8_t)st;
-        if (st != HF_OK) atomicMin(first_failed, (unsigned long long)i);
-    }
-    reduce_counters(c, c.steps, counters);
+// ------------------------------------------------------------------ chemistry
+// Fill-kernel reset of the failure index and counters (graph-capturable).
+__global__ void chem_zero_kernel(unsigned long long* first_failed, unsigned long long* counters) {
+    const int t = threadIdx.x;
+    if (t == 0) *first_failed = ~0ull;
+    if (t < 8) counters[t] = 0;
 }
 
 // ------------------------------------------------------------------ chemistry (batched)
@@ -207,12 +195,15 @@ __global__ void __launch_bounds__(256) chem_scatter_kernel(long ncell, int nbins
     if (i < ncell) w.order[w.offs[bin] + lbase[bin] + local] = (int32_t)i;
 }
 
-template <class TT, class MT>
+// One lane integrates one cell at a time. PLAIN: lane l of block b takes cell
+// 64 b + l and stops when it is done (one cell per lane, the drop-in form).
+// Otherwise the block is a persistent wave draining the binned queue.
+template <class TT, class MT, bool PLAIN>
 __global__ void __launch_bounds__(64)
-chem_persistent_kernel(const MechData<TT>* __restrict__ dm, const double* __restrict__ U, long ncell, long ld,
-                       const double* __restrict__ dt, int per_cell, double* __restrict__ out, long ldo,
-                       int8_t* __restrict__ status, unsigned long long* __restrict__ first_failed, ChemParams p,
-                       unsigned long long* __restrict__ counters, Workspace w, int refill) {
+chem_lane_kernel(const MechData<TT>* __restrict__ dm, const double* __restrict__ U, long ncell, long ld,
+                 const double* __restrict__ dt, int per_cell, double* __restrict__ out, long ldo,
+                 int8_t* __restrict__ status, unsigned long long* __restrict__ first_failed, ChemParams p,
+                 unsigned long long* __restrict__ counters, Workspace w, int refill) {
     constexpr int N = TT::NS - 1;
     __shared__ double zlds[(6 * N + 13) * 64];   // per-lane BDF slab: Nordsieck history, tau, l
     const int lane = threadIdx.x;
@@ -225,7 +216,19 @@ chem_persistent_kernel(const MechData<TT>* __restrict__ dm, const double* __rest
     // together once at least `refill` of them wait (or when the whole wave is
     // idle), so the cost of starting a cell (initial rate and step-size
     // estimate) is paid by many lanes in the same trip, not one at a time.
-    bool exhausted = false;
+    bool exhausted = PLAIN;
+    if constexpr (PLAIN) {
+        cell = (long)blockIdx.x * 64 + lane;
+        if (cell < ncell) {
+            const int st = run.start(U, cell, ld, per_cell ? dt[cell] : dt[0], p.maxsteps);
+            if (st == HF_OK) {
+                active = true;
+            } else {
+                status[cell] = (int8_t)st;
+                atomicMin(first_failed, (unsigned long long)cell);
+            }
+        }
+    }
     for (;;) {
         const unsigned long long want = __ballot(!active);
         const int idle = __popcll(want);
@@ -301,23 +304,25 @@ int chem(const ChemLaunch& L) {
             hipLaunchKernelGGL(chem_scatter_kernel, dim3(g256), dim3(256), 0, L.stream, (long)L.ncell, nbins, w);
             StreamTimer timer(L.kernel_ms, L.stream);
             if (L.precision == HF_PREC_MIXED)
-                hipLaunchKernelGGL((chem_persistent_kernel<Topo, float>), dim3((unsigned)(L.num_cus * wpc)), dim3(64),
+                hipLaunchKernelGGL((chem_lane_kernel<Topo, float, false>), dim3((unsigned)(L.num_cus * wpc)), dim3(64),
                                    0, L.stream, dm, L.U, (long)L.ncell, (long)L.ld, L.dt, L.dt_per_cell, L.out,
                                    (long)L.ldo, L.status, L.first_failed, p, L.counters, w, chunk);
             else
-                hipLaunchKernelGGL((chem_persistent_kernel<Topo, double>), dim3((unsigned)(L.num_cus * wpc)), dim3(64),
+                hipLaunchKernelGGL((chem_lane_kernel<Topo, double, false>), dim3((unsigned)(L.num_cus * wpc)), dim3(64),
                                    0, L.stream, dm, L.U, (long)L.ncell, (long)L.ld, L.dt, L.dt_per_cell, L.out,
                                    (long)L.ldo, L.status, L.first_failed, p, L.counters, w, chunk);
             if (hipGetLastError() != hipSuccess) return HF_EDEVICE;
             timer.stop();
             return HF_OK;
         }
+        // plain: FP64 throughout, one cell per lane, cells in index order
+        const Workspace none{};
+        hipLaunchKernelGGL(chem_zero_kernel, dim3(1), dim3(64), 0, L.stream, L.first_failed, L.counters);
         StreamTimer timer(L.kernel_ms, L.stream);
-        const int block = 64;
-        const long grid = (L.ncell + block - 1) / block;
-        hipLaunchKernelGGL(chem_plain_kernel<Topo>, dim3((unsigned)grid), dim3(block), 0, L.stream, dm, L.U,
+        const unsigned grid = (unsigned)((L.ncell + 63) / 64);
+        hipLaunchKernelGGL((chem_lane_kernel<Topo, double, true>), dim3(grid), dim3(64), 0, L.stream, dm, L.U,
                            (long)L.ncell, (long)L.ld, L.dt, L.dt_per_cell, L.out, (long)L.ldo, L.status,
-                           L.first_failed, p, L.counters);
+                           L.first_failed, p, L.counters, none, 64);
         if (hipGetLastError() != hipSuccess) return HF_EDEVICE;
         timer.stop();
         return HF_OK;
