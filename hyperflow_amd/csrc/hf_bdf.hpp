@@ -10,8 +10,13 @@
 // = dt, 5000 max steps). Step/order control, the WRMS error test with
 // w_i = 1/(rtol |y_i| + atol), the Newton convergence test and the exact stop
 // at tout follow CVODE's published algorithm (Hindmarsh et al., ACM TOMS
-// 31(3), 2005). GPU-specific choices: the Newton matrix I - gamma J is rebuilt
-// from an analytic Jacobian at every setup (no saved J), and it is factored
+// 31(3), 2005), including CVODE's Jacobian reuse: J is re-evaluated only on
+// the first step, after 51 steps, or after a Newton failure, and is otherwise
+// reloaded to re-form I - gamma J when gamma moves. GPU-specific choices: J is
+// analytic (not a difference quotient), it and the factored Newton matrix
+// are kept lane-contiguously in global memory (so they occupy registers only
+// while they are formed or applied, not across rate evaluations), the batched
+// form refreshes J for a whole wave at once, and I - gamma J is factored
 // without pivoting (a zero or non-finite pivot is a recoverable failure that
 // reduces the step, where I - gamma J tends to I).
 #pragma once
@@ -37,12 +42,17 @@ struct Bdf {
     // [j][i][lane], so consecutive lanes touch consecutive doubles.
     double* zs;
     double ewt[N], acor[N];
-    MT M[N * N];
     double tq1, tq2, tq3, tq4, tq5;
     double tn, h, hprime, hscale, eta, etamax, gamma, gammap, gamrat, rl1, crate, acnrm, saved_tq5;
     double rtol, atol;
-    int q, qprime, L, qwait, nst, nstlp;
+    int q, qprime, L, qwait, nst, nstlp, nstlj;
     bool force_setup, jfresh;
+    bool sync_jac = false;      // refresh Jacobians wave-wide (batched form)
+    // Saved Jacobian and factored Newton matrix: element k at [k * ZSTRIDE],
+    // the callers point them at this lane's column of a per-wave
+    // [element][64 lanes] tile, so the offsets are compile-time constants.
+    MT* jsave = nullptr;
+    MT* lu = nullptr;
     BdfCounters c;
 
     // ------------------------------------------------------------ helpers
@@ -224,61 +234,89 @@ struct Bdf {
     }
 
     // ------------------------------------------------------------ linear algebra
-    // In-place LU of M without pivoting; the diagonal keeps the reciprocal
-    // pivots. MT is the Newton-matrix precision (double, or float for the
-    // mixed form, where the residuals and the corrections stay double).
-    __device__ __forceinline__ bool lu_factor() {
-        #pragma unroll
-        for (int k = 0; k < N; ++k) {
-            const MT p = M[k * N + k];
-            if (!(fabs(p) > __FLT_MIN__) || !__builtin_isfinite(p)) return false;
-            const MT inv = MT(1) / p;
-            M[k * N + k] = inv;
-            #pragma unroll
-            for (int i = k + 1; i < N; ++i) {
-                const MT li = M[i * N + k] * inv;
-                M[i * N + k] = li;
-                #pragma unroll
-                for (int j = k + 1; j < N; ++j) M[i * N + j] = fma(-li, M[k * N + j], M[i * N + j]);
-            }
-        }
-        return true;
+    // The saved Jacobian and the LU factors live in global memory; `opaque`
+    // hides a pointer's provenance from the optimiser so that every use
+    // reloads from memory instead of keeping 64 matrix values (128 VGPRs in
+    // FP64) live in registers across the rate evaluations around it.
+    template <class P>
+    static __device__ __forceinline__ P* opaque(P* p) {
+        asm volatile("" : "+v"(p));
+        return p;
     }
+
+    // Forward and back substitution with the unit-lower / upper factors in
+    // `lu` (reciprocal pivots on the diagonal). MT is the Newton-matrix
+    // precision (double, or float for the mixed form, where the residuals and
+    // the corrections stay double).
     __device__ __forceinline__ void lu_solve(double* bd) const {
+        const MT* lp = opaque(lu);
         MT b[N];
         #pragma unroll
         for (int i = 0; i < N; ++i) b[i] = (MT)bd[i];
         #pragma unroll
         for (int k = 0; k < N; ++k)
             #pragma unroll
-            for (int i = k + 1; i < N; ++i) b[i] = fma(-M[i * N + k], b[k], b[i]);
+            for (int i = k + 1; i < N; ++i) b[i] = fma(-lp[(i * N + k) * ZSTRIDE], b[k], b[i]);
         #pragma unroll
         for (int i = N - 1; i >= 0; --i) {
             MT s = b[i];
             #pragma unroll
-            for (int j = i + 1; j < N; ++j) s = fma(-M[i * N + j], b[j], s);
-            b[i] = s * M[i * N + i];
+            for (int j = i + 1; j < N; ++j) s = fma(-lp[(i * N + j) * ZSTRIDE], b[j], s);
+            b[i] = s * lp[(i * N + i) * ZSTRIDE];
         }
         #pragma unroll
         for (int i = 0; i < N; ++i) bd[i] = (double)b[i];
     }
 
-    __device__ __forceinline__ bool lsetup(Sys& sys, const double* y) {
-        double f[N];
-        ++c.jac;
+    // Newton-matrix setup. A fresh Jacobian (jbad) is evaluated and saved in
+    // the lane's global slot (jsave, lane-contiguous); otherwise the saved one
+    // is reused, as CVODE reuses J across steps and only re-forms I - gamma J
+    // when gamma changes. I - gamma J is factored row by row (Doolittle, no
+    // pivoting) straight into the lane's `lu` slot: row i is formed from the
+    // saved J, eliminated against the finished rows 0..i-1 read back from
+    // `lu`, and stored, so only two rows are live in registers at a time.
+    __device__ __forceinline__ bool lsetup(Sys& sys, const double* y, bool jbad) {
         ++c.lu;
-        if (!sys.jac(y, M, f)) return false;
-        const MT g = (MT)gamma;
-        #pragma unroll
-        for (int i = 0; i < N * N; ++i) M[i] *= -g;
-        #pragma unroll
-        for (int i = 0; i < N; ++i) M[i * N + i] += MT(1);
-        jfresh = true;
+        if (jbad) {
+            MT J[N * N];
+            double f[N];
+            ++c.jac;
+            if (!sys.jac(y, J, f)) return false;
+            #pragma unroll
+            for (int i = 0; i < N * N; ++i) jsave[i * ZSTRIDE] = J[i];
+            nstlj = nst;
+            jfresh = true;
+        } else {
+            jfresh = false;
+        }
         gammap = gamma;
         gamrat = 1;
         crate = 1;
         nstlp = nst;
-        return lu_factor();
+        const MT g = (MT)gamma;
+        const MT* js = opaque(jsave);
+        bool ok = true;
+        #pragma unroll
+        for (int i = 0; i < N; ++i) {
+            MT row[N];
+            #pragma unroll
+            for (int j = 0; j < N; ++j) row[j] = (i == j ? MT(1) : MT(0)) - g * js[(i * N + j) * ZSTRIDE];
+            const MT* up = opaque(lu);
+            #pragma unroll
+            for (int k = 0; k < i; ++k) {
+                const MT l = row[k] * up[(k * N + k) * ZSTRIDE];
+                row[k] = l;
+                #pragma unroll
+                for (int j = k + 1; j < N; ++j) row[j] = fma(-l, up[(k * N + j) * ZSTRIDE], row[j]);
+            }
+            const MT p = row[i];
+            ok = ok && fabs(p) > __FLT_MIN__ && __builtin_isfinite(p);
+            row[i] = MT(1) / p;
+            MT* wp = opaque(lu);
+            #pragma unroll
+            for (int j = 0; j < N; ++j) wp[(i * N + j) * ZSTRIDE] = row[j];
+        }
+        return ok;
     }
 
     // 0 converged, 1 retry with a fresh matrix, 2 reduce the step, 3 rhs/setup failure
@@ -288,10 +326,19 @@ struct Bdf {
         for (int i = 0; i < N; ++i) y[i] = Z(0, i);
         ++c.rhs;
         if (!sys.rhs(y, f)) return 3;
-        const bool callsetup = nst == 0 || nst >= nstlp + 20 || fabs(gamrat - 1.0) > 0.3 || convfail != 0 || force_setup;
+        bool callsetup = nst == 0 || nst >= nstlp + 20 || fabs(gamrat - 1.0) > 0.3 || convfail != 0 || force_setup;
+        bool jbad = callsetup && (nst == 0 || nst >= nstlj + 51 || convfail == 2 ||
+                                  (convfail == 1 && fabs(gamma / gammap - 1.0) < 0.2));
+        // Wave-synchronised refresh: when any lane of the wave needs a new
+        // Jacobian, every lane takes one, so the expensive evaluation happens
+        // for the whole wave at once instead of in a different trip per lane.
+        if (sync_jac && __any(jbad)) {
+            jbad = true;
+            callsetup = true;
+        }
         force_setup = false;
         jfresh = false;
-        if (callsetup && !lsetup(sys, y)) return 3;
+        if (callsetup && !lsetup(sys, y, jbad)) return 3;
         #pragma unroll
         for (int i = 0; i < N; ++i) acor[i] = 0;
         double delp = 0;
@@ -550,7 +597,7 @@ struct Bdf {
         #pragma unroll
         for (int i = 0; i < QMAX + 2; ++i) TAU(i) = 0;
         tq1 = tq2 = tq3 = tq4 = tq5 = 0;
-        nst = nstlp = 0;
+        nst = nstlp = nstlj = 0;
         gammap = 0;
         gamrat = 1;
         crate = 1;

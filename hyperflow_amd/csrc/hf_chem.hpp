@@ -78,11 +78,17 @@ struct CellRun {
     Bdf<N, ChemSys<T>, MT> bdf;
     double y[N];
 
-    __device__ __forceinline__ CellRun(const MechData<T>& d, double* zs, const ChemParams& p)
+    // jsave/lu: this lane's saved-Jacobian and LU-factor slots (element k at
+    // [k * 64], a per-wave [element][lane] tile); sync_jac: wave-wide refresh.
+    __device__ __forceinline__ CellRun(const MechData<T>& d, double* zs, const ChemParams& p, MT* jsave, MT* lu,
+                                       bool sync_jac)
         : sys{d, 1.0, 0.0, 1000.0, nullptr} {
         bdf.zs = zs;
         bdf.rtol = p.rtol;
         bdf.atol = p.atol;
+        bdf.jsave = jsave;
+        bdf.lu = lu;
+        bdf.sync_jac = sync_jac;
     }
 
     // HF_OK when the integration is running, else the cell's error code.
@@ -132,24 +138,6 @@ struct CellRun {
         return HF_OK;
     }
 };
-
-// One cell start to end: Chemistry::advance (chemistry.cpp:55-76) on a lane.
-template <class T>
-__device__ __forceinline__ int advance_cell(const MechData<T>& d, const double* __restrict__ U, long i, long ld,
-                                            double dt, double* __restrict__ out, long ldo, const ChemParams& p,
-                                            BdfCounters& c, double* zs) {
-    CellRun<T> run(d, zs, p);
-    int st = run.start(U, i, ld, dt, p.maxsteps);
-    if (st == HF_OK) {
-        int r;
-        do {
-            r = run.step();
-        } while (r == BDF_RUNNING);
-        st = r == BDF_OK ? run.finish(U, i, ld, out, ldo) : HF_ECONV;
-    }
-    c = run.bdf.c;
-    return st;
-}
 
 // counters: [steps, rhs, jac, lu, newton, conv fails, err fails, max steps of a cell]
 __device__ __forceinline__ void reduce_counters(const BdfCounters& c, int max_cell_steps,

@@ -4,6 +4,7 @@
 // uploaded at run time.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -79,11 +80,21 @@ struct Workspace {
     unsigned* offs;
     unsigned* cursor;
     unsigned* head;
+    void* jsave;          // saved Jacobians: one [element][64 lanes] tile per wave
+    void* lu;             // factored Newton matrices, same layout
 };
 
 inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
-Workspace carve(void* base, int64_t ncell) {
+// Lanes that hold a Jacobian slot: every cell's lane (rounded up to whole
+// waves) for the plain form, every resident lane of the persistent grid for
+// the batched one; the larger of the two, so one workspace serves both.
+int64_t slot_lanes(int64_t ncell, int num_cus, int waves_per_cu) {
+    return std::max<int64_t>((ncell + 63) / 64 * 64, (int64_t)num_cus * (waves_per_cu > 0 ? waves_per_cu : 4) * 64);
+}
+
+Workspace carve(void* base, int64_t ncell, int64_t lanes) {
+    constexpr int N = Topo::NS - 1;
     char* p = static_cast<char*>(base);
     Workspace w;
     w.bins = reinterpret_cast<uint8_t*>(p);
@@ -97,11 +108,19 @@ Workspace carve(void* base, int64_t ncell) {
     w.cursor = reinterpret_cast<unsigned*>(p);
     p += align256(sizeof(unsigned) * kMaxBins);
     w.head = reinterpret_cast<unsigned*>(p);
+    p += 256;
+    w.jsave = p;
+    p += align256(sizeof(double) * N * N * lanes);
+    w.lu = p;
     return w;
 }
 
-size_t chem_workspace(int64_t ncell, int, int, int) {
-    return align256(ncell) + align256(sizeof(int32_t) * ncell) + 3 * align256(sizeof(unsigned) * kMaxBins) + 256;
+// Queue arrays plus a saved Jacobian and an LU-factor slot (FP64-sized) per lane.
+size_t chem_workspace(int64_t ncell, int, int num_cus, int waves_per_cu) {
+    constexpr int N = Topo::NS - 1;
+    const int64_t lanes = slot_lanes(ncell, num_cus, waves_per_cu);
+    return align256(ncell) + align256(sizeof(int32_t) * ncell) + 3 * align256(sizeof(unsigned) * kMaxBins) + 256 +
+           2 * align256(sizeof(double) * N * N * lanes);
 }
 
 __global__ void chem_init_kernel(Workspace w, unsigned long long* first_failed, unsigned long long* counters) {
@@ -207,7 +226,9 @@ chem_lane_kernel(const MechData<TT>* __restrict__ dm, const double* __restrict__
     constexpr int N = TT::NS - 1;
     __shared__ double zlds[(6 * N + 13) * 64];   // per-lane BDF slab: Nordsieck history, tau, l
     const int lane = threadIdx.x;
-    CellRun<TT, MT> run(*dm, zlds + lane, p);
+    const long tile = (long)blockIdx.x * N * N * 64;
+    CellRun<TT, MT> run(*dm, zlds + lane, p, static_cast<MT*>(w.jsave) + tile + lane,
+                        static_cast<MT*>(w.lu) + tile + lane, !PLAIN);
     BdfCounters tot;
     int max_steps_cell = 0;
     long cell = -1;
@@ -289,9 +310,11 @@ int chem(const ChemLaunch& L) {
     } else {
         const ChemParams p{L.rtol, L.atol, L.maxsteps};
         const MD* dm = static_cast<const MD*>(L.mech);
+        const int64_t lanes = slot_lanes(L.ncell, L.num_cus, L.waves_per_cu);
+        if (!L.workspace || L.workspace_bytes < chem_workspace(L.ncell, L.nbins, L.num_cus, L.waves_per_cu))
+            return HF_EINVAL;
         if (L.mode == HF_CHEM_BATCHED) {
-            if (!L.workspace) return HF_EINVAL;
-            const Workspace w = carve(L.workspace, L.ncell);
+            const Workspace w = carve(L.workspace, L.ncell, lanes);
             const int nbins = L.nbins > 0 ? min(L.nbins, kMaxBins) : 32;
             const int wpc = L.waves_per_cu > 0 ? L.waves_per_cu : 4;
             // queue chunking: idle lanes refilled together
@@ -316,7 +339,7 @@ int chem(const ChemLaunch& L) {
             return HF_OK;
         }
         // plain: FP64 throughout, one cell per lane, cells in index order
-        const Workspace none{};
+        const Workspace none = carve(L.workspace, L.ncell, lanes);
         hipLaunchKernelGGL(chem_zero_kernel, dim3(1), dim3(64), 0, L.stream, L.first_failed, L.counters);
         StreamTimer timer(L.kernel_ms, L.stream);
         const unsigned grid = (unsigned)((L.ncell + 63) / 64);
