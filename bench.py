@@ -130,6 +130,8 @@ def main():
     ap.add_argument("--cells", type=int, default=1 << 20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and parity legs")
     ap.add_argument("--skip-plain", action="store_true")
+    ap.add_argument("--copy-ref-mib", type=int, default=0,
+                    help="also run a device-to-device copy of this many MiB (reference for HBM counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build has no multi-GPU path); --gpus {args.gpus} "
@@ -148,6 +150,15 @@ def main():
     Ud = Ud0.clone()
     dtd = torch.from_numpy(dt).to("cuda")
     ns, reactions = F.topology_summary(GEN / "h2_li.inc")
+
+    if args.copy_ref_mib:
+        a = torch.empty(args.copy_ref_mib << 17, dtype=torch.float64, device="cuda").fill_(1.0)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"copy reference: {args.copy_ref_mib} MiB device-to-device, 3 times")
+        del a, b
 
     res = {}
     mech_mixed = hf.Mechanism(hf.H2_LI, precision=hf.HF_PREC_MIXED)

@@ -9,6 +9,7 @@ from pathlib import Path
 
 from ._lib import (HF_CHEM_BATCHED, HF_CHEM_PLAIN, HF_PREC_FP64, HF_PREC_MIXED, HyperflowError, LIB_PATH,
                    load)
+from .adjoint import ChemSensitivity, fixed_step_chemistry
 from .chemistry import ChemistryBatch, Mechanism
 from .physics import FluxSweep
 
@@ -16,5 +17,5 @@ MECHANISM_DIR = Path(__file__).resolve().parent / "mechanisms"
 H2_LI = MECHANISM_DIR / "H2_Li.yaml"
 DME_SK39 = MECHANISM_DIR / "dme_sk39.yaml"
 
-__all__ = ["ChemistryBatch", "Mechanism", "FluxSweep", "HyperflowError", "load", "LIB_PATH", "H2_LI", "DME_SK39",
-           "HF_CHEM_PLAIN", "HF_CHEM_BATCHED", "HF_PREC_FP64", "HF_PREC_MIXED"]
+__all__ = ["ChemistryBatch", "Mechanism", "ChemSensitivity", "fixed_step_chemistry", "FluxSweep", "HyperflowError",
+           "load", "LIB_PATH", "H2_LI", "DME_SK39", "HF_CHEM_PLAIN", "HF_CHEM_BATCHED", "HF_PREC_FP64", "HF_PREC_MIXED"]

@@ -87,6 +87,14 @@ def load():
         if not LIB_PATH.exists():
             raise RuntimeError(f"{LIB_PATH} is not built: run `python -m hyperflow_amd.build` "
                                "(there is no CPU fallback)")
+        # torch's ROCm wheel ships its own HIP runtime under the same soname
+        # (libamdhip64.so.7). Loading torch first makes this library bind to
+        # that runtime, so torch tensors and these kernels share one runtime
+        # (loaded the other way round, torch finds no device).
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(str(LIB_PATH))
         for name, (res, args) in _SIGNATURES.items():
             f = getattr(L, name)

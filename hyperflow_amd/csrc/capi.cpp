@@ -509,16 +509,131 @@ int hf_stable_dt(hf_ctx* c, const double* U, int64_t ncell, int64_t ld, double d
     return HF_OK;
 }
 
-int hf_chem_adjoint(hf_ctx* c, int64_t, const double*, const double*, const double*, double, int, int, const double*,
-                    double*, double*, double*, double*) {
+namespace {
+
+// Host or device arrays of the sensitivity entry points: device pointers are
+// used in place, host arrays are staged through one device buffer (inputs
+// copied in, outputs copied back after the kernels).
+struct Staging {
+    struct Item {
+        const void* host;
+        void* hout;
+        size_t off, bytes;
+    };
+    std::vector<Item> in, out;
+    size_t total = 0;
+    static size_t pad(size_t b) { return (b + 255) & ~size_t(255); }
+    // returns the offset (bytes) of a staged array, or SIZE_MAX when used in place
+    size_t add_in(const double* p, size_t n) {
+        if (!p || is_device_ptr(p)) return SIZE_MAX;
+        in.push_back({p, nullptr, total, n * sizeof(double)});
+        total += pad(n * sizeof(double));
+        return in.back().off;
+    }
+    size_t add_out(double* p, size_t n) {
+        if (!p || is_device_ptr(p)) return SIZE_MAX;
+        out.push_back({nullptr, p, total, n * sizeof(double)});
+        total += pad(n * sizeof(double));
+        return out.back().off;
+    }
+};
+
+int sens_common(hf_ctx* c, bool tangent, int64_t ncell, const double* rho, const double* T0, const double* Y0,
+                double h, int nsteps, int ckpt_every, const double* lnA_shift, const double* v_lnA,
+                const double* v_T0, const double* v_Y0, double* J, double* grad_lnA, double* grad_T0,
+                double* grad_Y0, double* dJ) {
     if (!c) return HF_EINVAL;
-    return fail(c, HF_EINVAL, "adjoint not built");
+    BusyGuard guard(c);
+    if (!guard.ok) return fail(c, HF_EBUSY, "concurrent call is unsupported");
+    if (ncell < 0) return fail(c, HF_EINVAL, "invalid extent");
+    if (!(h > 0) || !std::isfinite(h) || nsteps < 1) return fail(c, HF_EINVAL, "invalid step size or step count");
+    if (!tangent && ckpt_every < 1) return fail(c, HF_EINVAL, "invalid checkpoint interval");
+    if (ncell == 0) return HF_OK;
+    if (!rho || !T0 || !Y0 || !J) return fail(c, HF_EINVAL, "missing state or objective array");
+    if (tangent ? (!v_lnA || !v_T0 || !v_Y0 || !dJ) : (!grad_lnA || !grad_T0 || !grad_Y0))
+        return fail(c, HF_EINVAL, "missing direction or gradient array");
+    HF_DEVICE(c);
+    const size_t n = (size_t)ncell, ns = (size_t)c->ops->ns, nr = (size_t)c->ops->nr;
+    Staging st;
+    const size_t o_rho = st.add_in(rho, n), o_T0 = st.add_in(T0, n), o_Y0 = st.add_in(Y0, n * ns);
+    const size_t o_sh = st.add_in(lnA_shift, nr);
+    size_t o_vl = SIZE_MAX, o_vt = SIZE_MAX, o_vy = SIZE_MAX, o_gl = SIZE_MAX, o_gt = SIZE_MAX, o_gy = SIZE_MAX,
+           o_dj = SIZE_MAX;
+    const size_t o_J = st.add_out(J, n);
+    if (tangent) {
+        o_vl = st.add_in(v_lnA, n * nr);
+        o_vt = st.add_in(v_T0, n);
+        o_vy = st.add_in(v_Y0, n * ns);
+        o_dj = st.add_out(dJ, n);
+    } else {
+        o_gl = st.add_out(grad_lnA, n * nr);
+        o_gt = st.add_out(grad_T0, n);
+        o_gy = st.add_out(grad_Y0, n * ns);
+    }
+    const size_t wbytes = tangent ? 0 : c->ops->sens_workspace(ncell, nsteps, ckpt_every);
+    if (!c->aux[0].ensure(st.total + 256)) return fail(c, HF_ENOMEM, "device staging buffer");
+    if (wbytes && !c->work.ensure(wbytes)) return fail(c, HF_ENOMEM, "device checkpoint workspace");
+    if (!c->scal.ensure(256)) return fail(c, HF_ENOMEM, "device scalars");
+    char* base = c->aux[0].as<char>();
+    auto dev = [&](const double* p, size_t off) { return off == SIZE_MAX ? p : reinterpret_cast<const double*>(base + off); };
+    auto devo = [&](double* p, size_t off) { return off == SIZE_MAX ? p : reinterpret_cast<double*>(base + off); };
+    for (const auto& it : st.in)
+        HF_HIP(c, hipMemcpyAsync(base + it.off, it.host, it.bytes, hipMemcpyHostToDevice, c->stream));
+    AdjointLaunch L{};
+    L.mech = c->d_mech;
+    L.ncell = ncell;
+    L.rho = dev(rho, o_rho);
+    L.T0 = dev(T0, o_T0);
+    L.Y0 = dev(Y0, o_Y0);
+    L.h = h;
+    L.nsteps = nsteps;
+    L.ckpt_every = ckpt_every;
+    L.lnA_shift = dev(lnA_shift, o_sh);
+    L.J = devo(J, o_J);
+    if (tangent) {
+        L.v_lnA = dev(v_lnA, o_vl);
+        L.v_T0 = dev(v_T0, o_vt);
+        L.v_Y0 = dev(v_Y0, o_vy);
+        L.dJ = devo(dJ, o_dj);
+    } else {
+        L.grad_lnA = devo(grad_lnA, o_gl);
+        L.grad_T0 = devo(grad_T0, o_gt);
+        L.grad_Y0 = devo(grad_Y0, o_gy);
+        L.workspace = c->work.as<double>();
+        L.workspace_bytes = wbytes;
+    }
+    L.first_failed = c->scal.as<unsigned long long>();
+    L.stream = c->stream;
+    const int r = tangent ? c->ops->tangent(L) : c->ops->adjoint(L);
+    if (r != HF_OK) {
+        (void)hipStreamSynchronize(c->stream);
+        return fail(c, r, r == HF_EDEVICE ? "sensitivity kernel launch failed" : "invalid sensitivity arguments");
+    }
+    for (const auto& it : st.out)
+        HF_HIP(c, hipMemcpyAsync(it.hout, base + it.off, it.bytes, hipMemcpyDeviceToHost, c->stream));
+    unsigned long long ff = ~0ull;
+    HF_HIP(c, hipMemcpyAsync(&ff, L.first_failed, sizeof(ff), hipMemcpyDeviceToHost, c->stream));
+    HF_HIP(c, hipStreamSynchronize(c->stream));
+    if (ff != ~0ull)
+        return fail(c, HF_ECONV, "sensitivity cell " + std::to_string(ff) +
+                                     ": backward-Euler Newton iteration failed (results of that cell are NaN)");
+    return HF_OK;
 }
 
-int hf_chem_tangent(hf_ctx* c, int64_t, const double*, const double*, const double*, double, int, const double*,
-                    const double*, const double*, const double*, double*, double*) {
-    if (!c) return HF_EINVAL;
-    return fail(c, HF_EINVAL, "tangent not built");
+}  // namespace
+
+int hf_chem_adjoint(hf_ctx* c, int64_t ncell, const double* rho, const double* T0, const double* Y0, double h,
+                    int nsteps, int ckpt_every, const double* lnA_shift, double* J, double* grad_lnA,
+                    double* grad_T0, double* grad_Y0) {
+    return sens_common(c, false, ncell, rho, T0, Y0, h, nsteps, ckpt_every, lnA_shift, nullptr, nullptr, nullptr, J,
+                       grad_lnA, grad_T0, grad_Y0, nullptr);
+}
+
+int hf_chem_tangent(hf_ctx* c, int64_t ncell, const double* rho, const double* T0, const double* Y0, double h,
+                    int nsteps, const double* lnA_shift, const double* v_lnA, const double* v_T0,
+                    const double* v_Y0, double* J, double* dJ) {
+    return sens_common(c, true, ncell, rho, T0, Y0, h, nsteps, 1, lnA_shift, v_lnA, v_T0, v_Y0, J, nullptr, nullptr,
+                       nullptr, dJ);
 }
 
 int hf_synth_cells(hf_ctx* c, int which, uint64_t seed, int64_t ncell, double* U, double* dt) {

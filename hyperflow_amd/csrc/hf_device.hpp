@@ -415,10 +415,13 @@ __device__ __forceinline__ void production_rates(const MechData<T>& d, double t,
 //             + W_k/rho * dwdot_k/dT * dT/dy_j,  dT/dy_j = -(e_j - e_last)/cv.
 // Writes J (row-major N x N, N = NS-1) and f. Rate constants, their
 // derivatives and f are FP64; J is accumulated in MT (double, or float for
-// the mixed-precision Newton matrix).
+// the mixed-precision Newton matrix). Optional outputs for sensitivities:
+// fT[k] = W_k/rho dwdot_k/dT at fixed concentrations, qout[i] = net rate of
+// progress of reaction i.
 template <class T, class MT>
 __device__ __forceinline__ void chem_jacobian(const MechData<T>& d, double t, double rho, const double* Y,
-                                              MT* J, double* f, const double* lnA_shift = nullptr) {
+                                              MT* J, double* f, const double* lnA_shift = nullptr,
+                                              double* fT = nullptr, double* qout = nullptr) {
     constexpr int NS = T::NS, N = NS - 1, L = NS - 1;
     const double logt = log(t), it = 1.0 / t;
     const double logc0 = log(kP0 / (kR * t));
@@ -452,6 +455,7 @@ __device__ __forceinline__ void chem_jacobian(const MechData<T>& d, double t, do
             if constexpr (p >= 0) pb *= C[p];
         });
         const double q = k.kf * pf - kr * pb;
+        if (qout) qout[I] = q;
         const double dqdT = k.kf * k.dlnkf_dT * pf - kr * (k.dlnkf_dT + k.dlnrkc_dT) * pb;
         // d q / d C_s for the species in the reaction's concentration products
         // (the kf/kr factors that carry [M] are handled separately below).
@@ -525,6 +529,7 @@ __device__ __forceinline__ void chem_jacobian(const MechData<T>& d, double t, do
         f[k] = wd[k] * Wk * rrho;
         const MT cl = (MT)(dwdCL[k] * d.invW[L]);
         const MT tk = (MT)(Wk * rrho * dwdT[k]);
+        if (fT) fT[k] = Wk * rrho * dwdT[k];
         const MT wk = (MT)Wk;
         #pragma unroll
         for (int j = 0; j < N; ++j) {

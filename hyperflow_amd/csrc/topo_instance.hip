@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstring>
 
+#include "hf_adjoint.hpp"
 #include "hf_chem.hpp"
 #include "hf_flux.hpp"
 #include "topo_ops.hpp"
@@ -63,7 +64,7 @@ void fill(const Mechanism& m, void* host) {
 __global__ void chem_zero_kernel(unsigned long long* first_failed, unsigned long long* counters) {
     const int t = threadIdx.x;
     if (t == 0) *first_failed = ~0ull;
-    if (t < 8) counters[t] = 0;
+    if (counters && t < 8) counters[t] = 0;
 }
 
 // ------------------------------------------------------------------ chemistry (batched)
@@ -424,8 +425,257 @@ int props(const PropsLaunch& L) {
     return hipGetLastError() == hipSuccess ? HF_OK : HF_EDEVICE;
 }
 
-int adjoint(const AdjointLaunch&) { return HF_EINVAL; }
-int tangent(const AdjointLaunch&) { return HF_EINVAL; }
+// ------------------------------------------------------------------ sensitivities
+// Fixed-step backward-Euler chemistry with its discrete adjoint (reverse
+// sweep over checkpointed segments) and its forward tangent, one lane per
+// cell. Checkpoint and trajectory slots are [slot][element][cell]: element j
+// (y_0..y_{N-1}, then T) of slot m for cell c at [(m (N+1) + j) ncell + c].
+struct SensArgs {
+    long ncell;
+    const double *rho, *T0, *Y0;
+    double h;
+    int nsteps, ckpt;
+    const double* shift;
+    double *J, *g_lnA, *g_T0, *g_Y0;
+    const double *v_lnA, *v_T0, *v_Y0;
+    double* dJ;
+    double *ck, *traj;
+    unsigned long long* first_failed;
+};
+
+// e, cv and the species energies e_k of the initial state (T0, Y0), Y0 as given.
+template <class TT>
+__device__ __forceinline__ void initial_energy(const MechData<TT>& d, double T0, const double* Y0, double& e,
+                                               double& cv, double* ek) {
+    e = cv = 0;
+    #pragma unroll
+    for (int k = 0; k < TT::NS; ++k) {
+        const Nasa7 na = nasa_select(d, k, T0);
+        const double* a = na.a;
+        const double cpr = fma(T0, fma(T0, fma(T0, fma(T0, a[4], a[3]), a[2]), a[1]), a[0]);
+        const double hrt = fma(T0, fma(T0, fma(T0, fma(T0, a[4] * 0.2, a[3] * 0.25), a[2] * (1.0 / 3.0)), a[1] * 0.5),
+                               a[0]) + a[5] / T0;
+        ek[k] = (hrt - 1.0) * kR * T0 * d.invW[k];
+        e = fma(Y0[k], ek[k], e);
+        cv = fma(Y0[k], (cpr - 1.0) * kR * d.invW[k], cv);
+    }
+}
+
+template <class TT, bool TANGENT>
+__global__ void __launch_bounds__(64) sens_kernel(const MechData<TT>* __restrict__ dm, SensArgs a) {
+    constexpr int NS = TT::NS, N = NS - 1, NR = TT::NR;
+    const long c = (long)blockIdx.x * 64 + threadIdx.x;
+    if (c >= a.ncell) return;
+    const MechData<TT>& d = *dm;
+    const double rho = a.rho[c], h = a.h;
+    double Y0[NS], ek0[NS], e, cv0;
+    #pragma unroll
+    for (int k = 0; k < NS; ++k) Y0[k] = a.Y0[c * NS + k];
+    initial_energy(d, a.T0[c], Y0, e, cv0, ek0);
+    double y[N], t = a.T0[c];
+    #pragma unroll
+    for (int k = 0; k < N; ++k) y[k] = Y0[k];
+    BeState<TT> s;
+    const size_t nc = (size_t)a.ncell;
+    auto ck = [&](int m, int j) HF_AI -> double& { return a.ck[((size_t)m * (N + 1) + j) * nc + c]; };
+    auto tr = [&](int m, int j) HF_AI -> double& { return a.traj[((size_t)m * (N + 1) + j) * nc + c]; };
+    // tangent state
+    double dy[N], de0 = 0, dJ = 0;
+    if constexpr (TANGENT) {
+        de0 = cv0 * a.v_T0[c];
+        #pragma unroll
+        for (int k = 0; k < NS; ++k) de0 = fma(ek0[k], a.v_Y0[c * NS + k], de0);
+        #pragma unroll
+        for (int k = 0; k < N; ++k) dy[k] = a.v_Y0[c * NS + k];
+    }
+    double Jv = 0;
+    bool ok = true;
+    for (int n = 0; n < a.nsteps && ok; ++n) {
+        if constexpr (!TANGENT) {
+            if (n % a.ckpt == 0) {
+                #pragma unroll
+                for (int j = 0; j < N; ++j) ck(n / a.ckpt, j) = y[j];
+                ck(n / a.ckpt, N) = t;
+            }
+        }
+        double yold[N];
+        #pragma unroll
+        for (int k = 0; k < N; ++k) yold[k] = y[k];
+        ok = be_step(d, rho, e, a.shift, h, yold, y, t, s);
+        if (!ok) break;
+        Jv += h * be_qdot(s, rho);
+        if constexpr (TANGENT) {
+            // (I - hJ) dy' = dy + h (df/dtheta v_theta + fT/cv de0)
+            double rq[N];
+            #pragma unroll
+            for (int k = 0; k < N; ++k) rq[k] = 0;
+            double dq_theta = 0;   // dqdot/dtheta . v_theta
+            static_for<0, NR>([&](auto ic) HF_AI {
+                constexpr int I = decltype(ic)::value;
+                const double qv = s.q[I] * a.v_lnA[(size_t)c * NR + I];
+                static_for<0, TT::NRMAX>([&](auto jc) HF_AI {
+                    constexpr int r = TT::reac[I][decltype(jc)::value], p = TT::prod[I][decltype(jc)::value];
+                    if constexpr (r >= 0 && r < N) rq[r] -= qv;
+                    if constexpr (p >= 0 && p < N) rq[p] += qv;
+                });
+                dq_theta -= qv * nu_w_dot<TT, I>(d, s.de);
+            });
+            double b[N];
+            #pragma unroll
+            for (int k = 0; k < N; ++k) b[k] = dy[k] + h * (d.W[k] / rho * rq[k] + s.fT[k] / s.cv * de0);
+            double dqdy[N], dqde;
+            be_dqdot(s, rho, dqdy, dqde);
+            double M[N * N];
+            be_matrix(s, h, M);
+            ok = be_factor<N>(M);
+            be_solve<N>(M, b);
+            double acc = dqde * de0 + dq_theta;
+            #pragma unroll
+            for (int k = 0; k < N; ++k) {
+                dy[k] = b[k];
+                acc = fma(dqdy[k], b[k], acc);
+            }
+            dJ += h * acc;
+        }
+    }
+    if (ok) {
+        Jv += s.t;
+        if constexpr (TANGENT) {
+            double acc = de0 / s.cv;
+            #pragma unroll
+            for (int k = 0; k < N; ++k) acc -= s.de[k] / s.cv * dy[k];
+            dJ += acc;
+        }
+    }
+    if constexpr (TANGENT) {
+        a.J[c] = ok ? Jv : NAN;
+        a.dJ[c] = ok ? dJ : NAN;
+        if (!ok) atomicMin(a.first_failed, (unsigned long long)c);
+        return;
+    } else {
+        double lam[N], gth[NR], ge = 0;
+        #pragma unroll
+        for (int i = 0; i < NR; ++i) gth[i] = 0;
+        #pragma unroll
+        for (int k = 0; k < N; ++k) lam[k] = 0;
+        const int K = a.ckpt, nseg = (a.nsteps + K - 1) / K;
+        bool first = true;
+        for (int g = nseg - 1; g >= 0 && ok; --g) {
+            const int n0 = g * K, n1 = min(a.nsteps, n0 + K);
+            double yy[N], tt = ck(g, N);
+            #pragma unroll
+            for (int j = 0; j < N; ++j) yy[j] = ck(g, j);
+            for (int n = n0; n < n1 && ok; ++n) {
+                double yold[N];
+                #pragma unroll
+                for (int k = 0; k < N; ++k) yold[k] = yy[k];
+                ok = be_step(d, rho, e, a.shift, h, yold, yy, tt, s);
+                #pragma unroll
+                for (int j = 0; j < N; ++j) tr(n - n0, j) = yy[j];
+                tr(n - n0, N) = tt;
+            }
+            for (int n = n1; n > n0 && ok; --n) {
+                double yn[N], tn = tr(n - n0 - 1, N);
+                #pragma unroll
+                for (int j = 0; j < N; ++j) yn[j] = tr(n - n0 - 1, j);
+                ok = be_eval(d, rho, e, a.shift, yn, tn, s);
+                if (!ok) break;
+                if (first) {
+                    // terminal term T(y_S): dT/dy_j = -(e_j - e_N)/cv, dT/de = 1/cv
+                    #pragma unroll
+                    for (int j = 0; j < N; ++j) lam[j] = -s.de[j] / s.cv;
+                    ge = 1.0 / s.cv;
+                    first = false;
+                }
+                double dqdy[N], dqde, mu[N];
+                be_dqdot(s, rho, dqdy, dqde);
+                #pragma unroll
+                for (int j = 0; j < N; ++j) mu[j] = fma(h, dqdy[j], lam[j]);
+                double muft = 0, wv[N];
+                {
+                    double M[N * N];
+                    be_matrix(s, h, M);
+                    ok = be_factor<N>(M);
+                    be_solve_t<N>(M, mu);
+                }
+                #pragma unroll
+                for (int k = 0; k < N; ++k) {
+                    wv[k] = mu[k] / rho - s.de[k];
+                    muft = fma(mu[k], s.fT[k], muft);
+                    lam[k] = mu[k];
+                }
+                static_for<0, NR>([&](auto ic) HF_AI {
+                    constexpr int I = decltype(ic)::value;
+                    gth[I] = fma(h * s.q[I], nu_w_dot<TT, I>(d, wv), gth[I]);
+                });
+                ge += h * (muft / s.cv + dqde);
+            }
+        }
+        a.J[c] = ok ? Jv : NAN;
+        #pragma unroll
+        for (int i = 0; i < NR; ++i) a.g_lnA[(size_t)c * NR + i] = ok ? gth[i] : NAN;
+        a.g_T0[c] = ok ? ge * cv0 : NAN;
+        #pragma unroll
+        for (int k = 0; k < NS; ++k) a.g_Y0[(size_t)c * NS + k] = ok ? (k < N ? lam[k] : 0.0) + ge * ek0[k] : NAN;
+        if (!ok) atomicMin(a.first_failed, (unsigned long long)c);
+    }
+}
+
+// Workspace of the adjoint: checkpoints plus one segment of trajectory.
+size_t sens_workspace(int64_t ncell, int nsteps, int ckpt) {
+    constexpr int N = Topo::NS - 1;
+    const int64_t nseg = (nsteps + ckpt - 1) / ckpt;
+    return sizeof(double) * (size_t)(N + 1) * (size_t)ncell * (size_t)(nseg + ckpt);
+}
+
+// Lane-resident sensitivities (the N x N Jacobian lives in VGPRs): compiled
+// for small mechanisms only.
+constexpr bool kSens = Topo::NS <= 16;
+
+int sens(const AdjointLaunch& L, bool tangent) {
+    if (L.ncell == 0) return HF_OK;
+    if constexpr (!kSens) {
+        return HF_EINVAL;
+    } else {
+    if (!(L.h > 0) || L.nsteps < 1 || (!tangent && L.ckpt_every < 1)) return HF_EINVAL;
+    constexpr int N = Topo::NS - 1;
+    SensArgs a{};
+    a.ncell = (long)L.ncell;
+    a.rho = L.rho;
+    a.T0 = L.T0;
+    a.Y0 = L.Y0;
+    a.h = L.h;
+    a.nsteps = L.nsteps;
+    a.ckpt = tangent ? 1 : L.ckpt_every;
+    a.shift = L.lnA_shift;
+    a.J = L.J;
+    a.g_lnA = L.grad_lnA;
+    a.g_T0 = L.grad_T0;
+    a.g_Y0 = L.grad_Y0;
+    a.v_lnA = L.v_lnA;
+    a.v_T0 = L.v_T0;
+    a.v_Y0 = L.v_Y0;
+    a.dJ = L.dJ;
+    a.first_failed = L.first_failed;
+    if (!tangent) {
+        if (!L.workspace || L.workspace_bytes < sens_workspace(L.ncell, L.nsteps, L.ckpt_every)) return HF_EINVAL;
+        const int64_t nseg = (L.nsteps + L.ckpt_every - 1) / L.ckpt_every;
+        a.ck = L.workspace;
+        a.traj = L.workspace + (size_t)nseg * (N + 1) * (size_t)L.ncell;
+    }
+    hipLaunchKernelGGL(chem_zero_kernel, dim3(1), dim3(64), 0, L.stream, L.first_failed,
+                       static_cast<unsigned long long*>(nullptr));
+    const unsigned grid = (unsigned)((L.ncell + 63) / 64);
+    if (tangent)
+        hipLaunchKernelGGL((sens_kernel<Topo, true>), dim3(grid), dim3(64), 0, L.stream, static_cast<const MD*>(L.mech), a);
+    else
+        hipLaunchKernelGGL((sens_kernel<Topo, false>), dim3(grid), dim3(64), 0, L.stream, static_cast<const MD*>(L.mech), a);
+    return hipGetLastError() == hipSuccess ? HF_OK : HF_EDEVICE;
+    }
+}
+
+int adjoint(const AdjointLaunch& L) { return sens(L, false); }
+int tangent(const AdjointLaunch& L) { return sens(L, true); }
 
 }  // namespace
 
@@ -435,7 +685,7 @@ int tangent(const AdjointLaunch&) { return HF_EINVAL; }
 #define HF_OPS_NAME(x) HF_OPS_NAME2(x)
 extern const TopoOps HF_OPS_NAME(HF_TOPO);
 const TopoOps HF_OPS_NAME(HF_TOPO) = {HF_STR(HF_TOPO), Topo::topology, Topo::NS, Topo::NR, sizeof(MD), fill, chem,
-                                      chem_workspace, flux, props, adjoint, tangent};
+                                      chem_workspace, flux, props, adjoint, tangent, sens_workspace};
 #endif
 
 }  // namespace hf

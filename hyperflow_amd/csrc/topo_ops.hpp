@@ -73,9 +73,9 @@ struct AdjointLaunch {
     double *grad_lnA, *grad_T0, *grad_Y0;
     const double *v_lnA, *v_T0, *v_Y0;  // tangent directions (tangent mode)
     double* dJ;
-    double* workspace;              // checkpoints
+    double* workspace;              // checkpoints + one segment of trajectory (adjoint)
     size_t workspace_bytes;
-    unsigned long long* first_failed;
+    unsigned long long* first_failed;   // device scalar
     hipStream_t stream;
 };
 
@@ -91,6 +91,7 @@ struct TopoOps {
     int (*props)(const PropsLaunch& L);
     int (*adjoint)(const AdjointLaunch& L);
     int (*tangent)(const AdjointLaunch& L);
+    size_t (*sens_workspace)(int64_t ncell, int nsteps, int ckpt_every);
 };
 
 // All compiled topologies (null-terminated).

@@ -158,7 +158,10 @@ int hf_stable_dt(hf_ctx *ctx, const double *U, int64_t ncell, int64_t ld, double
  * qdot = -sum_k u_k wdot_k (u_k molar internal energy), in J[c].
  * Gradients: grad_lnA[c*nr + i] = dJ/d ln A_i (A_i scales k and k0 of
  * reaction i), grad_T0[c], grad_Y0[c*ns + k]. lnA_shift (optional, nr values)
- * perturbs every cell's rate constants. All arrays host or device. */
+ * perturbs every cell's rate constants. All arrays host or device. Each step
+ * is solved by Newton to a correction below 1e-15; a cell whose iteration
+ * fails gets NaN outputs and the call returns HF_ECONV naming the lowest such
+ * cell in hf_last_error. */
 int hf_chem_adjoint(hf_ctx *ctx, int64_t ncell, const double *rho, const double *T0, const double *Y0, double h,
                     int nsteps, int ckpt_every, const double *lnA_shift, double *J, double *grad_lnA,
                     double *grad_T0, double *grad_Y0);

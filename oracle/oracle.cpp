@@ -988,15 +988,17 @@ int or_chem_fixed_be(void* mp, long ncell, const double* rho, const double* T0, 
         bool ok = true;
         for (int s = 0; s < nsteps && ok; ++s) {
             yn = y;
-            // Newton with a finite-difference Jacobian, iterated to convergence
-            for (int it = 0; it < 50; ++it) {
+            // Newton with a finite-difference Jacobian, iterated to the
+            // roundoff floor: stop when every correction is below 1e-14 of its
+            // component (floor 1e-30), or when the largest relative correction
+            // stops halving once below 1e-8 (roundoff reached).
+            bool conv = false;
+            double prev = HUGE_VAL;
+            for (int it = 0; it < 60; ++it) {
                 if (cell.rhs(yn.data(), f.data())) { ok = false; break; }
-                for (int k = 0; k < N; ++k) r[k] = yn[k] - y[k] - h * f[k];
-                double rn = 0;
-                for (int k = 0; k < N; ++k) rn = std::max(rn, std::abs(r[k]));
-                if (rn <= 1e-15 && it > 0) break;
+                for (int k = 0; k < N; ++k) r[k] = -(yn[k] - y[k] - h * f[k]);
                 for (int j = 0; j < N; ++j) {
-                    const double inc = 1e-7 * std::max(std::abs(yn[j]), 1e-10);
+                    const double inc = 1e-8 * std::max(std::abs(yn[j]), 1e-6);
                     const double ys = yn[j];
                     yn[j] = ys + inc;
                     if (cell.rhs(yn.data(), f2.data())) { ok = false; break; }
@@ -1005,15 +1007,17 @@ int or_chem_fixed_be(void* mp, long ncell, const double* rho, const double* T0, 
                 }
                 if (!ok) break;
                 if (!lu_factor(N, Jm.data(), piv.data())) { ok = false; break; }
-                for (int k = 0; k < N; ++k) r[k] = -r[k];
                 lu_solve(N, Jm.data(), piv.data(), r.data());
-                double dn = 0;
-                for (int k = 0; k < N; ++k) { yn[k] += r[k]; dn = std::max(dn, std::abs(r[k])); }
-                if (dn <= 1e-16) {
-                    if (cell.rhs(yn.data(), f.data())) ok = false;
-                    break;
+                double wn = 0;
+                for (int k = 0; k < N; ++k) {
+                    yn[k] += r[k];
+                    wn = std::max(wn, std::abs(r[k]) / (std::abs(yn[k]) + 1e-30));
                 }
+                if (!std::isfinite(wn)) { ok = false; break; }
+                if (wn <= 1e-14 || (it >= 2 && wn < 1e-8 && wn >= 0.5 * prev)) { conv = true; break; }
+                prev = wn;
             }
+            if (ok && !conv) ok = false;
             if (!ok) break;
             y = yn;
             // heat release at the new state
