@@ -58,9 +58,9 @@ enum { HF_PREC_FP64 = 0, HF_PREC_MIXED = 1 };
 typedef struct hf_ctx hf_ctx;
 
 typedef struct hf_options {
-    double rtol;        /* relative tolerance; reference: 1e-8 (chemistry.cpp:114) */
-    double atol;        /* absolute tolerance; reference: 1e-15 (chemistry.cpp:114) */
-    int max_steps;      /* per cell; reference: 5000 (chemistry.cpp:120) */
+    double rtol;        /* relative tolerance; reference: 1e-8 (chemistry.cpp:19) */
+    double atol;        /* absolute tolerance; reference: 1e-15 (chemistry.cpp:19) */
+    int max_steps;      /* per cell; reference: 5000 (chemistry.cpp:25) */
     int precision;      /* HF_PREC_FP64 or HF_PREC_MIXED */
     int device;         /* HIP device ordinal */
     /* batched-form tuning; 0 selects the autotuned / default value */

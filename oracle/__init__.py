@@ -18,9 +18,11 @@ Parity status per function (how each is pinned; see the tests named):
     scipy Radau golden trajectories (tests/test_oracle_bdf.py, tests/golden/).
   * ChemistryBatch semantics: all-or-nothing, lowest index, invalid dt,
     count 0 (tests/test_oracle_chem.py).
-  * fixed-step backward-Euler objective: parity unpinned at the function level;
-    it is the forward map the adjoint's finite differences are taken of, and is
-    checked for step-size convergence against the adaptive BDF.
+  * fixed-step backward-Euler objective (no reference counterpart): the
+    energy identity h qdot(y_1) = rho cv (T_1 - T_0) on single steps,
+    first-order convergence of T(t_end) to the converged BDF solution, and
+    the ln A shift entering the rates (tests/test_oracle_adjoint.py). It is
+    the forward map whose central differences pin the GPU adjoint.
 """
 from __future__ import annotations
 
