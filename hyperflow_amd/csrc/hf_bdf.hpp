@@ -319,125 +319,125 @@ struct Bdf {
         return ok;
     }
 
-    // 0 converged, 1 retry with a fresh matrix, 2 reduce the step, 3 rhs/setup failure
-    __device__ __forceinline__ int newton(Sys& sys, int& convfail) {
-        double y[N], f[N], delta[N];
-        #pragma unroll
-        for (int i = 0; i < N; ++i) y[i] = Z(0, i);
-        ++c.rhs;
-        if (!sys.rhs(y, f)) return 3;
-        bool callsetup = nst == 0 || nst >= nstlp + 20 || fabs(gamrat - 1.0) > 0.3 || convfail != 0 || force_setup;
-        bool jbad = callsetup && (nst == 0 || nst >= nstlj + 51 || convfail == 2 ||
-                                  (convfail == 1 && fabs(gamma / gammap - 1.0) < 0.2));
-        // Wave-synchronised refresh: when any lane of the wave needs a new
-        // Jacobian, every lane takes one, so the expensive evaluation happens
-        // for the whole wave at once instead of in a different trip per lane.
-        if (sync_jac && __any(jbad)) {
-            jbad = true;
-            callsetup = true;
-        }
-        force_setup = false;
-        jfresh = false;
-        if (callsetup && !lsetup(sys, y, jbad)) return 3;
+    // ------------------------------------------------------------ trip-driven integration
+    // The integration is a state machine advanced by trip(): every call makes
+    // exactly one right-hand-side evaluation, at a point chosen by the lane's
+    // phase, and then the bookkeeping that evaluation enables. A wave of lanes
+    // at different points of their steps (first Newton iteration, a later one,
+    // a retry after a failed error test, a new cell's initial-step search)
+    // therefore shares one evaluation per trip instead of serialising each
+    // lane's whole step. The algorithm per lane is CVODE's: the order of
+    // evaluations and decisions is unchanged, only their grouping into calls.
+    enum { PH_F0 = 0, PH_HIN = 1, PH_START = 2, PH_NEWTON = 3, PH_ERR_RHS = 4 };
+    int phase, m, convfail, ncf, nef, hin_count, hin_tries;
+    bool retry, callsetup, jbad;
+    double saved_t, delp, hg, hub, hlb;
+    double tout;
+    int maxsteps;
+    bool first;
+
+    // Start of a Newton solve (iteration m = 0), after the predictor or for
+    // a retry with a fresh setup: CVODE's setup decision (MSBP 20, DGMAX 0.3)
+    // and Jacobian decision (MSBJ 51, DGMAXLSETUP 0.2).
+    __device__ __forceinline__ void begin_newton() {
+        m = 0;
+        delp = 0;
         #pragma unroll
         for (int i = 0; i < N; ++i) acor[i] = 0;
-        double delp = 0;
-        for (int m = 0; m < 3; ++m) {
-            ++c.newton;
-            #pragma unroll
-            for (int i = 0; i < N; ++i) delta[i] = -(fma(rl1, Z(1, i), acor[i]) - gamma * f[i]);
-            lu_solve(delta);
-            if (gamrat != 1.0) {
-                const double s = 2.0 / (1.0 + gamrat);
-                #pragma unroll
-                for (int i = 0; i < N; ++i) delta[i] *= s;
-            }
-            #pragma unroll
-            for (int i = 0; i < N; ++i) {
-                acor[i] += delta[i];
-                y[i] = Z(0, i) + acor[i];
-            }
-            const double del = wrms(delta);
-            if (m > 0) crate = fmax(0.3 * crate, del / delp);
-            const double dcon = del * fmin(1.0, crate) / tq4;
-            if (dcon <= 1.0) {
-                acnrm = (m == 0) ? del : wrms(acor);
-                return 0;
-            }
-            if (m == 2 || (m >= 1 && del > 2.0 * delp)) break;
-            delp = del;
-            ++c.rhs;
-            if (!sys.rhs(y, f)) return 3;
-        }
-        if (!jfresh) {
-            convfail = 1;
-            return 1;
-        }
-        return 2;
+        callsetup = nst == 0 || nst >= nstlp + 20 || fabs(gamrat - 1.0) > 0.3 || convfail != 0 || force_setup;
+        jbad = callsetup && (nst == 0 || nst >= nstlj + 51 || convfail == 2 ||
+                             (convfail == 1 && fabs(gamma / gammap - 1.0) < 0.2));
+        force_setup = false;
+        jfresh = false;
+        phase = PH_NEWTON;
     }
 
-    __device__ __forceinline__ int step(Sys& sys, double tout) {
-        const double saved_t = tn;
-        int ncf = 0, nef = 0, convfail = 0;
-        double dsm = 0;
-        for (;;) {
-            predict();
-            if ((tn - tout) * h > 0) tn = tout;
-            set_bdf();
-            rl1 = 1.0 / LC(1);
-            gamma = h * rl1;
-            if (nst == 0) gammap = gamma;
-            gamrat = (nst > 0) ? gamma / gammap : 1.0;
-            int nf = newton(sys, convfail);
-            if (nf == 1) nf = newton(sys, convfail);
-            if (nf != 0) {
-                restore(saved_t);
-                ++ncf;
-                ++c.ncf;
-                etamax = 1;
-                if (ncf >= 10) return BDF_FAIL;
-                eta = 0.25;
-                convfail = 2;
-                rescale();
-                continue;
+    // A step attempt: between accepted steps, the tstop clamp and pending
+    // order/step changes first; then the predictor and the BDF coefficients.
+    __device__ __forceinline__ bool start_attempt() {
+        if (!retry) {
+            if (nst >= maxsteps) return false;
+            if (!first) set_ewt();
+            if ((tn + hprime - tout) * hprime > 0) {
+                hprime = (tout - tn) * (1.0 - 4.0 * UROUND);
+                eta = hprime / h;
             }
+            if (hprime != h) {
+                if (!first && qprime != q) {
+                    adjust_order(qprime - q);
+                    q = qprime;
+                    L = q + 1;
+                    qwait = L;
+                }
+                rescale();
+            }
+            first = false;
+            ncf = nef = 0;
             convfail = 0;
-            dsm = acnrm * tq2;
-            if (dsm <= 1.0) break;
+        }
+        saved_t = tn;
+        predict();
+        if ((tn - tout) * h > 0) tn = tout;
+        set_bdf();
+        rl1 = 1.0 / LC(1);
+        gamma = h * rl1;
+        if (nst == 0) gammap = gamma;
+        gamrat = (nst > 0) ? gamma / gammap : 1.0;
+        begin_newton();
+        return true;
+    }
+
+    // The step after its last Newton failure (CVODE: eta 0.25, convfail OTHER).
+    __device__ __forceinline__ int newton_failed() {
+        restore(saved_t);
+        ++ncf;
+        ++c.ncf;
+        etamax = 1;
+        if (ncf >= 10) return BDF_FAIL;
+        eta = 0.25;
+        convfail = 2;
+        rescale();
+        retry = true;
+        phase = PH_START;
+        return BDF_RUNNING;
+    }
+
+    // Newton converged: the local error test, then acceptance or a retry.
+    __device__ __forceinline__ int error_test(double* yout) {
+        convfail = 0;
+        const double dsm = acnrm * tq2;
+        if (dsm > 1.0) {
             ++nef;
             ++c.netf;
             restore(saved_t);
             force_setup = true;
             if (nef >= 7) return BDF_FAIL;
             etamax = 1;
+            retry = true;
+            phase = PH_START;
             if (nef <= 3) {
                 eta = 1.0 / (pow(6.0 * dsm, 1.0 / L) + 1e-6);
                 eta = fmax(0.1, eta);
                 if (nef >= 2) eta = fmin(eta, 0.2);
                 rescale();
-                continue;
-            }
-            if (q > 1) {
+            } else if (q > 1) {
                 eta = 0.1;
                 adjust_order(-1);
                 L = q;
                 --q;
                 qwait = L;
                 rescale();
-                continue;
+            } else {
+                // order 1 after repeated failures: restart the history from a
+                // fresh derivative at the restored state (next trip)
+                eta = 0.1;
+                h *= eta;
+                hprime = h;
+                hscale = h;
+                qwait = 10;
+                phase = PH_ERR_RHS;
             }
-            eta = 0.1;
-            h *= eta;
-            hprime = h;
-            hscale = h;
-            qwait = 10;
-            double f[N], y0[N];
-            #pragma unroll
-            for (int i = 0; i < N; ++i) y0[i] = Z(0, i);
-            ++c.rhs;
-            if (!sys.rhs(y0, f)) return BDF_FAIL;
-            #pragma unroll
-            for (int i = 0; i < N; ++i) Z(1, i) = h * f[i];
+            return BDF_RUNNING;
         }
         // accept
         ++nst;
@@ -460,7 +460,22 @@ struct Bdf {
             saved_tq5 = tq5;
         }
         prepare_next(dsm);
-        return BDF_OK;
+        retry = false;
+        phase = PH_START;
+        const double troundoff = 100.0 * UROUND * (fabs(tn) + fabs(h));
+        if (fabs(tn - tout) <= troundoff || (tn - tout) * h >= 0) {
+            const double s = (tout - tn) / h;
+            #pragma unroll
+            for (int i = 0; i < N; ++i) {
+                double v = zrow(q, i);
+                #pragma unroll
+                for (int j = QMAX - 1; j >= 0; --j)
+                    if (j < q) v = fma(s, v, Z(j, i));
+                yout[i] = v;
+            }
+            return BDF_OK;
+        }
+        return BDF_RUNNING;
     }
 
     // Every branch computes into locals and the state is written once at the
@@ -519,59 +534,10 @@ struct Bdf {
         etamax = 10.0;
     }
 
-    __device__ __forceinline__ double initial_step(Sys& sys, const double* f0, double tout) {
-        const double tdist = fabs(tout);
-        const double tround = UROUND * tdist;
-        if (tdist < 2.0 * tround) return -1;
-        const double hlb = 100.0 * tround;
-        double hub_inv = 0;
-        #pragma unroll
-        for (int i = 0; i < N; ++i) hub_inv = fmax(hub_inv, fabs(f0[i]) / fma(0.1, fabs(Z(0, i)), atol));
-        double hub = 0.1 * tdist;
-        if (hub * hub_inv > 1.0) hub = 1.0 / hub_inv;
-        double hg = sqrt(hlb * hub);
-        if (hub < hlb) return hg;
-        double hnew = hg;
-        for (int count = 1;; ++count) {
-            bool ok = false;
-            double yddnrm = 0;
-            for (int tries = 0; tries < 4 && !ok; ++tries) {
-                double y[N], f1[N];
-                #pragma unroll
-                for (int i = 0; i < N; ++i) y[i] = fma(hg, f0[i], Z(0, i));
-                ++c.rhs;
-                if (sys.rhs(y, f1)) {
-                    #pragma unroll
-                    for (int i = 0; i < N; ++i) f1[i] = (f1[i] - f0[i]) / hg;
-                    yddnrm = wrms(f1);
-                    ok = __builtin_isfinite(yddnrm);
-                }
-                if (!ok) hg *= 0.2;
-            }
-            if (!ok) return -1;
-            hnew = (yddnrm * hub * hub > 2.0) ? sqrt(2.0 / yddnrm) : sqrt(hg * hub);
-            if (count >= 4) break;
-            const double hrat = hnew / hg;
-            if (hrat > 0.5 && hrat < 2.0) break;
-            if (count > 1 && hrat > 2.0) {
-                hnew = hg;
-                break;
-            }
-            hg = hnew;
-        }
-        return fmin(fmax(0.5 * hnew, hlb), hub);
-    }
-
-    // ------------------------------------------------------------ resumable driver
-    // begin() starts an integration of y from t = 0 to tout; each advance()
-    // call takes one accepted step (with its internal retries) and returns
-    // BDF_RUNNING, BDF_OK (tout reached, y written) or BDF_FAIL. integrate()
-    // strings them together; the persistent kernel interleaves cells by
-    // calling advance() once per loop trip.
-    double tout;
-    int maxsteps;
-    bool first;
-
+    // ------------------------------------------------------------ driver
+    // begin() starts an integration of y from t = 0 to t_out; every trip()
+    // makes one rate evaluation and returns BDF_RUNNING, BDF_OK (t_out
+    // reached, y written) or BDF_FAIL.
     __device__ __forceinline__ int begin(Sys& sys, const double* y, double t_out, int max_steps) {
         tout = t_out;
         maxsteps = max_steps;
@@ -579,15 +545,15 @@ struct Bdf {
         #pragma unroll
         for (int i = 0; i < N; ++i) Z(0, i) = y[i];
         tn = 0;
-        double f0[N];
-        ++c.rhs;
-        if (!sys.rhs(y, f0)) return BDF_FAIL;
-        set_ewt();
-        const double h0 = initial_step(sys, f0, tout);
-        if (!(h0 > 0)) return BDF_FAIL;
+        phase = PH_F0;
+        return BDF_RUNNING;
+    }
+
+    // Initial step accepted (CVODE's cvHin result): the history starts at order 1.
+    __device__ __forceinline__ void finish_begin(double h0) {
         h = h0;
         #pragma unroll
-        for (int i = 0; i < N; ++i) Z(1, i) = h * f0[i];
+        for (int i = 0; i < N; ++i) Z(1, i) *= h;   // Z(1) held f(y0) during the search
         q = qprime = 1;
         L = 2;
         qwait = 2;
@@ -605,49 +571,134 @@ struct Bdf {
         force_setup = false;
         jfresh = false;
         first = true;
-        return BDF_RUNNING;
+        retry = false;
+        phase = PH_START;
     }
 
-    __device__ __forceinline__ int advance(Sys& sys, double* y) {
-        if (nst >= maxsteps) return BDF_FAIL;
-        if (!first) set_ewt();
-        if ((tn + hprime - tout) * hprime > 0) {
-            hprime = (tout - tn) * (1.0 - 4.0 * UROUND);
-            eta = hprime / h;
-        }
-        if (hprime != h) {
-            if (!first && qprime != q) {
-                adjust_order(qprime - q);
-                q = qprime;
-                L = q + 1;
-                qwait = L;
+    __device__ __forceinline__ int trip(Sys& sys, double* yout) {
+        if (phase == PH_START && !start_attempt()) return BDF_FAIL;
+        // Wave-synchronised Jacobian refresh: when any lane starting a Newton
+        // solve needs a new Jacobian, every lane starting one takes it, so the
+        // expensive evaluation falls in few trips of the wave.
+        if (sync_jac) {
+            const bool at0 = phase == PH_NEWTON && m == 0;
+            if (__any(at0 && jbad) && at0) {
+                jbad = true;
+                callsetup = true;
             }
-            rescale();
         }
-        first = false;
-        if (step(sys, tout) != BDF_OK) return BDF_FAIL;
-        const double troundoff = 100.0 * UROUND * (fabs(tn) + fabs(h));
-        if (fabs(tn - tout) <= troundoff || (tn - tout) * h >= 0) {
-            const double s = (tout - tn) / h;
-            #pragma unroll
-            for (int i = 0; i < N; ++i) {
-                double v = zrow(q, i);
+        double yv[N], f[N];
+        #pragma unroll
+        for (int i = 0; i < N; ++i) {
+            const double z1 = Z(1, i);
+            const double dz = phase == PH_NEWTON ? acor[i] : phase == PH_HIN ? hg * z1 : 0.0;
+            yv[i] = Z(0, i) + dz;
+        }
+        ++c.rhs;
+        const bool ok = sys.rhs(yv, f);
+
+        if (phase == PH_NEWTON) {
+            if (ok && (m != 0 || !callsetup || lsetup(sys, yv, jbad))) {
+                ++c.newton;
+                double delta[N];
                 #pragma unroll
-                for (int j = QMAX - 1; j >= 0; --j)
-                    if (j < q) v = fma(s, v, Z(j, i));
-                y[i] = v;
+                for (int i = 0; i < N; ++i) delta[i] = -(fma(rl1, Z(1, i), acor[i]) - gamma * f[i]);
+                lu_solve(delta);
+                if (gamrat != 1.0) {
+                    const double s = 2.0 / (1.0 + gamrat);
+                    #pragma unroll
+                    for (int i = 0; i < N; ++i) delta[i] *= s;
+                }
+                #pragma unroll
+                for (int i = 0; i < N; ++i) acor[i] += delta[i];
+                const double del = wrms(delta);
+                if (m > 0) crate = fmax(0.3 * crate, del / delp);
+                const double dcon = del * fmin(1.0, crate) / tq4;
+                if (dcon <= 1.0) {
+                    acnrm = (m == 0) ? del : wrms(acor);
+                    return error_test(yout);
+                }
+                if (!(m == 2 || (m >= 1 && del > 2.0 * delp))) {
+                    delp = del;
+                    ++m;
+                    return BDF_RUNNING;
+                }
+                if (!jfresh) {
+                    // stale Jacobian: retry the solve with a fresh setup (CVODE's TRY_AGAIN)
+                    convfail = 1;
+                    begin_newton();
+                    return BDF_RUNNING;
+                }
             }
-            return BDF_OK;
+            return newton_failed();
         }
+        if (phase == PH_F0) {
+            if (!ok) return BDF_FAIL;
+            #pragma unroll
+            for (int i = 0; i < N; ++i) Z(1, i) = f[i];
+            set_ewt();
+            // CVODE's initial step (cvHin): bounds, then up to four refinements
+            const double tdist = fabs(tout), tround = UROUND * tdist;
+            if (tdist < 2.0 * tround) return BDF_FAIL;
+            hlb = 100.0 * tround;
+            double hub_inv = 0;
+            #pragma unroll
+            for (int i = 0; i < N; ++i) hub_inv = fmax(hub_inv, fabs(f[i]) / fma(0.1, fabs(yv[i]), atol));
+            hub = 0.1 * tdist;
+            if (hub * hub_inv > 1.0) hub = 1.0 / hub_inv;
+            hg = sqrt(hlb * hub);
+            if (hub < hlb) {
+                finish_begin(hg);
+            } else {
+                hin_count = 1;
+                hin_tries = 0;
+                phase = PH_HIN;
+            }
+            return BDF_RUNNING;
+        }
+        if (phase == PH_HIN) {
+            double yddnrm = 0;
+            bool good = false;
+            if (ok) {
+                double d2[N];
+                #pragma unroll
+                for (int i = 0; i < N; ++i) d2[i] = (f[i] - Z(1, i)) / hg;
+                yddnrm = wrms(d2);
+                good = __builtin_isfinite(yddnrm);
+            }
+            if (!good) {
+                hg *= 0.2;
+                return ++hin_tries >= 4 ? BDF_FAIL : BDF_RUNNING;
+            }
+            hin_tries = 0;
+            double hnew = (yddnrm * hub * hub > 2.0) ? sqrt(2.0 / yddnrm) : sqrt(hg * hub);
+            bool done = hin_count >= 4;
+            if (!done) {
+                const double hrat = hnew / hg;
+                if (hrat > 0.5 && hrat < 2.0) {
+                    done = true;
+                } else if (hin_count > 1 && hrat > 2.0) {
+                    hnew = hg;
+                    done = true;
+                }
+            }
+            if (done) {
+                finish_begin(fmin(fmax(0.5 * hnew, hlb), hub));
+            } else {
+                hg = hnew;
+                ++hin_count;
+            }
+            return BDF_RUNNING;
+        }
+        // PH_ERR_RHS: new first derivative at the restored state
+        if (!ok) return BDF_FAIL;
+        #pragma unroll
+        for (int i = 0; i < N; ++i) Z(1, i) = h * f[i];
+        retry = true;
+        phase = PH_START;
         return BDF_RUNNING;
-    }
-
-    // Integrate y from t = 0 to tout in place. Returns BDF_OK or BDF_FAIL.
-    __device__ __forceinline__ int integrate(Sys& sys, double* y, double t_out, int max_steps) {
-        int r = begin(sys, y, t_out, max_steps);
-        while (r == BDF_RUNNING) r = advance(sys, y);
-        return r;
     }
 };
+
 
 }  // namespace hf

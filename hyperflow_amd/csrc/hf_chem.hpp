@@ -68,8 +68,9 @@ struct ChemParams {
 // Returns HF_OK, HF_EINVAL (bad dt), HF_EDOMAIN (inadmissible state before or
 // after, or accepted species below -1e-13) or HF_ECONV (integrator failure).
 // A lane's integration of one cell, split so that a persistent kernel can
-// interleave cells: start() validates and begins, step() takes one BDF step,
-// finish() applies the reference's post-step checks and writes the species.
+// interleave cells: start() validates and begins, step() makes one trip of the
+// integrator (one rate evaluation and the bookkeeping it enables), finish()
+// applies the reference's post-step checks and writes the species.
 // zs: this lane's slot of the wave's Nordsieck slab ([order][unknown][64 lanes]).
 template <class T, class MT = double>
 struct CellRun {
@@ -112,7 +113,7 @@ struct CellRun {
         return bdf.begin(sys, y, dt, maxsteps) == BDF_RUNNING ? HF_OK : HF_ECONV;
     }
 
-    __device__ __forceinline__ int step() { return bdf.advance(sys, y); }
+    __device__ __forceinline__ int step() { return bdf.trip(sys, y); }
 
     // Post-step checks of chemistry.cpp:68-75 and the write of the species planes.
     __device__ __forceinline__ int finish(const double* __restrict__ U, long i, long ld, double* __restrict__ out,
