@@ -206,10 +206,11 @@ __device__ __forceinline__ bool properties(const MechData<T>& d, const double* u
     ok = ok && finite(u[NS]) && finite(u[NS + 1]) && finite(u[NS + 2]) && rho > 0;
     if (!ok) return false;
     double Y[NS];
+    const double rr = 1.0 / rho;
     #pragma unroll
-    for (int k = 0; k < NS; ++k) Y[k] = u[k] / rho;
-    const double ux = u[NS] / rho, vy = u[NS + 1] / rho;
-    const double e = u[NS + 2] / rho - 0.5 * (ux * ux + vy * vy);
+    for (int k = 0; k < NS; ++k) Y[k] = u[k] * rr;
+    const double ux = u[NS] * rr, vy = u[NS + 1] * rr;
+    const double e = u[NS + 2] * rr - 0.5 * (ux * ux + vy * vy);
     double t = t_guess;
     if (!temperature_from_e(d, Y, e, t)) return false;
     double cp, cv, rw;
@@ -218,6 +219,31 @@ __device__ __forceinline__ bool properties(const MechData<T>& d, const double* u
     out.p = rho * kR * t * rw;
     out.a = sqrt(cp / cv * kR * t * rw);
     return out.T > 0 && out.p > 0 && out.a > 0 && finite(out.a);
+}
+
+// Temperature of a conserved state (the inversion inside properties()), or
+// `fallback` when the state is not admissible. Used to warm-start the
+// inversions of nearby states, as Cantera's phase object keeps the last T.
+template <class T>
+__device__ __forceinline__ double state_temperature(const MechData<T>& d, const double* u, double fallback) {
+    constexpr int NS = T::NS;
+    double rho = 0;
+    bool ok = true;
+    #pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        rho += u[k];
+        ok = ok && finite(u[k]) && u[k] >= 0;
+    }
+    ok = ok && finite(u[NS]) && finite(u[NS + 1]) && finite(u[NS + 2]) && rho > 0;
+    if (!ok) return fallback;
+    double Y[NS];
+    const double rr = 1.0 / rho;
+    #pragma unroll
+    for (int k = 0; k < NS; ++k) Y[k] = u[k] * rr;
+    const double ux = u[NS] * rr, vy = u[NS + 1] * rr;
+    const double e = u[NS + 2] * rr - 0.5 * (ux * ux + vy * vy);
+    double t = fallback;
+    return temperature_from_e(d, Y, e, t) ? t : fallback;
 }
 
 __device__ __forceinline__ double minmod(double x, double y) {

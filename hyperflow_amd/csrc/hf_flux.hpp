@@ -28,16 +28,17 @@ struct FluxTile {
 
 // Reconstructed face state and its properties; falls back to the cell value
 // (first order) when the reconstruction is inadmissible (BlockAMR.cpp:221-222).
-// Returns false when even the cell value is inadmissible.
+// Returns false when even the cell value is inadmissible. t_hint: the
+// temperature of the adjacent cell, the warm start of the inversion.
 template <class T>
 __device__ __forceinline__ bool face_side(const MechData<T>& d, double* s, const double* cell, Props& pr,
-                                          int& fallbacks) {
+                                          int& fallbacks, double t_hint) {
     constexpr int NV = T::NS + 3;
-    if (properties(d, s, pr, 1000.0)) return true;
+    if (properties(d, s, pr, t_hint)) return true;
     ++fallbacks;
     #pragma unroll
     for (int c = 0; c < NV; ++c) s[c] = cell[c];
-    return properties(d, s, pr, 1000.0);
+    return properties(d, s, pr, t_hint);
 }
 
 template <class T, int TX, int TY, bool TWO_D>
@@ -51,6 +52,7 @@ flux_stage_kernel(const MechData<T>* __restrict__ dm, const hf_box* __restrict__
     constexpr int GX = FT::GX, GY = FT::GY, BX = FT::BX;
     __shared__ double s_in[FT::LDS_IN];
     __shared__ double s_fx[FT::LDS_FLUX];
+    __shared__ double s_t[GX * GY];   // temperature of each staged cell (warm starts)
     const MechData<T>& d = *dm;
 
     // XCD-aware tile order: workgroups are dealt round-robin to the 8 XCDs,
@@ -81,6 +83,13 @@ flux_stage_kernel(const MechData<T>* __restrict__ dm, const hf_box* __restrict__
     __syncthreads();
 
     auto cellv = [&](int gi, int gj, int n) HF_AI -> double { return s_in[n * GX * GY + gj * GX + gi]; };
+    for (int e = lane; e < GX * GY; e += FT::THREADS) {
+        double u[NV];
+        #pragma unroll
+        for (int c = 0; c < NV; ++c) u[c] = s_in[c * GX * GY + e];
+        s_t[e] = state_temperature(d, u, 1000.0);
+    }
+    __syncthreads();
     int fallbacks = 0, faces = 0;
     bool fail = false;
     unsigned long long fail_idx = ~0ull;
@@ -118,8 +127,8 @@ flux_stage_kernel(const MechData<T>* __restrict__ dm, const hf_box* __restrict__
                 }
             }
             Props pl, pr;
-            const bool okl = face_side(d, L, qb, pl, fallbacks);
-            const bool okr = face_side(d, R, qc, pr, fallbacks);
+            const bool okl = face_side(d, L, qb, pl, fallbacks, s_t[(gj - dj) * GX + gi - di]);
+            const bool okr = face_side(d, R, qc, pr, fallbacks, s_t[gj * GX + gi]);
             if (!okl || !okr) {
                 fail = true;
                 const int bad_i = okl ? fi : fi - di, bad_j = okl ? fj : fj - dj;
@@ -173,7 +182,7 @@ flux_stage_kernel(const MechData<T>* __restrict__ dm, const hf_box* __restrict__
         for (int c = 0; c < NV; ++c)
             Uout[B.out_offset + c * B.out_plane + (int64_t)cj * B.out_pitch + ci] = acc[c];
         Props p;
-        if (!properties(d, acc, p, 1000.0)) {
+        if (!properties(d, acc, p, s_t[gyc * GX + lx + 2])) {
             fail = true;
             fail_idx = min(fail_idx, ((unsigned long long)bi << 32) | (unsigned)(cj * B.nx + ci));
         }
