@@ -10,10 +10,12 @@ cell over its dt. metric value = cells x steps / wall time of the timed steps
 (cell-steps/s) for the batched (persistent, stiffness-binned) form; the plain
 one-cell-per-lane form runs the same workload beside it.
 
-Also reported: the 2-D flux sweep (one Euler stage over a 4096 x 1024 box,
+Also reported: the DME mixed-precision workload (2^18 DME/air cells, seed 4,
+39 species: batched with an FP32 Newton matrix and plain FP64, one step
+each), the 2-D flux sweep (one Euler stage over a 4096 x 1024 box,
 Mcell-updates/s, HBM roofline at 192 algorithmic bytes per cell-update),
 chemistry parity against the CPU oracle on a sample, and the CPU oracle's
-throughput on a bounded sample of the same workload (all host cores).
+throughput on a bounded sample of the same workloads (all host cores).
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1]
 """
@@ -86,12 +88,12 @@ def flux_run(torch, hf, mech, nx, ny, steps, warmup):
     return wall, ms, st
 
 
-def cpu_baseline(U, dt, target_s=15.0, rtol=1e-8, atol=1e-15):
+def cpu_baseline(U, dt, target_s=15.0, rtol=1e-8, atol=1e-15, mech_path=None, label="H2"):
     from oracle import Oracle
     from hyperflow_amd import H2_LI
-    o = Oracle(H2_LI)
+    o = Oracle(mech_path or H2_LI)
     cores = len(os.sched_getaffinity(0))
-    n0 = 512
+    n0 = 512 if mech_path is None else 64
     Us = U[:, :n0].copy()
     t = time.perf_counter()
     o.chem_advance(Us, dt[:n0], rtol=rtol, atol=atol, threads=cores)
@@ -102,15 +104,15 @@ def cpu_baseline(U, dt, target_s=15.0, rtol=1e-8, atol=1e-15):
     r, st, ff, msg = o.chem_advance(Us, dt[:n], rtol=rtol, atol=atol, threads=cores)
     t = time.perf_counter() - t
     return dict(value=n / t, unit="cell-steps/s", cores=cores, kind="port",
-                sample=f"first {n} cells of the same seeded 2^20-cell H2 workload, oracle BDF at "
+                sample=f"first {n} cells of the same seeded {label} workload, oracle BDF at "
                        f"rtol {rtol:g}/atol {atol:g}, OpenMP dynamic,8, {t:.1f} s")
 
 
-def parity(U0, dt, out, n=256):
+def parity(U0, dt, out, n=256, mech_path=None):
     """Max errors of the GPU result against the converged oracle (rtol 1e-12, atol 1e-20) on n cells."""
     from oracle import Oracle
     from hyperflow_amd import H2_LI
-    o = Oracle(H2_LI)
+    o = Oracle(mech_path or H2_LI)
     ref = U0[:, :n].copy()
     o.chem_advance(ref, dt[:n], rtol=1e-12, atol=1e-20, threads=len(os.sched_getaffinity(0)))
     ns = o.ns
@@ -130,6 +132,7 @@ def main():
     ap.add_argument("--cells", type=int, default=1 << 20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and parity legs")
     ap.add_argument("--skip-plain", action="store_true")
+    ap.add_argument("--dme-cells", type=int, default=1 << 18, help="DME mixed-precision workload size (0 skips it)")
     ap.add_argument("--copy-ref-mib", type=int, default=0,
                     help="also run a device-to-device copy of this many MiB (reference for HBM counter scaling)")
     args = ap.parse_args()
@@ -177,6 +180,30 @@ def main():
         log(f"chemistry {name}: {res[name]['value']:.4g} cell-steps/s, kernel {np.mean(kms):.2f} ms, "
             f"steps/cell {st['steps'] / args.cells:.1f}")
 
+    dme = None
+    if args.dme_cells:
+        # DME/air, 2^18 cells, seed 4 (39 species, N = 38). Tolerances rtol 1e-8, atol 1e-18: at the
+        # reference's atol 1e-15 the reference algorithm itself rejects this batch (a trace species is
+        # accepted below -1e-13, chemistry.cpp:69-71; the CPU oracle fails at cell 211).
+        dme = {}
+        for name, prec, mode in (("batched", hf.HF_PREC_MIXED, hf.HF_CHEM_BATCHED),
+                                 ("plain", hf.HF_PREC_FP64, hf.HF_CHEM_PLAIN)):
+            md = hf.Mechanism(hf.DME_SK39, atol=1e-18, precision=prec)
+            if name == "batched":
+                D0, ddt = md.synth_cells(1, 4, args.dme_cells)
+                Dd0 = torch.from_numpy(D0).to("cuda")
+                Dd = Dd0.clone()
+                ddtd = torch.from_numpy(ddt).to("cuda")
+            log(f"DME chemistry {name}: {args.dme_cells} cells, 1 step")
+            wall, kms, st = chem_run(torch, hf.ChemistryBatch(md, mode=mode), Dd, Dd0, ddtd, 1, 0)
+            dme[name] = {"value": args.dme_cells / wall, "kernel_ms": float(kms[0]),
+                         "steps_per_cell": st["steps"] / args.dme_cells, "rhs_per_step": st["rhs"] / st["steps"]}
+            if name == "batched":
+                dme_out = Dd.cpu().numpy()
+            log(f"DME chemistry {name}: {dme[name]['value']:.4g} cell-steps/s, kernel {kms[0]:.1f} ms")
+        dme["batched_over_plain"] = dme["batched"]["value"] / dme["plain"]["value"]
+        del Dd, Dd0
+
     log("flux sweep 4096 x 1024")
     fwall, fms, fst = flux_run(torch, hf, mech, 4096, 1024, max(args.steps, 5), args.warmup)
     fcells = fst["cells"]
@@ -222,11 +249,21 @@ def main():
                           "kernel_ms": float(np.mean(fms)),
                           "roofline": {"bound": "hbm", "achieved": f_kernel_gbs, "peak": HBM_PEAK_GBS,
                                        "unit": "GB/s", "frac": f_kernel_gbs / HBM_PEAK_GBS}}
+    if dme is not None:
+        line["dme_sk39"] = {"workload": f"batched chemistry, {args.dme_cells} DME/air cells (dme_sk39, 39 species), "
+                                        "seed 4, dt log-uniform [1e-8,1e-6] s, rtol 1e-8 atol 1e-18",
+                            "unit": "cell-steps/s", "precision": "batched: FP32 Newton matrix and LU; plain: FP64",
+                            **dme}
     if not args.no_cpu:
         log("parity sample vs the converged oracle")
         line["parity_vs_oracle"] = {k: parity(U0, dt, v) for k, v in outs.items()}
         log("CPU baseline")
         line["cpu_baseline"] = cpu_baseline(U0, dt)
+        if dme is not None:
+            log("DME parity sample and CPU baseline")
+            line["dme_sk39"]["parity_vs_oracle"] = parity(D0, ddt, dme_out, n=128, mech_path=hf.DME_SK39)
+            line["dme_sk39"]["cpu_baseline"] = cpu_baseline(D0, ddt, atol=1e-18, mech_path=hf.DME_SK39,
+                                                            label=f"{args.dme_cells}-cell DME")
     print(json.dumps(line), flush=True)
 
 

@@ -30,7 +30,7 @@ struct BdfCounters {
 
 enum { BDF_OK = 0, BDF_RUNNING = 1, BDF_FAIL = 3 };
 
-template <int N, class Sys, class MT = double, int ZSTRIDE = 64>
+template <int N, class Sys, class MT = double, int ZSTRIDE = 64, int ZAS = 3>
 struct Bdf {
     static constexpr int QMAX = 5;
     static constexpr double UROUND = 2.220446049250313e-16;
@@ -38,8 +38,10 @@ struct Bdf {
     static constexpr int SLAB = (QMAX + 1) * N + (QMAX + 2) + (QMAX + 1);
 
     // Nordsieck history: element (j, i) at zs[(j*N + i) * ZSTRIDE]; the caller
-    // points zs at this lane's slot of an LDS (or global) array laid out
-    // [j][i][lane], so consecutive lanes touch consecutive doubles.
+    // points zs at this lane's slot of an array laid out [j][i][lane], so
+    // consecutive lanes touch consecutive doubles. ZAS is its address space:
+    // 3 (LDS) for small mechanisms, 1 (a per-wave tile in global memory) when
+    // the history outgrows LDS.
     double* zs;
     double ewt[N], acor[N];
     double tq1, tq2, tq3, tq4, tq5;
@@ -56,18 +58,18 @@ struct Bdf {
     BdfCounters c;
 
     // ------------------------------------------------------------ helpers
-    // zs points into LDS: address it as LDS so the accesses are ds_read/ds_write
-    // rather than flat loads through a generic pointer.
+    // Address zs in its own address space so the accesses are ds_read/ds_write
+    // (or global loads) rather than flat loads through a generic pointer.
     __device__ __forceinline__ double& Z(int j, int i) const {
-        typedef __attribute__((address_space(3))) double lds_double;
-        return *(double*)((lds_double*)zs + (j * N + i) * ZSTRIDE);
+        typedef __attribute__((address_space(ZAS))) double zs_double;
+        return *(double*)((zs_double*)zs + (j * N + i) * ZSTRIDE);
     }
     __device__ __forceinline__ double zrow(int j, int i) const { return Z(j, i); }
     // Step-size history tau and BDF coefficients l share the lane's LDS slab,
     // after the Nordsieck array: they are indexed by the current order.
     __device__ __forceinline__ double& slab(int k) const {
-        typedef __attribute__((address_space(3))) double lds_double;
-        return *(double*)((lds_double*)zs + k * ZSTRIDE);
+        typedef __attribute__((address_space(ZAS))) double zs_double;
+        return *(double*)((zs_double*)zs + k * ZSTRIDE);
     }
     __device__ __forceinline__ double& TAU(int i) const { return slab((QMAX + 1) * N + i); }
     __device__ __forceinline__ double& LC(int i) const { return slab((QMAX + 1) * N + QMAX + 2 + i); }
@@ -244,6 +246,10 @@ struct Bdf {
         return p;
     }
 
+    // Small systems are unrolled into registers; large ones (N > 16) run
+    // rolled loops over per-lane arrays.
+    static constexpr bool SMALL = N <= 16;
+
     // Forward and back substitution with the unit-lower / upper factors in
     // `lu` (reciprocal pivots on the diagonal). MT is the Newton-matrix
     // precision (double, or float for the mixed form, where the residuals and
@@ -253,37 +259,105 @@ struct Bdf {
         MT b[N];
         #pragma unroll
         for (int i = 0; i < N; ++i) b[i] = (MT)bd[i];
-        #pragma unroll
-        for (int k = 0; k < N; ++k)
+        if constexpr (SMALL) {
             #pragma unroll
-            for (int i = k + 1; i < N; ++i) b[i] = fma(-lp[(i * N + k) * ZSTRIDE], b[k], b[i]);
-        #pragma unroll
-        for (int i = N - 1; i >= 0; --i) {
-            MT s = b[i];
+            for (int k = 0; k < N; ++k)
+                #pragma unroll
+                for (int i = k + 1; i < N; ++i) b[i] = fma(-lp[(i * N + k) * ZSTRIDE], b[k], b[i]);
             #pragma unroll
-            for (int j = i + 1; j < N; ++j) s = fma(-lp[(i * N + j) * ZSTRIDE], b[j], s);
-            b[i] = s * lp[(i * N + i) * ZSTRIDE];
+            for (int i = N - 1; i >= 0; --i) {
+                MT s = b[i];
+                #pragma unroll
+                for (int j = i + 1; j < N; ++j) s = fma(-lp[(i * N + j) * ZSTRIDE], b[j], s);
+                b[i] = s * lp[(i * N + i) * ZSTRIDE];
+            }
+        } else {
+            #pragma unroll 1
+            for (int i = 1; i < N; ++i) {
+                MT s = b[i];
+                #pragma unroll 1
+                for (int k = 0; k < i; ++k) s = fma(-lp[(i * N + k) * ZSTRIDE], b[k], s);
+                b[i] = s;
+            }
+            #pragma unroll 1
+            for (int i = N - 1; i >= 0; --i) {
+                MT s = b[i];
+                #pragma unroll 1
+                for (int j = i + 1; j < N; ++j) s = fma(-lp[(i * N + j) * ZSTRIDE], b[j], s);
+                b[i] = s * lp[(i * N + i) * ZSTRIDE];
+            }
         }
         #pragma unroll
         for (int i = 0; i < N; ++i) bd[i] = (double)b[i];
     }
 
+    // Row i of I - gamma J eliminated against the finished rows 0..i-1 of
+    // `lu` and stored; returns false on a zero or non-finite pivot.
+    __device__ __forceinline__ bool lu_row(int i, MT g, const MT* js) {
+        MT row[N];
+        #pragma unroll
+        for (int j = 0; j < N; ++j) row[j] = (i == j ? MT(1) : MT(0)) - g * js[(i * N + j) * ZSTRIDE];
+        const MT* up = opaque(lu);
+        #pragma unroll 1
+        for (int k = 0; k < i; ++k) {
+            const MT l = row[k] * up[(k * N + k) * ZSTRIDE];
+            row[k] = l;
+            #pragma unroll 1
+            for (int j = k + 1; j < N; ++j) row[j] = fma(-l, up[(k * N + j) * ZSTRIDE], row[j]);
+        }
+        const MT p = row[i];
+        const bool ok = fabs(p) > __FLT_MIN__ && __builtin_isfinite(p);
+        row[i] = MT(1) / p;
+        MT* wp = opaque(lu);
+        #pragma unroll
+        for (int j = 0; j < N; ++j) wp[(i * N + j) * ZSTRIDE] = row[j];
+        return ok;
+    }
+
     // Newton-matrix setup. A fresh Jacobian (jbad) is evaluated and saved in
     // the lane's global slot (jsave, lane-contiguous); otherwise the saved one
     // is reused, as CVODE reuses J across steps and only re-forms I - gamma J
-    // when gamma changes. I - gamma J is factored row by row (Doolittle, no
-    // pivoting) straight into the lane's `lu` slot: row i is formed from the
-    // saved J, eliminated against the finished rows 0..i-1 read back from
-    // `lu`, and stored, so only two rows are live in registers at a time.
-    __device__ __forceinline__ bool lsetup(Sys& sys, const double* y, bool jbad) {
+    // when gamma changes. Small mechanisms use the analytic Jacobian; large
+    // ones CVODE's own difference quotients (the reference's configuration,
+    // HyperFlow_V3/src/chemistry.cpp:20-24: dense matrix, no Jacobian
+    // function). f = f(y), already evaluated. I - gamma J is factored row by
+    // row (Doolittle, no pivoting) straight into the lane's `lu` slot: row i is
+    // formed from the saved J, eliminated against the finished rows 0..i-1
+    // read back from `lu`, and stored, so one row at a time is live.
+    __device__ __forceinline__ bool lsetup(Sys& sys, const double* y, const double* f, bool jbad) {
         ++c.lu;
         if (jbad) {
-            MT J[N * N];
-            double f[N];
             ++c.jac;
-            if (!sys.jac(y, J, f)) return false;
-            #pragma unroll
-            for (int i = 0; i < N * N; ++i) jsave[i * ZSTRIDE] = J[i];
+            if constexpr (Sys::kAnalyticJac) {
+                MT J[N * N];
+                double fj[N];
+                if (!sys.jac(y, J, fj)) return false;
+                #pragma unroll
+                for (int i = 0; i < N * N; ++i) jsave[i * ZSTRIDE] = J[i];
+            } else {
+                // cvLsDenseDQJac: column j from f(y + inc_j e_j) with
+                // inc_j = max(sqrt(uround) |y_j|, minInc / ewt_j),
+                // minInc = 1000 |h| uround N ||f||_wrms (1 when f = 0).
+                const double fnorm = wrms(f);
+                const double min_inc = fnorm != 0 ? 1000.0 * fabs(h) * UROUND * N * fnorm : 1.0;
+                const double srur = sqrt(UROUND);
+                double yj[N], fj[N];
+                #pragma unroll
+                for (int i = 0; i < N; ++i) yj[i] = y[i];
+                #pragma unroll 1
+                for (int j = 0; j < N; ++j) {
+                    const double ysave = yj[j];
+                    const double inc = fmax(srur * fabs(ysave), min_inc / ewt[j]);
+                    yj[j] = ysave + inc;
+                    ++c.rhs;
+                    if (!sys.rhs(yj, fj)) return false;
+                    yj[j] = ysave;
+                    const double rinc = 1.0 / inc;
+                    MT* js = opaque(jsave);
+                    #pragma unroll
+                    for (int i = 0; i < N; ++i) js[(i * N + j) * ZSTRIDE] = (MT)((fj[i] - f[i]) * rinc);
+                }
+            }
             nstlj = nst;
             jfresh = true;
         } else {
@@ -296,25 +370,30 @@ struct Bdf {
         const MT g = (MT)gamma;
         const MT* js = opaque(jsave);
         bool ok = true;
-        #pragma unroll
-        for (int i = 0; i < N; ++i) {
-            MT row[N];
+        if constexpr (SMALL) {
             #pragma unroll
-            for (int j = 0; j < N; ++j) row[j] = (i == j ? MT(1) : MT(0)) - g * js[(i * N + j) * ZSTRIDE];
-            const MT* up = opaque(lu);
-            #pragma unroll
-            for (int k = 0; k < i; ++k) {
-                const MT l = row[k] * up[(k * N + k) * ZSTRIDE];
-                row[k] = l;
+            for (int i = 0; i < N; ++i) {
+                MT row[N];
                 #pragma unroll
-                for (int j = k + 1; j < N; ++j) row[j] = fma(-l, up[(k * N + j) * ZSTRIDE], row[j]);
+                for (int j = 0; j < N; ++j) row[j] = (i == j ? MT(1) : MT(0)) - g * js[(i * N + j) * ZSTRIDE];
+                const MT* up = opaque(lu);
+                #pragma unroll
+                for (int k = 0; k < i; ++k) {
+                    const MT l = row[k] * up[(k * N + k) * ZSTRIDE];
+                    row[k] = l;
+                    #pragma unroll
+                    for (int j = k + 1; j < N; ++j) row[j] = fma(-l, up[(k * N + j) * ZSTRIDE], row[j]);
+                }
+                const MT p = row[i];
+                ok = ok && fabs(p) > __FLT_MIN__ && __builtin_isfinite(p);
+                row[i] = MT(1) / p;
+                MT* wp = opaque(lu);
+                #pragma unroll
+                for (int j = 0; j < N; ++j) wp[(i * N + j) * ZSTRIDE] = row[j];
             }
-            const MT p = row[i];
-            ok = ok && fabs(p) > __FLT_MIN__ && __builtin_isfinite(p);
-            row[i] = MT(1) / p;
-            MT* wp = opaque(lu);
-            #pragma unroll
-            for (int j = 0; j < N; ++j) wp[(i * N + j) * ZSTRIDE] = row[j];
+        } else {
+            #pragma unroll 1
+            for (int i = 0; i < N; ++i) ok = lu_row(i, g, js) && ok;
         }
         return ok;
     }
@@ -598,7 +677,7 @@ struct Bdf {
         const bool ok = sys.rhs(yv, f);
 
         if (phase == PH_NEWTON) {
-            if (ok && (m != 0 || !callsetup || lsetup(sys, yv, jbad))) {
+            if (ok && (m != 0 || !callsetup || lsetup(sys, yv, f, jbad))) {
                 ++c.newton;
                 double delta[N];
                 #pragma unroll

@@ -14,6 +14,8 @@ namespace hf {
 template <class T>
 struct ChemSys {
     static constexpr int NS = T::NS, N = NS - 1;
+    // analytic Jacobian for small mechanisms; difference quotients otherwise
+    static constexpr bool kAnalyticJac = NS <= 16;
     const MechData<T>& d;
     double rho, e, t;
     const double* shift;   // optional per-reaction ln A shift (sensitivities)
@@ -71,12 +73,13 @@ struct ChemParams {
 // interleave cells: start() validates and begins, step() makes one trip of the
 // integrator (one rate evaluation and the bookkeeping it enables), finish()
 // applies the reference's post-step checks and writes the species.
-// zs: this lane's slot of the wave's Nordsieck slab ([order][unknown][64 lanes]).
-template <class T, class MT = double>
+// zs: this lane's slot of the wave's Nordsieck slab ([order][unknown][64
+// lanes]), in LDS (ZAS 3) or in a global per-wave tile (ZAS 1).
+template <class T, class MT = double, int ZAS = 3>
 struct CellRun {
     static constexpr int NS = T::NS, NV = NS + 3, N = NS - 1;
     ChemSys<T> sys;
-    Bdf<N, ChemSys<T>, MT> bdf;
+    Bdf<N, ChemSys<T>, MT, 64, ZAS> bdf;
     double y[N];
 
     // jsave/lu: this lane's saved-Jacobian and LU-factor slots (element k at

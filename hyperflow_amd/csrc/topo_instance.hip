@@ -83,7 +83,15 @@ struct Workspace {
     unsigned* head;
     void* jsave;          // saved Jacobians: one [element][64 lanes] tile per wave
     void* lu;             // factored Newton matrices, same layout
+    void* zs;             // Nordsieck slabs, same layout, when they do not fit in LDS
 };
+
+// The per-lane BDF slab (Nordsieck history, step-size history, BDF
+// coefficients) of a wave: (6N + 13) doubles x 64 lanes. It lives in LDS
+// while that allows four waves per CU (31 KB per wave for H2), otherwise in a
+// per-wave tile of the workspace (123 KB per wave for DME, N = 38).
+constexpr int kSlab = 6 * (Topo::NS - 1) + 13;
+constexpr bool kSlabLds = kSlab * 64 * 8 <= 40 * 1024;
 
 inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
@@ -113,15 +121,18 @@ Workspace carve(void* base, int64_t ncell, int64_t lanes) {
     w.jsave = p;
     p += align256(sizeof(double) * N * N * lanes);
     w.lu = p;
+    p += align256(sizeof(double) * N * N * lanes);
+    w.zs = kSlabLds ? nullptr : p;
     return w;
 }
 
-// Queue arrays plus a saved Jacobian and an LU-factor slot (FP64-sized) per lane.
+// Queue arrays plus a saved Jacobian and an LU-factor slot (FP64-sized) per
+// lane, and the BDF slabs when they live in global memory.
 size_t chem_workspace(int64_t ncell, int, int num_cus, int waves_per_cu) {
     constexpr int N = Topo::NS - 1;
     const int64_t lanes = slot_lanes(ncell, num_cus, waves_per_cu);
     return align256(ncell) + align256(sizeof(int32_t) * ncell) + 3 * align256(sizeof(unsigned) * kMaxBins) + 256 +
-           2 * align256(sizeof(double) * N * N * lanes);
+           2 * align256(sizeof(double) * N * N * lanes) + (kSlabLds ? 0 : align256(sizeof(double) * kSlab * lanes));
 }
 
 __global__ void chem_init_kernel(Workspace w, unsigned long long* first_failed, unsigned long long* counters) {
@@ -225,11 +236,17 @@ chem_lane_kernel(const MechData<TT>* __restrict__ dm, const double* __restrict__
                  int8_t* __restrict__ status, unsigned long long* __restrict__ first_failed, ChemParams p,
                  unsigned long long* __restrict__ counters, Workspace w, int refill) {
     constexpr int N = TT::NS - 1;
-    __shared__ double zlds[(6 * N + 13) * 64];   // per-lane BDF slab: Nordsieck history, tau, l
+    double* zbase;
+    if constexpr (kSlabLds) {
+        __shared__ double zlds[kSlab * 64];
+        zbase = zlds;
+    } else {
+        zbase = static_cast<double*>(w.zs) + (long)blockIdx.x * kSlab * 64;
+    }
     const int lane = threadIdx.x;
     const long tile = (long)blockIdx.x * N * N * 64;
-    CellRun<TT, MT> run(*dm, zlds + lane, p, static_cast<MT*>(w.jsave) + tile + lane,
-                        static_cast<MT*>(w.lu) + tile + lane, !PLAIN);
+    CellRun<TT, MT, kSlabLds ? 3 : 1> run(*dm, zbase + lane, p, static_cast<MT*>(w.jsave) + tile + lane,
+                                        static_cast<MT*>(w.lu) + tile + lane, !PLAIN);
     BdfCounters tot;
     int max_steps_cell = 0;
     long cell = -1;
@@ -300,15 +317,9 @@ chem_lane_kernel(const MechData<TT>* __restrict__ dm, const double* __restrict__
     reduce_counters(tot, max_steps_cell, counters);
 }
 
-// Register-resident per-lane integration is compiled for small mechanisms
-// (the Newton matrix of N = NS-1 unknowns lives in VGPRs).
-constexpr bool kLaneChem = Topo::NS <= 16;
-
 int chem(const ChemLaunch& L) {
     if (L.ncell == 0) return HF_OK;
-    if constexpr (!kLaneChem) {
-        return HF_EINVAL;
-    } else {
+    {
         const ChemParams p{L.rtol, L.atol, L.maxsteps};
         const MD* dm = static_cast<const MD*>(L.mech);
         const int64_t lanes = slot_lanes(L.ncell, L.num_cus, L.waves_per_cu);

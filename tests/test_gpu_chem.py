@@ -126,3 +126,36 @@ def test_scalar_dt_matches_per_cell_dt():
     hf.ChemistryBatch(mech).advance(a, 3e-7)
     hf.ChemistryBatch(mech).advance(b, dev(np.full(128, 3e-7)))
     assert torch.equal(a, b)
+
+
+@pytest.mark.parametrize("mode,precision", [(hf.HF_CHEM_PLAIN, hf.HF_PREC_FP64), (hf.HF_CHEM_BATCHED, hf.HF_PREC_FP64),
+                                            (hf.HF_CHEM_BATCHED, hf.HF_PREC_MIXED)])
+def test_dme_chemistry_matches_oracle(oracle_dme, mode, precision):
+    """DME/air cells of the mixed-precision workload (39 species, N = 38:
+    difference-quotient Jacobian, global-memory Newton matrix and history)."""
+    print(f"[gpu] DME chemistry parity mode={mode} precision={precision} n=128 seed=4", flush=True)
+    mech = hf.Mechanism(hf.DME_SK39, precision=precision)
+    U0, dt = mech.synth_cells(1, 4, 128)
+    ref = U0.copy()
+    r, st, ff, msg = oracle_dme.chem_advance(ref, dt, rtol=ORACLE_RTOL, atol=ORACLE_ATOL)
+    assert r == 0, msg
+    Ud = dev(U0)
+    stats = hf.ChemistryBatch(mech, mode=mode).advance(Ud, dev(dt))
+    out = Ud.cpu().numpy()
+    eT, eY = compare(oracle_dme, out, ref, mech.W, oracle_dme.m["comp"])
+    print(f"[gpu] DME mode={mode} precision={precision}: T {eT:.3e} Y(floor {Y_FLOOR:g}) {eY:.3e}; stats {stats}",
+          flush=True)
+    assert eT < 1e-6 and eY < 1e-6
+    # Element masses: exact to 1e-12 up to the reference's own clip of accepted
+    # species in [-1e-13, 0) to zero (chemistry.cpp:69-73), which may move each
+    # species by 1e-13 in Y. (The oracle at the reference tolerance moves the H
+    # element of these cells by 3e-11 relative through that clip; at rtol
+    # 1e-12 by 2e-14.)
+    m = oracle_dme.m
+    ns = mech.ns
+    rho0 = U0[:ns].sum(0)
+    el0 = (U0[:ns] / m["W"][:, None]).T @ m["comp"]
+    el1 = (out[:ns] / m["W"][:, None]).T @ m["comp"]
+    clip = 1e-13 * rho0[:, None] * (m["comp"] / m["W"][:, None]).sum(0)[None, :]
+    assert np.all(np.abs(el1 - el0) <= 1e-12 * np.abs(el0) + clip)
+    assert np.array_equal(out[ns:], U0[ns:])
