@@ -182,26 +182,34 @@ def main():
 
     dme = None
     if args.dme_cells:
-        # DME/air, 2^18 cells, seed 4 (39 species, N = 38). Tolerances rtol 1e-8, atol 1e-18: at the
+        # DME/air, 2^18 cells, seed 4 (39 species, N = 38). Tolerances rtol 1e-8, atol 1e-20: at the
         # reference's atol 1e-15 the reference algorithm itself rejects this batch (a trace species is
-        # accepted below -1e-13, chemistry.cpp:69-71; the CPU oracle fails at cell 211).
+        # accepted below -1e-13, chemistry.cpp:69-71; the CPU oracle fails at cell 211), and at 1e-18 a
+        # cell of the 2^18 still does; at 1e-20 every trace species stays resolved.
         dme = {}
         for name, prec, mode in (("batched", hf.HF_PREC_MIXED, hf.HF_CHEM_BATCHED),
                                  ("plain", hf.HF_PREC_FP64, hf.HF_CHEM_PLAIN)):
-            md = hf.Mechanism(hf.DME_SK39, atol=1e-18, precision=prec)
+            md = hf.Mechanism(hf.DME_SK39, atol=1e-20, precision=prec)
             if name == "batched":
                 D0, ddt = md.synth_cells(1, 4, args.dme_cells)
                 Dd0 = torch.from_numpy(D0).to("cuda")
                 Dd = Dd0.clone()
                 ddtd = torch.from_numpy(ddt).to("cuda")
             log(f"DME chemistry {name}: {args.dme_cells} cells, 1 step")
-            wall, kms, st = chem_run(torch, hf.ChemistryBatch(md, mode=mode), Dd, Dd0, ddtd, 1, 0)
+            try:
+                wall, kms, st = chem_run(torch, hf.ChemistryBatch(md, mode=mode), Dd, Dd0, ddtd, 1, 0)
+            except hf.HyperflowError as e:
+                # the batch is rejected as a whole (all-or-nothing): report it, keep the other legs
+                dme[name] = {"error": str(e)}
+                log(f"DME chemistry {name}: {e}")
+                continue
             dme[name] = {"value": args.dme_cells / wall, "kernel_ms": float(kms[0]),
                          "steps_per_cell": st["steps"] / args.dme_cells, "rhs_per_step": st["rhs"] / st["steps"]}
             if name == "batched":
                 dme_out = Dd.cpu().numpy()
             log(f"DME chemistry {name}: {dme[name]['value']:.4g} cell-steps/s, kernel {kms[0]:.1f} ms")
-        dme["batched_over_plain"] = dme["batched"]["value"] / dme["plain"]["value"]
+        if "value" in dme["batched"] and "value" in dme["plain"]:
+            dme["batched_over_plain"] = dme["batched"]["value"] / dme["plain"]["value"]
         del Dd, Dd0
 
     log("flux sweep 4096 x 1024")
@@ -251,7 +259,7 @@ def main():
                                        "unit": "GB/s", "frac": f_kernel_gbs / HBM_PEAK_GBS}}
     if dme is not None:
         line["dme_sk39"] = {"workload": f"batched chemistry, {args.dme_cells} DME/air cells (dme_sk39, 39 species), "
-                                        "seed 4, dt log-uniform [1e-8,1e-6] s, rtol 1e-8 atol 1e-18",
+                                        "seed 4, dt log-uniform [1e-8,1e-6] s, rtol 1e-8 atol 1e-20",
                             "unit": "cell-steps/s", "precision": "batched: FP32 Newton matrix and LU; plain: FP64",
                             **dme}
     if not args.no_cpu:
@@ -261,8 +269,9 @@ def main():
         line["cpu_baseline"] = cpu_baseline(U0, dt)
         if dme is not None:
             log("DME parity sample and CPU baseline")
-            line["dme_sk39"]["parity_vs_oracle"] = parity(D0, ddt, dme_out, n=128, mech_path=hf.DME_SK39)
-            line["dme_sk39"]["cpu_baseline"] = cpu_baseline(D0, ddt, atol=1e-18, mech_path=hf.DME_SK39,
+            if "value" in dme["batched"]:
+                line["dme_sk39"]["parity_vs_oracle"] = parity(D0, ddt, dme_out, n=128, mech_path=hf.DME_SK39)
+            line["dme_sk39"]["cpu_baseline"] = cpu_baseline(D0, ddt, atol=1e-20, mech_path=hf.DME_SK39,
                                                             label=f"{args.dme_cells}-cell DME")
     print(json.dumps(line), flush=True)
 
