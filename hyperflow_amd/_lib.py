@@ -14,6 +14,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libhyperflow_hip.so"
 
 HF_OK, HF_EINVAL, HF_EDOMAIN, HF_ECONV, HF_ENOMEM, HF_EDEVICE, HF_EBUSY = range(7)
 HF_CHEM_PLAIN, HF_CHEM_BATCHED = 0, 1
+HF_BC_TRANSMISSIVE, HF_BC_REFLECTING, HF_BC_PERIODIC = 0, 1, 2
 HF_PREC_FP64, HF_PREC_MIXED = 0, 1
 
 ERROR_NAMES = {0: "HF_OK", 1: "HF_EINVAL", 2: "HF_EDOMAIN", 3: "HF_ECONV", 4: "HF_ENOMEM", 5: "HF_EDEVICE",
@@ -71,6 +72,10 @@ _SIGNATURES = {
                                 c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "hf_chem_tangent": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_double, c_int, c_void_p,
                                 c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "hf_fill_ghosts": (c_int, [c_void_p, c_void_p, POINTER(hf_box), POINTER(c_int32)]),
+    "hf_repair_roundoff": (c_int, [c_void_p, c_void_p, POINTER(hf_box), POINTER(c_int64), POINTER(c_int64)]),
+    "hf_validate_box": (c_int, [c_void_p, c_void_p, POINTER(hf_box), c_double, c_double, _dp, POINTER(c_int64)]),
+    "hf_lincomb": (c_int, [c_void_p, c_void_p, c_void_p, c_double, c_double, c_int64]),
     "hf_synth_cells": (c_int, [c_void_p, c_int, c_uint64, c_int64, c_void_p, c_void_p]),
     "hf_equilibrate_tp": (c_int, [c_void_p, c_double, c_double, _dp, _dp]),
     "hf_state_from_TPY": (c_int, [c_void_p, c_double, c_double, _dp, c_double, c_double, _dp]),

@@ -636,6 +636,127 @@ int hf_chem_tangent(hf_ctx* c, int64_t ncell, const double* rho, const double* T
                        nullptr, dJ);
 }
 
+// ------------------------------------------------------------------ single-box driver operations
+namespace {
+
+__global__ void __launch_bounds__(256) lincomb_kernel(double* __restrict__ y, const double* __restrict__ x, double a,
+                                                      double b, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        y[i] = fma(a, y[i], b * x[i]);
+}
+
+int box_common(hf_ctx* c, BoxLaunch& L, const hf_box* box, const void* U) {
+    if (!box || !U) return fail(c, HF_EINVAL, "missing box or array");
+    if (box->nx <= 0 || box->ny <= 0 || box->in_pitch < box->nx + 4 || box->in_plane < box->in_pitch * (box->ny + 4))
+        return fail(c, HF_EINVAL, "invalid box extent");
+    HF_DEVICE(c);
+    if (!is_device_ptr(U)) return fail(c, HF_EINVAL, "box operations need device memory");
+    if (!c->scal.ensure(sizeof(unsigned long long) * 256)) return fail(c, HF_ENOMEM, "device scalars");
+    L.mech = c->d_mech;
+    L.box = *box;
+    L.scal = c->scal.as<unsigned long long>() + 8;
+    L.first_failed = c->scal.as<unsigned long long>();
+    L.stream = c->stream;
+    return HF_OK;
+}
+
+}  // namespace
+
+int hf_fill_ghosts(hf_ctx* c, double* U, const hf_box* box, const int32_t bc[4]) {
+    if (!c) return HF_EINVAL;
+    BusyGuard guard(c);
+    if (!guard.ok) return fail(c, HF_EBUSY, "concurrent call is unsupported");
+    if (!bc) return fail(c, HF_EINVAL, "missing boundary codes");
+    for (int k = 0; k < 4; ++k)
+        if (bc[k] > HF_BC_PERIODIC || (k < 2 && bc[k] < 0)) return fail(c, HF_EINVAL, "unknown boundary code");
+    BoxLaunch L{};
+    const int r = box_common(c, L, box, U);
+    if (r != HF_OK) return r;
+    L.U = U;
+    L.op = BoxLaunch::FILL;
+    for (int k = 0; k < 4; ++k) L.bc[k] = bc[k];
+    L.two_d = bc[2] >= 0 && bc[3] >= 0;
+    if (c->ops->box_ops(L) != HF_OK) return fail(c, HF_EDEVICE, "ghost fill launch failed");
+    HF_HIP(c, hipStreamSynchronize(c->stream));
+    return HF_OK;
+}
+
+int hf_repair_roundoff(hf_ctx* c, double* U, const hf_box* box, int64_t* repairs, int64_t* first_failed) {
+    if (!c) return HF_EINVAL;
+    BusyGuard guard(c);
+    if (!guard.ok) return fail(c, HF_EBUSY, "concurrent call is unsupported");
+    if (first_failed) *first_failed = -1;
+    BoxLaunch L{};
+    const int r = box_common(c, L, box, U);
+    if (r != HF_OK) return r;
+    L.U = U;
+    L.op = BoxLaunch::REPAIR;
+    if (c->ops->box_ops(L) != HF_OK) return fail(c, HF_EDEVICE, "roundoff repair launch failed");
+    const int ns = c->ops->ns;
+    std::vector<unsigned long long> h(3 * ns + 2);
+    unsigned long long ff = ~0ull;
+    HF_HIP(c, hipMemcpyAsync(h.data(), L.scal, sizeof(unsigned long long) * h.size(), hipMemcpyDeviceToHost, c->stream));
+    HF_HIP(c, hipMemcpyAsync(&ff, L.first_failed, sizeof(ff), hipMemcpyDeviceToHost, c->stream));
+    HF_HIP(c, hipStreamSynchronize(c->stream));
+    if (repairs) *repairs = (int64_t)h[3 * ns];
+    if (ff != ~0ull || h[3 * ns + 1]) {
+        if (first_failed) *first_failed = ff == ~0ull ? 0 : (int64_t)ff;
+        return fail(c, HF_EDOMAIN, h[3 * ns + 1] ? "insufficient positive donor for roundoff redistribution"
+                                                 : "materially negative species after flow stage at cell " +
+                                                       std::to_string(ff));
+    }
+    return HF_OK;
+}
+
+int hf_validate_box(hf_ctx* c, const double* U, const hf_box* box, double dx, double dy, double* rate,
+                    int64_t* first_failed) {
+    if (!c) return HF_EINVAL;
+    BusyGuard guard(c);
+    if (!guard.ok) return fail(c, HF_EBUSY, "concurrent call is unsupported");
+    if (first_failed) *first_failed = -1;
+    if (!(dx > 0) || !(dy >= 0)) return fail(c, HF_EINVAL, "invalid cell size");
+    BoxLaunch L{};
+    const int r = box_common(c, L, box, U);
+    if (r != HF_OK) return r;
+    L.U = const_cast<double*>(U);
+    L.op = BoxLaunch::PROPS;
+    L.dx = dx;
+    L.dy = dy;
+    unsigned long long init[2] = {~0ull, 0};
+    HF_HIP(c, hipMemcpyAsync(L.first_failed, &init[0], sizeof(init[0]), hipMemcpyHostToDevice, c->stream));
+    HF_HIP(c, hipMemcpyAsync(L.scal, &init[1], sizeof(init[1]), hipMemcpyHostToDevice, c->stream));
+    if (c->ops->box_ops(L) != HF_OK) return fail(c, HF_EDEVICE, "validation launch failed");
+    unsigned long long out[2];
+    HF_HIP(c, hipMemcpyAsync(&out[0], L.first_failed, sizeof(out[0]), hipMemcpyDeviceToHost, c->stream));
+    HF_HIP(c, hipMemcpyAsync(&out[1], L.scal, sizeof(out[1]), hipMemcpyDeviceToHost, c->stream));
+    HF_HIP(c, hipStreamSynchronize(c->stream));
+    if (out[0] != ~0ull) {
+        if (first_failed) *first_failed = (int64_t)out[0];
+        return fail(c, HF_EDOMAIN, "inadmissible state at cell " + std::to_string(out[0]));
+    }
+    if (rate) {
+        double v;
+        std::memcpy(&v, &out[1], sizeof(v));
+        *rate = v;
+    }
+    return HF_OK;
+}
+
+int hf_lincomb(hf_ctx* c, double* y, const double* x, double a, double b, int64_t n) {
+    if (!c) return HF_EINVAL;
+    BusyGuard guard(c);
+    if (!guard.ok) return fail(c, HF_EBUSY, "concurrent call is unsupported");
+    if (n < 0 || (n > 0 && (!y || !x))) return fail(c, HF_EINVAL, "invalid extent");
+    if (n == 0) return HF_OK;
+    HF_DEVICE(c);
+    if (!is_device_ptr(y) || !is_device_ptr(x)) return fail(c, HF_EINVAL, "lincomb needs device memory");
+    const unsigned grid = (unsigned)std::min<int64_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(lincomb_kernel, dim3(grid), dim3(256), 0, c->stream, y, x, a, b, n);
+    if (hipGetLastError() != hipSuccess) return fail(c, HF_EDEVICE, "lincomb launch failed");
+    HF_HIP(c, hipStreamSynchronize(c->stream));
+    return HF_OK;
+}
+
 int hf_synth_cells(hf_ctx* c, int which, uint64_t seed, int64_t ncell, double* U, double* dt) {
     if (!c) return HF_EINVAL;
     if (ncell < 0 || (ncell > 0 && !U)) return fail(c, HF_EINVAL, "invalid extent");

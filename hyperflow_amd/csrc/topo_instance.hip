@@ -436,6 +436,194 @@ int props(const PropsLaunch& L) {
     return hipGetLastError() == hipSuccess ? HF_OK : HF_EDEVICE;
 }
 
+// ------------------------------------------------------------------ single-box driver operations
+// Ghost fill, roundoff repair and validation over one ghosted SoA box, for a
+// driver that advances a single-level box (BlockAMR::fill / repairRoundoff /
+// validate / stableDt, BlockAMR.cpp:137-152,286-346).
+__device__ __forceinline__ double& box_at(double* U, const hf_box& B, int n, int i, int j) {
+    return U[B.in_offset + n * B.in_plane + (int64_t)(j + 2) * B.in_pitch + (i + 2)];
+}
+
+// Source index of ghost g (outside [0, len)) for a boundary code:
+// HF_BC_TRANSMISSIVE copies the nearest valid cell (AMReX foextrap),
+// HF_BC_REFLECTING mirrors about the face (reflect_even / reflect_odd),
+// HF_BC_PERIODIC wraps.
+__device__ __forceinline__ int ghost_source(int g, int len, int bc) {
+    if (bc == HF_BC_PERIODIC) return ((g % len) + len) % len;
+    if (bc == HF_BC_REFLECTING) return g < 0 ? -g - 1 : 2 * len - g - 1;
+    return g < 0 ? 0 : len - 1;
+}
+
+// pass 0: x ghosts of the valid rows; pass 1: y ghosts of every column
+// (including the x ghosts, so corners come from the filled columns).
+__global__ void __launch_bounds__(256) ghost_kernel(double* __restrict__ U, hf_box B, int4 bc, int pass) {
+    constexpr int NV = Topo::NS + 3;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int len = pass == 0 ? B.nx : B.ny;
+    const int across = pass == 0 ? B.ny : B.nx + 4;
+    if (t >= (long)NV * across * 4) return;
+    const int g4 = (int)(t % 4);
+    const int a = (int)((t / 4) % across);
+    const int n = (int)(t / (4L * across));
+    const int g = g4 < 2 ? g4 - 2 : len + g4 - 2;          // -2, -1, len, len+1
+    const int code = g < 0 ? (pass == 0 ? bc.x : bc.z) : (pass == 0 ? bc.y : bc.w);
+    const int s = ghost_source(g, len, code);
+    const bool odd = code == HF_BC_REFLECTING && n == (pass == 0 ? Topo::NS : Topo::NS + 1);
+    if (pass == 0) {
+        const double v = box_at(U, B, n, s, a);
+        box_at(U, B, n, g, a) = odd ? -v : v;
+    } else {
+        const int i = a - 2;
+        const double v = box_at(U, B, n, i, s);
+        box_at(U, B, n, i, g) = odd ? -v : v;
+    }
+}
+
+// Roundoff repair (BlockAMR::repairRoundoff, BlockAMR.cpp:299-334) on one
+// level of equal cell volumes. scal layout: [0, NS) peak bits, [NS, 2NS)
+// donor index, [2NS, 3NS) deficit (double bits), 3NS repairs, 3NS+1 failure.
+__global__ void __launch_bounds__(256) repair_peak_kernel(const double* __restrict__ U, hf_box B,
+                                                          unsigned long long* __restrict__ scal) {
+    constexpr int NS = Topo::NS;
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= (long)B.nx * B.ny) return;
+    const int i = (int)(c % B.nx), j = (int)(c / B.nx);
+    #pragma unroll
+    for (int n = 0; n < NS; ++n) {
+        const double v = box_at(const_cast<double*>(U), B, n, i, j);
+        // peak starts at 0 (the reference's zero-initialised array): only positive values compete
+        if (v > 0) atomicMax(&scal[n], (unsigned long long)__double_as_longlong(v));
+    }
+}
+// donor: the first cell in BoxIterator order (i fastest) holding the peak mass
+__global__ void __launch_bounds__(256) repair_donor_kernel(const double* __restrict__ U, hf_box B,
+                                                           unsigned long long* __restrict__ scal) {
+    constexpr int NS = Topo::NS;
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= (long)B.nx * B.ny) return;
+    const int i = (int)(c % B.nx), j = (int)(c / B.nx);
+    #pragma unroll
+    for (int n = 0; n < NS; ++n) {
+        const double v = box_at(const_cast<double*>(U), B, n, i, j);
+        if (v > 0 && (unsigned long long)__double_as_longlong(v) == scal[n]) atomicMin(&scal[NS + n], (unsigned long long)c);
+    }
+}
+__global__ void __launch_bounds__(256) repair_zero_kernel(double* __restrict__ U, hf_box B,
+                                                          unsigned long long* __restrict__ scal,
+                                                          unsigned long long* __restrict__ first_failed) {
+    constexpr int NS = Topo::NS;
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= (long)B.nx * B.ny) return;
+    const int i = (int)(c % B.nx), j = (int)(c / B.nx);
+    unsigned long long repairs = 0;
+    #pragma unroll
+    for (int n = 0; n < NS; ++n) {
+        double& v = box_at(U, B, n, i, j);
+        if (v >= 0) continue;
+        const double peak = __longlong_as_double((long long)scal[n]);
+        if (!finite(v) || -v > 64.0 * 2.220446049250313e-16 * peak) {
+            atomicMin(first_failed, (unsigned long long)c);
+            continue;
+        }
+        atomicAdd(reinterpret_cast<double*>(&scal[2 * NS + n]), -v);
+        v = 0.0;
+        ++repairs;
+    }
+    if (repairs) atomicAdd(&scal[3 * NS], repairs);
+}
+__global__ void repair_donate_kernel(double* __restrict__ U, hf_box B, unsigned long long* __restrict__ scal,
+                                     unsigned long long* __restrict__ first_failed) {
+    constexpr int NS = Topo::NS;
+    const int n = threadIdx.x;
+    if (n >= NS) return;
+    const double deficit = __longlong_as_double((long long)scal[2 * NS + n]);
+    if (!(deficit > 0)) return;
+    const double peak = __longlong_as_double((long long)scal[n]);
+    const unsigned long long d = scal[NS + n];
+    if (!(peak > deficit) || d == ~0ull) {
+        atomicMin(first_failed, d == ~0ull ? 0ull : d);
+        scal[3 * NS + 1] = 1;
+        return;
+    }
+    box_at(U, B, n, (int)(d % B.nx), (int)(d / B.nx)) -= deficit;
+}
+__global__ void repair_init_kernel(unsigned long long* __restrict__ scal, unsigned long long* __restrict__ first_failed) {
+    constexpr int NS = Topo::NS;
+    const int t = threadIdx.x;
+    if (t < NS) {
+        scal[t] = 0;                 // peak 0.0
+        scal[NS + t] = ~0ull;        // no donor
+        scal[2 * NS + t] = 0;        // deficit 0.0
+    }
+    if (t == 0) {
+        scal[3 * NS] = 0;
+        scal[3 * NS + 1] = 0;
+        *first_failed = ~0ull;
+    }
+}
+
+// validate + stableDt over the box interior: properties of every valid cell
+// (lowest failing index in first_failed) and the max CFL rate.
+__global__ void __launch_bounds__(256) box_props_kernel(const MD* __restrict__ dm, const double* __restrict__ U,
+                                                        hf_box B, double dx, double dy,
+                                                        unsigned long long* __restrict__ first_failed,
+                                                        unsigned long long* __restrict__ rate_max) {
+    constexpr int NS = Topo::NS, NV = NS + 3;
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double rate = 0;
+    if (c < (long)B.nx * B.ny) {
+        const int i = (int)(c % B.nx), j = (int)(c / B.nx);
+        double u[NV];
+        #pragma unroll
+        for (int n = 0; n < NV; ++n) u[n] = box_at(const_cast<double*>(U), B, n, i, j);
+        Props p;
+        if (!properties(*dm, u, p, 1000.0)) {
+            atomicMin(first_failed, (unsigned long long)c);
+        } else {
+            double rho = 0;
+            #pragma unroll
+            for (int k = 0; k < NS; ++k) rho += u[k];
+            rate = (fabs(u[NS] / rho) + p.a) / dx + (dy > 0 ? (fabs(u[NS + 1] / rho) + p.a) / dy : 0.0);
+        }
+    }
+    #pragma unroll
+    for (int off = 32; off > 0; off >>= 1) rate = fmax(rate, __shfl_xor(rate, off, 64));
+    if ((threadIdx.x & 63) == 0 && rate > 0) atomicMax(rate_max, (unsigned long long)__double_as_longlong(rate));
+}
+
+int box_ops(const BoxLaunch& L) {
+    const long cells = (long)L.box.nx * L.box.ny;
+    const unsigned g256 = (unsigned)((cells + 255) / 256);
+    const MD* dm = static_cast<const MD*>(L.mech);
+    switch (L.op) {
+    case BoxLaunch::FILL: {
+        constexpr int NV = Topo::NS + 3;
+        const int4 bc = make_int4(L.bc[0], L.bc[1], L.bc[2], L.bc[3]);
+        const long t0 = (long)NV * L.box.ny * 4;
+        hipLaunchKernelGGL(ghost_kernel, dim3((unsigned)((t0 + 255) / 256)), dim3(256), 0, L.stream, L.U, L.box, bc, 0);
+        if (L.two_d) {
+            const long t1 = (long)NV * (L.box.nx + 4) * 4;
+            hipLaunchKernelGGL(ghost_kernel, dim3((unsigned)((t1 + 255) / 256)), dim3(256), 0, L.stream, L.U, L.box, bc,
+                               1);
+        }
+        break;
+    }
+    case BoxLaunch::REPAIR:
+        hipLaunchKernelGGL(repair_init_kernel, dim3(1), dim3(64), 0, L.stream, L.scal, L.first_failed);
+        hipLaunchKernelGGL(repair_peak_kernel, dim3(g256), dim3(256), 0, L.stream, L.U, L.box, L.scal);
+        hipLaunchKernelGGL(repair_donor_kernel, dim3(g256), dim3(256), 0, L.stream, L.U, L.box, L.scal);
+        hipLaunchKernelGGL(repair_zero_kernel, dim3(g256), dim3(256), 0, L.stream, L.U, L.box, L.scal,
+                           L.first_failed);
+        hipLaunchKernelGGL(repair_donate_kernel, dim3(1), dim3(64), 0, L.stream, L.U, L.box, L.scal, L.first_failed);
+        break;
+    case BoxLaunch::PROPS:
+        hipLaunchKernelGGL(box_props_kernel, dim3(g256), dim3(256), 0, L.stream, dm, L.U, L.box, L.dx, L.dy,
+                           L.first_failed, L.scal);
+        break;
+    }
+    return hipGetLastError() == hipSuccess ? HF_OK : HF_EDEVICE;
+}
+
 // ------------------------------------------------------------------ sensitivities
 // Fixed-step backward-Euler chemistry with its discrete adjoint (reverse
 // sweep over checkpointed segments) and its forward tangent, one lane per
@@ -696,7 +884,8 @@ int tangent(const AdjointLaunch& L) { return sens(L, true); }
 #define HF_OPS_NAME(x) HF_OPS_NAME2(x)
 extern const TopoOps HF_OPS_NAME(HF_TOPO);
 const TopoOps HF_OPS_NAME(HF_TOPO) = {HF_STR(HF_TOPO), Topo::topology, Topo::NS, Topo::NR, sizeof(MD), fill, chem,
-                                      chem_workspace, flux, props, adjoint, tangent, sens_workspace};
+                                      chem_workspace, flux, props, adjoint, tangent, sens_workspace,
+                                      box_ops};
 #endif
 
 }  // namespace hf

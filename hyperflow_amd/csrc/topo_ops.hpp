@@ -79,6 +79,22 @@ struct AdjointLaunch {
     hipStream_t stream;
 };
 
+// Single-box driver operations (ghost fill, roundoff repair, validation and
+// CFL rate) on a ghosted box in device memory.
+struct BoxLaunch {
+    enum Op { FILL, REPAIR, PROPS };
+    const void* mech;
+    double* U;
+    hf_box box;
+    int op;
+    int bc[4];                      // x-lo, x-hi, y-lo, y-hi boundary codes (y < 0: no y ghosts)
+    int two_d;
+    double dx, dy;
+    unsigned long long* scal;       // device scratch (repair: 3 NS + 2 words; props: rate bits)
+    unsigned long long* first_failed;
+    hipStream_t stream;
+};
+
 struct TopoOps {
     const char* name;
     const char* topology;
@@ -92,6 +108,7 @@ struct TopoOps {
     int (*adjoint)(const AdjointLaunch& L);
     int (*tangent)(const AdjointLaunch& L);
     size_t (*sens_workspace)(int64_t ncell, int nsteps, int ckpt_every);
+    int (*box_ops)(const BoxLaunch& L);
 };
 
 // All compiled topologies (null-terminated).

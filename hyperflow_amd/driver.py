@@ -1,0 +1,192 @@
+"""Strang-split reacting-flow steps on one single-level box, on the GPU.
+
+Mirrors the reference's step loop for a single level
+(HyperFlow_V3/src/BlockAMR.cpp):
+  run      :400-425  dt = min(target - t, stableDt()); flow(dt/2); react(dt);
+                     flow(dt/2); on any failure restore the state, halve dt
+                     and retry (18 attempts)
+  flow     :266-270  Heun / SSP-RK2: U1 = E(U), U2 = E(U1), U = U/2 + U2/2,
+                     then validate
+  euler    :202-265  ghost fill, NND + HLLC stage, repairRoundoff, validate;
+                     on domain_error redo the stage with first-order faces
+  react    :271-285  ChemistryBatch::advance over every valid cell
+  stableDt :335-346  cfl / max((|u|+a)/dx + (|v|+a)/dy)
+Every arithmetic step runs in libhyperflow_hip.so (hf_fill_ghosts,
+hf_flux_stage, hf_repair_roundoff, hf_validate_box, hf_lincomb,
+hf_chem_advance); torch only holds the device arrays.
+
+A box with ny = 1 and dy = 0 is one-dimensional: x faces only and an x-only
+CFL rate (the reference itself needs ny >= 2 with reflecting y walls).
+"""
+from __future__ import annotations
+
+from ctypes import byref, c_double, c_int32, c_int64
+
+from ._lib import (HF_BC_PERIODIC, HF_BC_REFLECTING, HF_BC_TRANSMISSIVE, HF_CHEM_BATCHED, HF_EDOMAIN, HF_OK,  # noqa: F401
+                   HyperflowError, check, hf_box, hf_flux_stats, load)
+from .chemistry import ChemistryBatch
+
+
+class BoxDriver:
+    """Advances the conserved state of one ghosted box.
+
+    bc: (x_lo, x_hi, y_lo, y_hi) boundary codes; y codes are ignored for a
+    1-D box. Reflecting sides are walls: their face fluxes of species,
+    tangential momentum and energy are zeroed (BlockAMR.cpp:224-229).
+    """
+
+    def __init__(self, mechanism, nx, ny, dx, dy, bc, cfl=0.5, chem_mode=HF_CHEM_BATCHED, max_attempts=18):
+        import torch
+        self.torch = torch
+        self.mech = mechanism
+        self.nx, self.ny, self.dx, self.dy = int(nx), int(ny), float(dx), float(dy)
+        self.one_d = self.ny == 1 and self.dy == 0.0
+        self.cfl = float(cfl)
+        self.max_attempts = max_attempts
+        nv = mechanism.nv
+        shape = (nv, self.ny + 4, self.nx + 4)
+        self.U = torch.zeros(shape, dtype=torch.float64, device="cuda")
+        self.U1 = torch.zeros_like(self.U)
+        self.U2 = torch.zeros_like(self.U)
+        self.old = torch.zeros_like(self.U)
+        self.backup = torch.zeros_like(self.U)
+        bc = list(bc)
+        if self.one_d:
+            bc[2] = bc[3] = -1
+        self.bc = (c_int32 * 4)(*bc)
+        walls = 0
+        for k in range(4):
+            if bc[k] == HF_BC_REFLECTING:
+                walls |= 1 << k
+        self.walls = walls
+        self.chem = ChemistryBatch(mechanism, mode=chem_mode)
+        self.time = 0.0
+        self.steps = 0
+        self.retries = 0
+        self.first_order_stages = 0
+        self.repairs = 0
+        self._h = mechanism.handle
+
+    # ------------------------------------------------------------ state
+    def interior(self, A=None):
+        A = self.U if A is None else A
+        return A[:, 2:self.ny + 2, 2:self.nx + 2]
+
+    def set_state(self, Q):
+        """Q: (nv, ny, nx) conserved values (numpy or torch)."""
+        t = self.torch
+        Qt = Q if isinstance(Q, t.Tensor) else t.from_numpy(Q)
+        self.interior().copy_(Qt.reshape(self.interior().shape))
+
+    def state(self):
+        return self.interior().cpu().numpy().copy()
+
+    def _box(self, h=0.0):
+        gx, gy = self.nx + 4, self.ny + 4
+        plane = gx * gy
+        return hf_box(self.nx, self.ny, self.walls, 0, 0, gx, plane, 2 * gx + 2, gx, plane, h / self.dx,
+                      0.0 if self.one_d else h / self.dy)
+
+    # ------------------------------------------------------------ stages
+    def _fill(self, A):
+        box = self._box()
+        check(self._h, load().hf_fill_ghosts(self._h, A.data_ptr(), byref(box), self.bc))
+
+    def _validate(self, A):
+        box = self._box()
+        rate, ff = c_double(0.0), c_int64(-1)
+        r = load().hf_validate_box(self._h, A.data_ptr(), byref(box), self.dx, 0.0 if self.one_d else self.dy,
+                                   byref(rate), byref(ff))
+        check(self._h, r, ff.value)
+        return rate.value
+
+    def _stage_once(self, src, dst, h, first_order):
+        box = self._box(h)
+        st, ff = hf_flux_stats(), c_int64(-1)
+        r = load().hf_flux_stage(self._h, byref(box), 1, src.data_ptr(), dst.data_ptr(), int(first_order), byref(st),
+                                 byref(ff))
+        # HF_EDOMAIN here only says an updated cell is inadmissible as written:
+        # the reference repairs roundoff before validating, so continue.
+        if r not in (HF_OK, HF_EDOMAIN):
+            check(self._h, r, ff.value)
+        rep, ff2 = c_int64(0), c_int64(-1)
+        r = load().hf_repair_roundoff(self._h, dst.data_ptr(), byref(box), byref(rep), byref(ff2))
+        if r == HF_OK:
+            self.repairs += rep.value
+            r = load().hf_validate_box(self._h, dst.data_ptr(), byref(box), self.dx, 0.0 if self.one_d else self.dy,
+                                       None, byref(ff2))
+        return r, ff2.value
+
+    def euler(self, src, dst, h):
+        """One Euler stage src -> dst (interior), with the first-order redo."""
+        self._fill(src)
+        r, ff = self._stage_once(src, dst, h, False)
+        if r == HF_EDOMAIN:
+            self.first_order_stages += 1
+            r, ff = self._stage_once(src, dst, h, True)
+        check(self._h, r, ff)
+
+    def flow(self, h):
+        self.old.copy_(self.U)
+        self.euler(self.U, self.U1, h)
+        self.euler(self.U1, self.U2, h)
+        n = self.U.numel()
+        check(self._h, load().hf_lincomb(self._h, self.U2.data_ptr(), self.old.data_ptr(), 0.5, 0.5, n))
+        self.U, self.U2 = self.U2, self.U
+        self._validate(self.U)
+
+    def react(self, h):
+        if self.one_d:
+            view = self.U[:, 2, 2:self.nx + 2]          # (nv, nx), row-contiguous, ld = plane
+            self.chem.advance(view, h)
+        else:
+            cells = self.interior().reshape(self.mech.nv, -1).contiguous()
+            self.chem.advance(cells, h)
+            self.interior().copy_(cells.reshape(self.interior().shape))
+
+    def stable_dt(self):
+        return self.cfl / self._validate(self.U)
+
+    def step(self, target=None):
+        """One Strang step; returns the dt taken."""
+        dt = self.stable_dt()
+        if target is not None:
+            dt = min(target - self.time, dt)
+        self.backup.copy_(self.U)
+        for _ in range(self.max_attempts):
+            try:
+                self.flow(0.5 * dt)
+                self.react(dt)
+                self.flow(0.5 * dt)
+                break
+            except HyperflowError:
+                self.U.copy_(self.backup)
+                self.retries += 1
+                dt *= 0.5
+        else:
+            raise RuntimeError("time advancement failed")
+        self.time = target if target is not None and dt >= target - self.time else self.time + dt
+        self.steps += 1
+        return dt
+
+
+def shock_tube_1d(mech, nx=2000, length=0.1, hot_x=0.01, hot=(2000.0, 20.0), cold=(300.0, 1.0)):
+    """The 1-D plain workload: stoichiometric H2/air (X H2:O2:N2 = 2:1:3.76,
+    BlockAMR.cpp:67); x < hot_x holds the TP-equilibrium composition at
+    (T, p atm) = hot, the rest the unburnt mixture at cold; at rest.
+    Returns (Q (nv, 1, nx), dx)."""
+    import numpy as np
+    atm = 101325.0
+    X = np.zeros(mech.ns)
+    for name, x in (("H2", 2.0), ("O2", 1.0), ("N2", 3.76)):
+        X[mech.species.index(name)] = x
+    X /= X.sum()
+    W = mech.W
+    Y_fresh = X * W / (X * W).sum()
+    Y_hot = mech.equilibrate_tp(hot[0], hot[1] * atm, X)
+    q_hot = mech.state_from_TPY(hot[0], hot[1] * atm, Y_hot)
+    q_cold = mech.state_from_TPY(cold[0], cold[1] * atm, Y_fresh)
+    dx = length / nx
+    xc = (np.arange(nx) + 0.5) * dx
+    Q = np.where((xc < hot_x)[None, :], q_hot[:, None], q_cold[:, None])
+    return Q.reshape(mech.nv, 1, nx).copy(), dx

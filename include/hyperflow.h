@@ -173,6 +173,37 @@ int hf_chem_tangent(hf_ctx *ctx, int64_t ncell, const double *rho, const double 
                     int nsteps, const double *lnA_shift, const double *v_lnA, const double *v_T0,
                     const double *v_Y0, double *J, double *dJ);
 
+/* ------------------------------------------------------------------ single-box driver operations
+ * Device-memory helpers for a driver that advances one single-level ghosted
+ * box (the hf_box input layout; the output fields of the box are unused).
+ * All pointers are device pointers (HF_EINVAL otherwise). */
+enum { HF_BC_TRANSMISSIVE = 0, HF_BC_REFLECTING = 1, HF_BC_PERIODIC = 2 };
+
+/* Fill the two ghost layers of U (BlockAMR::fill, BlockAMR.cpp:137-152, with
+ * the boundary types of BlockAMR.cpp:48-59): transmissive copies the nearest
+ * valid cell (AMReX foextrap); reflecting mirrors about the face, negating the
+ * normal momentum (reflect_even / reflect_odd); periodic wraps. bc = {x-lo,
+ * x-hi, y-lo, y-hi}; y-lo < 0 fills x ghosts only (a 1-D box). */
+int hf_fill_ghosts(hf_ctx *ctx, double *U, const hf_box *box, const int32_t bc[4]);
+
+/* BlockAMR::repairRoundoff (BlockAMR.cpp:299-334) on the valid cells of one
+ * box of equal cell volumes: negative species within 64 eps of the species'
+ * peak are zeroed and their total is taken from the first cell holding the
+ * peak; a larger negative value, a non-finite one or a missing donor returns
+ * HF_EDOMAIN with the cell (j*nx + i) in first_failed. repairs (optional)
+ * receives the number of zeroed values. */
+int hf_repair_roundoff(hf_ctx *ctx, double *U, const hf_box *box, int64_t *repairs, int64_t *first_failed);
+
+/* BlockAMR::validate and stableDt (BlockAMR.cpp:286-298,335-346) over the
+ * valid cells of one box: HF_EDOMAIN with the lowest inadmissible cell in
+ * first_failed, else *rate = max((|u|+a)/dx + (|v|+a)/dy) (dy = 0: x only). */
+int hf_validate_box(hf_ctx *ctx, const double *U, const hf_box *box, double dx, double dy, double *rate,
+                    int64_t *first_failed);
+
+/* y = a y + b x over n doubles (MultiFab::LinComb of the Heun average,
+ * BlockAMR.cpp:268). */
+int hf_lincomb(hf_ctx *ctx, double *y, const double *x, double a, double b, int64_t n);
+
 /* Synthetic inputs (no reference counterpart). Seeded splitmix64/xoshiro256**
  * cells for the batched-chemistry workloads. Per cell, in draw order:
  * T, ln p, phi, f, ln dt (uniform doubles u = (x >> 11) 2^-53).

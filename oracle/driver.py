@@ -1,0 +1,165 @@
+"""CPU restatement of the reference's single-level step loop — TEST
+INFRASTRUCTURE ONLY (the checker of hyperflow_amd.driver.BoxDriver).
+
+Follows HyperFlow_V3/src/BlockAMR.cpp:
+  fill            :137-152 with the boundary types of :48-59 (AMReX foextrap,
+                  reflect_even / reflect_odd with the normal momentum odd,
+                  periodic)
+  euler           :202-265 (stage, repairRoundoff, validate, first-order redo)
+  repairRoundoff  :299-334
+  flow            :266-270 (Heun / SSP-RK2 average, validate)
+  react           :271-285
+  stableDt        :335-346
+  run             :400-425 (dt, Strang splitting, halve-and-retry)
+on one box of equal cell volumes, with the oracle's flux stage, properties and
+chemistry. Parity status: pinned by the functions it composes (flux stage,
+properties, chemistry: see the package docstring) and, for the composition
+itself, by the tests in tests/test_oracle_driver.py (conservation on a
+periodic box, uniform state preserved, wall symmetry, repair semantics).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+TRANSMISSIVE, REFLECTING, PERIODIC = 0, 1, 2
+EPS = np.finfo(float).eps
+
+
+class DomainError(Exception):
+    pass
+
+
+def _src(g, n, bc):
+    if bc == PERIODIC:
+        return g % n
+    if bc == REFLECTING:
+        return -g - 1 if g < 0 else 2 * n - g - 1
+    return 0 if g < 0 else n - 1
+
+
+def fill_ghosts(G, ns, bc, one_d):
+    """Two ghost layers of G (nv, ny+4, nx+4) from its valid cells, in place."""
+    nv, gy, gx = G.shape
+    nx, ny = gx - 4, gy - 4
+    for g in (-2, -1, nx, nx + 1):
+        code = bc[0] if g < 0 else bc[1]
+        s = _src(g, nx, code)
+        G[:, 2:ny + 2, g + 2] = G[:, 2:ny + 2, s + 2]
+        if code == REFLECTING:
+            G[ns, 2:ny + 2, g + 2] *= -1.0
+    if one_d:
+        return
+    for g in (-2, -1, ny, ny + 1):
+        code = bc[2] if g < 0 else bc[3]
+        s = _src(g, ny, code)
+        G[:, g + 2, :] = G[:, s + 2, :]
+        if code == REFLECTING:
+            G[ns + 1, g + 2, :] *= -1.0
+
+
+def repair_roundoff(Q, ns):
+    """repairRoundoff on Q (nv, ny, nx) of equal cell volumes, in place."""
+    rho = Q[:ns]
+    flat = rho.reshape(ns, -1)
+    repairs = 0
+    peak = np.maximum(flat.max(1), 0.0)
+    donors = []
+    for n in range(ns):
+        pos = np.flatnonzero((flat[n] > 0) & (flat[n] == peak[n]))
+        donors.append(pos[0] if pos.size else -1)
+    deficit = np.zeros(ns)
+    for n in range(ns):
+        neg = np.flatnonzero(~(flat[n] >= 0))
+        for c in neg:
+            v = flat[n, c]
+            if not np.isfinite(v) or -v > 64 * EPS * peak[n]:
+                raise DomainError("materially negative species after composite flow stage")
+            deficit[n] += -v
+            flat[n, c] = 0.0
+            repairs += 1
+    for n in range(ns):
+        if deficit[n] > 0:
+            if donors[n] < 0 or not (peak[n] > deficit[n]):
+                raise DomainError("insufficient positive donor for roundoff redistribution")
+            flat[n, donors[n]] -= deficit[n]
+    Q[:ns] = flat.reshape(Q[:ns].shape)
+    return repairs
+
+
+class OracleBoxDriver:
+    def __init__(self, oracle, nx, ny, dx, dy, bc, cfl=0.5, rtol=1e-12, atol=1e-20, max_attempts=18):
+        self.o = oracle
+        self.ns = oracle.ns
+        self.nv = self.ns + 3
+        self.nx, self.ny, self.dx, self.dy = nx, ny, dx, dy
+        self.one_d = ny == 1 and dy == 0.0
+        self.bc = list(bc)
+        self.walls = sum(1 << k for k in range(4) if bc[k] == REFLECTING and (k < 2 or not self.one_d))
+        self.cfl, self.rtol, self.atol, self.max_attempts = cfl, rtol, atol, max_attempts
+        self.Q = np.zeros((self.nv, ny, nx))
+        self.time, self.steps, self.retries, self.first_order_stages = 0.0, 0, 0, 0
+
+    def _ghosted(self, Q):
+        G = np.zeros((self.nv, self.ny + 4, self.nx + 4))
+        G[:, 2:self.ny + 2, 2:self.nx + 2] = Q
+        fill_ghosts(G, self.ns, self.bc, self.one_d)
+        return G
+
+    def _validate(self, Q):
+        p, a, T, st = self.o.properties(Q.reshape(self.nv, -1).T.copy())
+        if np.any(st != 0):
+            raise DomainError("inadmissible state")
+        rho = Q[:self.ns].sum(0).reshape(-1)
+        rate = (np.abs(Q[self.ns].reshape(-1) / rho) + a) / self.dx
+        if not self.one_d:
+            rate = rate + (np.abs(Q[self.ns + 1].reshape(-1) / rho) + a) / self.dy
+        return rate.max()
+
+    def euler(self, Q, h):
+        G = self._ghosted(Q)
+        for first_order in (False, True):
+            r, out, fb, ff = self.o.flux_stage(G, h / self.dx, 0.0 if self.one_d else h / self.dy,
+                                               first_order=first_order, walls=self.walls,
+                                               dims=1 if self.one_d else 2)
+            try:
+                repair_roundoff(out, self.ns)
+                self._validate(out)
+                return out
+            except DomainError:
+                if first_order:
+                    raise
+                self.first_order_stages += 1
+        raise DomainError("unreachable")
+
+    def flow(self, h):
+        old = self.Q.copy()
+        Q1 = self.euler(self.Q, h)
+        Q2 = self.euler(Q1, h)
+        self.Q = 0.5 * old + 0.5 * Q2
+        self._validate(self.Q)
+
+    def react(self, h):
+        cells = self.Q.reshape(self.nv, -1).copy()
+        r, st, ff, msg = self.o.chem_advance(cells, np.full(cells.shape[1], h), rtol=self.rtol, atol=self.atol)
+        if r != 0:
+            raise DomainError(msg)
+        self.Q = cells.reshape(self.Q.shape)
+
+    def step(self):
+        dt = self.cfl / self._validate(self.Q)
+        backup = self.Q.copy()
+        for _ in range(self.max_attempts):
+            try:
+                self.flow(0.5 * dt)
+                self.react(dt)
+                self.flow(0.5 * dt)
+                break
+            except DomainError:
+                self.Q = backup.copy()
+                self.retries += 1
+                dt *= 0.5
+        else:
+            raise RuntimeError("time advancement failed")
+        self.time += dt
+        self.steps += 1
+        return dt

@@ -1,0 +1,112 @@
+"""GPU single-box Strang driver (hyperflow_amd.driver.BoxDriver over the C
+ABI) against the oracle's restatement of the same loop (oracle/driver.py),
+and its building blocks one by one.
+
+Tolerances: ghost fill exact; roundoff repair to 1e-15 of the donor; the
+driven state after a few Strang steps to 1e-8 of each component's scale,
+with both sides' chemistry run converged (rtol 1e-12, atol 1e-20) so that the
+comparison tests the flow, splitting and boundary logic rather than two
+integrators' truncation errors.
+"""
+import numpy as np
+import pytest
+
+import hyperflow_amd as hf
+from hyperflow_amd.driver import (HF_BC_PERIODIC, HF_BC_REFLECTING, HF_BC_TRANSMISSIVE, BoxDriver,
+                                  shock_tube_1d)
+from oracle.driver import OracleBoxDriver, fill_ghosts, repair_roundoff
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("bc,one_d", [((HF_BC_REFLECTING, HF_BC_TRANSMISSIVE, -1, -1), True),
+                                      ((HF_BC_TRANSMISSIVE, HF_BC_REFLECTING, HF_BC_REFLECTING, HF_BC_PERIODIC), False),
+                                      ((HF_BC_PERIODIC, HF_BC_PERIODIC, HF_BC_TRANSMISSIVE, HF_BC_REFLECTING), False)])
+def test_ghost_fill_matches_oracle(mech_h2, bc, one_d):
+    print(f"[gpu] ghost fill bc={bc} one_d={one_d}", flush=True)
+    nx, ny = 11, (1 if one_d else 7)
+    d = BoxDriver(mech_h2, nx, ny, 1e-4, 0.0 if one_d else 1e-4, bc)
+    G = np.random.default_rng(4).standard_normal(tuple(d.U.shape))
+    d.U.copy_(torch.from_numpy(G))
+    d._fill(d.U)
+    fill_ghosts(G, mech_h2.ns, bc, one_d)
+    out = d.U.cpu().numpy()
+    rows = slice(2, ny + 2) if one_d else slice(None)
+    np.testing.assert_array_equal(out[:, rows, :], G[:, rows, :])
+
+
+def test_repair_roundoff_matches_oracle(mech_h2):
+    print("[gpu] repair roundoff", flush=True)
+    nx, ny, ns = 9, 6, mech_h2.ns
+    d = BoxDriver(mech_h2, nx, ny, 1e-4, 1e-4, (HF_BC_TRANSMISSIVE,) * 4)
+    Q = np.abs(np.random.default_rng(5).standard_normal((mech_h2.nv, ny, nx)))
+    eps = np.finfo(float).eps
+    for n, cells in ((2, [(1, 1), (4, 7)]), (5, [(0, 0)]), (7, [(5, 8), (2, 3), (3, 3)])):
+        for (j, i) in cells:
+            Q[n, j, i] = -(3 + j) * eps * Q[n].max()
+    d.set_state(Q)
+    from ctypes import byref, c_int64
+    from hyperflow_amd._lib import load
+    box = d._box()
+    rep, ff = c_int64(0), c_int64(-1)
+    assert load().hf_repair_roundoff(d._h, d.U.data_ptr(), byref(box), byref(rep), byref(ff)) == 0
+    n_or = repair_roundoff(Q, ns)
+    assert rep.value == n_or == 6
+    np.testing.assert_allclose(d.state(), Q, rtol=1e-15, atol=0)
+    Q2 = d.state()
+    Q2[1, 2, 2] = -1e-9 * Q2[1].max()
+    d.set_state(Q2)
+    r = load().hf_repair_roundoff(d._h, d.U.data_ptr(), byref(box), byref(rep), byref(ff))
+    assert r == hf._lib.HF_EDOMAIN and ff.value == 2 * nx + 2
+
+
+def _compare(a, b, rel, ns):
+    """Per-component error over the component's scale; species scales are
+    floored at 1e-6 of the largest density (values below the chemistry's
+    atol carry no information, as in the chemistry parity tests)."""
+    scale = np.abs(b).max(axis=tuple(range(1, b.ndim)), keepdims=True)
+    rho = b[:ns].sum(0).max()
+    scale[:ns] = np.maximum(scale[:ns], 1e-6 * rho)
+    err = np.abs(a - b) / np.maximum(scale, 1e-300)
+    assert err.max() <= rel, (err.max(), np.unravel_index(err.argmax(), err.shape))
+
+
+def test_shock_tube_1d_matches_oracle(mech_h2, oracle_h2):
+    """The 1-D workload (2000 cells, 0.1 m, hot equilibrium slab at 2000 K /
+    20 atm next to a reflecting wall, transmissive outflow): 3 Strang steps."""
+    print("[gpu] 1-D Strang driver: 2000 cells, 3 steps", flush=True)
+    Q, dx = shock_tube_1d(mech_h2)
+    m = hf.Mechanism(hf.H2_LI, rtol=1e-12, atol=1e-20)
+    g = BoxDriver(m, 2000, 1, dx, 0.0, (HF_BC_REFLECTING, HF_BC_TRANSMISSIVE, -1, -1), cfl=0.5)
+    o = OracleBoxDriver(oracle_h2, 2000, 1, dx, 0.0, (HF_BC_REFLECTING, HF_BC_TRANSMISSIVE, -1, -1), cfl=0.5)
+    g.set_state(Q)
+    o.Q[:] = Q
+    for _ in range(3):
+        dtg, dto = g.step(), o.step()
+        assert dtg == pytest.approx(dto, rel=1e-12)
+    print(f"[gpu] 1-D driver: t={g.time:.4e} first-order stages {g.first_order_stages}/{o.first_order_stages} "
+          f"repairs {g.repairs} retries {g.retries}/{o.retries}", flush=True)
+    _compare(g.state(), o.Q, 1e-8, mech_h2.ns)
+
+
+def test_box_2d_matches_oracle(mech_h2, oracle_h2):
+    """A 2-D box: hot high-pressure square in a cold mixture, reflecting and
+    transmissive x sides, reflecting / periodic y: 2 Strang steps."""
+    print("[gpu] 2-D Strang driver: 40 x 24 box, 2 steps", flush=True)
+    nx, ny, dx = 40, 24, 1e-4
+    bc = (HF_BC_TRANSMISSIVE, HF_BC_REFLECTING, HF_BC_REFLECTING, HF_BC_PERIODIC)
+    Q1, _ = shock_tube_1d(mech_h2, nx=4, length=0.04)   # cell 0 hot (x < 0.01 m), cell 3 cold
+    cold, hot = Q1[:, 0, 3], Q1[:, 0, 0]
+    assert hot[:mech_h2.ns].sum() != cold[:mech_h2.ns].sum()
+    Q = np.repeat(np.repeat(cold[:, None, None], ny, 1), nx, 2)
+    Q[:, 8:16, 10:20] = hot[:, None, None]
+    m = hf.Mechanism(hf.H2_LI, rtol=1e-12, atol=1e-20)
+    g = BoxDriver(m, nx, ny, dx, dx, bc, cfl=0.5)
+    o = OracleBoxDriver(oracle_h2, nx, ny, dx, dx, bc, cfl=0.5)
+    g.set_state(Q)
+    o.Q[:] = Q
+    for _ in range(2):
+        dtg, dto = g.step(), o.step()
+        assert dtg == pytest.approx(dto, rel=1e-12)
+    _compare(g.state(), o.Q, 1e-8, mech_h2.ns)
