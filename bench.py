@@ -46,14 +46,16 @@ def chem_run(torch, batch, Ud, Ud0, dtd, steps, warmup):
         Ud.copy_(Ud0)
         batch.advance(Ud, dtd)
     torch.cuda.synchronize()
-    kernel_ms, stats = [], None
+    kernel_ms, chunks, stats = [], [], None
     t0 = time.perf_counter()
     for _ in range(steps):
         Ud.copy_(Ud0)
         stats = batch.advance(Ud, dtd)
         kernel_ms.append(stats["kernel_ms"])
+        chunks.append(stats["chunk"])
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    stats = dict(stats, chunks=chunks)
     return wall, kernel_ms, stats
 
 
@@ -251,6 +253,7 @@ def main():
     if "plain" in res:
         line["batched_over_plain"] = head["value"] / res["plain"]["value"]
     line["batched_kernel_ms"] = float(np.mean(head["kernel_ms"]))
+    line["batched_chunk_per_step"] = head["stats"]["chunks"]
     line["batched_stats"] = {k: head["stats"][k] for k in ("steps", "rhs", "jac", "lu", "newton",
                                                              "conv_failures", "err_failures", "max_cell_steps")}
     line["flux_sweep"] = {"workload": "one Euler stage, 4096 x 1024 box, H2_Li", "Mcell_updates_per_s": f_upd / 1e6,

@@ -23,13 +23,14 @@ ERROR_NAMES = {0: "HF_OK", 1: "HF_EINVAL", 2: "HF_EDOMAIN", 3: "HF_ECONV", 4: "H
 
 class hf_options(Structure):
     _fields_ = [("rtol", c_double), ("atol", c_double), ("max_steps", c_int), ("precision", c_int),
-                ("device", c_int), ("waves_per_cu", c_int), ("chunk", c_int), ("nbins", c_int)]
+                ("device", c_int), ("waves_per_cu", c_int), ("chunk", c_int), ("nbins", c_int), ("eager", c_int)]
 
 
 class hf_chem_stats(Structure):
     _fields_ = [("cells", c_int64), ("steps", c_int64), ("rhs", c_int64), ("jac", c_int64), ("lu", c_int64),
                 ("newton", c_int64), ("conv_failures", c_int64), ("err_failures", c_int64),
-                ("max_cell_steps", c_int64), ("team", c_int), ("kernel_ms", c_double)]
+                ("max_cell_steps", c_int64), ("team", c_int), ("kernel_ms", c_double), ("chunk", c_int),
+                ("graph", c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

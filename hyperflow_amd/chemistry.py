@@ -44,9 +44,9 @@ class Mechanism:
     """A loaded mechanism bound to one HIP device (owns an hf_ctx)."""
 
     def __init__(self, path, rtol=1e-8, atol=1e-15, max_steps=5000, precision=HF_PREC_FP64, device=0,
-                 waves_per_cu=0, chunk=0, nbins=0):
+                 waves_per_cu=0, chunk=0, nbins=0, eager=False):
         L = load()
-        opt = hf_options(rtol, atol, max_steps, precision, device, waves_per_cu, chunk, nbins)
+        opt = hf_options(rtol, atol, max_steps, precision, device, waves_per_cu, chunk, nbins, int(bool(eager)))
         h = c_void_p()
         check(None, L.hf_create(str(path).encode(), byref(opt), byref(h)))
         self._h = h

@@ -57,8 +57,58 @@ bool is_device_ptr(const void* ptr) {
 
 }  // namespace
 
+// Online tuning of the batched form's refill group (hf_options.chunk = 0):
+// the first calls of a size class (cell counts within a factor of two) try
+// each candidate once and the fastest per cell is kept for the following
+// calls. Results depend on the choice only within the integration tolerance
+// (the wave-synchronised Jacobian refresh groups lanes differently).
+struct ChemTuner {
+    static constexpr int kN = 3;
+    static constexpr int kCand[kN] = {8, 4, 16};
+    int64_t size = 0;
+    int calls = 0, best = 0;
+    double per_cell[kN] = {};
+    int pick(int64_t ncell) {
+        if (size == 0 || ncell > 2 * size || 2 * ncell < size) {
+            size = ncell;
+            calls = 0;
+        }
+        return kCand[calls < kN ? calls : best];
+    }
+    void record(double ms, int64_t ncell) {
+        if (calls >= kN || ncell <= 0) return;
+        per_cell[calls] = ms / (double)ncell;
+        if (++calls == kN)
+            for (int k = 1; k < kN; ++k)
+                if (per_cell[k] < per_cell[best]) best = k;
+    }
+};
+
+// One captured chemistry call (a HIP graph of its kernels), replayed while a
+// device-resident call repeats the same arguments and buffers.
+struct ChemGraph {
+    std::vector<uint64_t> key;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    void reset() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+        key.clear();
+    }
+    ~ChemGraph() {
+        reset();
+        if (t0) (void)hipEventDestroy(t0);
+        if (t1) (void)hipEventDestroy(t1);
+    }
+};
+
 struct hf_ctx {
     Mechanism mech;
+    ChemTuner tuner;
+    ChemGraph graph;
     const TopoOps* ops = nullptr;
     hf_options opt{};
     int num_cus = 256;
@@ -190,6 +240,7 @@ void hf_destroy(hf_ctx* c) {
     if (c->d_mech || c->stream) {
         (void)hipSetDevice(c->opt.device);
         if (c->stream) (void)hipStreamSynchronize(c->stream);
+        c->graph.reset();
         if (c->d_mech) (void)hipFree(c->d_mech);
         if (c->stream) (void)hipStreamDestroy(c->stream);
     }
@@ -219,6 +270,58 @@ int hf_get_options(const hf_ctx* c, hf_options* o) {
     *o = c->opt;
     return HF_OK;
 }
+
+namespace {
+
+// Replays the cached graph when the launch repeats the cached arguments
+// (pointers, sizes, tolerances and tuning values), otherwise captures the
+// kernels of this launch into a new graph first. The graph holds kernel
+// nodes only (no memset or memcpy nodes); *ms is the device time of the
+// whole graph, measured with events around its launch.
+int chem_graph_launch(hf_ctx* c, ChemLaunch L, float* ms, int* how) {
+    auto bits = [](double x) {
+        uint64_t u;
+        std::memcpy(&u, &x, sizeof(u));
+        return u;
+    };
+    const std::vector<uint64_t> key = {
+        (uint64_t)(uintptr_t)L.mech, (uint64_t)(uintptr_t)L.U, (uint64_t)L.ncell, (uint64_t)L.ld,
+        (uint64_t)(uintptr_t)L.dt, (uint64_t)L.dt_per_cell, (uint64_t)(uintptr_t)L.out, (uint64_t)L.ldo,
+        (uint64_t)(uintptr_t)L.status, (uint64_t)(uintptr_t)L.first_failed, (uint64_t)(uintptr_t)L.counters,
+        bits(L.rtol), bits(L.atol), (uint64_t)L.maxsteps, (uint64_t)L.mode, (uint64_t)L.precision,
+        (uint64_t)L.num_cus, (uint64_t)L.waves_per_cu, (uint64_t)L.chunk, (uint64_t)L.nbins,
+        (uint64_t)(uintptr_t)L.workspace, (uint64_t)L.workspace_bytes, (uint64_t)(uintptr_t)L.commit,
+        (uint64_t)L.commit_ld};
+    ChemGraph& g = c->graph;
+    if (!g.t0 && (hipEventCreate(&g.t0) != hipSuccess || hipEventCreate(&g.t1) != hipSuccess)) return HF_EDEVICE;
+    *how = 1;
+    if (!g.exec || g.key != key) {
+        *how = 2;
+        g.reset();
+        L.kernel_ms = nullptr;   // events are recorded around the graph launch instead
+        if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return HF_EDEVICE;
+        const int r = c->ops->chem(L);
+        hipGraph_t graph = nullptr;
+        const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+        if (r != HF_OK || e != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return r != HF_OK ? r : HF_EDEVICE;
+        }
+        g.graph = graph;
+        if (hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0) != hipSuccess) {
+            g.reset();
+            return HF_EDEVICE;
+        }
+        g.key = key;
+    }
+    if (hipEventRecord(g.t0, c->stream) != hipSuccess || hipGraphLaunch(g.exec, c->stream) != hipSuccess ||
+        hipEventRecord(g.t1, c->stream) != hipSuccess)
+        return HF_EDEVICE;
+    if (hipEventSynchronize(g.t1) != hipSuccess || hipEventElapsedTime(ms, g.t0, g.t1) != hipSuccess) return HF_EDEVICE;
+    return HF_OK;
+}
+
+}  // namespace
 
 int hf_chem_advance(hf_ctx* c, double* U, int64_t ncell, int64_t ld, const double* dt, int dt_per_cell, int mode,
                     hf_chem_stats* stats, int64_t* first_failed) {
@@ -283,7 +386,8 @@ int hf_chem_advance(hf_ctx* c, double* U, int64_t ncell, int64_t ld, const doubl
     L.precision = c->opt.precision;
     L.num_cus = c->num_cus;
     L.waves_per_cu = c->opt.waves_per_cu;
-    L.chunk = c->opt.chunk;
+    const bool tune = mode == HF_CHEM_BATCHED && c->opt.chunk == 0;
+    L.chunk = tune ? c->tuner.pick(ncell) : c->opt.chunk;
     L.nbins = c->opt.nbins;
     const size_t wbytes = c->ops->chem_workspace(ncell, c->opt.nbins, c->num_cus, c->opt.waves_per_cu);
     if (wbytes) {
@@ -294,8 +398,19 @@ int hf_chem_advance(hf_ctx* c, double* U, int64_t ncell, int64_t ld, const doubl
     L.stream = c->stream;
     float ms = 0;
     L.kernel_ms = &ms;
-    const int r = c->ops->chem(L);
-    if (r != HF_OK) return fail(c, r, "chemistry kernel launch failed");
+    if (dev_U) {
+        // device-resident field: the commit is a kernel that runs only when
+        // no cell failed, so the whole call is one capturable sequence
+        L.commit = U;
+        L.commit_ld = ld;
+    }
+    int r, how = 0;
+    if (dev_U && dev_dt && !c->opt.eager) {
+        r = chem_graph_launch(c, L, &ms, &how);
+    } else {
+        r = c->ops->chem(L);
+    }
+    if (r != HF_OK) return fail(c, r, r == HF_EDEVICE ? "chemistry kernel launch failed" : "invalid chemistry launch");
     unsigned long long host_scal[9];
     HF_HIP(c, hipMemcpyAsync(host_scal, scal, sizeof(host_scal), hipMemcpyDeviceToHost, c->stream));
     HF_HIP(c, hipStreamSynchronize(c->stream));
@@ -311,7 +426,10 @@ int hf_chem_advance(hf_ctx* c, double* U, int64_t ncell, int64_t ld, const doubl
         stats->max_cell_steps = (int64_t)host_scal[8];
         stats->team = (int)std::min<int64_t>(ncell, INT32_MAX);
         stats->kernel_ms = ms;
+        stats->chunk = mode == HF_CHEM_BATCHED ? L.chunk : 0;
+        stats->graph = how;
     }
+    if (tune && host_scal[0] == ~0ull) c->tuner.record(ms, ncell);
     if (host_scal[0] != ~0ull) {
         const int64_t k = (int64_t)host_scal[0];
         int8_t code = 0;
@@ -322,15 +440,12 @@ int hf_chem_advance(hf_ctx* c, double* U, int64_t ncell, int64_t ld, const doubl
                                                 : "integrator failed (CVode-equivalent error)";
         return fail(c, code ? code : HF_ECONV, "chemistry cell " + std::to_string(k) + ": " + what);
     }
-    // commit the species planes
-    if (dev_U) {
-        HF_HIP(c, hipMemcpy2DAsync(U, ld * sizeof(double), c->out.p, ncell * sizeof(double), ncell * sizeof(double), ns,
-                                   hipMemcpyDeviceToDevice, c->stream));
-    } else {
+    // commit the species planes (device-resident fields were committed by the kernels)
+    if (!dev_U) {
         HF_HIP(c, hipMemcpy2DAsync(U, ld * sizeof(double), c->out.p, ncell * sizeof(double), ncell * sizeof(double), ns,
                                    hipMemcpyDeviceToHost, c->stream));
+        HF_HIP(c, hipStreamSynchronize(c->stream));
     }
-    HF_HIP(c, hipStreamSynchronize(c->stream));
     return HF_OK;
 }
 

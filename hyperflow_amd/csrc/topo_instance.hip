@@ -317,6 +317,26 @@ chem_lane_kernel(const MechData<TT>* __restrict__ dm, const double* __restrict__
     reduce_counters(tot, max_steps_cell, counters);
 }
 
+// All-or-nothing commit on the device: the advanced species planes replace
+// the caller's only when no cell failed (chemistry_batch.cpp:40-49).
+__global__ void __launch_bounds__(256) chem_commit_kernel(const double* __restrict__ out, long ldo,
+                                                          double* __restrict__ U, long ld, long ncell,
+                                                          const unsigned long long* __restrict__ first_failed) {
+    if (*first_failed != ~0ull) return;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ncell * Topo::NS) return;
+    const long k = t / ncell, i = t - k * ncell;
+    U[k * ld + i] = out[k * ldo + i];
+}
+
+int chem_commit(const ChemLaunch& L) {
+    if (!L.commit) return HF_OK;
+    const long n = (long)L.ncell * Topo::NS;
+    hipLaunchKernelGGL(chem_commit_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, L.stream, L.out,
+                       (long)L.ldo, L.commit, (long)L.commit_ld, (long)L.ncell, L.first_failed);
+    return hipGetLastError() == hipSuccess ? HF_OK : HF_EDEVICE;
+}
+
 int chem(const ChemLaunch& L) {
     if (L.ncell == 0) return HF_OK;
     {
@@ -330,7 +350,7 @@ int chem(const ChemLaunch& L) {
             const int nbins = L.nbins > 0 ? min(L.nbins, kMaxBins) : 32;
             const int wpc = L.waves_per_cu > 0 ? L.waves_per_cu : 4;
             // queue chunking: idle lanes refilled together
-            const int chunk = L.chunk > 0 ? min(L.chunk, 64) : 16;
+            const int chunk = L.chunk > 0 ? min(L.chunk, 64) : 8;
             const unsigned g256 = (unsigned)((L.ncell + 255) / 256);
             hipLaunchKernelGGL(chem_init_kernel, dim3(1), dim3(256), 0, L.stream, w, L.first_failed, L.counters);
             hipLaunchKernelGGL(chem_estimate_kernel<Topo>, dim3(g256), dim3(256), 0, L.stream, dm, L.U,
@@ -348,7 +368,7 @@ int chem(const ChemLaunch& L) {
                                    (long)L.ldo, L.status, L.first_failed, p, L.counters, w, chunk);
             if (hipGetLastError() != hipSuccess) return HF_EDEVICE;
             timer.stop();
-            return HF_OK;
+            return chem_commit(L);
         }
         // plain: FP64 throughout, one cell per lane, cells in index order
         const Workspace none = carve(L.workspace, L.ncell, lanes);
@@ -360,7 +380,7 @@ int chem(const ChemLaunch& L) {
                            L.first_failed, p, L.counters, none, 64);
         if (hipGetLastError() != hipSuccess) return HF_EDEVICE;
         timer.stop();
-        return HF_OK;
+        return chem_commit(L);
     }
 }
 

@@ -33,6 +33,8 @@ struct ChemLaunch {
     size_t workspace_bytes;
     hipStream_t stream;
     float* kernel_ms;               // optional: measured with events around the integration kernel
+    double* commit;                 // optional device U: species planes written back (stride commit_ld)
+    int64_t commit_ld;              //   only when no cell failed (all-or-nothing on the device)
 };
 
 struct FluxLaunch {

@@ -63,10 +63,15 @@ typedef struct hf_options {
     int max_steps;      /* per cell; reference: 5000 (chemistry.cpp:25) */
     int precision;      /* HF_PREC_FP64 or HF_PREC_MIXED */
     int device;         /* HIP device ordinal */
-    /* batched-form tuning; 0 selects the autotuned / default value */
+    /* batched-form tuning; 0 selects the default (waves_per_cu 4, nbins 32) or,
+     * for chunk, online tuning over the first calls of a size class */
     int waves_per_cu;   /* resident waves per CU of the persistent kernel */
-    int chunk;          /* cells a wave claims from the queue at once */
+    int chunk;          /* idle lanes of a wave refilled together from the queue */
     int nbins;          /* stiffness bins */
+    /* 0: a device-resident hf_chem_advance call runs as a captured HIP graph,
+     * replayed while calls repeat their arguments and buffers; 1: launch the
+     * kernels one by one */
+    int eager;
 } hf_options;
 
 typedef struct hf_chem_stats {
@@ -75,6 +80,10 @@ typedef struct hf_chem_stats {
     int64_t max_cell_steps; /* largest step count of any cell */
     int team;               /* lanes of work (ChemistryBatch::lastTeamSize analogue) */
     double kernel_ms;       /* device time of the integration kernel(s) */
+    int chunk;              /* batched form: idle lanes refilled per queue claim in this call
+                               (the online-tuned value when hf_options.chunk is 0) */
+    int graph;              /* 0 kernels launched one by one, 1 cached HIP graph replayed,
+                               2 graph captured by this call and launched */
 } hf_chem_stats;
 
 /* One ghosted SoA box for a flux stage. Element (component n, cell i, j) of

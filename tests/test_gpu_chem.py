@@ -159,3 +159,55 @@ def test_dme_chemistry_matches_oracle(oracle_dme, mode, precision):
     clip = 1e-13 * rho0[:, None] * (m["comp"] / m["W"][:, None]).sum(0)[None, :]
     assert np.all(np.abs(el1 - el0) <= 1e-12 * np.abs(el0) + clip)
     assert np.array_equal(out[ns:], U0[ns:])
+
+
+@pytest.mark.parametrize("mode", [hf.HF_CHEM_PLAIN, hf.HF_CHEM_BATCHED])
+def test_graph_replays_match_eager(mode):
+    """A device-resident call is captured once as a HIP graph and replayed;
+    every replay, with eager torch work and synchronisations between them,
+    matches the kernels launched one by one. The plain form is deterministic
+    (bit-identical); the batched form's lane grouping depends on the queue
+    timing, so it matches to the chemistry tolerance."""
+    print(f"[gpu] graph replay mode={mode} n=4096 seed=2", flush=True)
+    m_eager = hf.Mechanism(H2, eager=True, chunk=8)
+    m_graph = hf.Mechanism(H2, chunk=8)
+    U0, dt = m_eager.synth_cells(0, 2, 4096)
+    Ue, dte = dev(U0), dev(dt)
+    st = hf.ChemistryBatch(m_eager, mode=mode).advance(Ue, dte)
+    assert st["graph"] == 0
+    ref = Ue.cpu().numpy()
+    Ug0, dtg = dev(U0), dev(dt)
+    Ug = Ug0.clone()
+    b = hf.ChemistryBatch(m_graph, mode=mode)
+    hows = []
+    for rep in range(4):
+        Ug.copy_(Ug0)
+        (Ug0 * 2.0).sum().item()          # eager work on the stream between replays
+        torch.cuda.synchronize()
+        hows.append(b.advance(Ug, dtg)["graph"])
+        out = Ug.cpu().numpy()
+        if mode == hf.HF_CHEM_PLAIN:
+            np.testing.assert_array_equal(out, ref)
+        else:
+            ns = m_eager.ns
+            Yo, Yr = out[:ns] / out[:ns].sum(0), ref[:ns] / ref[:ns].sum(0)
+            assert np.max(np.abs(Yo - Yr) / np.maximum(np.abs(Yr), Y_FLOOR)) < 1e-6
+            assert np.array_equal(out[ns:], ref[ns:])
+    assert hows == [2, 1, 1, 1]
+
+
+def test_graph_replay_keeps_field_on_failure():
+    """All-or-nothing holds inside the graph: a replay whose batch has a
+    failing cell leaves the device field unchanged and names the cell."""
+    print("[gpu] graph replay failure path", flush=True)
+    m = hf.Mechanism(H2, chunk=8)
+    U0, dt = m.synth_cells(0, 2, 512)
+    Ud, dtd = dev(U0), dev(dt)
+    b = hf.ChemistryBatch(m, mode=hf.HF_CHEM_BATCHED)
+    b.advance(Ud, dtd)
+    Ud.copy_(dev(U0))
+    dtd[37] = -1.0                        # same buffers: the cached graph is replayed
+    with pytest.raises(hf.HyperflowError) as ei:
+        b.advance(Ud, dtd)
+    assert ei.value.first_failed == 37
+    np.testing.assert_array_equal(Ud.cpu().numpy(), U0)
