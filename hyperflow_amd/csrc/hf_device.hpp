@@ -44,6 +44,15 @@ struct MechData {
     double effd[NR][NEFF];      // listed efficiency minus the default
     int common_tmid;            // every species switches range at the same T
     double tmid;
+    // Runtime copy of the topology, for kernels that assign reactions and
+    // species to lanes (indexed by a lane-dependent reaction or species):
+    int8_t rk_kind[NR], rk_troe[NR], rk_rev[NR];
+    int16_t rk_reac[NR][T::NRMAX], rk_prod[NR][T::NRMAX], rk_eff[NR][NEFF > 0 ? NEFF : 1];
+    // species k appears in reactions inc_rxn[inc_start[k] .. inc_start[k+1]) with net
+    // stoichiometric coefficient inc_nu (products minus reactants)
+    int16_t inc_start[NS + 1];
+    int16_t inc_rxn[2 * NR * T::NRMAX];
+    int8_t inc_nu[2 * NR * T::NRMAX];
 };
 
 // ----------------------------------------------------------------- utilities

@@ -1,0 +1,739 @@
+// Wave-per-cell chemistry for large mechanisms (DME: 39 species, 175
+// reactions): one wave integrates one cell. Lane k < N owns unknown k (its
+// value, Nordsieck history, error weight and correction live in that lane's
+// registers; norms are wave reductions), lane N holds the closing species
+// for the thermodynamics, and the reactions are spread over the lanes (lane
+// l evaluates reactions l, l + 64, ...; production rates are then gathered
+// per species in a fixed order). Every step-control decision is uniform
+// across the wave, so the wave never diverges on control flow.
+//
+// The integrator is the same CVODE-equivalent BDF as hf_bdf.hpp (the
+// reference's CVODE call, HyperFlow_V3/src/chemistry.cpp:10-26,64-68):
+// Nordsieck form, orders 1-5, WRMS error test, Newton with CVODE's
+// convergence test and Jacobian reuse, difference-quotient Jacobian
+// (cvLsDenseDQJac, the reference's configuration). The Jacobian is kept in
+// LDS column by column; I - gamma J is factored with each lane holding one
+// row in registers and the pivot rows broadcast with readlane.
+#pragma once
+#include "hf_device.hpp"
+
+namespace hf {
+
+__device__ __forceinline__ double wave_sum(double x) {
+    #pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    return x;
+}
+__device__ __forceinline__ double wave_max(double x) {
+    #pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x = fmax(x, __shfl_xor(x, off, 64));
+    return x;
+}
+// value of x in lane l (l wave-uniform)
+__device__ __forceinline__ double lane_of(double x, int l) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(x), l);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(x), l);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ float lane_of(float x, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
+}
+
+// The cell's thermochemistry with one species per lane.
+template <class T>
+struct WaveGas {
+    static constexpr int NS = T::NS, N = NS - 1, NR = T::NR;
+    const MechData<T>& d;
+    double* lds;                 // per wave: C[NS], grt[NS], q[NR]
+    int lane;
+    bool sp;                     // this lane holds a species (lane < NS)
+    double W, invW, tmid, lo[7], hi[7];
+    double rho, e, t;            // cell constants and the warm-started temperature
+
+    __device__ __forceinline__ WaveGas(const MechData<T>& dd, double* scratch) : d(dd), lds(scratch) {
+        lane = threadIdx.x & 63;
+        sp = lane < NS;
+        const int k = sp ? lane : 0;
+        W = d.W[k];
+        invW = d.invW[k];
+        tmid = d.Tmid[k];
+        #pragma unroll
+        for (int i = 0; i < 7; ++i) {
+            lo[i] = d.lo[k][i];
+            hi[i] = d.hi[k][i];
+        }
+        rho = 1;
+        e = 0;
+        t = 1000;
+    }
+
+    // Mass fraction of this lane's species from the unknowns (lane N closes the mass).
+    __device__ __forceinline__ double mass_fraction(double y) const {
+        const double s = wave_sum(lane < N ? y : 0.0);
+        return lane < N ? y : (lane == N ? 1.0 - s : 0.0);
+    }
+    // T from e at fixed Y (Newton, warm start), as temperature_from_e.
+    __device__ __forceinline__ bool temperature(double Yk, double e_target, double& tt) const {
+        if (!(tt > 0) || !finite(tt)) tt = 1000.0;
+        const double yw = sp ? Yk * invW : 0.0;
+        for (int it = 0; it < 60; ++it) {
+            const bool low = tt <= tmid;
+            double a[6];
+            #pragma unroll
+            for (int i = 0; i < 6; ++i) a[i] = low ? lo[i] : hi[i];
+            const double cpr = fma(tt, fma(tt, fma(tt, fma(tt, a[4], a[3]), a[2]), a[1]), a[0]);
+            const double er = fma(tt, fma(tt, fma(tt, fma(tt, fma(tt, a[4] * 0.2, a[3] * 0.25), a[2] * (1.0 / 3.0)),
+                                                   a[1] * 0.5), a[0]) - 1.0, a[5]);
+            const double ev = kR * wave_sum(yw * er), cv = kR * wave_sum(yw * (cpr - 1.0));
+            const double dT = (ev - e_target) / cv;
+            double tn = tt - dT;
+            tn = tn > 0 ? tn : 0.5 * tt;
+            const bool done = fabs(dT) <= 1e-9 * tt;
+            tt = tn;
+            if (done) return finite(tt) && cv > 0;
+        }
+        return false;
+    }
+
+    // Net rate of progress of reaction i from C and g/RT in LDS (the runtime
+    // form of RateI and production_rates in hf_device.hpp).
+    __device__ __forceinline__ double rate(int i, double logt, double it, double logc0, double ctot) const {
+        const double* C = lds;
+        const double* grt = lds + NS;
+        const int kind = d.rk_kind[i];
+        const double kinf = exp(d.lnA[i] + d.b[i] * logt - d.EaR[i] * it);
+        double Mc = 0;
+        if (kind != 0) {
+            Mc = d.defeff[i] * ctot;
+            #pragma unroll
+            for (int j = 0; j < (T::NEFF > 0 ? T::NEFF : 1); ++j) {
+                const int s = d.rk_eff[i][j];
+                if (s >= 0) Mc = fma(d.effd[i][j], C[s], Mc);
+            }
+        }
+        double kf = kinf;
+        if (kind == 1) {
+            kf = kinf * Mc;
+        } else if (kind == 2) {
+            const double k0 = exp(d.lnA0[i] + d.b0[i] * logt - d.Ea0R[i] * it);
+            const double pr = Mc * k0 / (kinf + kSmall);
+            double F = 1.0;
+            const int troe = d.rk_troe[i];
+            if (troe > 0) {
+                const double A = d.troeA[i];
+                double fc = (1.0 - A) * exp(-t * d.troeRT3[i]) + A * exp(-t * d.troeRT1[i]);
+                if (troe == 4) fc += exp(-d.troeT2[i] * it);
+                const double lc = log10(fmax(fc, kSmall));
+                const double lpr = log10(fmax(pr, kSmall));
+                const double c = -0.4 - 0.67 * lc, n = 0.75 - 1.27 * lc;
+                const double u = lpr + c;
+                const double f1 = u / (n - 0.14 * u);
+                F = exp(kLn10 * (lc / (1.0 + f1 * f1)));
+            }
+            kf = kinf * (pr / (1.0 + pr)) * F;
+        }
+        double pf = 1.0, pb = 1.0, dg = 0.0;
+        int dn = 0;
+        #pragma unroll
+        for (int j = 0; j < T::NRMAX; ++j) {
+            const int r = d.rk_reac[i][j], p = d.rk_prod[i][j];
+            if (r >= 0) {
+                pf *= C[r];
+                dg -= grt[r];
+                --dn;
+            }
+            if (p >= 0) {
+                pb *= C[p];
+                dg += grt[p];
+                ++dn;
+            }
+        }
+        const double kr = d.rk_rev[i] ? kf * exp(dg - dn * logc0) : 0.0;
+        return kf * pf - kr * pb;
+    }
+
+    // f_k = wdot_k W_k / rho on lane k < N (chemistry.cpp:36-53); y on lanes
+    // k < N. Returns the wave-uniform success flag.
+    __device__ __forceinline__ bool rhs(double y, double& f) {
+        const double Yk = mass_fraction(y);
+        double tt = t;
+        if (!temperature(Yk, e, tt)) return false;
+        t = tt;
+        const double logt = log(t), it = 1.0 / t;
+        const double logc0 = log(kP0 / (kR * t));
+        double C = 0;
+        if (sp) {
+            const bool low = t <= tmid;
+            double a[7];
+            #pragma unroll
+            for (int i = 0; i < 7; ++i) a[i] = low ? lo[i] : hi[i];
+            const double hrt = fma(t, fma(t, fma(t, fma(t, a[4] * 0.2, a[3] * 0.25), a[2] * (1.0 / 3.0)), a[1] * 0.5),
+                                   a[0]) + a[5] * it;
+            const double sr = fma(a[0], logt, fma(t, fma(t, fma(t, fma(t, a[4] * 0.25, a[3] * (1.0 / 3.0)), a[2] * 0.5),
+                                                     a[1]), a[6]));
+            C = rho * Yk * invW;
+            lds[lane] = C;
+            lds[NS + lane] = hrt - sr;
+        }
+        const double ctot = wave_sum(C);
+        __syncthreads();
+        for (int i = lane; i < NR; i += 64) lds[2 * NS + i] = rate(i, logt, it, logc0, ctot);
+        __syncthreads();
+        double w = 0;
+        if (lane < N) {
+            const int b = d.inc_start[lane], e2 = d.inc_start[lane + 1];
+            for (int p = b; p < e2; ++p) w = fma((double)d.inc_nu[p], lds[2 * NS + d.inc_rxn[p]], w);
+            f = w * W / rho;
+        } else {
+            f = 0;
+        }
+        __syncthreads();   // the scratch is rewritten by the next evaluation
+        return !__any(!finite(f));
+    }
+
+    // properties() on the cell state held one species per lane: rhoY_k on
+    // lane k, momenta and energy (uniform). Returns admissibility; T warm-started from t.
+    __device__ __forceinline__ bool properties(double rhoY, double mx, double my, double E, double& tt) const {
+        bool ok = !sp || (finite(rhoY) && rhoY >= 0);
+        ok = !__any(!ok) && finite(mx) && finite(my) && finite(E);
+        const double r = wave_sum(sp ? rhoY : 0.0);
+        if (!ok || !(r > 0)) return false;
+        const double Yk = sp ? rhoY / r : 0.0;
+        const double ux = mx / r, vy = my / r;
+        const double ee = E / r - 0.5 * (ux * ux + vy * vy);
+        if (!temperature(Yk, ee, tt)) return false;
+        const bool low = tt <= tmid;
+        double a[5];
+        #pragma unroll
+        for (int i = 0; i < 5; ++i) a[i] = low ? lo[i] : hi[i];
+        const double yw = sp ? Yk * invW : 0.0;
+        const double cp = wave_sum(yw * fma(tt, fma(tt, fma(tt, fma(tt, a[4], a[3]), a[2]), a[1]), a[0]));
+        const double rw = wave_sum(yw);
+        const double p = r * kR * tt * rw;
+        const double cv = cp - rw;
+        const double aa = sqrt(cp / cv * kR * tt * rw);
+        return tt > 0 && p > 0 && aa > 0 && finite(aa);
+    }
+};
+
+struct WaveCounters {
+    int steps = 0, rhs = 0, jac = 0, lu = 0, newton = 0, ncf = 0, netf = 0;
+};
+
+// CVODE-equivalent BDF over one cell, the unknown of lane k in lane k.
+// MT: precision of the Jacobian and the Newton matrix factors.
+template <class T, class MT>
+struct WaveBdf {
+    static constexpr int NS = T::NS, N = NS - 1, QMAX = 5;
+    static constexpr double UROUND = 2.220446049250313e-16;
+    WaveGas<T> g;                // held by value: the whole integrator stays in registers
+    MT* jcol;                    // LDS: Jacobian column j at jcol[j * 64 + lane]
+    double* coef;                // LDS: tau[QMAX + 2], l[QMAX + 1]
+    int lane;
+    bool own;                    // lane < N
+    double z[QMAX + 1];          // Nordsieck history of this lane's unknown
+    double ewt, acor;
+    MT row[N];                   // this lane's row of the factored Newton matrix
+    double tq1, tq2, tq3, tq4, tq5;
+    double tn, h, hprime, hscale, eta, etamax, gamma, gammap, gamrat, rl1, crate, acnrm, saved_tq5;
+    double rtol, atol;
+    int q, qprime, L, qwait, nst, nstlp, nstlj;
+    bool force_setup, jfresh;
+    WaveCounters c;
+
+    __device__ __forceinline__ WaveBdf(const MechData<T>& d, double* scratch, MT* jc, double* cf, double rt, double at)
+        : g(d, scratch), jcol(jc), coef(cf), rtol(rt), atol(at) {
+        lane = threadIdx.x & 63;
+        own = lane < N;
+    }
+    __device__ __forceinline__ double& TAU(int i) const { return coef[i]; }
+    __device__ __forceinline__ double& LC(int i) const { return coef[QMAX + 2 + i]; }
+    __device__ __forceinline__ double wrms(double v) const {
+        const double x = own ? v * ewt : 0.0;
+        return sqrt(wave_sum(x * x) * (1.0 / N));
+    }
+    __device__ __forceinline__ void set_ewt() { ewt = own ? 1.0 / fma(rtol, fabs(z[0]), atol) : 0.0; }
+    // z[q] with a wave-uniform q, as a sum of values selected against zero
+    // (exact): a select between array elements would be folded into one
+    // load through a selected address, which keeps the history in scratch.
+    __device__ __forceinline__ double zq(int qq) const {
+        double v = 0;
+        #pragma unroll
+        for (int j = 0; j <= QMAX; ++j) v += j == qq ? z[j] : 0.0;
+        return v;
+    }
+
+    __device__ __forceinline__ void rescale() {
+        double factor = eta;
+        #pragma unroll
+        for (int j = 1; j <= QMAX; ++j) {
+            if (j <= q) z[j] *= factor;
+            factor *= eta;
+        }
+        h = hscale * eta;
+        hprime = h;
+        hscale = h;
+    }
+    __device__ __forceinline__ void predict() {
+        tn += h;
+        #pragma unroll
+        for (int k = 1; k <= QMAX; ++k)
+            #pragma unroll
+            for (int j = QMAX; j >= k; --j)
+                if (k <= q && j <= q) z[j - 1] += z[j];
+    }
+    __device__ __forceinline__ void restore(double saved_t) {
+        tn = saved_t;
+        #pragma unroll
+        for (int k = 1; k <= QMAX; ++k)
+            #pragma unroll
+            for (int j = QMAX; j >= k; --j)
+                if (k <= q && j <= q) z[j - 1] -= z[j];
+    }
+    __device__ __forceinline__ void decrease_order() {
+        if (q <= 2) return;
+        double lc[QMAX + 1];
+        #pragma unroll
+        for (int j = 0; j <= QMAX; ++j) lc[j] = 0;
+        lc[2] = 1;
+        double hsum = 0;
+        #pragma unroll
+        for (int j = 1; j <= QMAX - 2; ++j) {
+            if (j <= q - 2) {
+                hsum += TAU(j);
+                const double xi = hsum / hscale;
+                #pragma unroll
+                for (int i = j + 2; i >= 2; --i) lc[i] = lc[i] * xi + lc[i - 1];
+            }
+        }
+        const double top = zq(q);
+        #pragma unroll
+        for (int j = 2; j < QMAX; ++j)
+            if (j < q) z[j] -= top * lc[j];
+    }
+    __device__ __forceinline__ void increase_order() {
+        double lc[QMAX + 1];
+        #pragma unroll
+        for (int j = 0; j <= QMAX; ++j) lc[j] = 0;
+        lc[2] = 1;
+        double alpha1 = 1, prod = 1, xiold = 1, alpha0 = -1, hsum = hscale;
+        #pragma unroll
+        for (int j = 1; j < QMAX; ++j) {
+            if (j < q) {
+                hsum += TAU(j + 1);
+                const double xi = hsum / hscale;
+                prod *= xi;
+                alpha0 -= 1.0 / (j + 1);
+                alpha1 += 1.0 / xi;
+                #pragma unroll
+                for (int i = j + 2; i >= 2; --i) lc[i] = lc[i] * xiold + lc[i - 1];
+                xiold = xi;
+            }
+        }
+        const double A1 = (-alpha0 - alpha1) / prod;
+        const double nz = A1 * z[QMAX];
+        #pragma unroll
+        for (int j = 2; j <= QMAX; ++j) {
+            if (j <= q) z[j] += lc[j] * nz;
+            if (j == q + 1) z[j] = nz;
+        }
+    }
+    __device__ __forceinline__ void adjust_order(int dq) {
+        if (q == 2 && dq != 1) return;
+        if (dq == 1) increase_order();
+        else if (dq == -1) decrease_order();
+    }
+    __device__ __forceinline__ void set_bdf() {
+        // coefficients are uniform: lane 0 computes them into LDS
+        if (lane == 0) {
+            #pragma unroll
+            for (int i = 0; i <= QMAX; ++i) LC(i) = 0;
+            LC(0) = LC(1) = 1;
+        }
+        __syncthreads();
+        double xi_inv = 1, xistar_inv = 1, alpha0 = -1, alpha0_hat = -1, hsum = h;
+        if (q > 1) {
+            for (int j = 2; j < q; ++j) {
+                hsum += TAU(j - 1);
+                xi_inv = h / hsum;
+                alpha0 -= 1.0 / j;
+                __syncthreads();
+                if (lane == 0)
+                    for (int i = j; i >= 1; --i) LC(i) += LC(i - 1) * xi_inv;
+                __syncthreads();
+            }
+            alpha0 -= 1.0 / q;
+            xistar_inv = -LC(1) - alpha0;
+            hsum += TAU(q - 1);
+            xi_inv = h / hsum;
+            alpha0_hat = -LC(1) - xi_inv;
+            __syncthreads();
+            if (lane == 0)
+                for (int i = q; i >= 1; --i) LC(i) += LC(i - 1) * xistar_inv;
+            __syncthreads();
+        }
+        const double lq = LC(q);
+        const double A1 = 1.0 - alpha0_hat + alpha0;
+        const double A2 = 1.0 + q * A1;
+        tq2 = fabs(A1 / (alpha0 * A2));
+        tq5 = fabs(A2 * xistar_inv / (lq * xi_inv));
+        if (qwait == 1) {
+            if (q > 1) {
+                const double C = xistar_inv / lq;
+                const double A3 = alpha0 + 1.0 / q;
+                const double A4 = alpha0_hat + xi_inv;
+                tq1 = fabs(C * (1.0 - A4 + A3) / A3);
+            } else {
+                tq1 = 1.0;
+            }
+            hsum += TAU(q);
+            xi_inv = h / hsum;
+            const double A5 = alpha0 - 1.0 / (q + 1);
+            const double A6 = alpha0_hat - xi_inv;
+            const double Cppinv = (1.0 - A6 + A5) / A2;
+            tq3 = fabs(Cppinv / (xi_inv * (q + 2) * A5));
+        }
+        tq4 = 0.1 / tq2;
+    }
+
+    // ---------------------------------------------------------------- linear algebra
+    // Difference-quotient Jacobian (cvLsDenseDQJac) into the LDS columns;
+    // y, f: this lane's unknown and f(y).
+    __device__ __forceinline__ bool dq_jacobian(double y, double f) {
+        const double fnorm = wrms(f);
+        const double min_inc = fnorm != 0 ? 1000.0 * fabs(h) * UROUND * N * fnorm : 1.0;
+        const double srur = sqrt(UROUND);
+        for (int j = 0; j < N; ++j) {
+            const double yj = lane_of(y, j), ewj = lane_of(ewt, j);
+            const double inc = fmax(srur * fabs(yj), min_inc / ewj);
+            double fj;
+            ++c.rhs;
+            if (!g.rhs(lane == j ? yj + inc : y, fj)) return false;
+            jcol[j * 64 + lane] = (MT)((fj - f) * (1.0 / inc));
+        }
+        return true;
+    }
+    // I - gamma J factored without pivoting, one row per lane (reciprocal pivots on the diagonal).
+    __device__ __forceinline__ bool factor() {
+        const MT gm = (MT)gamma;
+        #pragma unroll
+        for (int j = 0; j < N; ++j) row[j] = (lane == j ? MT(1) : MT(0)) - gm * jcol[j * 64 + lane];
+        bool ok = true;
+        #pragma unroll
+        for (int k = 0; k < N; ++k) {
+            const MT p = lane_of(row[k], k);
+            ok = ok && fabs(p) > __FLT_MIN__ && __builtin_isfinite(p);
+            const MT inv = MT(1) / p;
+            if (lane == k) row[k] = inv;
+            const MT l = row[k] * inv;
+            const bool below = lane > k;
+            if (below) row[k] = l;
+            #pragma unroll
+            for (int j = k + 1; j < N; ++j) {
+                const MT pj = lane_of(row[j], k);
+                if (below) row[j] = fma(-l, pj, row[j]);
+            }
+        }
+        return ok;
+    }
+    // Solve (I - gamma J) x = b with b, x one component per lane.
+    __device__ __forceinline__ double solve(double bd) const {
+        MT b = (MT)bd;
+        #pragma unroll
+        for (int k = 0; k < N; ++k) {
+            const MT bk = lane_of(b, k);
+            if (lane > k) b = fma(-row[k], bk, b);
+        }
+        #pragma unroll
+        for (int k = N - 1; k >= 0; --k) {
+            if (lane == k) b *= row[k];
+            const MT bk = lane_of(b, k);
+            if (lane < k) b = fma(-row[k], bk, b);
+        }
+        return own ? (double)b : 0.0;
+    }
+    __device__ __forceinline__ bool lsetup(double y, double f, bool jbad) {
+        ++c.lu;
+        if (jbad) {
+            ++c.jac;
+            if (!dq_jacobian(y, f)) return false;
+            nstlj = nst;
+            jfresh = true;
+        } else {
+            jfresh = false;
+        }
+        __syncthreads();
+        gammap = gamma;
+        gamrat = 1;
+        crate = 1;
+        nstlp = nst;
+        return factor();
+    }
+
+    // ---------------------------------------------------------------- Newton and step
+    // 0 converged, 1 retry with a fresh matrix, 2 reduce the step, 3 rhs/setup failure
+    __device__ __forceinline__ int newton(int& convfail) {
+        double y = own ? z[0] : 0.0, f;
+        ++c.rhs;
+        if (!g.rhs(y, f)) return 3;
+        const bool callsetup = nst == 0 || nst >= nstlp + 20 || fabs(gamrat - 1.0) > 0.3 || convfail != 0 || force_setup;
+        const bool jbad = callsetup && (nst == 0 || nst >= nstlj + 51 || convfail == 2 ||
+                                        (convfail == 1 && fabs(gamma / gammap - 1.0) < 0.2));
+        force_setup = false;
+        jfresh = false;
+        if (callsetup && !lsetup(y, f, jbad)) return 3;
+        acor = 0;
+        double delp = 0;
+        for (int m = 0; m < 3; ++m) {
+            ++c.newton;
+            double delta = own ? -(fma(rl1, z[1], acor) - gamma * f) : 0.0;
+            delta = solve(delta);
+            if (gamrat != 1.0) delta *= 2.0 / (1.0 + gamrat);
+            acor += delta;
+            y = own ? z[0] + acor : 0.0;
+            const double del = wrms(delta);
+            if (m > 0) crate = fmax(0.3 * crate, del / delp);
+            const double dcon = del * fmin(1.0, crate) / tq4;
+            if (dcon <= 1.0) {
+                acnrm = (m == 0) ? del : wrms(acor);
+                return 0;
+            }
+            if (m == 2 || (m >= 1 && del > 2.0 * delp)) break;
+            delp = del;
+            ++c.rhs;
+            if (!g.rhs(y, f)) return 3;
+        }
+        if (!jfresh) {
+            convfail = 1;
+            return 1;
+        }
+        return 2;
+    }
+
+    __device__ __forceinline__ bool step(double tout) {
+        const double saved_t = tn;
+        int ncf = 0, nef = 0, convfail = 0;
+        double dsm = 0;
+        for (;;) {
+            predict();
+            if ((tn - tout) * h > 0) tn = tout;
+            set_bdf();
+            rl1 = 1.0 / LC(1);
+            gamma = h * rl1;
+            if (nst == 0) gammap = gamma;
+            gamrat = (nst > 0) ? gamma / gammap : 1.0;
+            int nf = newton(convfail);
+            while (nf == 1) nf = newton(convfail);
+            if (nf != 0) {
+                restore(saved_t);
+                ++ncf;
+                ++c.ncf;
+                etamax = 1;
+                if (ncf >= 10) return false;
+                eta = 0.25;
+                convfail = 2;
+                rescale();
+                continue;
+            }
+            convfail = 0;
+            dsm = acnrm * tq2;
+            if (dsm <= 1.0) break;
+            ++nef;
+            ++c.netf;
+            restore(saved_t);
+            force_setup = true;
+            if (nef >= 7) return false;
+            etamax = 1;
+            if (nef <= 3) {
+                eta = 1.0 / (pow(6.0 * dsm, 1.0 / L) + 1e-6);
+                eta = fmax(0.1, eta);
+                if (nef >= 2) eta = fmin(eta, 0.2);
+                rescale();
+                continue;
+            }
+            if (q > 1) {
+                eta = 0.1;
+                adjust_order(-1);
+                L = q;
+                --q;
+                qwait = L;
+                rescale();
+                continue;
+            }
+            eta = 0.1;
+            h *= eta;
+            hprime = h;
+            hscale = h;
+            qwait = 10;
+            double f;
+            ++c.rhs;
+            if (!g.rhs(own ? z[0] : 0.0, f)) return false;
+            z[1] = h * f;
+        }
+        // accept
+        ++nst;
+        ++c.steps;
+        __syncthreads();
+        if (lane == 0) {
+            for (int i = q; i >= 2; --i) TAU(i) = TAU(i - 1);
+            if (q == 1 && nst > 1) TAU(2) = TAU(1);
+            TAU(1) = h;
+        }
+        __syncthreads();
+        #pragma unroll
+        for (int j = 0; j <= QMAX; ++j)
+            if (j <= q) z[j] = fma(LC(j), acor, z[j]);
+        --qwait;
+        if (qwait == 1 && q != QMAX) {
+            z[QMAX] = acor;
+            saved_tq5 = tq5;
+        }
+        prepare_next(dsm);
+        return true;
+    }
+
+    __device__ __forceinline__ void prepare_next(double dsm) {
+        if (etamax == 1.0) {
+            qwait = max(qwait, 2);
+            hprime = h;
+            eta = 1.0;
+            etamax = 10.0;
+            return;
+        }
+        const double etaq = 1.0 / (pow(6.0 * dsm, 1.0 / L) + 1e-6);
+        double e = etaq;
+        if (qwait == 0) {
+            qwait = 2;
+            double etaqm1 = 0, etaqp1 = 0;
+            if (q > 1) {
+                const double ddn = wrms(zq(q)) * tq1;
+                etaqm1 = 1.0 / (pow(6.0 * ddn, 1.0 / q) + 1e-6);
+            }
+            if (q != QMAX && saved_tq5 != 0) {
+                const double cquot = (tq5 / saved_tq5) * pow(h / TAU(2), (double)L);
+                const double dup = wrms(acor - cquot * z[QMAX]) * tq3;
+                etaqp1 = 1.0 / (pow(10.0 * dup, 1.0 / (L + 1)) + 1e-6);
+            }
+            const double etam = fmax(etaqm1, fmax(etaq, etaqp1));
+            if (etam < 1.5) {
+                e = 1.0;
+            } else if (etam == etaq) {
+                e = etaq;
+            } else if (etam == etaqm1) {
+                e = etaqm1;
+                qprime = q - 1;
+            } else {
+                e = etaqp1;
+                qprime = q + 1;
+                z[QMAX] = acor;
+            }
+        }
+        eta = e < 1.5 ? 1.0 : fmin(e, etamax);
+        hprime = e < 1.5 ? h : h * eta;
+        etamax = 10.0;
+    }
+
+    __device__ __forceinline__ double initial_step(double f0, double tout) {
+        const double tdist = fabs(tout);
+        const double tround = UROUND * tdist;
+        if (tdist < 2.0 * tround) return -1;
+        const double hlb = 100.0 * tround;
+        const double hub_inv = wave_max(own ? fabs(f0) / fma(0.1, fabs(z[0]), atol) : 0.0);
+        double hub = 0.1 * tdist;
+        if (hub * hub_inv > 1.0) hub = 1.0 / hub_inv;
+        double hg = sqrt(hlb * hub);
+        if (hub < hlb) return hg;
+        double hnew = hg;
+        for (int count = 1;; ++count) {
+            bool ok = false;
+            double yddnrm = 0;
+            for (int tries = 0; tries < 4 && !ok; ++tries) {
+                double f1;
+                ++c.rhs;
+                if (g.rhs(own ? fma(hg, f0, z[0]) : 0.0, f1)) {
+                    yddnrm = wrms((f1 - f0) / hg);
+                    ok = __builtin_isfinite(yddnrm);
+                }
+                if (!ok) hg *= 0.2;
+            }
+            if (!ok) return -1;
+            hnew = (yddnrm * hub * hub > 2.0) ? sqrt(2.0 / yddnrm) : sqrt(hg * hub);
+            if (count >= 4) break;
+            const double hrat = hnew / hg;
+            if (hrat > 0.5 && hrat < 2.0) break;
+            if (count > 1 && hrat > 2.0) {
+                hnew = hg;
+                break;
+            }
+            hg = hnew;
+        }
+        return fmin(fmax(0.5 * hnew, hlb), hub);
+    }
+
+    // Integrate this lane's unknown y from 0 to tout; false on failure.
+    __device__ __forceinline__ bool integrate(double& y, double tout, int maxsteps) {
+        c = WaveCounters{};
+        z[0] = own ? y : 0.0;
+        #pragma unroll
+        for (int j = 1; j <= QMAX; ++j) z[j] = 0;
+        tn = 0;
+        double f0;
+        ++c.rhs;
+        if (!g.rhs(z[0], f0)) return false;
+        set_ewt();
+        const double h0 = initial_step(f0, tout);
+        if (!(h0 > 0)) return false;
+        h = h0;
+        z[1] = h * f0;
+        q = qprime = 1;
+        L = 2;
+        qwait = 2;
+        hscale = hprime = h;
+        eta = 1;
+        etamax = 10000.0;
+        __syncthreads();
+        if (lane == 0)
+            for (int i = 0; i < QMAX + 2; ++i) TAU(i) = 0;
+        __syncthreads();
+        tq1 = tq2 = tq3 = tq4 = tq5 = 0;
+        nst = nstlp = nstlj = 0;
+        gammap = 0;
+        gamrat = 1;
+        crate = 1;
+        saved_tq5 = 0;
+        force_setup = false;
+        jfresh = false;
+        bool first = true;
+        for (;;) {
+            if (nst >= maxsteps) return false;
+            if (!first) set_ewt();
+            if ((tn + hprime - tout) * hprime > 0) {
+                hprime = (tout - tn) * (1.0 - 4.0 * UROUND);
+                eta = hprime / h;
+            }
+            if (hprime != h) {
+                if (!first && qprime != q) {
+                    adjust_order(qprime - q);
+                    q = qprime;
+                    L = q + 1;
+                    qwait = L;
+                }
+                rescale();
+            }
+            first = false;
+            if (!step(tout)) return false;
+            const double troundoff = 100.0 * UROUND * (fabs(tn) + fabs(h));
+            if (fabs(tn - tout) <= troundoff || (tn - tout) * h >= 0) {
+                const double s = (tout - tn) / h;
+                double v = zq(q);
+                #pragma unroll
+                for (int j = QMAX - 1; j >= 0; --j)
+                    if (j < q) v = fma(s, v, z[j]);
+                y = v;
+                return true;
+            }
+        }
+    }
+};
+
+}  // namespace hf

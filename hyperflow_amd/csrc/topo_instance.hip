@@ -10,6 +10,7 @@
 
 #include "hf_adjoint.hpp"
 #include "hf_chem.hpp"
+#include "hf_wave.hpp"
 #include "hf_flux.hpp"
 #include "topo_ops.hpp"
 
@@ -57,6 +58,35 @@ void fill(const Mechanism& m, void* host) {
         for (size_t j = 0; j < r.efficiencies.size(); ++j)
             d.effd[i][j] = r.efficiencies[j].second - r.default_efficiency;
     }
+    // runtime topology (the same tables the kernels unroll over at compile time)
+    constexpr int NEFF = Topo::NEFF > 0 ? Topo::NEFF : 1;
+    for (int i = 0; i < nr; ++i) {
+        d.rk_kind[i] = (int8_t)Topo::kind[i];
+        d.rk_troe[i] = (int8_t)Topo::troe[i];
+        d.rk_rev[i] = (int8_t)Topo::rev[i];
+        for (int j = 0; j < Topo::NRMAX; ++j) {
+            d.rk_reac[i][j] = (int16_t)Topo::reac[i][j];
+            d.rk_prod[i][j] = (int16_t)Topo::prod[i][j];
+        }
+        for (int j = 0; j < NEFF; ++j) d.rk_eff[i][j] = Topo::NEFF > 0 ? (int16_t)Topo::effsp[i][j] : (int16_t)-1;
+    }
+    int n = 0;
+    for (int k = 0; k < ns; ++k) {
+        d.inc_start[k] = (int16_t)n;
+        for (int i = 0; i < nr; ++i) {
+            int nu = 0;
+            for (int j = 0; j < Topo::NRMAX; ++j) {
+                if (Topo::reac[i][j] == k) --nu;
+                if (Topo::prod[i][j] == k) ++nu;
+            }
+            if (nu != 0) {
+                d.inc_rxn[n] = (int16_t)i;
+                d.inc_nu[n] = (int8_t)nu;
+                ++n;
+            }
+        }
+    }
+    d.inc_start[ns] = (int16_t)n;
 }
 
 // ------------------------------------------------------------------ chemistry
@@ -317,6 +347,84 @@ chem_lane_kernel(const MechData<TT>* __restrict__ dm, const double* __restrict__
     reduce_counters(tot, max_steps_cell, counters);
 }
 
+// Wave-per-cell batched form for large mechanisms: each wave of the
+// persistent grid takes the next cell of the binned queue and integrates it
+// with all 64 lanes (hf_wave.hpp); the post-step checks and the write of
+// the species planes follow Chemistry::advance (chemistry.cpp:68-75).
+constexpr bool kWaveChem = Topo::NS > 16;
+
+template <class TT, class MT>
+__global__ void __launch_bounds__(64)
+chem_wave_kernel(const MechData<TT>* __restrict__ dm, const double* __restrict__ U, long ncell, long ld,
+                 const double* __restrict__ dt, int per_cell, double* __restrict__ out, long ldo,
+                 int8_t* __restrict__ status, unsigned long long* __restrict__ first_failed, ChemParams p,
+                 unsigned long long* __restrict__ counters, Workspace w) {
+    constexpr int NS = TT::NS, N = NS - 1, NR = TT::NR;
+    __shared__ double scratch[2 * NS + NR];
+    __shared__ double coef[13];
+    __shared__ MT jcol[N * 64];
+    const int lane = threadIdx.x;
+    WaveBdf<TT, MT> bdf(*dm, scratch, jcol, coef, p.rtol, p.atol);
+    WaveGas<TT>& gas = bdf.g;
+    unsigned long long tot[7] = {0, 0, 0, 0, 0, 0, 0};
+    int max_steps_cell = 0;
+    for (;;) {
+        unsigned slot = 0;
+        if (lane == 0) slot = atomicAdd(w.head, 1u);
+        slot = (unsigned)__builtin_amdgcn_readfirstlane((int)slot);
+        if (slot >= (unsigned)ncell) break;
+        const long cell = w.order[slot];
+        const double rhoY = lane < NS ? U[lane * ld + cell] : 0.0;
+        const double mx = U[NS * ld + cell], my = U[(NS + 1) * ld + cell], E = U[(NS + 2) * ld + cell];
+        const double h = per_cell ? dt[cell] : dt[0];
+        int st = HF_OK;
+        double tt = 1000.0;
+        if (!(h > 0) || !finite(h)) {
+            st = HF_EINVAL;
+        } else if (!gas.properties(rhoY, mx, my, E, tt)) {
+            st = HF_EDOMAIN;
+        } else {
+            const double rho = wave_sum(lane < NS ? rhoY : 0.0);
+            const double ux = mx / rho, vy = my / rho;
+            gas.rho = rho;
+            gas.e = E / rho - 0.5 * (ux * ux + vy * vy);
+            gas.t = tt;
+            double y = lane < N ? rhoY / rho : 0.0;
+            if (!bdf.integrate(y, h, p.maxsteps)) {
+                st = HF_ECONV;
+            } else if (__any(lane < N && !(finite(y) && y >= -1e-13))) {
+                st = HF_EDOMAIN;
+            } else {
+                double r = lane < N ? rho * fmax(0.0, y) : 0.0;
+                const double sum = wave_sum(r);
+                if (lane == N) r = rho - sum;
+                double t2 = gas.t;
+                if (!gas.properties(r, mx, my, E, t2)) st = HF_EDOMAIN;
+                else if (lane < NS) out[lane * ldo + cell] = r;
+            }
+            const WaveCounters& c = bdf.c;
+            tot[0] += c.steps;
+            tot[1] += c.rhs;
+            tot[2] += c.jac;
+            tot[3] += c.lu;
+            tot[4] += c.newton;
+            tot[5] += c.ncf;
+            tot[6] += c.netf;
+            max_steps_cell = max(max_steps_cell, c.steps);
+        }
+        if (lane == 0) {
+            status[cell] = (int8_t)st;
+            if (st != HF_OK) atomicMin(first_failed, (unsigned long long)cell);
+        }
+    }
+    if (lane == 0) {
+        #pragma unroll
+        for (int k = 0; k < 7; ++k)
+            if (tot[k]) atomicAdd(&counters[k], tot[k]);
+        atomicMax(&counters[7], (unsigned long long)max_steps_cell);
+    }
+}
+
 // All-or-nothing commit on the device: the advanced species planes replace
 // the caller's only when no cell failed (chemistry_batch.cpp:40-49).
 __global__ void __launch_bounds__(256) chem_commit_kernel(const double* __restrict__ out, long ldo,
@@ -358,17 +466,33 @@ int chem(const ChemLaunch& L) {
             hipLaunchKernelGGL(chem_scan_kernel, dim3(1), dim3(64), 0, L.stream, w, nbins);
             hipLaunchKernelGGL(chem_scatter_kernel, dim3(g256), dim3(256), 0, L.stream, (long)L.ncell, nbins, w);
             StreamTimer timer(L.kernel_ms, L.stream);
-            if (L.precision == HF_PREC_MIXED)
-                hipLaunchKernelGGL((chem_lane_kernel<Topo, float, false>), dim3((unsigned)(L.num_cus * wpc)), dim3(64),
-                                   0, L.stream, dm, L.U, (long)L.ncell, (long)L.ld, L.dt, L.dt_per_cell, L.out,
-                                   (long)L.ldo, L.status, L.first_failed, p, L.counters, w, chunk);
-            else
-                hipLaunchKernelGGL((chem_lane_kernel<Topo, double, false>), dim3((unsigned)(L.num_cus * wpc)), dim3(64),
-                                   0, L.stream, dm, L.U, (long)L.ncell, (long)L.ld, L.dt, L.dt_per_cell, L.out,
-                                   (long)L.ldo, L.status, L.first_failed, p, L.counters, w, chunk);
-            if (hipGetLastError() != hipSuccess) return HF_EDEVICE;
-            timer.stop();
-            return chem_commit(L);
+            if constexpr (kWaveChem) {
+                // one cell per wave: more resident waves per CU (LDS ~12-22 KB per wave)
+                const int wwpc = L.waves_per_cu > 0 ? L.waves_per_cu : 8;
+                if (L.precision == HF_PREC_MIXED)
+                    hipLaunchKernelGGL((chem_wave_kernel<Topo, float>), dim3((unsigned)(L.num_cus * wwpc)), dim3(64), 0,
+                                       L.stream, dm, L.U, (long)L.ncell, (long)L.ld, L.dt, L.dt_per_cell, L.out,
+                                       (long)L.ldo, L.status, L.first_failed, p, L.counters, w);
+                else
+                    hipLaunchKernelGGL((chem_wave_kernel<Topo, double>), dim3((unsigned)(L.num_cus * wwpc)), dim3(64), 0,
+                                       L.stream, dm, L.U, (long)L.ncell, (long)L.ld, L.dt, L.dt_per_cell, L.out,
+                                       (long)L.ldo, L.status, L.first_failed, p, L.counters, w);
+                if (hipGetLastError() != hipSuccess) return HF_EDEVICE;
+                timer.stop();
+                return chem_commit(L);
+            } else {
+                if (L.precision == HF_PREC_MIXED)
+                    hipLaunchKernelGGL((chem_lane_kernel<Topo, float, false>), dim3((unsigned)(L.num_cus * wpc)),
+                                       dim3(64), 0, L.stream, dm, L.U, (long)L.ncell, (long)L.ld, L.dt, L.dt_per_cell,
+                                       L.out, (long)L.ldo, L.status, L.first_failed, p, L.counters, w, chunk);
+                else
+                    hipLaunchKernelGGL((chem_lane_kernel<Topo, double, false>), dim3((unsigned)(L.num_cus * wpc)),
+                                       dim3(64), 0, L.stream, dm, L.U, (long)L.ncell, (long)L.ld, L.dt, L.dt_per_cell,
+                                       L.out, (long)L.ldo, L.status, L.first_failed, p, L.counters, w, chunk);
+                if (hipGetLastError() != hipSuccess) return HF_EDEVICE;
+                timer.stop();
+                return chem_commit(L);
+            }
         }
         // plain: FP64 throughout, one cell per lane, cells in index order
         const Workspace none = carve(L.workspace, L.ncell, lanes);

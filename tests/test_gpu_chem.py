@@ -128,14 +128,68 @@ def test_scalar_dt_matches_per_cell_dt():
     assert torch.equal(a, b)
 
 
-@pytest.mark.parametrize("mode,precision", [(hf.HF_CHEM_PLAIN, hf.HF_PREC_FP64), (hf.HF_CHEM_BATCHED, hf.HF_PREC_FP64),
-                                            (hf.HF_CHEM_BATCHED, hf.HF_PREC_MIXED)])
+DME_FORMS = [(hf.HF_CHEM_PLAIN, hf.HF_PREC_FP64), (hf.HF_CHEM_BATCHED, hf.HF_PREC_FP64),
+             (hf.HF_CHEM_BATCHED, hf.HF_PREC_MIXED)]
+
+
+def element_drift(m, U0, U1):
+    """Relative change of each element mass per cell, (ncell, nelem)."""
+    ns = len(m["W"])
+    el0 = (U0[:ns] / m["W"][:, None]).T @ m["comp"]
+    el1 = (U1[:ns] / m["W"][:, None]).T @ m["comp"]
+    return np.abs(el1 - el0) / np.abs(el0)
+
+
+@pytest.mark.parametrize("mode,precision", DME_FORMS)
 def test_dme_chemistry_matches_oracle(oracle_dme, mode, precision):
-    """DME/air cells of the mixed-precision workload (39 species, N = 38:
-    difference-quotient Jacobian, global-memory Newton matrix and history)."""
+    """DME/air cells of the mixed-precision workload (39 species, N = 38;
+    plain: one cell per lane; batched: one cell per wave) at rtol 1e-10.
+    atol is 1e-20: at atol 1e-14 (and at the reference's 1e-15) the
+    reference algorithm itself rejects this batch (an accepted species below
+    -1e-13, chemistry.cpp:69-73), so no run at that setting exists to match.
+
+    Element masses: a Newton update of the BDF corrector keeps the linear
+    element invariants c exactly only if c^T J = 0; the difference-quotient
+    Jacobian carries roundoff/increment errors E, and each step leaves a
+    drift of about gamma c^T E delta, delta being the last Newton correction
+    (rtol-sized). The reference algorithm on these cells drifts 3e-10 at
+    rtol 1e-8, 2e-12 at rtol 1e-10 and 2e-14 at rtol 1e-12. At rtol 1e-10
+    the bound here is therefore the oracle's own drift on the same cells at
+    the same tolerances (x2); the 1e-12 bar itself is checked at rtol 1e-12
+    by test_dme_element_masses_tight."""
     print(f"[gpu] DME chemistry parity mode={mode} precision={precision} n=128 seed=4", flush=True)
-    mech = hf.Mechanism(hf.DME_SK39, precision=precision)
+    mech = hf.Mechanism(hf.DME_SK39, precision=precision, rtol=1e-10, atol=1e-20)
     U0, dt = mech.synth_cells(1, 4, 128)
+    ref = U0.copy()
+    r, st, ff, msg = oracle_dme.chem_advance(ref, dt, rtol=ORACLE_RTOL, atol=ORACLE_ATOL)
+    assert r == 0, msg
+    same = U0.copy()
+    r, st, ff, msg = oracle_dme.chem_advance(same, dt, rtol=1e-10, atol=1e-20)
+    assert r == 0, msg
+    Ud = dev(U0)
+    stats = hf.ChemistryBatch(mech, mode=mode).advance(Ud, dev(dt))
+    out = Ud.cpu().numpy()
+    eT, eY = compare(oracle_dme, out, ref, mech.W, oracle_dme.m["comp"])
+    eT2, eY2 = compare(oracle_dme, same, ref, mech.W, oracle_dme.m["comp"])
+    m = oracle_dme.m
+    d_gpu = element_drift(m, U0, out).max()
+    d_orc = element_drift(m, U0, same).max()
+    print(f"[gpu] DME mode={mode} precision={precision}: T {eT:.3e} Y(floor {Y_FLOOR:g}) {eY:.3e} "
+          f"(oracle at the same tolerances: T {eT2:.3e} Y {eY2:.3e}); element drift {d_gpu:.3e} "
+          f"(oracle {d_orc:.3e}); stats {stats}", flush=True)
+    assert eT < 1e-6 and eY < 1e-6
+    assert d_gpu <= max(1e-12, 2.0 * d_orc)
+    assert np.array_equal(out[mech.ns:], U0[mech.ns:])
+
+
+@pytest.mark.parametrize("mode,precision", DME_FORMS)
+def test_dme_element_masses_tight(oracle_dme, mode, precision):
+    """At rtol 1e-12 / atol 1e-20 every DME form conserves each element mass
+    to 1e-12 relative (no accepted species is clipped at this atol), and
+    matches the converged oracle to 1e-6 in T and Y."""
+    print(f"[gpu] DME element masses mode={mode} precision={precision} n=64 seed=5", flush=True)
+    mech = hf.Mechanism(hf.DME_SK39, precision=precision, rtol=1e-12, atol=1e-20)
+    U0, dt = mech.synth_cells(1, 5, 64)
     ref = U0.copy()
     r, st, ff, msg = oracle_dme.chem_advance(ref, dt, rtol=ORACLE_RTOL, atol=ORACLE_ATOL)
     assert r == 0, msg
@@ -143,22 +197,11 @@ def test_dme_chemistry_matches_oracle(oracle_dme, mode, precision):
     stats = hf.ChemistryBatch(mech, mode=mode).advance(Ud, dev(dt))
     out = Ud.cpu().numpy()
     eT, eY = compare(oracle_dme, out, ref, mech.W, oracle_dme.m["comp"])
-    print(f"[gpu] DME mode={mode} precision={precision}: T {eT:.3e} Y(floor {Y_FLOOR:g}) {eY:.3e}; stats {stats}",
-          flush=True)
+    d_gpu = element_drift(oracle_dme.m, U0, out).max()
+    print(f"[gpu] DME tight mode={mode} precision={precision}: T {eT:.3e} Y {eY:.3e} element drift {d_gpu:.3e}; "
+          f"stats {stats}", flush=True)
     assert eT < 1e-6 and eY < 1e-6
-    # Element masses: exact to 1e-12 up to the reference's own clip of accepted
-    # species in [-1e-13, 0) to zero (chemistry.cpp:69-73), which may move each
-    # species by 1e-13 in Y. (The oracle at the reference tolerance moves the H
-    # element of these cells by 3e-11 relative through that clip; at rtol
-    # 1e-12 by 2e-14.)
-    m = oracle_dme.m
-    ns = mech.ns
-    rho0 = U0[:ns].sum(0)
-    el0 = (U0[:ns] / m["W"][:, None]).T @ m["comp"]
-    el1 = (out[:ns] / m["W"][:, None]).T @ m["comp"]
-    clip = 1e-13 * rho0[:, None] * (m["comp"] / m["W"][:, None]).sum(0)[None, :]
-    assert np.all(np.abs(el1 - el0) <= 1e-12 * np.abs(el0) + clip)
-    assert np.array_equal(out[ns:], U0[ns:])
+    assert d_gpu <= 1e-12
 
 
 @pytest.mark.parametrize("mode", [hf.HF_CHEM_PLAIN, hf.HF_CHEM_BATCHED])
