@@ -19,15 +19,29 @@
 
 namespace hf {
 
+// The value of x in the first active lane, in scalar registers. Every
+// step-control decision of the integrator (Newton exits, error tests, the
+// temperature iteration's stop) must see bitwise the same value on all 64
+// lanes: if two lanes disagree in the last bit on a test that sits on its
+// threshold, the wave splits, and the reductions of the lanes still running
+// read stale values from the lanes that left (seen as a temperature solve
+// jumping by 1000 K). A butterfly sum is not bitwise uniform by itself once
+// the compiler fuses its first add with the product that feeds it, so each
+// reduction ends with this broadcast.
+__device__ __forceinline__ double uniform(double x) {
+    const int lo = __builtin_amdgcn_readfirstlane(__double2loint(x));
+    const int hi = __builtin_amdgcn_readfirstlane(__double2hiint(x));
+    return __hiloint2double(hi, lo);
+}
 __device__ __forceinline__ double wave_sum(double x) {
     #pragma unroll
     for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
-    return x;
+    return uniform(x);
 }
 __device__ __forceinline__ double wave_max(double x) {
     #pragma unroll
     for (int off = 32; off > 0; off >>= 1) x = fmax(x, __shfl_xor(x, off, 64));
-    return x;
+    return uniform(x);
 }
 // value of x in lane l (l wave-uniform)
 __device__ __forceinline__ double lane_of(double x, int l) {
