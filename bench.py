@@ -14,8 +14,10 @@ Also reported: the DME mixed-precision workload (2^18 DME/air cells, seed 4,
 39 species: batched with an FP32 Newton matrix and plain FP64, one step
 each), the 2-D flux sweep (one Euler stage over a 4096 x 1024 box,
 Mcell-updates/s, HBM roofline at 192 algorithmic bytes per cell-update),
-chemistry parity against the CPU oracle on a sample, and the CPU oracle's
-throughput on a bounded sample of the same workloads (all host cores).
+the 1-D plain workload end to end (2000 cells, 1000 Strang steps through the
+step loop), chemistry parity against the CPU oracle on a sample, and the CPU
+oracle's throughput on a bounded sample of the same workloads (all host
+cores).
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1]
 """
@@ -90,6 +92,62 @@ def flux_run(torch, hf, mech, nx, ny, steps, warmup):
     return wall, ms, st
 
 
+def shock_tube_setup(hf):
+    """The 1-D plain workload: 2000 cells over 0.1 m of H2/air, x < 0.01 m at
+    2000 K / 20 atm TP equilibrium, the rest at 300 K / 1 atm; reflecting left,
+    transmissive right; CFL 0.5; chemistry rtol 1e-8, atol 1e-12."""
+    from hyperflow_amd.driver import shock_tube_1d
+    mech = hf.Mechanism(hf.H2_LI, rtol=1e-8, atol=1e-12)
+    Q, dx = shock_tube_1d(mech)
+    bc = (hf.HF_BC_REFLECTING, hf.HF_BC_TRANSMISSIVE, -1, -1)
+    return mech, Q, dx, bc
+
+
+def shock_tube_run(torch, hf, steps, warmup):
+    """Strang steps of the 1-D workload through BoxDriver (every stage on the
+    GPU: ghost fill, flux stage, repair, validate, Heun average, plain FP64
+    chemistry). Returns (wall of the timed steps, driver, state after them)."""
+    from hyperflow_amd.driver import BoxDriver
+    mech, Q, dx, bc = shock_tube_setup(hf)
+    d = BoxDriver(mech, 2000, 1, dx, 0.0, bc, cfl=0.5, chem_mode=hf.HF_CHEM_PLAIN)
+    d.set_state(Q)
+    for _ in range(warmup):
+        d.step()
+    d = BoxDriver(mech, 2000, 1, dx, 0.0, bc, cfl=0.5, chem_mode=hf.HF_CHEM_PLAIN)
+    d.set_state(Q)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        d.step()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, d, d.state()
+
+
+def shock_tube_cpu(hf, target_s=12.0):
+    """The oracle's restatement of the same step loop on all host cores: as
+    many steps as fit in about target_s. Returns (steps, wall, state)."""
+    from oracle import Oracle
+    from oracle.driver import OracleBoxDriver
+    mech, Q, dx, bc = shock_tube_setup(hf)
+    o = OracleBoxDriver(Oracle(hf.H2_LI), 2000, 1, dx, 0.0, bc, cfl=0.5, rtol=1e-8, atol=1e-12)
+    o.Q = Q.copy()
+    n, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < target_s:
+        o.step()
+        n += 1
+    return n, time.perf_counter() - t0, o.Q.copy()
+
+
+def state_difference(a, b, ns):
+    """Max over cells and components of |a - b| / scale: species scaled by
+    max(|rhoY|, 1e-6 rho), momenta by max(|rho u|, rho x 1 m/s), energy by |rhoE|."""
+    rho = np.abs(b[:ns]).sum(0)
+    scale = np.abs(b).copy()
+    scale[:ns] = np.maximum(scale[:ns], 1e-6 * rho)
+    scale[ns:ns + 2] = np.maximum(scale[ns:ns + 2], rho)
+    return float((np.abs(a - b) / np.maximum(scale, 1e-300)).max())
+
+
 def cpu_baseline(U, dt, target_s=15.0, rtol=1e-8, atol=1e-15, mech_path=None, label="H2"):
     from oracle import Oracle
     from hyperflow_amd import H2_LI
@@ -137,6 +195,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and parity legs")
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--dme-cells", type=int, default=1 << 18, help="DME mixed-precision workload size (0 skips it)")
+    ap.add_argument("--shock-tube-steps", type=int, default=1000,
+                    help="Strang steps of the 1-D 2000-cell workload through the step loop (0 skips it)")
     ap.add_argument("--copy-ref-mib", type=int, default=0,
                     help="also run a device-to-device copy of this many MiB (reference for HBM counter scaling)")
     args = ap.parse_args()
@@ -222,6 +282,19 @@ def main():
     f_upd = fcells * max(args.steps, 5) / fwall
     f_kernel_gbs = 192.0 * fcells / (np.mean(fms) * 1e-3) / 1e9
 
+    tube = None
+    if args.shock_tube_steps:
+        log(f"1-D shock tube: 2000 cells, {args.shock_tube_steps} Strang steps")
+        twall, tdrv, tstate = shock_tube_run(torch, hf, args.shock_tube_steps, 2)
+        tube = {"workload": "1-D reacting shock tube, 2000 cells over 0.1 m (H2_Li), 2000 K / 20 atm equilibrium "
+                            "driver in x < 0.01 m, 300 K / 1 atm stoichiometric H2/air elsewhere; NND + HLLC, Heun, "
+                            "CFL 0.5, Strang chemistry (plain FP64 form, rtol 1e-8 atol 1e-12)",
+                "steps": args.shock_tube_steps, "steps_per_s": args.shock_tube_steps / twall,
+                "Mcell_updates_per_s": 2000 * 4 * args.shock_tube_steps / twall / 1e6,
+                "ms_per_step": twall / args.shock_tube_steps * 1e3, "simulated_time_s": tdrv.time,
+                "first_order_stages": tdrv.first_order_stages, "retries": tdrv.retries, "repairs": tdrv.repairs}
+        log(f"1-D shock tube: {tube['steps_per_s']:.4g} steps/s, {tube['ms_per_step']:.3f} ms/step")
+
     head = res["batched"]
     kern_s = np.mean(head["kernel_ms"]) * 1e-3
     achieved = head["flops"] / kern_s / 1e12
@@ -243,8 +316,10 @@ def main():
                    "cells": args.cells, "form": "batched"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                     "note": "FP64 algorithmic operations of the chemistry kernel (hyperflow_amd/flops.py) "
-                             "over its event-timed duration; peak is the FP64 rate"},
+                     "note": "FP64 algorithmic operations of the chemistry kernel (hyperflow_amd/flops.py: "
+                             "the kernel's operation counters x per-operation counts, exp/log/log10/division "
+                             "at the FP64 arithmetic of their gfx950 code) over its event-timed duration; "
+                             "peak is the FP64 rate; HBM traffic per launch is in profiles/"},
     }
     line["config"]["precision"] = "FP64 state, residuals, rates and error control; FP32 Newton matrix and LU"
     for name in ("plain", "batched_fp64"):
@@ -267,7 +342,20 @@ def main():
                                         "seed 4, dt log-uniform [1e-8,1e-6] s, rtol 1e-8 atol 1e-20",
                             "unit": "cell-steps/s", "precision": "batched: FP32 Newton matrix and LU; plain: FP64",
                             **dme}
+    if tube is not None:
+        line["shock_tube_1d"] = tube
     if not args.no_cpu:
+        if tube is not None:
+            log("1-D shock tube: CPU oracle step loop")
+            nc, cwall, cstate = shock_tube_cpu(hf)
+            _, _, gstate = shock_tube_run(torch, hf, nc, 0)
+            line["shock_tube_1d"]["cpu_baseline"] = {
+                "value": nc / cwall, "unit": "Strang steps/s", "cores": len(os.sched_getaffinity(0)), "kind": "port",
+                "sample": f"the first {nc} steps of the same run, the oracle's step loop (C++ flux stage and BDF, "
+                          f"OpenMP chemistry), {cwall:.1f} s"}
+            line["shock_tube_1d"]["parity_vs_oracle"] = {
+                "steps": nc, "max_rel_state_difference": state_difference(gstate.reshape(cstate.shape), cstate,
+                                                                           hf.Mechanism(hf.H2_LI).ns)}
         log("parity sample vs the converged oracle")
         line["parity_vs_oracle"] = {k: parity(U0, dt, v) for k, v in outs.items()}
         log("CPU baseline")

@@ -7,8 +7,8 @@ from __future__ import annotations
 
 from pathlib import Path
 
-from ._lib import (HF_CHEM_BATCHED, HF_CHEM_PLAIN, HF_PREC_FP64, HF_PREC_MIXED, HyperflowError, LIB_PATH,
-                   load)
+from ._lib import (HF_BC_PERIODIC, HF_BC_REFLECTING, HF_BC_TRANSMISSIVE, HF_CHEM_BATCHED, HF_CHEM_PLAIN,
+                   HF_PREC_FP64, HF_PREC_MIXED, HyperflowError, LIB_PATH, load)
 from .adjoint import ChemSensitivity, fixed_step_chemistry
 from .chemistry import ChemistryBatch, Mechanism
 from .physics import FluxSweep
@@ -18,4 +18,5 @@ H2_LI = MECHANISM_DIR / "H2_Li.yaml"
 DME_SK39 = MECHANISM_DIR / "dme_sk39.yaml"
 
 __all__ = ["ChemistryBatch", "Mechanism", "ChemSensitivity", "fixed_step_chemistry", "FluxSweep", "HyperflowError",
-           "load", "LIB_PATH", "H2_LI", "DME_SK39", "HF_CHEM_PLAIN", "HF_CHEM_BATCHED", "HF_PREC_FP64", "HF_PREC_MIXED"]
+           "load", "LIB_PATH", "H2_LI", "DME_SK39", "HF_CHEM_PLAIN", "HF_CHEM_BATCHED", "HF_PREC_FP64", "HF_PREC_MIXED",
+           "HF_BC_TRANSMISSIVE", "HF_BC_REFLECTING", "HF_BC_PERIODIC"]

@@ -29,6 +29,8 @@ constexpr double kR = 8314.462618;
 constexpr double kP0 = 101325.0;
 constexpr double kSmall = 1e-300;
 constexpr double kLn10 = 2.302585092994045684;
+// ln(p0 / R): ln(p0 / (R T)) = kLnP0R - ln T saves a logarithm per rate evaluation
+constexpr double kLnP0R = 2.500336689979609446;
 
 template <class T>
 struct MechData {
@@ -415,7 +417,7 @@ __device__ __forceinline__ void production_rates(const MechData<T>& d, double t,
                                                  const double* lnA_shift = nullptr, double* qout = nullptr) {
     constexpr int NS = T::NS;
     const double logt = log(t), it = 1.0 / t;
-    const double logc0 = log(kP0 / (kR * t));
+    const double logc0 = kLnP0R - logt;
     double hrt[NS], grt[NS];
     species_h_g(d, t, logt, hrt, grt);
     double Ctot = 0;
@@ -459,7 +461,7 @@ __device__ __forceinline__ void chem_jacobian(const MechData<T>& d, double t, do
                                               double* fT = nullptr, double* qout = nullptr) {
     constexpr int NS = T::NS, N = NS - 1, L = NS - 1;
     const double logt = log(t), it = 1.0 / t;
-    const double logc0 = log(kP0 / (kR * t));
+    const double logc0 = kLnP0R - logt;
     double hrt[NS], grt[NS], C[NS], wd[NS], dwdT[NS], dwdCL[N];
     species_h_g(d, t, logt, hrt, grt);
     double Ctot = 0;

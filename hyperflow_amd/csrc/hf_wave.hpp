@@ -174,7 +174,7 @@ struct WaveGas {
         if (!temperature(Yk, e, tt)) return false;
         t = tt;
         const double logt = log(t), it = 1.0 / t;
-        const double logc0 = log(kP0 / (kR * t));
+        const double logc0 = kLnP0R - logt;
         double C = 0;
         if (sp) {
             const bool low = t <= tmid;
