@@ -138,14 +138,62 @@ def shock_tube_cpu(hf, target_s=12.0):
     return n, time.perf_counter() - t0, o.Q.copy()
 
 
+CHANNEL = dict(nx=1024, ny=256)
+
+
+def channel_run(torch, hf, steps, warmup):
+    """Strang steps of the 2-D channel's base level through BoxDriver (every
+    stage on the GPU; batched FP64 chemistry over all cells each step).
+    Returns (wall of the timed steps, driver, state after them)."""
+    from hyperflow_amd.driver import BoxDriver, channel_2d
+    mech = hf.Mechanism(hf.H2_LI, rtol=1e-8, atol=1e-12)
+    Q, dx = channel_2d(mech, **CHANNEL)
+    bc = (hf.HF_BC_REFLECTING, hf.HF_BC_TRANSMISSIVE, hf.HF_BC_REFLECTING, hf.HF_BC_REFLECTING)
+    nx, ny = CHANNEL["nx"], CHANNEL["ny"]
+
+    def fresh():
+        d = BoxDriver(mech, nx, ny, dx, dx, bc, cfl=0.5, chem_mode=hf.HF_CHEM_BATCHED)
+        d.set_state(Q)
+        return d
+    d = fresh()
+    for _ in range(warmup):
+        d.step()
+    d = fresh()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        d.step()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, d, d.state()
+
+
+def channel_cpu(hf, target_s=12.0):
+    """The oracle's step loop on the same channel: whole steps until about
+    target_s has passed (at least one). Returns (steps, wall, state)."""
+    from hyperflow_amd.driver import channel_2d
+    from oracle import Oracle
+    from oracle.driver import OracleBoxDriver, REFLECTING, TRANSMISSIVE
+    mech = hf.Mechanism(hf.H2_LI, rtol=1e-8, atol=1e-12)
+    Q, dx = channel_2d(mech, **CHANNEL)
+    o = OracleBoxDriver(Oracle(hf.H2_LI), CHANNEL["nx"], CHANNEL["ny"], dx, dx,
+                        (REFLECTING, TRANSMISSIVE, REFLECTING, REFLECTING), cfl=0.5, rtol=1e-8, atol=1e-12)
+    o.Q = Q.copy()
+    n, t0 = 0, time.perf_counter()
+    while n == 0 or time.perf_counter() - t0 < target_s:
+        o.step()
+        n += 1
+    return n, time.perf_counter() - t0, o.Q.copy()
+
+
 def state_difference(a, b, ns):
-    """Max over cells and components of |a - b| / scale: species scaled by
-    max(|rhoY|, 1e-6 rho), momenta by max(|rho u|, rho x 1 m/s), energy by |rhoE|."""
+    """Largest differences between two conserved states (nv, ...), per kind:
+    species |d rhoY| / max(|rhoY|, 1e-6 rho); velocity |d(rho u)| / rho in m/s
+    (both directions); total energy |d rhoE| / |rhoE|."""
     rho = np.abs(b[:ns]).sum(0)
-    scale = np.abs(b).copy()
-    scale[:ns] = np.maximum(scale[:ns], 1e-6 * rho)
-    scale[ns:ns + 2] = np.maximum(scale[ns:ns + 2], rho)
-    return float((np.abs(a - b) / np.maximum(scale, 1e-300)).max())
+    sp = np.abs(a[:ns] - b[:ns]) / np.maximum(np.abs(b[:ns]), 1e-6 * rho)
+    vel = np.abs(a[ns:ns + 2] - b[ns:ns + 2]) / rho
+    en = np.abs(a[ns + 2] - b[ns + 2]) / np.abs(b[ns + 2])
+    return {"species_rel": float(sp.max()), "velocity_abs_m_s": float(vel.max()), "energy_rel": float(en.max())}
 
 
 def cpu_baseline(U, dt, target_s=15.0, rtol=1e-8, atol=1e-15, mech_path=None, label="H2"):
@@ -197,6 +245,8 @@ def main():
     ap.add_argument("--dme-cells", type=int, default=1 << 18, help="DME mixed-precision workload size (0 skips it)")
     ap.add_argument("--shock-tube-steps", type=int, default=1000,
                     help="Strang steps of the 1-D 2000-cell workload through the step loop (0 skips it)")
+    ap.add_argument("--channel-steps", type=int, default=200,
+                    help="Strang steps of the 2-D channel's 1024 x 256 base level (0 skips it)")
     ap.add_argument("--copy-ref-mib", type=int, default=0,
                     help="also run a device-to-device copy of this many MiB (reference for HBM counter scaling)")
     args = ap.parse_args()
@@ -295,6 +345,22 @@ def main():
                 "first_order_stages": tdrv.first_order_stages, "retries": tdrv.retries, "repairs": tdrv.repairs}
         log(f"1-D shock tube: {tube['steps_per_s']:.4g} steps/s, {tube['ms_per_step']:.3f} ms/step")
 
+    chan = None
+    if args.channel_steps:
+        log(f"2-D channel: 1024 x 256 base level, {args.channel_steps} Strang steps")
+        cwall, cdrv, _ = channel_run(torch, hf, args.channel_steps, 1)
+        ncell = CHANNEL["nx"] * CHANNEL["ny"]
+        chan = {"workload": "2-D reacting channel, base level only (no refinement): 1024 x 256 cells over 0.2 x 0.05 m "
+                            "(H2_Li), stoichiometric H2/air at 300 K / 1 atm with a 2500 K / 30 atm strip in "
+                            "x < 0.01 m, +-1 % density blocks of 4 x 4 cells at fixed p (seed 3); reflecting left, top "
+                            "and bottom, transmissive right; NND + HLLC, Heun, CFL 0.5, Strang chemistry (batched "
+                            "FP64 form, rtol 1e-8 atol 1e-12) over every cell",
+                "steps": args.channel_steps, "steps_per_s": args.channel_steps / cwall,
+                "Mcell_updates_per_s": ncell * 4 * args.channel_steps / cwall / 1e6,
+                "ms_per_step": cwall / args.channel_steps * 1e3, "simulated_time_s": cdrv.time,
+                "first_order_stages": cdrv.first_order_stages, "retries": cdrv.retries, "repairs": cdrv.repairs}
+        log(f"2-D channel: {chan['steps_per_s']:.4g} steps/s, {chan['Mcell_updates_per_s']:.4g} Mcell-updates/s")
+
     head = res["batched"]
     kern_s = np.mean(head["kernel_ms"]) * 1e-3
     achieved = head["flops"] / kern_s / 1e12
@@ -344,6 +410,8 @@ def main():
                             **dme}
     if tube is not None:
         line["shock_tube_1d"] = tube
+    if chan is not None:
+        line["channel_2d"] = chan
     if not args.no_cpu:
         if tube is not None:
             log("1-D shock tube: CPU oracle step loop")
@@ -354,8 +422,16 @@ def main():
                 "sample": f"the first {nc} steps of the same run, the oracle's step loop (C++ flux stage and BDF, "
                           f"OpenMP chemistry), {cwall:.1f} s"}
             line["shock_tube_1d"]["parity_vs_oracle"] = {
-                "steps": nc, "max_rel_state_difference": state_difference(gstate.reshape(cstate.shape), cstate,
-                                                                           hf.Mechanism(hf.H2_LI).ns)}
+                "steps": nc, **state_difference(gstate.reshape(cstate.shape), cstate, hf.Mechanism(hf.H2_LI).ns)}
+        if chan is not None:
+            log("2-D channel: CPU oracle step loop")
+            nc, cw, cstate = channel_cpu(hf)
+            _, _, gstate = channel_run(torch, hf, nc, 0)
+            line["channel_2d"]["cpu_baseline"] = {
+                "value": nc / cw, "unit": "Strang steps/s", "cores": len(os.sched_getaffinity(0)), "kind": "port",
+                "sample": f"the first {nc} steps of the same run, the oracle's step loop, {cw:.1f} s"}
+            line["channel_2d"]["parity_vs_oracle"] = {
+                "steps": nc, **state_difference(gstate, cstate, hf.Mechanism(hf.H2_LI).ns)}
         log("parity sample vs the converged oracle")
         line["parity_vs_oracle"] = {k: parity(U0, dt, v) for k, v in outs.items()}
         log("CPU baseline")

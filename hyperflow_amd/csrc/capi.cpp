@@ -474,7 +474,7 @@ int hf_flux_stage(hf_ctx* c, const hf_box* boxes, int nbox, const double* Uin, d
         out_extent = std::max<int64_t>(out_extent, B.out_offset + (nv - 1) * B.out_plane + (B.ny - 1) * B.out_pitch + B.nx);
         cells += (int64_t)B.nx * B.ny;
     }
-    const int TX = one_d ? 255 : 15, TY = one_d ? 1 : 15;
+    const int TX = one_d ? c->ops->flux_tx1 : c->ops->flux_tx2, TY = one_d ? 1 : c->ops->flux_ty2;
     std::vector<int32_t> tiles;
     for (int b = 0; b < nbox; ++b)
         for (int tj = 0; tj < (boxes[b].ny + TY - 1) / TY; ++tj)

@@ -509,16 +509,19 @@ int chem(const ChemLaunch& L) {
 }
 
 // ------------------------------------------------------------------ flux stage
-constexpr int TX2 = 15, TY2 = 15, TX1 = 255;
-
-// The LDS-staged tiles hold (TX+4)(TY+4) cells of NV doubles: built for small mechanisms.
-constexpr bool kFlux = Topo::NS <= 16;
+// The tile and its face fluxes are staged in LDS: (TX+4)(TY+4) cells and
+// (TX+1)(TY+1) faces of NV doubles. 15 x 15 (one 256-lane workgroup) for small
+// mechanisms; 7 x 7 (one wave, 63 KB at NV = 42) for large ones; 1-D rows of
+// 255 or 63 cells.
+constexpr bool kLargeState = Topo::NS > 16;
+constexpr int TX2 = kLargeState ? 7 : 15, TY2 = TX2, TX1 = kLargeState ? 63 : 255;
+static_assert(sizeof(double) * (Topo::NS + 3) * ((TX2 + 4) * (TY2 + 4) + (TX2 + 1) * (TY2 + 1)) +
+                  sizeof(double) * (TX2 + 4) * (TY2 + 4) <= 65536,
+              "flux tile exceeds 64 KB of LDS");
 
 int flux(const FluxLaunch& L) {
     if (L.ntiles == 0) return HF_OK;
-    if constexpr (!kFlux) {
-        return HF_EINVAL;
-    } else {
+    {
     StreamTimer timer(L.kernel_ms, L.stream);
     const unsigned grid = (unsigned)((L.ntiles + 7) / 8) * 8;   // XCD remap needs a multiple of 8
     if (L.one_d) {
@@ -1029,7 +1032,7 @@ int tangent(const AdjointLaunch& L) { return sens(L, true); }
 extern const TopoOps HF_OPS_NAME(HF_TOPO);
 const TopoOps HF_OPS_NAME(HF_TOPO) = {HF_STR(HF_TOPO), Topo::topology, Topo::NS, Topo::NR, sizeof(MD), fill, chem,
                                       chem_workspace, flux, props, adjoint, tangent, sens_workspace,
-                                      box_ops};
+                                      box_ops, TX2, TY2, TX1};
 #endif
 
 }  // namespace hf

@@ -46,7 +46,7 @@ struct FluxLaunch {
     const double* Uin;
     double* Uout;
     int first_order;
-    int one_d;                      // x-only stage: tiles are TX1 x 1 (see flux_tile_shape)
+    int one_d;                      // x-only stage: tiles are flux_tx1 x 1, else flux_tx2 x flux_ty2
     unsigned long long* first_failed;  // device scalar
     unsigned long long* counters;   // device: [faces, first-order sides]
     hipStream_t stream;
@@ -111,6 +111,7 @@ struct TopoOps {
     int (*tangent)(const AdjointLaunch& L);
     size_t (*sens_workspace)(int64_t ncell, int nsteps, int ckpt_every);
     int (*box_ops)(const BoxLaunch& L);
+    int flux_tx2, flux_ty2, flux_tx1;               // flux tile shapes: 2-D TX x TY, 1-D TX x 1
 };
 
 // All compiled topologies (null-terminated).

@@ -190,3 +190,42 @@ def shock_tube_1d(mech, nx=2000, length=0.1, hot_x=0.01, hot=(2000.0, 20.0), col
     xc = (np.arange(nx) + 0.5) * dx
     Q = np.where((xc < hot_x)[None, :], q_hot[:, None], q_cold[:, None])
     return Q.reshape(mech.nv, 1, nx).copy(), dx
+
+
+def channel_2d(mech, nx=1024, ny=256, length=0.2, seed=3, hot_x=0.01, hot=(2500.0, 30.0), cold=(300.0, 1.0),
+               block=4, amplitude=0.01):
+    """The 2-D channel's base level: nx x ny cells over length x (length ny/nx)
+    of stoichiometric H2/air (X H2:O2:N2 = 2:1:3.76, BlockAMR.cpp:67) at
+    (T, p atm) = cold, with a hot strip x < hot_x at (T, p atm) = hot of the
+    same fresh composition; at rest. Density is multiplied by 1 + amplitude r,
+    r uniform in [-1, 1] per block x block cells (numpy default_rng(seed)), at
+    fixed p (T scales by 1 / (1 + amplitude r)). Returns (Q (nv, ny, nx), dx)."""
+    import numpy as np
+    atm = 101325.0
+    X = np.zeros(mech.ns)
+    for name, x in (("H2", 2.0), ("O2", 1.0), ("N2", 3.76)):
+        X[mech.species.index(name)] = x
+    X /= X.sum()
+    W = mech.W
+    Y = X * W / (X * W).sum()
+    dx = length / nx
+    rng = np.random.default_rng(seed)
+    nbx, nby = (nx + block - 1) // block, (ny + block - 1) // block
+    r = rng.uniform(-1.0, 1.0, size=(nby, nbx))
+    xc = (np.arange(nx) + 0.5) * dx
+    hot_col = xc < hot_x
+    Q = np.empty((mech.nv, ny, nx))
+    # one state per (block, hot/cold) pair: blocks are uniform in r
+    for by in range(nby):
+        for bx in range(nbx):
+            f = 1.0 + amplitude * r[by, bx]
+            j0, i0 = by * block, bx * block
+            cols = np.arange(i0, min(i0 + block, nx))
+            for is_hot in (False, True):
+                sel = cols[hot_col[cols] == is_hot]
+                if sel.size == 0:
+                    continue
+                T, p = hot if is_hot else cold
+                q = mech.state_from_TPY(T / f, p * atm, Y)
+                Q[:, j0:j0 + block, sel] = q[:, None, None]
+    return Q, dx

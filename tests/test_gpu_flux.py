@@ -114,3 +114,34 @@ def test_properties_and_stable_dt_match_oracle(oracle_h2, mech_h2):
     rho = U[:9].sum(0)
     rate = (np.abs(U[9] / rho) + ar) / 1e-4 + (np.abs(U[10] / rho) + ar) / 2e-4
     assert dt == pytest.approx(0.5 / rate.max(), rel=1e-13)
+
+
+@pytest.mark.parametrize("nx,ny,walls,first_order", [(23, 19, WALL_XLO | WALL_YHI, False), (16, 9, 0, True)])
+def test_dme_flux_stage_matches_oracle(oracle_dme, nx, ny, walls, first_order):
+    """The 39-species DME mechanism (NV = 42): 7 x 7 tiles of one wave."""
+    print(f"[gpu] DME flux parity {nx}x{ny} walls={walls} first_order={first_order}", flush=True)
+    mech = hf.Mechanism(hf.DME_SK39)
+    G = smooth_box(oracle_dme, nx, ny, seed=nx * 100 + ny + 7)
+    dtdx, dtdy = 2e-4, 1.5e-4
+    r, ref, fb, ff = oracle_dme.flux_stage(G, dtdx, dtdy, first_order=first_order, walls=walls)
+    assert r == 0
+    out = torch.zeros((G.shape[0], ny, nx), dtype=torch.float64, device="cuda")
+    st = FluxSweep(mech).stage([box_for_arrays(G, out, dtdx, dtdy, walls)], dev(G), out, first_order)
+    err = normalised_error(out.cpu().numpy(), ref)
+    print(f"[gpu] DME max normalised error {err:.3e} stats {st}", flush=True)
+    assert err < TOL
+    assert st["first_order_sides"] == fb
+
+
+def test_dme_flux_stage_one_dimensional(oracle_dme):
+    mech = hf.Mechanism(hf.DME_SK39)
+    nx = 150
+    q = random_states(oracle_dme, 2, 23, umax=50.0)
+    G = np.empty((42, 5, nx + 4))
+    G[:, :, : (nx + 4) // 3] = q[0][:, None, None]
+    G[:, :, (nx + 4) // 3:] = q[1][:, None, None]
+    r, ref, fb, ff = oracle_dme.flux_stage(G, 3e-4, 0.0, walls=WALL_XLO, dims=1)
+    assert r == 0
+    out = torch.zeros((42, 1, nx), dtype=torch.float64, device="cuda")
+    FluxSweep(mech).stage([box_for_arrays(G, out, 3e-4, 0.0, WALL_XLO)], dev(G), out)
+    assert normalised_error(out.cpu().numpy(), ref) < TOL
