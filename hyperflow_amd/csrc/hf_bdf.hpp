@@ -371,25 +371,31 @@ struct Bdf {
         const MT* js = opaque(jsave);
         bool ok = true;
         if constexpr (SMALL) {
+            // The finished rows stay in registers for the rows below them
+            // (N*N values, local to this block); reading them back from the
+            // slot would put a dependent memory round trip on every row.
+            MT fin[N * N];
             #pragma unroll
             for (int i = 0; i < N; ++i) {
                 MT row[N];
                 #pragma unroll
                 for (int j = 0; j < N; ++j) row[j] = (i == j ? MT(1) : MT(0)) - g * js[(i * N + j) * ZSTRIDE];
-                const MT* up = opaque(lu);
                 #pragma unroll
                 for (int k = 0; k < i; ++k) {
-                    const MT l = row[k] * up[(k * N + k) * ZSTRIDE];
+                    const MT l = row[k] * fin[k * N + k];
                     row[k] = l;
                     #pragma unroll
-                    for (int j = k + 1; j < N; ++j) row[j] = fma(-l, up[(k * N + j) * ZSTRIDE], row[j]);
+                    for (int j = k + 1; j < N; ++j) row[j] = fma(-l, fin[k * N + j], row[j]);
                 }
                 const MT p = row[i];
                 ok = ok && fabs(p) > __FLT_MIN__ && __builtin_isfinite(p);
                 row[i] = MT(1) / p;
                 MT* wp = opaque(lu);
                 #pragma unroll
-                for (int j = 0; j < N; ++j) wp[(i * N + j) * ZSTRIDE] = row[j];
+                for (int j = 0; j < N; ++j) {
+                    fin[i * N + j] = row[j];
+                    wp[(i * N + j) * ZSTRIDE] = row[j];
+                }
             }
         } else {
             #pragma unroll 1
