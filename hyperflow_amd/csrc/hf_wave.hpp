@@ -241,13 +241,13 @@ struct WaveBdf {
     static constexpr int NS = T::NS, N = NS - 1, QMAX = 5;
     static constexpr double UROUND = 2.220446049250313e-16;
     WaveGas<T> g;                // held by value: the whole integrator stays in registers
-    MT* jcol;                    // LDS: Jacobian column j at jcol[j * 64 + lane]
+    MT* jg;                      // global, this wave's slot: Jacobian entry (lane, j) at jg[j * 64 + lane]
+    MT* lu;                      // LDS: factored Newton matrix, entry (lane, j) at lu[j * 64 + lane]
     double* coef;                // LDS: tau[QMAX + 2], l[QMAX + 1]
     int lane;
     bool own;                    // lane < N
     double z[QMAX + 1];          // Nordsieck history of this lane's unknown
     double ewt, acor;
-    MT row[N];                   // this lane's row of the factored Newton matrix
     double tq1, tq2, tq3, tq4, tq5;
     double tn, h, hprime, hscale, eta, etamax, gamma, gammap, gamrat, rl1, crate, acnrm, saved_tq5;
     double rtol, atol;
@@ -255,8 +255,9 @@ struct WaveBdf {
     bool force_setup, jfresh;
     WaveCounters c;
 
-    __device__ __forceinline__ WaveBdf(const MechData<T>& d, double* scratch, MT* jc, double* cf, double rt, double at)
-        : g(d, scratch), jcol(jc), coef(cf), rtol(rt), atol(at) {
+    __device__ __forceinline__ WaveBdf(const MechData<T>& d, double* scratch, MT* jslot, MT* lds_lu, double* cf,
+                                       double rt, double at)
+        : g(d, scratch), jg(jslot), lu(lds_lu), coef(cf), rtol(rt), atol(at) {
         lane = threadIdx.x & 63;
         own = lane < N;
     }
@@ -411,8 +412,9 @@ struct WaveBdf {
     }
 
     // ---------------------------------------------------------------- linear algebra
-    // Difference-quotient Jacobian (cvLsDenseDQJac) into the LDS columns;
-    // y, f: this lane's unknown and f(y).
+    // Difference-quotient Jacobian (cvLsDenseDQJac) into the wave's global
+    // slot (read back only when the Newton matrix is re-formed); y, f: this
+    // lane's unknown and f(y).
     __device__ __forceinline__ bool dq_jacobian(double y, double f) {
         const double fnorm = wrms(f);
         const double min_inc = fnorm != 0 ? 1000.0 * fabs(h) * UROUND * N * fnorm : 1.0;
@@ -423,46 +425,56 @@ struct WaveBdf {
             double fj;
             ++c.rhs;
             if (!g.rhs(lane == j ? yj + inc : y, fj)) return false;
-            jcol[j * 64 + lane] = (MT)((fj - f) * (1.0 / inc));
+            jg[j * 64 + lane] = (MT)((fj - f) * (1.0 / inc));
         }
         return true;
     }
-    // I - gamma J factored without pivoting, one row per lane (reciprocal pivots on the diagonal).
+    // I - gamma J factored in LDS without pivoting (reciprocal pivots on the
+    // diagonal), one row per lane. Step k reads pivot row k as broadcasts
+    // (one address for the whole wave) and each lane below it updates its own
+    // row; rolled loops keep the register footprint small, which the
+    // unrolled register form (38 values per lane and ~700 lane broadcasts
+    // into scalar registers) did not.
     __device__ __forceinline__ bool factor() {
         const MT gm = (MT)gamma;
-        #pragma unroll
-        for (int j = 0; j < N; ++j) row[j] = (lane == j ? MT(1) : MT(0)) - gm * jcol[j * 64 + lane];
+        #pragma unroll 2
+        for (int j = 0; j < N; ++j) lu[j * 64 + lane] = (lane == j ? MT(1) : MT(0)) - gm * jg[j * 64 + lane];
+        __syncthreads();
         bool ok = true;
-        #pragma unroll
+        #pragma unroll 1
         for (int k = 0; k < N; ++k) {
-            const MT p = lane_of(row[k], k);
+            const MT p = lu[k * 64 + k];
             ok = ok && fabs(p) > __FLT_MIN__ && __builtin_isfinite(p);
             const MT inv = MT(1) / p;
-            if (lane == k) row[k] = inv;
-            const MT l = row[k] * inv;
             const bool below = lane > k;
-            if (below) row[k] = l;
-            #pragma unroll
-            for (int j = k + 1; j < N; ++j) {
-                const MT pj = lane_of(row[j], k);
-                if (below) row[j] = fma(-l, pj, row[j]);
+            MT l = 0;
+            if (below) {
+                l = lu[k * 64 + lane] * inv;
+                lu[k * 64 + lane] = l;
             }
+            #pragma unroll 4
+            for (int j = k + 1; j < N; ++j) {
+                const MT pj = lu[j * 64 + k];
+                if (below) lu[j * 64 + lane] = fma(-l, pj, lu[j * 64 + lane]);
+            }
+            if (lane == k) lu[k * 64 + k] = inv;
+            __syncthreads();
         }
         return ok;
     }
     // Solve (I - gamma J) x = b with b, x one component per lane.
     __device__ __forceinline__ double solve(double bd) const {
         MT b = (MT)bd;
-        #pragma unroll
+        #pragma unroll 2
         for (int k = 0; k < N; ++k) {
             const MT bk = lane_of(b, k);
-            if (lane > k) b = fma(-row[k], bk, b);
+            if (lane > k) b = fma(-lu[k * 64 + lane], bk, b);
         }
-        #pragma unroll
+        #pragma unroll 2
         for (int k = N - 1; k >= 0; --k) {
-            if (lane == k) b *= row[k];
+            if (lane == k) b *= lu[k * 64 + k];
             const MT bk = lane_of(b, k);
-            if (lane < k) b = fma(-row[k], bk, b);
+            if (lane < k) b = fma(-lu[k * 64 + lane], bk, b);
         }
         return own ? (double)b : 0.0;
     }

@@ -354,7 +354,7 @@ chem_lane_kernel(const MechData<TT>* __restrict__ dm, const double* __restrict__
 constexpr bool kWaveChem = Topo::NS > 16;
 
 template <class TT, class MT>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(64, 2)
 chem_wave_kernel(const MechData<TT>* __restrict__ dm, const double* __restrict__ U, long ncell, long ld,
                  const double* __restrict__ dt, int per_cell, double* __restrict__ out, long ldo,
                  int8_t* __restrict__ status, unsigned long long* __restrict__ first_failed, ChemParams p,
@@ -362,9 +362,13 @@ chem_wave_kernel(const MechData<TT>* __restrict__ dm, const double* __restrict__
     constexpr int NS = TT::NS, N = NS - 1, NR = TT::NR;
     __shared__ double scratch[2 * NS + NR];
     __shared__ double coef[13];
-    __shared__ MT jcol[N * 64];
+    __shared__ MT lu[N * 64];
     const int lane = threadIdx.x;
-    WaveBdf<TT, MT> bdf(*dm, scratch, jcol, coef, p.rtol, p.atol);
+    // this wave's Jacobian slot: N x 64 entries inside the per-lane Jacobian
+    // region, which holds N x N entries for each of at least 64 x (the waves
+    // per CU of the launch, or 4) lanes per CU: room for every wave of the grid
+    MT* jslot = static_cast<MT*>(w.jsave) + (long)blockIdx.x * N * 64;
+    WaveBdf<TT, MT> bdf(*dm, scratch, jslot, lu, coef, p.rtol, p.atol);
     WaveGas<TT>& gas = bdf.g;
     unsigned long long tot[7] = {0, 0, 0, 0, 0, 0, 0};
     int max_steps_cell = 0;
