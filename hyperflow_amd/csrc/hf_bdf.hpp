@@ -65,6 +65,17 @@ struct Bdf {
         return *(double*)((zs_double*)zs + (j * N + i) * ZSTRIDE);
     }
     __device__ __forceinline__ double zrow(int j, int i) const { return Z(j, i); }
+    // The Nordsieck column of unknown i staged in registers: the updates below
+    // read and write it with one batch of slab loads and one of stores per
+    // unknown, instead of a dependent slab round trip per update.
+    __device__ __forceinline__ void zload(int i, double* zz) const {
+        #pragma unroll
+        for (int j = 0; j <= QMAX; ++j) zz[j] = Z(j, i);
+    }
+    __device__ __forceinline__ void zstore(int i, const double* zz) const {
+        #pragma unroll
+        for (int j = 0; j <= QMAX; ++j) Z(j, i) = zz[j];
+    }
     // Step-size history tau and BDF coefficients l share the lane's LDS slab,
     // after the Nordsieck array: they are indexed by the current order.
     __device__ __forceinline__ double& slab(int k) const {
@@ -88,14 +99,19 @@ struct Bdf {
     }
 
     __device__ __forceinline__ void rescale() {
-        double factor = eta;
+        double fac[QMAX + 1];
+        fac[0] = 1.0;
+        fac[1] = eta;
         #pragma unroll
-        for (int j = 1; j <= QMAX; ++j) {
-            if (j <= q) {
-                #pragma unroll
-                for (int i = 0; i < N; ++i) Z(j, i) *= factor;
-            }
-            factor *= eta;
+        for (int j = 2; j <= QMAX; ++j) fac[j] = fac[j - 1] * eta;
+        #pragma unroll
+        for (int i = 0; i < N; ++i) {
+            double zz[QMAX + 1];
+            zload(i, zz);
+            #pragma unroll
+            for (int j = 1; j <= QMAX; ++j)
+                if (j <= q) zz[j] *= fac[j];
+            zstore(i, zz);
         }
         h = hscale * eta;
         hprime = h;
@@ -104,24 +120,30 @@ struct Bdf {
     __device__ __forceinline__ void predict() {
         tn += h;
         #pragma unroll
-        for (int k = 1; k <= QMAX; ++k)
+        for (int i = 0; i < N; ++i) {
+            double zz[QMAX + 1];
+            zload(i, zz);
             #pragma unroll
-            for (int j = QMAX; j >= k; --j)
-                if (k <= q && j <= q) {
-                    #pragma unroll
-                    for (int i = 0; i < N; ++i) Z(j - 1, i) += Z(j, i);
-                }
+            for (int k = 1; k <= QMAX; ++k)
+                #pragma unroll
+                for (int j = QMAX; j >= k; --j)
+                    if (k <= q && j <= q) zz[j - 1] += zz[j];
+            zstore(i, zz);
+        }
     }
     __device__ __forceinline__ void restore(double saved_t) {
         tn = saved_t;
         #pragma unroll
-        for (int k = 1; k <= QMAX; ++k)
+        for (int i = 0; i < N; ++i) {
+            double zz[QMAX + 1];
+            zload(i, zz);
             #pragma unroll
-            for (int j = QMAX; j >= k; --j)
-                if (k <= q && j <= q) {
-                    #pragma unroll
-                    for (int i = 0; i < N; ++i) Z(j - 1, i) -= Z(j, i);
-                }
+            for (int k = 1; k <= QMAX; ++k)
+                #pragma unroll
+                for (int j = QMAX; j >= k; --j)
+                    if (k <= q && j <= q) zz[j - 1] -= zz[j];
+            zstore(i, zz);
+        }
     }
 
     // Order q -> q-1: subtract the top column's contribution.
@@ -143,10 +165,15 @@ struct Bdf {
         }
         #pragma unroll
         for (int i = 0; i < N; ++i) {
-            const double top = zrow(q, i);
+            double zz[QMAX + 1];
+            zload(i, zz);
+            double top = 0;
+            #pragma unroll
+            for (int j = 0; j <= QMAX; ++j) top += j == q ? zz[j] : 0.0;
             #pragma unroll
             for (int j = 2; j < QMAX; ++j)
-                if (j < q) Z(j, i) -= top * lc[j];
+                if (j < q) zz[j] -= top * lc[j];
+            zstore(i, zz);
         }
     }
     // Order q -> q+1: new top column from the saved correction in zn[QMAX].
@@ -172,12 +199,15 @@ struct Bdf {
         const double A1 = (-alpha0 - alpha1) / prod;
         #pragma unroll
         for (int i = 0; i < N; ++i) {
-            const double nz = A1 * Z(QMAX, i);
+            double zz[QMAX + 1];
+            zload(i, zz);
+            const double nz = A1 * zz[QMAX];
             #pragma unroll
             for (int j = 2; j <= QMAX; ++j) {
-                if (j <= q) Z(j, i) += lc[j] * nz;
-                if (j == q + 1) Z(j, i) = nz;
+                if (j <= q) zz[j] += lc[j] * nz;
+                if (j == q + 1) zz[j] = nz;
             }
+            zstore(i, zz);
         }
     }
     __device__ __forceinline__ void adjust_order(int dq) {
@@ -532,18 +562,22 @@ struct Bdf {
             if (i <= q) TAU(i) = TAU(i - 1);
         if (q == 1 && nst > 1) TAU(2) = TAU(1);
         TAU(1) = h;
+        double lcv[QMAX + 1];
         #pragma unroll
-        for (int j = 0; j <= QMAX; ++j)
-            if (j <= q) {
-                #pragma unroll
-                for (int i = 0; i < N; ++i) Z(j, i) = fma(LC(j), acor[i], Z(j, i));
-            }
+        for (int j = 0; j <= QMAX; ++j) lcv[j] = LC(j);
         --qwait;
-        if (qwait == 1 && q != QMAX) {
+        const bool save_acor = qwait == 1 && q != QMAX;
+        #pragma unroll
+        for (int i = 0; i < N; ++i) {
+            double zz[QMAX + 1];
+            zload(i, zz);
             #pragma unroll
-            for (int i = 0; i < N; ++i) Z(QMAX, i) = acor[i];
-            saved_tq5 = tq5;
+            for (int j = 0; j <= QMAX; ++j)
+                if (j <= q) zz[j] = fma(lcv[j], acor[i], zz[j]);
+            if (save_acor) zz[QMAX] = acor[i];
+            zstore(i, zz);
         }
+        if (save_acor) saved_tq5 = tq5;
         prepare_next(dsm);
         retry = false;
         phase = PH_START;
