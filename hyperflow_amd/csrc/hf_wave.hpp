@@ -59,15 +59,27 @@ struct WaveGas {
     static constexpr int NS = T::NS, N = NS - 1, NR = T::NR;
     const MechData<T>& d;
     double* lds;                 // per wave: C[NS], grt[NS], q[NR]
+    const int* inc;              // per wave (LDS): species incidences, reaction + nu * 65536
+    int inc_b, inc_e;            // this lane's species: inc[inc_b .. inc_e)
     int lane;
     bool sp;                     // this lane holds a species (lane < NS)
     double W, invW, tmid, lo[7], hi[7];
     double rho, e, t;            // cell constants and the warm-started temperature
 
-    __device__ __forceinline__ WaveGas(const MechData<T>& dd, double* scratch) : d(dd), lds(scratch) {
+    // inc_lds: LDS room for the incidence lists (2 NR NRMAX ints), filled
+    // here once per wave: the production-rate gather walks them on every
+    // evaluation, and from global memory each step of that walk was a
+    // dependent load.
+    __device__ __forceinline__ WaveGas(const MechData<T>& dd, double* scratch, int* inc_lds)
+        : d(dd), lds(scratch), inc(inc_lds) {
         lane = threadIdx.x & 63;
         sp = lane < NS;
         const int k = sp ? lane : 0;
+        const int ntot = d.inc_start[NS];
+        for (int p = lane; p < ntot; p += 64) inc_lds[p] = (int)d.inc_rxn[p] + (int)d.inc_nu[p] * 65536;
+        inc_b = lane < N ? (int)d.inc_start[lane] : 0;
+        inc_e = lane < N ? (int)d.inc_start[lane + 1] : 0;
+        __syncthreads();
         W = d.W[k];
         invW = d.invW[k];
         tmid = d.Tmid[k];
@@ -195,8 +207,10 @@ struct WaveGas {
         __syncthreads();
         double w = 0;
         if (lane < N) {
-            const int b = d.inc_start[lane], e2 = d.inc_start[lane + 1];
-            for (int p = b; p < e2; ++p) w = fma((double)d.inc_nu[p], lds[2 * NS + d.inc_rxn[p]], w);
+            for (int p = inc_b; p < inc_e; ++p) {
+                const int v = inc[p];
+                w = fma((double)(v >> 16), lds[2 * NS + (v & 0xffff)], w);
+            }
             f = w * W / rho;
         } else {
             f = 0;
@@ -255,9 +269,9 @@ struct WaveBdf {
     bool force_setup, jfresh;
     WaveCounters c;
 
-    __device__ __forceinline__ WaveBdf(const MechData<T>& d, double* scratch, MT* jslot, MT* lds_lu, double* cf,
-                                       double rt, double at)
-        : g(d, scratch), jg(jslot), lu(lds_lu), coef(cf), rtol(rt), atol(at) {
+    __device__ __forceinline__ WaveBdf(const MechData<T>& d, double* scratch, int* inc_lds, MT* jslot, MT* lds_lu,
+                                       double* cf, double rt, double at)
+        : g(d, scratch, inc_lds), jg(jslot), lu(lds_lu), coef(cf), rtol(rt), atol(at) {
         lane = threadIdx.x & 63;
         own = lane < N;
     }

@@ -363,12 +363,13 @@ chem_wave_kernel(const MechData<TT>* __restrict__ dm, const double* __restrict__
     __shared__ double scratch[2 * NS + NR];
     __shared__ double coef[13];
     __shared__ MT lu[N * 64];
+    __shared__ int inc[2 * NR * TT::NRMAX];
     const int lane = threadIdx.x;
     // this wave's Jacobian slot: N x 64 entries inside the per-lane Jacobian
     // region, which holds N x N entries for each of at least 64 x (the waves
     // per CU of the launch, or 4) lanes per CU: room for every wave of the grid
     MT* jslot = static_cast<MT*>(w.jsave) + (long)blockIdx.x * N * 64;
-    WaveBdf<TT, MT> bdf(*dm, scratch, jslot, lu, coef, p.rtol, p.atol);
+    WaveBdf<TT, MT> bdf(*dm, scratch, inc, jslot, lu, coef, p.rtol, p.atol);
     WaveGas<TT>& gas = bdf.g;
     unsigned long long tot[7] = {0, 0, 0, 0, 0, 0, 0};
     int max_steps_cell = 0;
