@@ -12,8 +12,9 @@
 // Nordsieck form, orders 1-5, WRMS error test, Newton with CVODE's
 // convergence test and Jacobian reuse, difference-quotient Jacobian
 // (cvLsDenseDQJac, the reference's configuration). The Jacobian is kept in
-// LDS column by column; I - gamma J is factored with each lane holding one
-// row in registers and the pivot rows broadcast with readlane.
+// a per-wave global slot (row = lane, one entry per column); I - gamma J is
+// formed and factored in LDS, each lane updating its own row against the
+// broadcast pivot row.
 #pragma once
 #include "hf_device.hpp"
 
