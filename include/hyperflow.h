@@ -170,7 +170,9 @@ int hf_stable_dt(hf_ctx *ctx, const double *U, int64_t ncell, int64_t ld, double
  * perturbs every cell's rate constants. All arrays host or device. Each step
  * is solved by Newton to a correction below 1e-15; a cell whose iteration
  * fails gets NaN outputs and the call returns HF_ECONV naming the lowest such
- * cell in hf_last_error. */
+ * cell in hf_last_error. Built for mechanisms of at most 16 species (the
+ * per-cell backward-Euler matrix is held in registers); a larger mechanism
+ * returns HF_EINVAL. */
 int hf_chem_adjoint(hf_ctx *ctx, int64_t ncell, const double *rho, const double *T0, const double *Y0, double h,
                     int nsteps, int ckpt_every, const double *lnA_shift, double *J, double *grad_lnA,
                     double *grad_T0, double *grad_Y0);
