@@ -15,8 +15,10 @@ Also reported: the DME mixed-precision workload (2^18 DME/air cells, seed 4,
 each), the 2-D flux sweep (one Euler stage over a 4096 x 1024 box,
 Mcell-updates/s, HBM roofline at 192 algorithmic bytes per cell-update),
 the 1-D plain workload end to end (2000 cells, 1000 Strang steps through the
-step loop), chemistry parity against the CPU oracle on a sample, and the CPU
-oracle's throughput on a bounded sample of the same workloads (all host
+step loop), the 2-D channel's base level end to end (1024 x 256, 200 Strang
+steps), the discrete adjoint of fixed-step chemistry (65536 cells, 200 steps,
+31 gradient components), parity against the CPU oracle on samples, and the
+CPU oracle's throughput on a bounded sample of the same workloads (all host
 cores).
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1]
@@ -185,6 +187,23 @@ def channel_cpu(hf, target_s=12.0):
     return n, time.perf_counter() - t0, o.Q.copy()
 
 
+ADJ_H, ADJ_STEPS, ADJ_CKPT = 1e-8, 200, 20
+
+
+def adjoint_cells(torch, hf, n):
+    """The adjoint workload's cells (H2/air, T0 in [1000,1500] K, p in [1,10]
+    atm, phi in [0.5,2], fresh mixture; seed 5) on the device: rho, T0 (from
+    the properties kernel), Y0 (n, ns)."""
+    mech = hf.Mechanism(hf.H2_LI)
+    U, _ = mech.synth_cells(2, 5, n, with_dt=False)
+    Ud = torch.from_numpy(U).to("cuda")
+    ns = mech.ns
+    rho = Ud[:ns].sum(0).contiguous()
+    Y0 = (Ud[:ns] / rho).T.contiguous()
+    _, _, T0 = mech.properties(Ud)
+    return mech, rho, T0.contiguous(), Y0
+
+
 def state_difference(a, b, ns):
     """Largest differences between two conserved states (nv, ...), per kind:
     species |d rhoY| / max(|rhoY|, 1e-6 rho); velocity |d(rho u)| / rho in m/s
@@ -247,6 +266,8 @@ def main():
                     help="Strang steps of the 1-D 2000-cell workload through the step loop (0 skips it)")
     ap.add_argument("--channel-steps", type=int, default=200,
                     help="Strang steps of the 2-D channel's 1024 x 256 base level (0 skips it)")
+    ap.add_argument("--adjoint-cells", type=int, default=65536,
+                    help="cells of the adjoint workload (200 fixed steps, checkpoints every 20; 0 skips it)")
     ap.add_argument("--copy-ref-mib", type=int, default=0,
                     help="also run a device-to-device copy of this many MiB (reference for HBM counter scaling)")
     args = ap.parse_args()
@@ -361,6 +382,26 @@ def main():
                 "first_order_stages": cdrv.first_order_stages, "retries": cdrv.retries, "repairs": cdrv.repairs}
         log(f"2-D channel: {chan['steps_per_s']:.4g} steps/s, {chan['Mcell_updates_per_s']:.4g} Mcell-updates/s")
 
+    adj = None
+    if args.adjoint_cells:
+        log(f"adjoint: {args.adjoint_cells} cells, {ADJ_STEPS} steps of {ADJ_H:g} s, checkpoints every {ADJ_CKPT}")
+        amech, arho, aT0, aY0 = adjoint_cells(torch, hf, args.adjoint_cells)
+        sens = hf.ChemSensitivity(amech)
+        sens.adjoint(arho, aT0, aY0, ADJ_H, ADJ_STEPS, ADJ_CKPT)       # warmup (workspace)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        aJ, ag_lnA, ag_T0, ag_Y0 = sens.adjoint(arho, aT0, aY0, ADJ_H, ADJ_STEPS, ADJ_CKPT)
+        torch.cuda.synchronize()
+        awall = time.perf_counter() - t0
+        nparam = ag_lnA.shape[1] + 1 + ag_Y0.shape[1]
+        adj = {"workload": f"discrete adjoint of fixed-step backward-Euler chemistry, {args.adjoint_cells} H2/air "
+                           f"cells (T0 [1000,1500] K, p [1,10] atm, phi [0.5,2], fresh mixture; seed 5), "
+                           f"{ADJ_STEPS} steps of {ADJ_H:g} s, checkpoints every {ADJ_CKPT}; J = T(t_end) + "
+                           f"integrated heat release and its gradient in {nparam} parameters "
+                           f"(ln A of {ag_lnA.shape[1]} reactions, T0, Y0)",
+               "cells_per_s": args.adjoint_cells / awall, "ms": awall * 1e3, "parameters": nparam}
+        log(f"adjoint: {adj['cells_per_s']:.4g} cells/s ({awall * 1e3:.1f} ms)")
+
     head = res["batched"]
     kern_s = np.mean(head["kernel_ms"]) * 1e-3
     achieved = head["flops"] / kern_s / 1e12
@@ -412,6 +453,8 @@ def main():
         line["shock_tube_1d"] = tube
     if chan is not None:
         line["channel_2d"] = chan
+    if adj is not None:
+        line["adjoint"] = adj
     if not args.no_cpu:
         if tube is not None:
             log("1-D shock tube: CPU oracle step loop")
@@ -432,6 +475,34 @@ def main():
                 "sample": f"the first {nc} steps of the same run, the oracle's step loop, {cw:.1f} s"}
             line["channel_2d"]["parity_vs_oracle"] = {
                 "steps": nc, **state_difference(gstate, cstate, hf.Mechanism(hf.H2_LI).ns)}
+        if adj is not None:
+            log("adjoint: oracle objective sample and CPU baseline")
+            from oracle import Oracle
+            o = Oracle(hf.H2_LI)
+            rho_h, T0_h, Y0_h = arho.cpu().numpy(), aT0.cpu().numpy(), aY0.cpu().numpy()
+            k = 256
+            r, Jo, _, _ = o.chem_fixed_be(rho_h[:k], T0_h[:k], Y0_h[:k], ADJ_H, ADJ_STEPS)
+            # the oracle's Newton (difference-quotient Jacobian) does not reach its
+            # roundoff-floor stop on a few hot cells; compare where it does
+            okc = np.isfinite(Jo)
+            line["adjoint"]["parity_vs_oracle"] = {
+                "cells": int(okc.sum()), "of": k,
+                "objective_rel": float(np.max(np.abs(aJ.cpu().numpy()[:k][okc] - Jo[okc]) / np.abs(Jo[okc])))}
+            n, t = 256, 0.0
+            while True:
+                t = time.perf_counter()
+                o.chem_fixed_be(rho_h[:n], T0_h[:n], Y0_h[:n], ADJ_H, ADJ_STEPS)
+                t = time.perf_counter() - t
+                if t >= 5.0 or n >= rho_h.size:
+                    break
+                n = int(min(rho_h.size, max(2 * n, n * 10.0 / max(t, 1e-3))))
+            fwd = n / t
+            line["adjoint"]["cpu_baseline"] = {
+                "value": fwd, "unit": "cells/s (objective only)", "cores": len(os.sched_getaffinity(0)), "kind": "port",
+                "sample": f"first {n} cells, the oracle's fixed-step integration (OpenMP), {t:.1f} s; a central-"
+                          f"difference gradient in the same {adj['parameters']} parameters needs "
+                          f"{2 * adj['parameters'] + 1} such integrations per cell: {fwd / (2 * adj['parameters'] + 1):.4g} "
+                          "cells/s"}
         log("parity sample vs the converged oracle")
         line["parity_vs_oracle"] = {k: parity(U0, dt, v) for k, v in outs.items()}
         log("CPU baseline")
