@@ -14,7 +14,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libhyperflow_hip.so"
 
 HF_OK, HF_EINVAL, HF_EDOMAIN, HF_ECONV, HF_ENOMEM, HF_EDEVICE, HF_EBUSY = range(7)
 HF_CHEM_PLAIN, HF_CHEM_BATCHED = 0, 1
-HF_BC_TRANSMISSIVE, HF_BC_REFLECTING, HF_BC_PERIODIC = 0, 1, 2
+HF_BC_TRANSMISSIVE, HF_BC_REFLECTING, HF_BC_PERIODIC, HF_BC_INFLOW, HF_BC_WEDGE = 0, 1, 2, 3, 4
 HF_PREC_FP64, HF_PREC_MIXED = 0, 1
 
 ERROR_NAMES = {0: "HF_OK", 1: "HF_EINVAL", 2: "HF_EDOMAIN", 3: "HF_ECONV", 4: "HF_ENOMEM", 5: "HF_EDEVICE",
@@ -41,6 +41,10 @@ class hf_box(Structure):
                 ("in_offset", c_int64), ("in_pitch", c_int64), ("in_plane", c_int64),
                 ("out_offset", c_int64), ("out_pitch", c_int64), ("out_plane", c_int64),
                 ("dtdx", c_double), ("dtdy", c_double)]
+
+
+class hf_patch(Structure):
+    _fields_ = [("offset", c_int64), ("pitch", c_int64), ("plane", c_int64)]
 
 
 class hf_flux_stats(Structure):
@@ -74,6 +78,11 @@ _SIGNATURES = {
     "hf_chem_tangent": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_double, c_int, c_void_p,
                                 c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "hf_fill_ghosts": (c_int, [c_void_p, c_void_p, POINTER(hf_box), POINTER(c_int32)]),
+    "hf_fill_ghosts_flow": (c_int, [c_void_p, c_void_p, POINTER(hf_box), POINTER(c_int32), _dp]),
+    "hf_interp_coarse_to_fine": (c_int, [c_void_p, c_void_p, POINTER(hf_patch), c_void_p, POINTER(hf_patch),
+                                         POINTER(c_int32), POINTER(c_int64), POINTER(c_int64)]),
+    "hf_average_down": (c_int, [c_void_p, c_void_p, POINTER(hf_patch), c_void_p, POINTER(hf_patch),
+                                POINTER(c_int32)]),
     "hf_repair_roundoff": (c_int, [c_void_p, c_void_p, POINTER(hf_box), POINTER(c_int64), POINTER(c_int64)]),
     "hf_validate_box": (c_int, [c_void_p, c_void_p, POINTER(hf_box), c_double, c_double, _dp, POINTER(c_int64)]),
     "hf_lincomb": (c_int, [c_void_p, c_void_p, c_void_p, c_double, c_double, c_int64]),

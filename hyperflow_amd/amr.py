@@ -1,0 +1,125 @@
+"""Python mirror of the level transfers around the flux sweep, on the HIP
+kernels: the physical ghost fill with FlowBoundary's inflow and wedge bottom
+(HyperFlow_V3/src/FlowBoundary.hpp:7-37, applied by BlockAMR::fill,
+BlockAMR.cpp:137-152), coarse-to-fine conservative interpolation
+(ConservativeInterp::interp, ConservativeInterp.cpp:18-51) and average-down
+(BlockAMR::average, BlockAMR.cpp:157-159).
+
+Arrays are torch float64 device tensors in the reference's SoA layout, one
+(nv, h, w) tensor per patch, i fastest. A patch tensor carries the index of
+its element [:, 0, 0] as ``lo = (i0, j0)`` in its level's index space. Every
+computation runs in libhyperflow_hip.so; there is no CPU path.
+"""
+from __future__ import annotations
+
+from ctypes import byref, c_double, c_int32, c_int64
+
+from ._lib import HF_BC_INFLOW, HF_BC_WEDGE, check, hf_box, hf_patch, load  # noqa: F401
+from .chemistry import Mechanism
+
+
+def _mech(mechanism):
+    return mechanism if isinstance(mechanism, Mechanism) else Mechanism(mechanism)
+
+
+def _device_f64(t, name):
+    if not (getattr(t, "is_cuda", False) and str(t.dtype) == "torch.float64" and t.is_contiguous() and t.dim() == 3):
+        raise ValueError(f"{name} must be a contiguous float64 (nv, h, w) device tensor")
+
+
+def _patch(t, lo):
+    """hf_patch addressing t[n, j - lo[1], i - lo[0]] as cell (i, j)."""
+    h, w = t.shape[1], t.shape[2]
+    return hf_patch(-(int(lo[1]) * w + int(lo[0])), w, h * w)
+
+
+def wedge_column(prob_lo_x, dx, wedge_start):
+    """Index of the first column whose centre lies at or past the wedge tip
+    (x = lo + (i + 0.5) dx >= wedgeStart, BlockAMR.cpp:225-226; FlowBoundary.hpp:27-28).
+    The reference requires the tip on a base-grid x face (BlockAMR.cpp:41-47)."""
+    index = (wedge_start - prob_lo_x) / dx
+    if abs(index - round(index)) > 1e-10:
+        raise ValueError("wedge tip must align with a base-grid x face")
+    return int(round(index))
+
+
+def fill_ghosts(mechanism, U, bc, inlet=None, wedge_i=0):
+    """Fill the two ghost layers of the ghosted box U (nv, ny+4, nx+4) in place.
+
+    bc = (x_lo, x_hi, y_lo, y_hi) from HF_BC_TRANSMISSIVE, HF_BC_REFLECTING,
+    HF_BC_PERIODIC, HF_BC_INFLOW (needs inlet: nv conserved values) and
+    HF_BC_WEDGE (y_lo only; wall from column wedge_i on).
+    """
+    m = _mech(mechanism)
+    _device_f64(U, "U")
+    nv, gy, gx = U.shape
+    if nv != m.nv:
+        raise ValueError("U must hold ns+3 components")
+    nx, ny = gx - 4, gy - 4
+    box = hf_box(nx, ny, 0, int(wedge_i), 0, gx, gx * gy, 0, nx, nx * ny, 0.0, 0.0)
+    codes = (c_int32 * 4)(*[int(b) for b in bc])
+    q = None
+    if inlet is not None:
+        vals = [float(v) for v in inlet]
+        if len(vals) != nv:
+            raise ValueError("inlet must hold ns+3 conserved values")
+        q = (c_double * nv)(*vals)
+    check(m.handle, load().hf_fill_ghosts_flow(m.handle, U.data_ptr(), byref(box), codes, q))
+
+
+class ConservativeInterp:
+    """Coarse-to-fine interpolation at ratio (2, 2) (ConservativeInterp.hpp:56-68)."""
+
+    def __init__(self, mechanism):
+        self.mech = _mech(mechanism)
+        self.last_limited = 0
+
+    def interp(self, coarse, coarse_lo, fine, fine_lo, region):
+        """Write the fine cells region = (ilo, jlo, ihi, jhi) (inclusive) of
+        ``fine`` from ``coarse``, which must hold grow(coarsen(region), 1).
+        Returns the number of parents whose slopes were limited (theta < 1);
+        an inadmissible parent raises HyperflowError(HF_EDOMAIN)."""
+        _device_f64(coarse, "coarse")
+        _device_f64(fine, "fine")
+        ilo, jlo, ihi, jhi = (int(v) for v in region)
+        if ihi < ilo or jhi < jlo:
+            raise ValueError("empty region")
+        ci0, cj0, ci1, cj1 = ilo >> 1, jlo >> 1, ihi >> 1, jhi >> 1
+        cx0, cy0 = coarse_lo
+        fx0, fy0 = fine_lo
+        if (ci0 - 1 < cx0 or cj0 - 1 < cy0 or ci1 + 1 >= cx0 + coarse.shape[2] or cj1 + 1 >= cy0 + coarse.shape[1]):
+            raise ValueError("coarse patch must hold grow(coarsen(region), 1)")
+        if ilo < fx0 or jlo < fy0 or ihi >= fx0 + fine.shape[2] or jhi >= fy0 + fine.shape[1]:
+            raise ValueError("fine patch must hold region")
+        if coarse.shape[0] != self.mech.nv or fine.shape[0] != self.mech.nv:
+            raise ValueError("patches must hold ns+3 components")
+        cp, fp = _patch(coarse, coarse_lo), _patch(fine, fine_lo)
+        reg = (c_int32 * 4)(ilo, jlo, ihi, jhi)
+        limited, ff = c_int64(0), c_int64(-1)
+        r = load().hf_interp_coarse_to_fine(self.mech.handle, coarse.data_ptr(), byref(cp), fine.data_ptr(),
+                                            byref(fp), reg, byref(limited), byref(ff))
+        self.last_limited = limited.value
+        check(self.mech.handle, r, ff.value)
+        return limited.value
+
+
+def average_down(mechanism, fine, fine_lo, coarse, coarse_lo, cregion):
+    """Replace the coarse cells cregion = (ilo, jlo, ihi, jhi) (inclusive) of
+    ``coarse`` by the mean of their 2 x 2 children in ``fine``."""
+    m = _mech(mechanism)
+    _device_f64(coarse, "coarse")
+    _device_f64(fine, "fine")
+    ilo, jlo, ihi, jhi = (int(v) for v in cregion)
+    if ihi < ilo or jhi < jlo:
+        raise ValueError("empty region")
+    cx0, cy0 = coarse_lo
+    fx0, fy0 = fine_lo
+    if ilo < cx0 or jlo < cy0 or ihi >= cx0 + coarse.shape[2] or jhi >= cy0 + coarse.shape[1]:
+        raise ValueError("coarse patch must hold cregion")
+    if 2 * ilo < fx0 or 2 * jlo < fy0 or 2 * ihi + 1 >= fx0 + fine.shape[2] or 2 * jhi + 1 >= fy0 + fine.shape[1]:
+        raise ValueError("fine patch must hold refine(cregion)")
+    if coarse.shape[0] != m.nv or fine.shape[0] != m.nv:
+        raise ValueError("patches must hold ns+3 components")
+    fp, cp = _patch(fine, fine_lo), _patch(coarse, coarse_lo)
+    reg = (c_int32 * 4)(ilo, jlo, ihi, jhi)
+    check(m.handle, load().hf_average_down(m.handle, fine.data_ptr(), byref(fp), coarse.data_ptr(), byref(cp), reg))

@@ -779,19 +779,97 @@ int box_common(hf_ctx* c, BoxLaunch& L, const hf_box* box, const void* U) {
 
 int hf_fill_ghosts(hf_ctx* c, double* U, const hf_box* box, const int32_t bc[4]) {
     if (!c) return HF_EINVAL;
+    if (bc)
+        for (int k = 0; k < 4; ++k)
+            if (bc[k] == HF_BC_INFLOW || bc[k] == HF_BC_WEDGE)
+                return fail(c, HF_EINVAL, "inflow and wedge boundaries need hf_fill_ghosts_flow");
+    return hf_fill_ghosts_flow(c, U, box, bc, nullptr);
+}
+
+int hf_fill_ghosts_flow(hf_ctx* c, double* U, const hf_box* box, const int32_t bc[4], const double* inlet) {
+    if (!c) return HF_EINVAL;
     BusyGuard guard(c);
     if (!guard.ok) return fail(c, HF_EBUSY, "concurrent call is unsupported");
     if (!bc) return fail(c, HF_EINVAL, "missing boundary codes");
-    for (int k = 0; k < 4; ++k)
-        if (bc[k] > HF_BC_PERIODIC || (k < 2 && bc[k] < 0)) return fail(c, HF_EINVAL, "unknown boundary code");
+    const bool two_d = bc[2] >= 0 && bc[3] >= 0;
+    for (int k = 0; k < 4; ++k) {
+        if (bc[k] > HF_BC_WEDGE || (k < 2 && bc[k] < 0)) return fail(c, HF_EINVAL, "unknown boundary code");
+        if (bc[k] == HF_BC_WEDGE && (k != 2 || !two_d))
+            return fail(c, HF_EINVAL, "the wedge boundary exists only on the y-lo side of a 2-D box");
+        if (bc[k] == HF_BC_INFLOW && !inlet) return fail(c, HF_EINVAL, "an inflow boundary needs the inlet state");
+    }
+    if (bc[2] == HF_BC_WEDGE && box && box->ny < 2)
+        return fail(c, HF_EINVAL, "a wedge bottom needs at least two rows");
     BoxLaunch L{};
     const int r = box_common(c, L, box, U);
     if (r != HF_OK) return r;
     L.U = U;
     L.op = BoxLaunch::FILL;
     for (int k = 0; k < 4; ++k) L.bc[k] = bc[k];
-    L.two_d = bc[2] >= 0 && bc[3] >= 0;
+    L.two_d = two_d;
+    L.inlet = inlet;
     if (c->ops->box_ops(L) != HF_OK) return fail(c, HF_EDEVICE, "ghost fill launch failed");
+    HF_HIP(c, hipStreamSynchronize(c->stream));
+    return HF_OK;
+}
+
+namespace {
+int patch_common(hf_ctx* c, BoxLaunch& L, const double* src, const hf_patch* sp, double* dst, const hf_patch* dp,
+                 const int32_t region[4]) {
+    if (!src || !sp || !dst || !dp || !region) return fail(c, HF_EINVAL, "missing patch, array or region");
+    if (region[2] < region[0] || region[3] < region[1]) return fail(c, HF_EINVAL, "empty region");
+    if (sp->pitch <= 0 || sp->plane <= 0 || dp->pitch <= 0 || dp->plane <= 0)
+        return fail(c, HF_EINVAL, "invalid patch stride");
+    HF_DEVICE(c);
+    if (!is_device_ptr(src) || !is_device_ptr(dst)) return fail(c, HF_EINVAL, "patch operations need device memory");
+    if (!c->scal.ensure(sizeof(unsigned long long) * 256)) return fail(c, HF_ENOMEM, "device scalars");
+    L.mech = c->d_mech;
+    L.src = src;
+    L.dst = dst;
+    L.sp = *sp;
+    L.dp = *dp;
+    for (int k = 0; k < 4; ++k) L.region[k] = region[k];
+    L.scal = c->scal.as<unsigned long long>() + 8;
+    L.first_failed = c->scal.as<unsigned long long>();
+    L.stream = c->stream;
+    return HF_OK;
+}
+}  // namespace
+
+int hf_interp_coarse_to_fine(hf_ctx* c, const double* coarse, const hf_patch* cp, double* fine, const hf_patch* fp,
+                             const int32_t region[4], int64_t* limited, int64_t* first_failed) {
+    if (!c) return HF_EINVAL;
+    BusyGuard guard(c);
+    if (!guard.ok) return fail(c, HF_EBUSY, "concurrent call is unsupported");
+    if (limited) *limited = 0;
+    if (first_failed) *first_failed = -1;
+    BoxLaunch L{};
+    const int r = patch_common(c, L, coarse, cp, fine, fp, region);
+    if (r != HF_OK) return r;
+    L.op = BoxLaunch::INTERP;
+    if (c->ops->box_ops(L) != HF_OK) return fail(c, HF_EDEVICE, "interpolation launch failed");
+    unsigned long long h[2] = {0, ~0ull};
+    HF_HIP(c, hipMemcpyAsync(&h[0], L.scal, sizeof(h[0]), hipMemcpyDeviceToHost, c->stream));
+    HF_HIP(c, hipMemcpyAsync(&h[1], L.first_failed, sizeof(h[1]), hipMemcpyDeviceToHost, c->stream));
+    HF_HIP(c, hipStreamSynchronize(c->stream));
+    if (limited) *limited = (int64_t)h[0];
+    if (h[1] != ~0ull) {
+        if (first_failed) *first_failed = (int64_t)h[1];
+        return fail(c, HF_EDOMAIN, "inadmissible coarse parent " + std::to_string(h[1]) + " in interpolation");
+    }
+    return HF_OK;
+}
+
+int hf_average_down(hf_ctx* c, const double* fine, const hf_patch* fp, double* coarse, const hf_patch* cp,
+                    const int32_t cregion[4]) {
+    if (!c) return HF_EINVAL;
+    BusyGuard guard(c);
+    if (!guard.ok) return fail(c, HF_EBUSY, "concurrent call is unsupported");
+    BoxLaunch L{};
+    const int r = patch_common(c, L, fine, fp, coarse, cp, cregion);
+    if (r != HF_OK) return r;
+    L.op = BoxLaunch::AVGDOWN;
+    if (c->ops->box_ops(L) != HF_OK) return fail(c, HF_EDEVICE, "average-down launch failed");
     HF_HIP(c, hipStreamSynchronize(c->stream));
     return HF_OK;
 }

@@ -609,9 +609,21 @@ __device__ __forceinline__ int ghost_source(int g, int len, int bc) {
     return g < 0 ? 0 : len - 1;
 }
 
+// The fixed inflow state of FlowBoundary (FlowBoundary.hpp:8), passed by value.
+struct InletState {
+    double q[Topo::NS + 3];
+};
+
 // pass 0: x ghosts of the valid rows; pass 1: y ghosts of every column
-// (including the x ghosts, so corners come from the filled columns).
-__global__ void __launch_bounds__(256) ghost_kernel(double* __restrict__ U, hf_box B, int4 bc, int pass) {
+// (including the x ghosts, so corners come from the filled columns, as AMReX's
+// filcc applies the y rule after the x rule). FlowBoundary (FlowBoundary.hpp:
+// 12-36) then overrides: a ghost beyond any inflow side is the inlet state; a
+// wedge-bottom ghost copies row 0 before the tip and mirrors (negated y
+// momentum) from column wedge_i on, reading the clamped valid column. Both
+// read only valid cells or pass-0 ghosts, so no pass-1 thread reads another's
+// output.
+__global__ void __launch_bounds__(256) ghost_kernel(double* __restrict__ U, hf_box B, int4 bc, InletState inlet,
+                                                    int pass) {
     constexpr int NV = Topo::NS + 3;
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const int len = pass == 0 ? B.nx : B.ny;
@@ -622,16 +634,136 @@ __global__ void __launch_bounds__(256) ghost_kernel(double* __restrict__ U, hf_b
     const int n = (int)(t / (4L * across));
     const int g = g4 < 2 ? g4 - 2 : len + g4 - 2;          // -2, -1, len, len+1
     const int code = g < 0 ? (pass == 0 ? bc.x : bc.z) : (pass == 0 ? bc.y : bc.w);
-    const int s = ghost_source(g, len, code);
-    const bool odd = code == HF_BC_REFLECTING && n == (pass == 0 ? Topo::NS : Topo::NS + 1);
     if (pass == 0) {
+        if (code == HF_BC_INFLOW) {
+            box_at(U, B, n, g, a) = inlet.q[n];
+            return;
+        }
+        const int s = ghost_source(g, len, code);
+        const bool odd = code == HF_BC_REFLECTING && n == Topo::NS;
         const double v = box_at(U, B, n, s, a);
         box_at(U, B, n, g, a) = odd ? -v : v;
-    } else {
-        const int i = a - 2;
-        const double v = box_at(U, B, n, i, s);
-        box_at(U, B, n, i, g) = odd ? -v : v;
+        return;
     }
+    const int i = a - 2;
+    const bool beyond_x_inflow = (i < 0 && bc.x == HF_BC_INFLOW) || (i >= B.nx && bc.y == HF_BC_INFLOW);
+    if (beyond_x_inflow || code == HF_BC_INFLOW) {
+        box_at(U, B, n, i, g) = inlet.q[n];
+        return;
+    }
+    if (code == HF_BC_WEDGE) {                             // g < 0 here: y-lo only
+        const bool wall = i >= B.wedge_i;
+        const int ci = min(max(i, 0), B.nx - 1);
+        const double v = box_at(U, B, n, ci, wall ? -g - 1 : 0);
+        box_at(U, B, n, i, g) = wall && n == Topo::NS + 1 ? -v : v;
+        return;
+    }
+    const int s = ghost_source(g, len, code);
+    const bool odd = code == HF_BC_REFLECTING && n == Topo::NS + 1;
+    const double v = box_at(U, B, n, i, s);
+    box_at(U, B, n, i, g) = odd ? -v : v;
+}
+
+// ------------------------------------------------------------------ coarse / fine transfer
+__device__ __forceinline__ double& patch_at(double* p, const hf_patch& P, int n, int i, int j) {
+    return p[P.offset + n * P.plane + (long)j * P.pitch + i];
+}
+
+// One child of ConservativeInterp.cpp:37,47 without contraction:
+// q + theta * ((+-0.25 sx) + (+-0.25 sy)).
+__device__ __forceinline__ double interp_child(double q, double theta, double cx, double sx, double cy, double sy) {
+#pragma clang fp contract(off)
+    return q + theta * (cx * sx + cy * sy);
+}
+
+__device__ __forceinline__ double minmod1(double x, double y) {
+    return x * y > 0 ? copysign(fmin(fabs(x), fabs(y)), x) : 0.0;
+}
+
+// Coarse-to-fine conservative interpolation at ratio (2, 2)
+// (ConservativeInterp::interp, ConservativeInterp.cpp:27-50): one lane per
+// coarse cell of coarsen(region), the theta search over all four children,
+// then the children inside region. scal[0] counts limited cells.
+__global__ void __launch_bounds__(256) interp_kernel(const MD* __restrict__ dm, const double* __restrict__ C,
+                                                     hf_patch cp, double* __restrict__ F, hf_patch fp, int4 region,
+                                                     unsigned long long* __restrict__ scal,
+                                                     unsigned long long* __restrict__ first_failed) {
+    constexpr int NV = Topo::NS + 3;
+    const int cilo = region.x >> 1, cjlo = region.y >> 1;  // floor division (coarsen)
+    const int cnx = (region.z >> 1) - cilo + 1, cny = (region.w >> 1) - cjlo + 1;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)cnx * cny) return;
+    const int i = cilo + (int)(t % cnx), j = cjlo + (int)(t / cnx);
+    double* c = const_cast<double*>(C);
+    double q[NV], sx[NV], sy[NV];
+    #pragma unroll
+    for (int n = 0; n < NV; ++n) {
+        q[n] = patch_at(c, cp, n, i, j);
+        sx[n] = minmod1(q[n] - patch_at(c, cp, n, i - 1, j), patch_at(c, cp, n, i + 1, j) - q[n]);
+        sy[n] = minmod1(q[n] - patch_at(c, cp, n, i, j - 1), patch_at(c, cp, n, i, j + 1) - q[n]);
+    }
+    double theta = 1.0;
+    bool accepted = false;
+    for (int attempt = 0; attempt < 48 && !accepted; ++attempt) {
+        bool good = true;
+        for (int k = 0; k < 4 && good; ++k) {              // (a, b) = (0,0), (1,0), (0,1), (1,1)
+            const double cx = (k & 1) ? 0.25 : -0.25, cy = (k & 2) ? 0.25 : -0.25;
+            double child[NV];
+            #pragma unroll
+            for (int n = 0; n < NV; ++n) child[n] = interp_child(q[n], theta, cx, sx[n], cy, sy[n]);
+            Props p;
+            good = properties(*dm, child, p, 1000.0);
+        }
+        if (good)
+            accepted = true;
+        else
+            theta *= 0.5;
+    }
+    if (!accepted) {
+        theta = 0.0;
+        Props p;
+        if (!properties(*dm, q, p, 1000.0)) {
+            atomicMin(first_failed, (unsigned long long)t);
+            return;
+        }
+    }
+    if (theta < 1.0) atomicAdd(&scal[0], 1ull);
+    for (int k = 0; k < 4; ++k) {
+        const int fi = 2 * i + (k & 1), fj = 2 * j + (k >> 1);
+        if (fi < region.x || fi > region.z || fj < region.y || fj > region.w) continue;
+        const double cx = (k & 1) ? 0.25 : -0.25, cy = (k & 2) ? 0.25 : -0.25;
+        #pragma unroll
+        for (int n = 0; n < NV; ++n) patch_at(F, fp, n, fi, fj) = interp_child(q[n], theta, cx, sx[n], cy, sy[n]);
+    }
+}
+
+// Clears one counter and the failure index (a kernel, not a memset, so the
+// sequence stays valid inside a captured graph).
+__global__ void counter_init_kernel(unsigned long long* __restrict__ counter, unsigned long long* __restrict__ first_failed) {
+    if (threadIdx.x == 0) {
+        *counter = 0;
+        *first_failed = ~0ull;
+    }
+}
+
+// Average-down at ratio (2, 2): coarse = 0.25 * (((0 + f00) + f10) + f01) + f11),
+// one lane per (component, coarse cell); i fastest for coalesced rows.
+__global__ void __launch_bounds__(256) avgdown_kernel(const double* __restrict__ F, hf_patch fp, double* __restrict__ C,
+                                                      hf_patch cp, int4 region) {
+    constexpr int NV = Topo::NS + 3;
+    const int cnx = region.z - region.x + 1, cny = region.w - region.y + 1;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)NV * cnx * cny) return;
+    const int i = region.x + (int)(t % cnx);
+    const int j = region.y + (int)((t / cnx) % cny);
+    const int n = (int)(t / ((long)cnx * cny));
+    double* f = const_cast<double*>(F);
+    double sum = 0.0;
+    sum += patch_at(f, fp, n, 2 * i, 2 * j);
+    sum += patch_at(f, fp, n, 2 * i + 1, 2 * j);
+    sum += patch_at(f, fp, n, 2 * i, 2 * j + 1);
+    sum += patch_at(f, fp, n, 2 * i + 1, 2 * j + 1);
+    patch_at(C, cp, n, i, j) = 0.25 * sum;
 }
 
 // Roundoff repair (BlockAMR::repairRoundoff, BlockAMR.cpp:299-334) on one
@@ -754,13 +886,33 @@ int box_ops(const BoxLaunch& L) {
     case BoxLaunch::FILL: {
         constexpr int NV = Topo::NS + 3;
         const int4 bc = make_int4(L.bc[0], L.bc[1], L.bc[2], L.bc[3]);
+        InletState inlet{};
+        if (L.inlet)
+            for (int n = 0; n < NV; ++n) inlet.q[n] = L.inlet[n];
         const long t0 = (long)NV * L.box.ny * 4;
-        hipLaunchKernelGGL(ghost_kernel, dim3((unsigned)((t0 + 255) / 256)), dim3(256), 0, L.stream, L.U, L.box, bc, 0);
+        hipLaunchKernelGGL(ghost_kernel, dim3((unsigned)((t0 + 255) / 256)), dim3(256), 0, L.stream, L.U, L.box, bc,
+                           inlet, 0);
         if (L.two_d) {
             const long t1 = (long)NV * (L.box.nx + 4) * 4;
             hipLaunchKernelGGL(ghost_kernel, dim3((unsigned)((t1 + 255) / 256)), dim3(256), 0, L.stream, L.U, L.box, bc,
-                               1);
+                               inlet, 1);
         }
+        break;
+    }
+    case BoxLaunch::INTERP: {
+        const int4 r = make_int4(L.region[0], L.region[1], L.region[2], L.region[3]);
+        const long cells = (long)((r.z >> 1) - (r.x >> 1) + 1) * ((r.w >> 1) - (r.y >> 1) + 1);
+        hipLaunchKernelGGL(counter_init_kernel, dim3(1), dim3(64), 0, L.stream, L.scal, L.first_failed);
+        hipLaunchKernelGGL(interp_kernel, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, L.stream, dm, L.src, L.sp,
+                           L.dst, L.dp, r, L.scal, L.first_failed);
+        break;
+    }
+    case BoxLaunch::AVGDOWN: {
+        constexpr int NV = Topo::NS + 3;
+        const int4 r = make_int4(L.region[0], L.region[1], L.region[2], L.region[3]);
+        const long t = (long)NV * (r.z - r.x + 1) * (r.w - r.y + 1);
+        hipLaunchKernelGGL(avgdown_kernel, dim3((unsigned)((t + 255) / 256)), dim3(256), 0, L.stream, L.src, L.sp, L.dst,
+                           L.dp, r);
         break;
     }
     case BoxLaunch::REPAIR:

@@ -84,15 +84,22 @@ struct AdjointLaunch {
 // Single-box driver operations (ghost fill, roundoff repair, validation and
 // CFL rate) on a ghosted box in device memory.
 struct BoxLaunch {
-    enum Op { FILL, REPAIR, PROPS };
+    enum Op { FILL, REPAIR, PROPS, INTERP, AVGDOWN };
     const void* mech;
     double* U;
     hf_box box;
     int op;
     int bc[4];                      // x-lo, x-hi, y-lo, y-hi boundary codes (y < 0: no y ghosts)
     int two_d;
+    const double* inlet;            // FILL: host inflow state (ns+3 values), null without HF_BC_INFLOW
     double dx, dy;
-    unsigned long long* scal;       // device scratch (repair: 3 NS + 2 words; props: rate bits)
+    // INTERP (coarse -> fine) and AVGDOWN (fine -> coarse): source and target patches
+    const double* src;
+    double* dst;
+    hf_patch sp, dp;
+    int region[4];                  // INTERP: fine cells; AVGDOWN: coarse cells (ilo, jlo, ihi, jhi)
+    unsigned long long* scal;       // device scratch (repair: 3 NS + 2 words; props: rate bits;
+                                    // interp: limited-cell count)
     unsigned long long* first_failed;
     hipStream_t stream;
 };

@@ -11,7 +11,7 @@ Mirrors the reference's step loop for a single level
                      on domain_error redo the stage with first-order faces
   react    :271-285  ChemistryBatch::advance over every valid cell
   stableDt :335-346  cfl / max((|u|+a)/dx + (|v|+a)/dy)
-Every arithmetic step runs in libhyperflow_hip.so (hf_fill_ghosts,
+Every arithmetic step runs in libhyperflow_hip.so (hf_fill_ghosts_flow,
 hf_flux_stage, hf_repair_roundoff, hf_validate_box, hf_lincomb,
 hf_chem_advance); torch only holds the device arrays.
 
@@ -22,7 +22,8 @@ from __future__ import annotations
 
 from ctypes import byref, c_double, c_int32, c_int64
 
-from ._lib import (HF_BC_PERIODIC, HF_BC_REFLECTING, HF_BC_TRANSMISSIVE, HF_CHEM_BATCHED, HF_EDOMAIN, HF_OK,  # noqa: F401
+from ._lib import (HF_BC_INFLOW, HF_BC_PERIODIC, HF_BC_REFLECTING, HF_BC_TRANSMISSIVE, HF_BC_WEDGE,  # noqa: F401
+                   HF_CHEM_BATCHED, HF_EDOMAIN, HF_OK,
                    HyperflowError, check, hf_box, hf_flux_stats, load)
 from .chemistry import ChemistryBatch
 
@@ -33,9 +34,14 @@ class BoxDriver:
     bc: (x_lo, x_hi, y_lo, y_hi) boundary codes; y codes are ignored for a
     1-D box. Reflecting sides are walls: their face fluxes of species,
     tangential momentum and energy are zeroed (BlockAMR.cpp:224-229).
+    HF_BC_INFLOW sides hold the fixed state ``inlet`` (nv conserved values);
+    an HF_BC_WEDGE bottom is an outflow before column ``wedge_i`` and a wall
+    from it on, in the ghost fill and in the face fluxes alike
+    (FlowBoundary.hpp:25-34, BlockAMR.cpp:225-229).
     """
 
-    def __init__(self, mechanism, nx, ny, dx, dy, bc, cfl=0.5, chem_mode=HF_CHEM_BATCHED, max_attempts=18):
+    def __init__(self, mechanism, nx, ny, dx, dy, bc, cfl=0.5, chem_mode=HF_CHEM_BATCHED, max_attempts=18,
+                 inlet=None, wedge_i=0):
         import torch
         self.torch = torch
         self.mech = mechanism
@@ -56,8 +62,15 @@ class BoxDriver:
         self.bc = (c_int32 * 4)(*bc)
         walls = 0
         for k in range(4):
-            if bc[k] == HF_BC_REFLECTING:
+            if bc[k] == HF_BC_REFLECTING or (k == 2 and bc[k] == HF_BC_WEDGE):
                 walls |= 1 << k
+        self.wedge_i = int(wedge_i)
+        self.inlet = None
+        if inlet is not None:
+            vals = [float(v) for v in inlet]
+            if len(vals) != nv:
+                raise ValueError("inlet must hold ns+3 conserved values")
+            self.inlet = (c_double * nv)(*vals)
         self.walls = walls
         self.chem = ChemistryBatch(mechanism, mode=chem_mode)
         self.time = 0.0
@@ -84,13 +97,13 @@ class BoxDriver:
     def _box(self, h=0.0):
         gx, gy = self.nx + 4, self.ny + 4
         plane = gx * gy
-        return hf_box(self.nx, self.ny, self.walls, 0, 0, gx, plane, 2 * gx + 2, gx, plane, h / self.dx,
+        return hf_box(self.nx, self.ny, self.walls, self.wedge_i, 0, gx, plane, 2 * gx + 2, gx, plane, h / self.dx,
                       0.0 if self.one_d else h / self.dy)
 
     # ------------------------------------------------------------ stages
     def _fill(self, A):
         box = self._box()
-        check(self._h, load().hf_fill_ghosts(self._h, A.data_ptr(), byref(box), self.bc))
+        check(self._h, load().hf_fill_ghosts_flow(self._h, A.data_ptr(), byref(box), self.bc, self.inlet))
 
     def _validate(self, A):
         box = self._box()
@@ -229,3 +242,31 @@ def channel_2d(mech, nx=1024, ny=256, length=0.2, seed=3, hot_x=0.01, hot=(2500.
                 q = mech.state_from_TPY(T / f, p * atm, Y)
                 Q[:, j0:j0 + block, sel] = q[:, None, None]
     return Q, dx
+
+
+def teng_osw_base(mech, nx=144, ny=64):
+    """The base level of the reference's oblique-shock wedge case
+    (HyperFlow_V3/cases/teng_case1: 144 x 64 cells over 2.304 x 1.024 mm from
+    origin (-0.128 mm, 0), inflow left and top, transmissive right, a wedge
+    bottom whose tip is at x = 0; BlockAMR.cpp:62-67): premixed H2/air
+    (X H2:O2:N2 = 2:1:3.76) at 958.1 K and 1 202 900 Pa moving at 2737.7 m/s,
+    19 degrees below the x axis, fills the domain and the inflow. nx, ny scale
+    the grid over the same domain. Returns a dict with Q (nv, ny, nx), inlet
+    (nv), dx, dy, bc and wedge_i."""
+    import math
+
+    import numpy as np
+    X = np.zeros(mech.ns)
+    for name, x in (("H2", 2.0), ("O2", 1.0), ("N2", 3.76)):
+        X[mech.species.index(name)] = x
+    X /= X.sum()
+    W = mech.W
+    Y = X * W / (X * W).sum()
+    speed, angle = 2737.7, 19.0 * math.acos(-1.0) / 180.0      # the reference's rounding (BlockAMR.cpp:64)
+    inlet = mech.state_from_TPY(958.1, 1202900.0, Y, speed * math.cos(angle), -speed * math.sin(angle))
+    lx, ly, x0 = 0.002304, 0.001024, -0.000128
+    dx, dy = lx / nx, ly / ny
+    from .amr import wedge_column
+    Q = np.repeat(np.repeat(np.asarray(inlet)[:, None, None], ny, 1), nx, 2)
+    return {"Q": Q, "inlet": np.asarray(inlet), "dx": dx, "dy": dy, "wedge_i": wedge_column(x0, dx, 0.0),
+            "bc": (HF_BC_INFLOW, HF_BC_TRANSMISSIVE, HF_BC_WEDGE, HF_BC_INFLOW)}

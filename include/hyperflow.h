@@ -188,14 +188,58 @@ int hf_chem_tangent(hf_ctx *ctx, int64_t ncell, const double *rho, const double 
  * Device-memory helpers for a driver that advances one single-level ghosted
  * box (the hf_box input layout; the output fields of the box are unused).
  * All pointers are device pointers (HF_EINVAL otherwise). */
-enum { HF_BC_TRANSMISSIVE = 0, HF_BC_REFLECTING = 1, HF_BC_PERIODIC = 2 };
+enum { HF_BC_TRANSMISSIVE = 0, HF_BC_REFLECTING = 1, HF_BC_PERIODIC = 2, HF_BC_INFLOW = 3, HF_BC_WEDGE = 4 };
 
 /* Fill the two ghost layers of U (BlockAMR::fill, BlockAMR.cpp:137-152, with
  * the boundary types of BlockAMR.cpp:48-59): transmissive copies the nearest
  * valid cell (AMReX foextrap); reflecting mirrors about the face, negating the
  * normal momentum (reflect_even / reflect_odd); periodic wraps. bc = {x-lo,
- * x-hi, y-lo, y-hi}; y-lo < 0 fills x ghosts only (a 1-D box). */
+ * x-hi, y-lo, y-hi}; y-lo < 0 fills x ghosts only (a 1-D box). At a corner
+ * the y rule is applied to the filled x ghost column (AMReX filcc order).
+ * Inflow and wedge codes need hf_fill_ghosts_flow (HF_EINVAL here). */
 int hf_fill_ghosts(hf_ctx *ctx, double *U, const hf_box *box, const int32_t bc[4]);
+
+/* hf_fill_ghosts plus the two conditions of FlowBoundary (FlowBoundary.hpp:
+ * 7-37, filled after the rules above as GpuBndryFuncFab does, BlockAMR.cpp:
+ * 137-152): HF_BC_INFLOW (ext_dir) sets every ghost cell that lies beyond an
+ * inflow side, corners included, to inlet (host, ns+3 conserved values: the
+ * reference's premixed `fresh` state, BlockAMR.cpp:132-136); HF_BC_WEDGE (y-lo
+ * only, 2-D, ny >= 2) makes the bottom ghost rows an outflow before the wedge
+ * tip (copy of row 0) and a slip wall from column box->wedge_i on (mirror,
+ * negated y momentum), reading the valid cell whose column is the ghost's
+ * clamped to [0, nx). A corner beyond an inflow side is inlet even on a wedge
+ * bottom. inlet may be NULL when no side is HF_BC_INFLOW. */
+int hf_fill_ghosts_flow(hf_ctx *ctx, double *U, const hf_box *box, const int32_t bc[4], const double *inlet);
+
+/* One SoA array of a level patch: element (component n, cell i, j), i and j
+ * in the level's own index space, is at base[offset + n*plane + j*pitch + i]. */
+typedef struct hf_patch {
+    int64_t offset, pitch, plane;
+} hf_patch;
+
+/* Coarse-to-fine conservative interpolation at refinement ratio (2, 2)
+ * (ConservativeInterp::interp, ConservativeInterp.cpp:18-51). region =
+ * {ilo, jlo, ihi, jhi}: the fine cells to write, inclusive, in the fine
+ * index space. For every coarse cell (i, j) of coarsen(region): minmod slopes
+ * sx, sy from its four neighbours (all must be readable in coarse:
+ * grow(coarsen(region), 1)); the first theta in 1, 1/2, ..., 2^-47 for which
+ * all four children q + theta (+-sx/4 +- sy/4) are admissible, else 0; the
+ * children inside region are written. No fused multiply-add in the child
+ * arithmetic. limited (optional) receives the number of coarse cells with
+ * theta < 1. A parent that is itself inadmissible returns HF_EDOMAIN with the
+ * lowest such coarse cell (row-major over coarsen(region), i fastest) in
+ * first_failed (optional); fine cells are then written for the other parents. */
+int hf_interp_coarse_to_fine(hf_ctx *ctx, const double *coarse, const hf_patch *cp, double *fine,
+                             const hf_patch *fp, const int32_t region[4], int64_t *limited,
+                             int64_t *first_failed);
+
+/* Average-down at ratio (2, 2) (BlockAMR::average, BlockAMR.cpp:157-159, over
+ * AMReX average_down, amrex @2d8c2e7e): every coarse cell of cregion = {ilo,
+ * jlo, ihi, jhi} (inclusive, coarse index space) becomes 0.25 times the sum of
+ * its four fine children, summed from 0 in the order (2i, 2j), (2i+1, 2j),
+ * (2i, 2j+1), (2i+1, 2j+1). */
+int hf_average_down(hf_ctx *ctx, const double *fine, const hf_patch *fp, double *coarse, const hf_patch *cp,
+                    const int32_t cregion[4]);
 
 /* BlockAMR::repairRoundoff (BlockAMR.cpp:299-334) on the valid cells of one
  * box of equal cell volumes: negative species within 64 eps of the species'

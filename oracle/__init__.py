@@ -133,7 +133,7 @@ class Oracle:
         return w
 
     # ------------------------------------------------------------ flux stage
-    def flux_stage(self, Uin, dtdx, dtdy, first_order=False, walls=0, dims=2):
+    def flux_stage(self, Uin, dtdx, dtdy, first_order=False, walls=0, dims=2, wedge_i=0):
         """Uin: (nv, ny+4, nx+4) ghosted SoA -> (status, Uout (nv, ny, nx), fallbacks, first_failed)."""
         Uin = np.ascontiguousarray(Uin, dtype=np.float64)
         nv, gy, gx = Uin.shape
@@ -141,7 +141,7 @@ class Oracle:
         out = np.zeros((nv, ny, nx))
         fb, ff = ctypes.c_long(0), ctypes.c_long(-1)
         r = lib().or_flux_stage(self.h, nx, ny, _d(Uin), out.ctypes.data_as(_dp), ctypes.c_double(dtdx),
-                                ctypes.c_double(dtdy), int(first_order), int(walls), int(dims),
+                                ctypes.c_double(dtdy), int(first_order), int(walls), int(wedge_i), int(dims),
                                 ctypes.byref(fb), ctypes.byref(ff))
         return r, out, fb.value, ff.value
 

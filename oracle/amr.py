@@ -1,0 +1,120 @@
+"""CPU restatement of the coarse/fine transfers between block-AMR levels —
+TEST INFRASTRUCTURE ONLY (the checker of hyperflow_amd.amr).
+
+  interp        ConservativeInterp::interp (HyperFlow_V3/src/ConservativeInterp.cpp:18-51):
+                minmod slopes of every coarse parent (:30-31), one theta for
+                all components and all four children, halved up to 48 times
+                until every child is admissible (:32-42), 0 with the parent
+                itself checked otherwise (:43), children q + theta (+-sx/4
+                +- sy/4) written where the fine region holds them (:44-49).
+  average_down  BlockAMR::average (HyperFlow_V3/src/BlockAMR.cpp:157-159) over
+                AMReX average_down (amrex @2d8c2e7e, fetched by
+                HyperFlow_V3/CMakeLists.txt:22-26, not vendored): for equal
+                cell volumes, coarse = (1/4) * sum of the 2 x 2 children,
+                accumulated from 0 with i fastest.
+
+Admissibility is the oracle's properties() (physics.cpp:4-15, 46-48).
+
+Parity status: interp is pinned by its invariants (a uniform field is
+reproduced, a linear field is reproduced exactly where minmod keeps the
+slope, every parent's four children average back to it), by a closed form of
+the theta search on species-gradient fields (theta = the largest power of two
+keeping every child's partial densities non-negative), and by the known
+failure (an inadmissible parent is an error). average_down is pinned by
+exact integer-valued sums and by conservation.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+class DomainError(Exception):
+    pass
+
+
+def minmod(a, b):
+    """physics.cpp:40-44, elementwise."""
+    return np.where(a * b > 0, np.copysign(np.minimum(np.abs(a), np.abs(b)), a), 0.0)
+
+
+def _admissible(o, Q):
+    """Q (nv, m) -> bool (m,): the reference's admissible() per column."""
+    _, _, _, st = o.properties(np.ascontiguousarray(Q.T))
+    return st == 0
+
+
+def _child(q, theta, cx, sx, cy, sy):
+    """ConservativeInterp.cpp:37,47: q + theta ((+-0.25) sx + (+-0.25) sy), no fusion."""
+    return q + theta * (cx * sx + cy * sy)
+
+
+_CHILDREN = ((0, 0), (1, 0), (0, 1), (1, 1))     # (a, b): fine (2i + a, 2j + b)
+
+
+def interp(o, C, clo, region):
+    """Interpolate coarse C onto the fine cells of region.
+
+    C: (nv, cny, cnx); C[:, j - clo[1], i - clo[0]] is coarse cell (i, j).
+    region: (ilo, jlo, ihi, jhi), fine cells, inclusive.
+    Returns (F, theta, failed): F (nv, jhi-jlo+1, ihi-ilo+1) the fine values
+    (NaN where the parent failed), theta (cny', cnx') per parent of
+    coarsen(region), failed the row-major indices of inadmissible parents.
+    """
+    ilo, jlo, ihi, jhi = region
+    ci0, cj0, ci1, cj1 = ilo >> 1, jlo >> 1, ihi >> 1, jhi >> 1
+    nv = C.shape[0]
+
+    def at(di, dj):
+        return C[:, cj0 - clo[1] + dj:cj1 - clo[1] + 1 + dj, ci0 - clo[0] + di:ci1 - clo[0] + 1 + di]
+
+    if ci0 - 1 < clo[0] or cj0 - 1 < clo[1] or ci1 + 1 >= clo[0] + C.shape[2] or cj1 + 1 >= clo[1] + C.shape[1]:
+        raise ValueError("coarse array must hold grow(coarsen(region), 1)")
+    shape = (cj1 - cj0 + 1, ci1 - ci0 + 1)
+    q = at(0, 0).reshape(nv, -1)
+    sx = minmod(q - at(-1, 0).reshape(nv, -1), at(1, 0).reshape(nv, -1) - q)
+    sy = minmod(q - at(0, -1).reshape(nv, -1), at(0, 1).reshape(nv, -1) - q)
+    m = q.shape[1]
+    theta = np.ones(m)
+    accepted = np.zeros(m, dtype=bool)
+    pending = np.arange(m)
+    for _ in range(48):
+        if pending.size == 0:
+            break
+        good = np.ones(pending.size, dtype=bool)
+        for a, b in _CHILDREN:
+            ch = _child(q[:, pending], theta[pending], 0.25 if a else -0.25, sx[:, pending],
+                        0.25 if b else -0.25, sy[:, pending])
+            good &= _admissible(o, ch)
+        accepted[pending[good]] = True
+        theta[pending[~good]] *= 0.5
+        pending = pending[~good]
+    theta[~accepted] = 0.0
+    failed = np.zeros(m, dtype=bool)
+    if (~accepted).any():
+        idx = np.flatnonzero(~accepted)
+        failed[idx] = ~_admissible(o, q[:, idx])
+    F = np.full((nv, jhi - jlo + 1, ihi - ilo + 1), np.nan)
+    for a, b in _CHILDREN:
+        ch = _child(q, theta, 0.25 if a else -0.25, sx, 0.25 if b else -0.25, sy).reshape(nv, *shape)
+        ch[:, failed.reshape(shape)] = np.nan
+        fi = 2 * np.arange(ci0, ci1 + 1) + a
+        fj = 2 * np.arange(cj0, cj1 + 1) + b
+        ki = (fi >= ilo) & (fi <= ihi)
+        kj = (fj >= jlo) & (fj <= jhi)
+        F[:, (fj[kj] - jlo)[:, None], (fi[ki] - ilo)[None, :]] = ch[:, kj][:, :, ki]
+    return F, theta.reshape(shape), np.flatnonzero(failed)
+
+
+def average_down(F, flo, cregion):
+    """Coarse cells of cregion (ilo, jlo, ihi, jhi) from the fine array F
+    (F[:, j - flo[1], i - flo[0]] is fine cell (i, j)): (nv, cny, cnx)."""
+    ilo, jlo, ihi, jhi = cregion
+    fi0, fj0 = 2 * ilo - flo[0], 2 * jlo - flo[1]
+    fi1, fj1 = 2 * ihi + 1 - flo[0], 2 * jhi + 1 - flo[1]
+    if fi0 < 0 or fj0 < 0 or fi1 >= F.shape[2] or fj1 >= F.shape[1]:
+        raise ValueError("fine array must hold refine(cregion)")
+    s = np.zeros((F.shape[0], jhi - jlo + 1, ihi - ilo + 1))
+    for b in (0, 1):
+        for a in (0, 1):
+            s = s + F[:, fj0 + b:fj1 + 1:2, fi0 + a:fi1 + 1:2]
+    return 0.25 * s

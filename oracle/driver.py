@@ -4,7 +4,8 @@ INFRASTRUCTURE ONLY (the checker of hyperflow_amd.driver.BoxDriver).
 Follows HyperFlow_V3/src/BlockAMR.cpp:
   fill            :137-152 with the boundary types of :48-59 (AMReX foextrap,
                   reflect_even / reflect_odd with the normal momentum odd,
-                  periodic)
+                  periodic, and FlowBoundary.hpp:12-36's fixed inflow and
+                  wedge bottom applied after them)
   euler           :202-265 (stage, repairRoundoff, validate, first-order redo)
   repairRoundoff  :299-334
   flow            :266-270 (Heun / SSP-RK2 average, validate)
@@ -15,13 +16,15 @@ on one box of equal cell volumes, with the oracle's flux stage, properties and
 chemistry. Parity status: pinned by the functions it composes (flux stage,
 properties, chemistry: see the package docstring) and, for the composition
 itself, by the tests in tests/test_oracle_driver.py (conservation on a
-periodic box, uniform state preserved, wall symmetry, repair semantics).
+periodic box, uniform state preserved, wall symmetry, repair semantics). The
+inflow / wedge fill is pinned by the known answers of the reference's own
+HyperFlow_V3/tests/boundary_check.cpp:15-36 (tests/test_oracle_amr.py).
 """
 from __future__ import annotations
 
 import numpy as np
 
-TRANSMISSIVE, REFLECTING, PERIODIC = 0, 1, 2
+TRANSMISSIVE, REFLECTING, PERIODIC, INFLOW, WEDGE = 0, 1, 2, 3, 4
 EPS = np.finfo(float).eps
 
 
@@ -37,24 +40,55 @@ def _src(g, n, bc):
     return 0 if g < 0 else n - 1
 
 
-def fill_ghosts(G, ns, bc, one_d):
-    """Two ghost layers of G (nv, ny+4, nx+4) from its valid cells, in place."""
+def fill_ghosts(G, ns, bc, one_d, inlet=None, wedge_i=0):
+    """Two ghost layers of G (nv, ny+4, nx+4) from its valid cells, in place.
+
+    First AMReX's filcc rules, cell by cell: x ghosts of the valid rows, then
+    y ghosts of every column, the y rule reading the filled x-ghost column at
+    a corner. Then FlowBoundary::operator() (FlowBoundary.hpp:12-36) over the
+    ghost cells: a cell beyond an inflow side (ext_dir; a wedge bottom is not
+    one for this test, :20) takes inlet; a wedge-bottom cell (y < 0) copies the
+    valid cell (clamp(i), 0) before the tip or mirrors (clamp(i), -j-1) with
+    the y momentum negated from column wedge_i on (:25-34).
+    """
     nv, gy, gx = G.shape
     nx, ny = gx - 4, gy - 4
     for g in (-2, -1, nx, nx + 1):
         code = bc[0] if g < 0 else bc[1]
+        if code == INFLOW:
+            continue                      # the functor below sets it
         s = _src(g, nx, code)
         G[:, 2:ny + 2, g + 2] = G[:, 2:ny + 2, s + 2]
         if code == REFLECTING:
             G[ns, 2:ny + 2, g + 2] *= -1.0
-    if one_d:
-        return
-    for g in (-2, -1, ny, ny + 1):
-        code = bc[2] if g < 0 else bc[3]
-        s = _src(g, ny, code)
-        G[:, g + 2, :] = G[:, s + 2, :]
-        if code == REFLECTING:
-            G[ns + 1, g + 2, :] *= -1.0
+    if not one_d:
+        for g in (-2, -1, ny, ny + 1):
+            code = bc[2] if g < 0 else bc[3]
+            if code in (INFLOW, WEDGE):
+                continue
+            s = _src(g, ny, code)
+            G[:, g + 2, :] = G[:, s + 2, :]
+            if code == REFLECTING:
+                G[ns + 1, g + 2, :] *= -1.0
+    # FlowBoundary over every ghost cell, reading valid cells only
+    ys = range(-2, ny + 2) if not one_d else range(0, ny)
+    for j in ys:
+        for i in range(-2, nx + 2):
+            if 0 <= i < nx and 0 <= j < ny:
+                continue
+            below = not one_d and j < 0
+            fixed = (i < 0 and bc[0] == INFLOW) or (i >= nx and bc[1] == INFLOW)
+            if not one_d:
+                fixed = fixed or (j < 0 and bc[2] == INFLOW) or (j >= ny and bc[3] == INFLOW)
+            if fixed:
+                G[:, j + 2, i + 2] = inlet
+            elif below and bc[2] == WEDGE:
+                wall = i >= wedge_i
+                sj = -j - 1 if wall else 0
+                si = min(max(i, 0), nx - 1)
+                G[:, j + 2, i + 2] = G[:, sj + 2, si + 2]
+                if wall:
+                    G[ns + 1, j + 2, i + 2] *= -1.0
 
 
 def repair_roundoff(Q, ns):
@@ -87,14 +121,19 @@ def repair_roundoff(Q, ns):
 
 
 class OracleBoxDriver:
-    def __init__(self, oracle, nx, ny, dx, dy, bc, cfl=0.5, rtol=1e-12, atol=1e-20, max_attempts=18):
+    def __init__(self, oracle, nx, ny, dx, dy, bc, cfl=0.5, rtol=1e-12, atol=1e-20, max_attempts=18,
+                 inlet=None, wedge_i=0):
         self.o = oracle
         self.ns = oracle.ns
         self.nv = self.ns + 3
         self.nx, self.ny, self.dx, self.dy = nx, ny, dx, dy
         self.one_d = ny == 1 and dy == 0.0
         self.bc = list(bc)
-        self.walls = sum(1 << k for k in range(4) if bc[k] == REFLECTING and (k < 2 or not self.one_d))
+        self.inlet = None if inlet is None else np.asarray(inlet, dtype=float)
+        self.wedge_i = int(wedge_i)
+        # reflecting sides and the wedge bottom are walls (BlockAMR.cpp:52-53)
+        self.walls = sum(1 << k for k in range(4)
+                         if (bc[k] == REFLECTING or (k == 2 and bc[k] == WEDGE)) and (k < 2 or not self.one_d))
         self.cfl, self.rtol, self.atol, self.max_attempts = cfl, rtol, atol, max_attempts
         self.Q = np.zeros((self.nv, ny, nx))
         self.time, self.steps, self.retries, self.first_order_stages = 0.0, 0, 0, 0
@@ -102,7 +141,7 @@ class OracleBoxDriver:
     def _ghosted(self, Q):
         G = np.zeros((self.nv, self.ny + 4, self.nx + 4))
         G[:, 2:self.ny + 2, 2:self.nx + 2] = Q
-        fill_ghosts(G, self.ns, self.bc, self.one_d)
+        fill_ghosts(G, self.ns, self.bc, self.one_d, self.inlet, self.wedge_i)
         return G
 
     def _validate(self, Q):
@@ -120,7 +159,7 @@ class OracleBoxDriver:
         for first_order in (False, True):
             r, out, fb, ff = self.o.flux_stage(G, h / self.dx, 0.0 if self.one_d else h / self.dy,
                                                first_order=first_order, walls=self.walls,
-                                               dims=1 if self.one_d else 2)
+                                               dims=1 if self.one_d else 2, wedge_i=self.wedge_i)
             try:
                 repair_roundoff(out, self.ns)
                 self._validate(out)

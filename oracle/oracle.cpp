@@ -841,11 +841,13 @@ void or_wdot(void* mp, double T, const double* C, double* w, const double* lnA_s
 //  Uin : ghosted SoA, component plane n at Uin + n*(nx+4)*(ny+4), i fastest, 2 ghosts.
 //  Uout: nx*ny per component plane, i fastest.
 //  walls: bit0 x-lo, bit1 x-hi, bit2 y-lo, bit3 y-hi faces of this box are walls.
+//  wedge_i: the y-lo wall holds only for faces of columns i >= wedge_i (the
+//           wedge bottom, BlockAMR.cpp:225-226: x = lo + (i + 0.5) dx >= wedgeStart).
 //  dims: 1 -> x sweep only (1-D), 2 -> both.
 //  fallbacks: number of face sides that fell back to first order.
 // Returns 0, or EDOMAIN with *first_failed = lowest cell index whose updated state is inadmissible.
 int or_flux_stage(void* mp, int nx, int ny, const double* Uin, double* Uout, double dtdx, double dtdy,
-                  int first_order, int walls, int dims, long* fallbacks, long* first_failed) {
+                  int first_order, int walls, int wedge_i, int dims, long* fallbacks, long* first_failed) {
     const Mech& m = *static_cast<Mech*>(mp);
     const int ns = m.ns, nv = ns + 3;
     const int gx = nx + 4, gy = ny + 4;
@@ -885,7 +887,7 @@ int or_flux_stage(void* mp, int nx, int ny, const double* Uin, double* Uout, dou
                 hllc(ns, L, Rr, nrm, pl, pr, f);
                 const int coord = d == 0 ? i : j, len = d == 0 ? nx : ny;
                 const bool lo = (walls >> (2 * d)) & 1, hi = (walls >> (2 * d + 1)) & 1;
-                if ((coord == 0 && lo) || (coord == len && hi)) {
+                if ((coord == 0 && lo && (d == 0 || i >= wedge_i)) || (coord == len && hi)) {
                     for (int k = 0; k < ns; ++k) f[k] = 0;
                     f[tan] = 0;
                     f[ns + 2] = 0;

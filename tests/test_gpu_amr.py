@@ -1,0 +1,185 @@
+"""GPU level transfers (hyperflow_amd.amr over the C ABI) against the oracle
+(oracle/amr.py, oracle/driver.py), and the wedge / inflow boundary path end
+to end.
+
+Tolerances: ghost fill, interpolation and average-down bit-exact (same
+arithmetic, no contraction, integer-free index work); the driven state of the
+wedge case after a few Strang steps to 1e-8 of each component's scale with
+both chemistries converged (rtol 1e-12, atol 1e-20), as for the other
+single-box driver tests.
+"""
+import numpy as np
+import pytest
+
+import hyperflow_amd as hf
+from hyperflow_amd.amr import ConservativeInterp, average_down, fill_ghosts
+from hyperflow_amd.driver import BoxDriver, teng_osw_base
+from oracle import amr as oamr
+from oracle.driver import OracleBoxDriver
+from oracle.driver import fill_ghosts as ofill
+
+from test_oracle_amr import limited_field, smooth_field
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+I, T, R_, P, W = hf.HF_BC_INFLOW, hf._lib.HF_BC_TRANSMISSIVE, hf._lib.HF_BC_REFLECTING, hf._lib.HF_BC_PERIODIC, \
+    hf.HF_BC_WEDGE
+
+
+def _dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def test_boundary_check_known_answers(mech_h2):
+    """HyperFlow_V3/tests/boundary_check.cpp:15-36 on the device."""
+    print("[gpu] flow fill: boundary_check known answers", flush=True)
+    ns, nx, ny = mech_h2.ns, 8, 4
+    nv = ns + 3
+    value = lambda i, j, n: 1000.0 * n + 10.0 * j + i + 1.0
+    G = np.full((nv, ny + 4, nx + 4), -1e100)
+    for n in range(nv):
+        for j in range(ny):
+            for i in range(nx):
+                G[n, j + 2, i + 2] = value(i, j, n)
+    inlet = 20000.0 + np.arange(nv)
+    U = _dev(G)
+    fill_ghosts(mech_h2, U, (I, T, W, I), inlet, wedge_i=hf.wedge_column(-0.002, 0.001, 0.0))
+    out = U.cpu().numpy()
+    a = lambda i, j, n: out[n, j + 2, i + 2]
+    for n in range(nv):
+        assert a(-1, 1, n) == inlet[n] and a(3, 4, n) == inlet[n] and a(-1, -1, n) == inlet[n]
+        assert a(8, 2, n) == value(7, 2, n)
+        assert a(0, -1, n) == value(0, 0, n) and a(0, -2, n) == value(0, 0, n)
+        sign = -1.0 if n == ns + 1 else 1.0
+        assert a(3, -1, n) == sign * value(3, 0, n) and a(3, -2, n) == sign * value(3, 1, n)
+    assert np.all(out > -1e99)
+
+
+@pytest.mark.parametrize("mech", ["h2", "dme"])
+@pytest.mark.parametrize("bc,wedge_i", [((I, T, W, I), 3), ((R_, I, W, T), -1), ((T, T, W, R_), 99),
+                                        ((P, P, W, I), 2), ((I, I, I, I), 0), ((R_, T, I, P), 0)])
+def test_flow_fill_matches_oracle(mech, bc, wedge_i, mech_h2, mech_dme):
+    m = mech_h2 if mech == "h2" else mech_dme
+    print(f"[gpu] flow fill {mech} bc={bc} wedge_i={wedge_i}", flush=True)
+    nx, ny = 9, 6
+    rng = np.random.default_rng(11)
+    G = rng.standard_normal((m.nv, ny + 4, nx + 4))
+    inlet = rng.standard_normal(m.nv)
+    U = _dev(G)
+    fill_ghosts(m, U, bc, inlet, wedge_i)
+    ofill(G, m.ns, bc, False, inlet, wedge_i)
+    np.testing.assert_array_equal(U.cpu().numpy(), G)
+
+
+def test_flow_fill_rejects_bad_codes(mech_h2):
+    U = _dev(np.zeros((mech_h2.nv, 8, 9)))
+    for bc, inlet in (((W, T, T, T), None), ((T, T, T, W), None), ((I, T, T, T), None), ((T, T, 5, T), None)):
+        with pytest.raises(hf.HyperflowError):
+            fill_ghosts(mech_h2, U, bc, inlet)
+    with pytest.raises(hf.HyperflowError):       # plain fill refuses the flow codes
+        from ctypes import byref, c_int32
+        from hyperflow_amd._lib import check, hf_box, load
+        box = hf_box(5, 4, 0, 0, 0, 9, 72, 0, 5, 20, 0.0, 0.0)
+        check(mech_h2.handle, load().hf_fill_ghosts(mech_h2.handle, U.data_ptr(), byref(box),
+                                                    (c_int32 * 4)(T, T, W, T)))
+
+
+def _interp_both(m, o, C, clo, region, flo, fshape):
+    F = torch.full((m.nv,) + fshape, float("nan"), dtype=torch.float64, device="cuda")
+    limited = ConservativeInterp(m).interp(_dev(C), clo, F, flo, region)
+    Fo, theta, failed = oamr.interp(o, C, clo, region)
+    assert failed.size == 0
+    out = F.cpu().numpy()
+    ilo, jlo, ihi, jhi = region
+    sub = out[:, jlo - flo[1]:jhi - flo[1] + 1, ilo - flo[0]:ihi - flo[0] + 1]
+    np.testing.assert_array_equal(sub, Fo)
+    mask = np.ones(fshape, dtype=bool)
+    mask[jlo - flo[1]:jhi - flo[1] + 1, ilo - flo[0]:ihi - flo[0] + 1] = False
+    assert np.isnan(out[:, mask]).all()
+    assert limited == int((theta < 1).sum())
+    return limited
+
+
+def test_interp_matches_oracle_h2(mech_h2, oracle_h2):
+    print("[gpu] interp H2: smooth field, odd region", flush=True)
+    C = smooth_field(oracle_h2, 14, 11, 1)
+    # coarse patch indexed from (-1, 2); fine patch from (0, 5) with spare cells
+    _interp_both(mech_h2, oracle_h2, C, (-1, 2), (1, 7, 22, 20), (0, 5), (18, 26))
+
+
+def test_interp_limited_matches_oracle(mech_h2, oracle_h2):
+    print("[gpu] interp H2: limited trace-species field", flush=True)
+    C, expect = limited_field(oracle_h2)
+    # the cells between the designed parents hold negative H: each parent alone
+    limited = 0
+    for (i, j), t in expect.items():
+        limited += _interp_both(mech_h2, oracle_h2, C, (0, 0), (2 * i, 2 * j, 2 * i + 1, 2 * j + 1), (0, 0), (30, 30))
+    assert limited == sum(1 for t in expect.values() if t < 1)
+
+
+def test_interp_matches_oracle_dme(mech_dme, oracle_dme):
+    print("[gpu] interp DME: random mixtures", flush=True)
+    rng = np.random.default_rng(3)
+    o = oracle_dme
+    nx, ny = 8, 7
+    C = np.zeros((o.ns + 3, ny, nx))
+    from test_oracle_amr import state
+    for j in range(ny):
+        for i in range(nx):
+            Y = rng.uniform(0, 1, o.ns) ** 4
+            Y /= Y.sum()
+            C[:, j, i] = state(o, rng.uniform(800, 2200), rng.uniform(1e5, 2e6), Y, rng.uniform(-300, 300),
+                               rng.uniform(-300, 300))
+    _interp_both(mech_dme, o, C, (0, 0), (2, 2, 13, 11), (2, 2), (10, 12))
+
+
+def test_interp_inadmissible_parent(mech_h2, oracle_h2):
+    print("[gpu] interp H2: inadmissible parent", flush=True)
+    C = smooth_field(oracle_h2, 8, 8, 3)
+    C[4, 3, 5] = -1e-3 * C[:oracle_h2.ns, 3, 5].sum()
+    F = torch.zeros((mech_h2.nv, 12, 12), dtype=torch.float64, device="cuda")
+    with pytest.raises(hf.HyperflowError) as e:
+        ConservativeInterp(mech_h2).interp(_dev(C), (0, 0), F, (2, 2), (2, 2, 13, 13))
+    assert e.value.code == hf._lib.HF_EDOMAIN and e.value.first_failed == 2 * 6 + 4
+
+
+@pytest.mark.parametrize("mech", ["h2", "dme"])
+def test_average_down_matches_oracle(mech, mech_h2, mech_dme):
+    m = mech_h2 if mech == "h2" else mech_dme
+    print(f"[gpu] average-down {mech}", flush=True)
+    rng = np.random.default_rng(8)
+    Fh = rng.standard_normal((m.nv, 22, 30)) * 10.0 ** rng.uniform(-8, 8, size=(m.nv, 1, 1))
+    Cd = torch.full((m.nv, 12, 16), float("nan"), dtype=torch.float64, device="cuda")
+    average_down(m, _dev(Fh), (-4, 2), Cd, (-3, 1), (-2, 1, 12, 11))
+    exp = oamr.average_down(Fh, (-4, 2), (-2, 1, 12, 11))
+    out = Cd.cpu().numpy()
+    np.testing.assert_array_equal(out[:, 0:11, 1:16], exp)
+    assert np.isnan(out[:, :, 0]).all() and np.isnan(out[:, 11, :]).all()
+
+
+def _compare(a, b, rel, ns):
+    scale = np.abs(b).max(axis=tuple(range(1, b.ndim)), keepdims=True)
+    rho = b[:ns].sum(0).max()
+    scale[:ns] = np.maximum(scale[:ns], 1e-6 * rho)
+    err = np.abs(a - b) / np.maximum(scale, 1e-300)
+    assert err.max() <= rel, (err.max(), np.unravel_index(err.argmax(), err.shape))
+
+
+def test_teng_osw_base_matches_oracle(mech_h2, oracle_h2):
+    """The wedge case's base level at half resolution (72 x 32): supersonic
+    inflow over the wedge, 4 Strang steps (the oblique shock forms at the
+    tip)."""
+    print("[gpu] wedge case base level 72 x 32, 4 steps", flush=True)
+    m = hf.Mechanism(hf.H2_LI, rtol=1e-12, atol=1e-20)
+    w = teng_osw_base(m, nx=72, ny=32)
+    g = BoxDriver(m, 72, 32, w["dx"], w["dy"], w["bc"], cfl=0.6, inlet=w["inlet"], wedge_i=w["wedge_i"])
+    o = OracleBoxDriver(oracle_h2, 72, 32, w["dx"], w["dy"], w["bc"], cfl=0.6, inlet=w["inlet"], wedge_i=w["wedge_i"])
+    g.set_state(w["Q"])
+    o.Q[:] = w["Q"]
+    for _ in range(4):
+        dtg, dto = g.step(), o.step()
+        assert dtg == pytest.approx(dto, rel=1e-12)
+    Qg = g.state()
+    assert not np.allclose(Qg, w["Q"])                 # the wall turned the flow
+    _compare(Qg, o.Q, 1e-8, mech_h2.ns)

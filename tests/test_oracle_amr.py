@@ -1,0 +1,185 @@
+"""Pins of the oracle's level transfers (oracle/amr.py) and of its inflow /
+wedge ghost fill (oracle/driver.py fill_ghosts):
+
+* ghost fill: the known answers of the reference's own boundary test
+  (HyperFlow_V3/tests/boundary_check.cpp:15-36: an 8 x 4 domain, value
+  1000 n + 10 j + i + 1, fixed inflow left and top, outflow right, a wedge
+  bottom whose tip lies on the x = 0 face, inlet 20000 + n);
+* interpolation: a uniform field is reproduced; a linear field is reproduced
+  exactly at the fine centres; every parent's four children average back to
+  it; on trace-species fields whose neighbours push minmod slopes past the
+  parent's density, theta is the closed form max{2^-k : q - 2^-k (sx + sy)/4
+  >= 0}; an inadmissible parent is reported;
+* average-down: exact sums of integer-valued fields, mass conservation, and
+  the round trip average_down(interp(C)) == C to roundoff.
+"""
+import numpy as np
+import pytest
+
+from oracle.amr import average_down, interp
+from oracle.driver import INFLOW, PERIODIC, REFLECTING, TRANSMISSIVE, WEDGE, fill_ghosts
+
+R = 8314.462618
+
+
+def test_boundary_check_known_answers():
+    ns, nx, ny = 9, 8, 4
+    nv = ns + 3
+    value = lambda i, j, n: 1000.0 * n + 10.0 * j + i + 1.0
+    G = np.full((nv, ny + 4, nx + 4), -1e100)
+    for n in range(nv):
+        for j in range(ny):
+            for i in range(nx):
+                G[n, j + 2, i + 2] = value(i, j, n)
+    inlet = 20000.0 + np.arange(nv)
+    # prob lo x = -0.002, dx = 0.001, wedge start 0: the wall starts at column 2
+    fill_ghosts(G, ns, (INFLOW, TRANSMISSIVE, WEDGE, INFLOW), False, inlet, wedge_i=2)
+    a = lambda i, j, n: G[n, j + 2, i + 2]
+    my = ns + 1
+    for n in range(nv):
+        assert a(-1, 1, n) == inlet[n] and a(3, 4, n) == inlet[n] and a(-1, -1, n) == inlet[n]
+        assert a(8, 2, n) == value(7, 2, n)
+        assert a(0, -1, n) == value(0, 0, n) and a(0, -2, n) == value(0, 0, n)
+        sign = -1.0 if n == my else 1.0
+        assert a(3, -1, n) == sign * value(3, 0, n) and a(3, -2, n) == sign * value(3, 1, n)
+    # every ghost cell is written
+    assert np.all(G > -1e99)
+
+
+def test_flow_fill_corners_and_clamp():
+    """Corners beyond an inflow side are inlet; a wedge-bottom corner beyond a
+    non-inflow side reads the clamped column; plain corners follow filcc."""
+    rng = np.random.default_rng(4)
+    ns, nx, ny = 9, 5, 4
+    nv = ns + 3
+    G = rng.standard_normal((nv, ny + 4, nx + 4))
+    inlet = rng.standard_normal(nv)
+    fill_ghosts(G, ns, (REFLECTING, INFLOW, WEDGE, TRANSMISSIVE), False, inlet, wedge_i=1)
+    a = lambda i, j: G[:, j + 2, i + 2]
+    my = ns + 1
+    for g in (-1, -2):
+        # x-lo reflecting + wedge bottom (column -1 < wedge_i: outflow from row 0, column clamped to 0)
+        assert np.array_equal(a(-1, g), a(0, 0)) and np.array_equal(a(-2, g), a(0, 0))
+        # x-hi inflow corners
+        assert np.array_equal(a(nx, g), inlet) and np.array_equal(a(nx + 1, ny - 1 - g), inlet)
+        # wall columns mirror
+        w = a(3, g).copy()
+        w[my] *= -1
+        assert np.array_equal(w, a(3, -g - 1))
+    # x-lo reflecting + y-hi transmissive corner: the y rule on the filled x-ghost column
+    assert np.array_equal(a(-1, ny), a(-1, ny - 1)) and np.array_equal(a(-2, ny + 1), a(-2, ny - 1))
+
+
+def state(o, T, p, Y, u=0.0, v=0.0):
+    """Conserved state of an ideal gas at (T, p, Y, u, v) (BlockAMR.cpp:20-24)."""
+    Y = np.asarray(Y, dtype=float)
+    rho = p / (R * T * np.sum(Y / o.m["W"]))
+    e, _ = o.e_cv(T, Y)
+    return np.concatenate([rho * Y, [rho * u, rho * v, rho * (e + 0.5 * (u * u + v * v))]])
+
+
+Y_FRESH = np.array([0.028, 0.226, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.746])
+Y_BURNT = np.array([0.002, 0.01, 0.22, 2e-4, 1.5e-3, 1.5e-2, 2e-5, 2e-6, 0.7515])
+
+
+def smooth_field(o, nx, ny, seed):
+    """An admissible H2/air field with steep but smooth T, p, Y and velocity variation."""
+    rng = np.random.default_rng(seed)
+    C = np.zeros((o.ns + 3, ny, nx))
+    for j in range(ny):
+        for i in range(nx):
+            f = 0.5 + 0.5 * np.sin(0.9 * i + 0.6 * j + rng.uniform(-0.3, 0.3))
+            Y = (1 - f) * Y_FRESH + f * Y_BURNT
+            C[:, j, i] = state(o, 900.0 + 1400.0 * f, 1e5 * (1 + 0.5 * f), Y / Y.sum(),
+                               80.0 * np.cos(0.5 * i), 40.0 * np.sin(0.7 * j))
+    return C
+
+
+def test_interp_uniform_and_linear(oracle_h2):
+    o = oracle_h2
+    q = state(o, 1500.0, 2e5, [0.01, 0.2, 0.1, 1e-4, 1e-3, 1e-2, 1e-5, 1e-6, 0.67889], 100.0, -40.0)
+    C = np.repeat(np.repeat(q[:, None, None], 6, 1), 7, 2)
+    F, theta, failed = interp(o, C, (0, 0), (2, 2, 9, 7))
+    assert failed.size == 0 and np.all(theta == 1.0)
+    assert np.array_equal(F, np.broadcast_to(q[:, None, None], F.shape))
+    # a linear field in every component: minmod keeps the slope, the children
+    # are the field at the fine centres ((I + 1/2)/2 - 1/2 in coarse units)
+    ii, jj = np.meshgrid(np.arange(7), np.arange(6))
+    L = q[:, None, None] * (1 + 0.01 * ii[None] + 0.02 * jj[None])
+    F, theta, failed = interp(o, L, (0, 0), (2, 2, 11, 9))
+    assert failed.size == 0 and np.all(theta == 1.0)
+    fi, fj = np.meshgrid(np.arange(2, 12), np.arange(2, 10))
+    exact = q[:, None, None] * (1 + 0.01 * ((fi + 0.5) / 2 - 0.5)[None] + 0.02 * ((fj + 0.5) / 2 - 0.5)[None])
+    assert np.allclose(F, exact, rtol=1e-14, atol=0)
+
+
+def test_interp_children_average_to_parent(oracle_h2):
+    o = oracle_h2
+    C = smooth_field(o, 12, 10, 1)
+    F, theta, failed = interp(o, C, (0, 0), (2, 2, 21, 17))
+    assert failed.size == 0
+    mean = average_down(F, (2, 2), (1, 1, 10, 8))
+    assert np.allclose(mean, C[:, 1:9, 1:11], rtol=2e-16 * 4, atol=1e-300)
+    # odd region bounds: only the children inside are written, same values
+    F2, theta2, _ = interp(o, C, (0, 0), (3, 2, 20, 16))
+    assert np.array_equal(F2, F[:, 0:15, 1:19]) and np.array_equal(theta2, theta)
+
+
+def limited_field(o, seed=2):
+    """Trace H (index 3) densities whose left / lower neighbours are negative:
+    minmod slopes s > 2 q push the (-,-) child negative, everything else stays
+    admissible; returns the field and the closed-form theta of each parent."""
+    rng = np.random.default_rng(seed)
+    base = state(o, 1400.0, 1.5e5, [0.02, 0.2, 0.05, 0.0, 1e-3, 1e-2, 1e-5, 1e-6, 0.71889], 20.0, 10.0)
+    nx = ny = 15
+    C = np.repeat(np.repeat(base[:, None, None], ny, 1), nx, 2)
+    h = 1e-6 * base[:o.ns].sum()
+    theta = {}
+    for j in range(1, ny - 1, 3):                       # parents three apart: no shared neighbours
+        for i in range(1, nx - 1, 3):
+            q = h * rng.uniform(0.5, 2.0)
+            kx, ky = np.exp(rng.uniform(np.log(0.2), np.log(60.0), size=2))   # slopes kx q, ky q after minmod
+            C[3, j, i] = q
+            C[3, j, i - 1] = q - kx * q
+            C[3, j, i + 1] = q + kx * q * 1.5
+            C[3, j - 1, i] = q - ky * q
+            C[3, j + 1, i] = q + ky * q * 1.5
+            sx, sy = min(kx * q, 1.5 * kx * q), min(ky * q, 1.5 * ky * q)
+            t = 1.0
+            while q - t * (0.25 * sx + 0.25 * sy) < 0:
+                t *= 0.5
+            theta[(i, j)] = t
+    return C, theta
+
+
+def test_interp_theta_closed_form(oracle_h2):
+    o = oracle_h2
+    C, expect = limited_field(o)
+    # region = the parent's own children
+    for (i, j), t in expect.items():
+        F, theta, failed = interp(o, C, (0, 0), (2 * i, 2 * j, 2 * i + 1, 2 * j + 1))
+        assert failed.size == 0
+        assert theta[0, 0] == t, (i, j, theta[0, 0], t)
+    assert min(expect.values()) <= 1 / 8 and any(t == 1 for t in expect.values())
+
+
+def test_interp_inadmissible_parent(oracle_h2):
+    o = oracle_h2
+    C = smooth_field(o, 8, 8, 3)
+    C[4, 3, 5] = -1e-3 * C[:o.ns, 3, 5].sum()             # parent (5, 3) holds negative O
+    F, theta, failed = interp(o, C, (0, 0), (2, 2, 13, 13))
+    # parents of region: i, j in 1..6; (5, 3) is row 2, column 4
+    assert list(failed) == [2 * 6 + 4]
+    assert np.isnan(F[:, 4:6, 8:10]).all() and not np.isnan(np.delete(F, np.s_[8:10], axis=2)).any()
+
+
+def test_average_down_exact_and_conservative():
+    rng = np.random.default_rng(7)
+    F = rng.integers(-1000, 1000, size=(12, 10, 14)).astype(float)
+    Cd = average_down(F, (4, 6), (2, 3, 8, 7))
+    for n in (0, 5, 11):
+        for j in range(3, 8):
+            for i in range(2, 9):
+                s = sum(F[n, 2 * j + b - 6, 2 * i + a - 4] for b in (0, 1) for a in (0, 1))
+                assert Cd[n, j - 3, i - 2] == 0.25 * s
+    assert np.array_equal(4 * Cd.sum(axis=(1, 2)), F.sum(axis=(1, 2)))
