@@ -187,6 +187,109 @@ def channel_cpu(hf, target_s=12.0):
     return n, time.perf_counter() - t0, o.Q.copy()
 
 
+def wedge_run(torch, hf, steps, warmup):
+    """Strang steps of the wedge case's base level (144 x 64, inflow left and
+    top, wedge bottom from x = 0, outflow right) through BoxDriver, batched
+    FP64 chemistry. Returns (wall of the timed steps, driver, state after them)."""
+    from hyperflow_amd.driver import BoxDriver, teng_osw_base
+    mech = hf.Mechanism(hf.H2_LI, rtol=1e-8, atol=1e-12)
+    w = teng_osw_base(mech)
+
+    def fresh():
+        d = BoxDriver(mech, 144, 64, w["dx"], w["dy"], w["bc"], cfl=0.6, chem_mode=hf.HF_CHEM_BATCHED,
+                      inlet=w["inlet"], wedge_i=w["wedge_i"])
+        d.set_state(w["Q"])
+        return d
+    d = fresh()
+    for _ in range(warmup):
+        d.step()
+    d = fresh()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        d.step()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, d, d.state()
+
+
+def wedge_cpu(hf, target_s=12.0):
+    """The oracle's step loop on the same wedge case: whole steps until about
+    target_s has passed. Returns (steps, wall, state)."""
+    from hyperflow_amd.driver import teng_osw_base
+    from oracle import Oracle
+    from oracle.driver import OracleBoxDriver
+    w = teng_osw_base(hf.Mechanism(hf.H2_LI))
+    o = OracleBoxDriver(Oracle(hf.H2_LI), 144, 64, w["dx"], w["dy"], w["bc"], cfl=0.6, rtol=1e-8, atol=1e-12,
+                        inlet=w["inlet"], wedge_i=w["wedge_i"])
+    o.Q = w["Q"].copy()
+    n, t0 = 0, time.perf_counter()
+    while n == 0 or time.perf_counter() - t0 < target_s:
+        o.step()
+        n += 1
+    return n, time.perf_counter() - t0, o.Q.copy()
+
+
+def transfer_field(hf, nx, ny, lattice=(33, 17), seed=6):
+    """A smooth admissible H2/air coarse level with one ghost ring,
+    (nv, ny+2, nx+2): conserved states on a lattice of (T in [900, 2300] K,
+    p in [1, 20] atm, fresh-to-burnt composition, |u|, |v| < 400 m/s; seed)
+    blended bilinearly (a convex combination of admissible states is
+    admissible)."""
+    rng = np.random.default_rng(seed)
+    mech = hf.Mechanism(hf.H2_LI)
+    Yf = np.zeros(mech.ns)
+    Yb = np.zeros(mech.ns)
+    for name, y in (("H2", 0.028), ("O2", 0.226), ("N2", 0.746)):
+        Yf[mech.species.index(name)] = y
+    for name, y in (("H2", 0.002), ("O2", 0.01), ("H2O", 0.22), ("H", 2e-4), ("O", 1.5e-3), ("OH", 1.5e-2),
+                    ("HO2", 2e-5), ("H2O2", 2e-6), ("N2", 0.7515)):
+        Yb[mech.species.index(name)] = y
+    Yb /= Yb.sum()
+    lx, ly = lattice
+    L = np.empty((mech.nv, ly, lx))
+    for j in range(ly):
+        for i in range(lx):
+            f = rng.uniform()
+            L[:, j, i] = mech.state_from_TPY(900.0 + 1400.0 * f, 101325.0 * np.exp(rng.uniform(0, np.log(20.0))),
+                                             (1 - f) * Yf + f * Yb, rng.uniform(-400, 400), rng.uniform(-400, 400))
+    x = np.linspace(0, lx - 1, nx + 2)
+    y = np.linspace(0, ly - 1, ny + 2)
+    i0 = np.minimum(x.astype(int), lx - 2)
+    j0 = np.minimum(y.astype(int), ly - 2)
+    fx, fy = x - i0, y - j0
+    A = L[:, j0][:, :, i0] * (1 - fx) + L[:, j0][:, :, i0 + 1] * fx
+    B = L[:, j0 + 1][:, :, i0] * (1 - fx) + L[:, j0 + 1][:, :, i0 + 1] * fx
+    return np.ascontiguousarray(A * (1 - fy)[None, :, None] + B * fy[None, :, None])
+
+
+def transfer_run(torch, hf, C, reps):
+    """reps coarse-to-fine interpolations of the whole coarse level C (with its
+    ghost ring) onto a fine level twice as fine, then reps average-downs back.
+    Returns (interp wall per call, average-down wall per call, limited parents,
+    fine level, coarse result)."""
+    from hyperflow_amd.amr import ConservativeInterp, average_down
+    mech = hf.Mechanism(hf.H2_LI)
+    nv, gy, gx = C.shape
+    nx, ny = gx - 2, gy - 2
+    Cd = torch.from_numpy(C).to("cuda")
+    Fd = torch.empty((nv, 2 * ny, 2 * nx), dtype=torch.float64, device="cuda")
+    Ad = torch.empty((nv, ny, nx), dtype=torch.float64, device="cuda")
+    ci = ConservativeInterp(mech)
+    region = (0, 0, 2 * nx - 1, 2 * ny - 1)
+    ci.interp(Cd, (-1, -1), Fd, (0, 0), region)
+    average_down(mech, Fd, (0, 0), Ad, (0, 0), (0, 0, nx - 1, ny - 1))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        limited = ci.interp(Cd, (-1, -1), Fd, (0, 0), region)
+    t_i = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        average_down(mech, Fd, (0, 0), Ad, (0, 0), (0, 0, nx - 1, ny - 1))
+    t_a = (time.perf_counter() - t0) / reps
+    return t_i, t_a, limited, Fd, Ad
+
+
 ADJ_H, ADJ_STEPS, ADJ_CKPT = 1e-8, 200, 20
 
 
@@ -268,6 +371,10 @@ def main():
                     help="Strang steps of the 2-D channel's 1024 x 256 base level (0 skips it)")
     ap.add_argument("--adjoint-cells", type=int, default=65536,
                     help="cells of the adjoint workload (200 fixed steps, checkpoints every 20; 0 skips it)")
+    ap.add_argument("--wedge-steps", type=int, default=200,
+                    help="Strang steps of the wedge case's 144 x 64 base level (0 skips it)")
+    ap.add_argument("--transfer-level", type=int, default=2048,
+                    help="coarse level width of the level-transfer leg (height half of it; 0 skips it)")
     ap.add_argument("--copy-ref-mib", type=int, default=0,
                     help="also run a device-to-device copy of this many MiB (reference for HBM counter scaling)")
     args = ap.parse_args()
@@ -382,6 +489,49 @@ def main():
                 "first_order_stages": cdrv.first_order_stages, "retries": cdrv.retries, "repairs": cdrv.repairs}
         log(f"2-D channel: {chan['steps_per_s']:.4g} steps/s, {chan['Mcell_updates_per_s']:.4g} Mcell-updates/s")
 
+    wedge = None
+    if args.wedge_steps:
+        log(f"wedge case: 144 x 64 base level, {args.wedge_steps} Strang steps")
+        wwall, wdrv, _ = wedge_run(torch, hf, args.wedge_steps, 1)
+        wedge = {"workload": "oblique-shock wedge case, base level only (no refinement): 144 x 64 cells over "
+                             "2.304 x 1.024 mm from x = -0.128 mm (H2_Li, the reference's cases/teng_case1); "
+                             "premixed H2/air (2:1:3.76) at 958.1 K / 1202900 Pa, 2737.7 m/s at 19 deg below x, in "
+                             "the domain and through the inflow left and top; wedge bottom (outflow before x = 0, "
+                             "slip wall after), transmissive right; NND + HLLC, Heun, CFL 0.6, Strang chemistry "
+                             "(batched FP64 form, rtol 1e-8 atol 1e-12) over every cell",
+                 "steps": args.wedge_steps, "steps_per_s": args.wedge_steps / wwall,
+                 "Mcell_updates_per_s": 144 * 64 * 4 * args.wedge_steps / wwall / 1e6,
+                 "ms_per_step": wwall / args.wedge_steps * 1e3, "simulated_time_s": wdrv.time,
+                 "first_order_stages": wdrv.first_order_stages, "retries": wdrv.retries, "repairs": wdrv.repairs}
+        log(f"wedge case: {wedge['steps_per_s']:.4g} steps/s")
+
+    xfer = None
+    if args.transfer_level:
+        tnx, tny = args.transfer_level, args.transfer_level // 2
+        log(f"level transfers: {tnx} x {tny} coarse level -> {2 * tnx} x {2 * tny}, 5 calls each")
+        Cx = transfer_field(hf, tnx, tny)
+        t_i, t_a, limited, Fx, Ax = transfer_run(torch, hf, Cx, 5)
+        nvx = Cx.shape[0]
+        b_i = 5 * nvx * 8 * tnx * tny / t_i / 1e9         # one coarse read + four children written per parent
+        b_a = 5 * nvx * 8 * tnx * tny / t_a / 1e9         # four children read + one coarse written
+        xfer = {"workload": f"coarse-to-fine conservative interpolation (ratio 2) of a {tnx} x {tny} H2/air level "
+                            "(smooth: bilinear blend of a 33 x 17 lattice of admissible states, seed 6) and "
+                            "average-down of the result",
+                "interp": {"Mparents_per_s": tnx * tny / t_i / 1e6, "ms_per_call": t_i * 1e3,
+                           "limited_parents": int(limited),
+                           "hbm_algorithmic": {"achieved": b_i, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                               "frac": b_i / HBM_PEAK_GBS, "bytes_per_parent": 5 * nvx * 8,
+                                               "timing": "wall per call (launch, one kernel, counter copy)"}},
+                "average_down": {"Mcoarse_cells_per_s": tnx * tny / t_a / 1e6, "ms_per_call": t_a * 1e3,
+                                 "hbm_algorithmic": {"achieved": b_a, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                                     "frac": b_a / HBM_PEAK_GBS, "bytes_per_coarse_cell": 5 * nvx * 8,
+                                                     "timing": "wall per call (launch, one kernel)"}}}
+        log(f"level transfers: interp {t_i * 1e3:.2f} ms ({b_i:.0f} GB/s), average-down {t_a * 1e3:.2f} ms "
+            f"({b_a:.0f} GB/s), limited parents {limited}")
+        xfer_fine = Fx[:, :64, :64].cpu().numpy()
+        xfer_coarse = Ax.cpu().numpy()
+        del Fx, Ax
+
     adj = None
     if args.adjoint_cells:
         log(f"adjoint: {args.adjoint_cells} cells, {ADJ_STEPS} steps of {ADJ_H:g} s, checkpoints every {ADJ_CKPT}")
@@ -453,9 +603,51 @@ def main():
         line["shock_tube_1d"] = tube
     if chan is not None:
         line["channel_2d"] = chan
+    if wedge is not None:
+        line["wedge_2d"] = wedge
+    if xfer is not None:
+        line["level_transfers"] = xfer
     if adj is not None:
         line["adjoint"] = adj
     if not args.no_cpu:
+        if wedge is not None:
+            log("wedge case: CPU oracle step loop")
+            nc, cw, cstate = wedge_cpu(hf)
+            _, _, gstate = wedge_run(torch, hf, nc, 0)
+            line["wedge_2d"]["cpu_baseline"] = {
+                "value": nc / cw, "unit": "Strang steps/s", "cores": len(os.sched_getaffinity(0)), "kind": "port",
+                "sample": f"the first {nc} steps of the same run, the oracle's step loop, {cw:.1f} s"}
+            line["wedge_2d"]["parity_vs_oracle"] = {
+                "steps": nc, **state_difference(gstate, cstate, hf.Mechanism(hf.H2_LI).ns)}
+        if xfer is not None:
+            log("level transfers: oracle sample and CPU baseline")
+            from oracle import Oracle
+            from oracle import amr as oamr
+            o = Oracle(hf.H2_LI)
+            # parity: the fine cells of the first 32 x 32 parents, and the whole average-down
+            Fo, _, failed = oamr.interp(o, Cx[:, :34, :34], (-1, -1), (0, 0, 63, 63))
+            line["level_transfers"]["interp"]["parity_vs_oracle"] = {
+                "parents": 32 * 32, "failed": int(failed.size), "bit_exact": bool(np.array_equal(Fo, xfer_fine))}
+            Ao = oamr.average_down(xfer_fine, (0, 0), (0, 0, 31, 31))
+            line["level_transfers"]["average_down"]["parity_vs_oracle"] = {
+                "coarse_cells": 32 * 32, "bit_exact": bool(np.array_equal(Ao, xfer_coarse[:, :32, :32]))}
+            # average_down(interp(C)) returns C up to roundoff (every parent's children average to it)
+            inner = Cx[:, 1:-1, 1:-1]
+            line["level_transfers"]["average_down"]["round_trip_rel"] = float(
+                np.max(np.abs(xfer_coarse - inner) / np.abs(inner).max(axis=(1, 2), keepdims=True)))
+            rows = 8
+            while True:
+                t = time.perf_counter()
+                oamr.interp(o, Cx[:, :rows + 2, :], (-1, -1), (0, 0, 2 * tnx - 1, 2 * rows - 1))
+                t = time.perf_counter() - t
+                if t >= 3.0 or rows >= tny:
+                    break
+                rows = int(min(tny, max(2 * rows, rows * 5.0 / max(t, 1e-3))))
+            line["level_transfers"]["interp"]["cpu_baseline"] = {
+                "value": tnx * rows / t / 1e6, "unit": "Mparents/s", "cores": 1, "kind": "port",
+                "sample": f"the first {rows} rows ({tnx * rows} parents) of the same level through the oracle "
+                          f"(numpy with the C++ properties), {t:.2f} s"}
+            del xfer_coarse
         if tube is not None:
             log("1-D shock tube: CPU oracle step loop")
             nc, cwall, cstate = shock_tube_cpu(hf)

@@ -704,6 +704,9 @@ __global__ void __launch_bounds__(256) interp_kernel(const MD* __restrict__ dm, 
     }
     double theta = 1.0;
     bool accepted = false;
+    // Each temperature inversion starts from the last converged one, as the
+    // reference's per-thread phase does; only admissibility is used.
+    double t_guess = 1000.0;
     for (int attempt = 0; attempt < 48 && !accepted; ++attempt) {
         bool good = true;
         for (int k = 0; k < 4 && good; ++k) {              // (a, b) = (0,0), (1,0), (0,1), (1,1)
@@ -712,7 +715,8 @@ __global__ void __launch_bounds__(256) interp_kernel(const MD* __restrict__ dm, 
             #pragma unroll
             for (int n = 0; n < NV; ++n) child[n] = interp_child(q[n], theta, cx, sx[n], cy, sy[n]);
             Props p;
-            good = properties(*dm, child, p, 1000.0);
+            good = properties(*dm, child, p, t_guess);
+            if (good) t_guess = p.T;
         }
         if (good)
             accepted = true;
