@@ -47,6 +47,11 @@ class hf_patch(Structure):
     _fields_ = [("offset", c_int64), ("pitch", c_int64), ("plane", c_int64)]
 
 
+class hf_face_fluxes(Structure):
+    _fields_ = [("fx", c_void_p), ("fx_offset", c_int64), ("fx_pitch", c_int64), ("fx_plane", c_int64),
+                ("fy", c_void_p), ("fy_offset", c_int64), ("fy_pitch", c_int64), ("fy_plane", c_int64)]
+
+
 class hf_flux_stats(Structure):
     _fields_ = [("cells", c_int64), ("faces", c_int64), ("first_order_sides", c_int64), ("kernel_ms", c_double)]
 
@@ -70,6 +75,10 @@ _SIGNATURES = {
                                 POINTER(hf_chem_stats), POINTER(c_int64)]),
     "hf_flux_stage": (c_int, [c_void_p, POINTER(hf_box), c_int, c_void_p, c_void_p, c_int,
                               POINTER(hf_flux_stats), POINTER(c_int64)]),
+    "hf_flux_stage_fluxes": (c_int, [c_void_p, POINTER(hf_box), c_int, c_void_p, c_void_p, c_int,
+                                     POINTER(hf_face_fluxes), POINTER(hf_flux_stats), POINTER(c_int64)]),
+    "hf_reflux": (c_int, [c_void_p, c_void_p, POINTER(hf_patch), POINTER(hf_face_fluxes), POINTER(hf_face_fluxes),
+                          POINTER(c_int32), c_int32, c_double, c_double, c_double]),
     "hf_properties": (c_int, [c_void_p, c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_void_p,
                               POINTER(c_int64)]),
     "hf_stable_dt": (c_int, [c_void_p, c_void_p, c_int64, c_int64, c_double, c_double, c_double, _dp]),

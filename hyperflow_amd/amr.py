@@ -3,7 +3,8 @@ kernels: the physical ghost fill with FlowBoundary's inflow and wedge bottom
 (HyperFlow_V3/src/FlowBoundary.hpp:7-37, applied by BlockAMR::fill,
 BlockAMR.cpp:137-152), coarse-to-fine conservative interpolation
 (ConservativeInterp::interp, ConservativeInterp.cpp:18-51) and average-down
-(BlockAMR::average, BlockAMR.cpp:157-159).
+(BlockAMR::average, BlockAMR.cpp:157-159), and the flux-register correction
+across levels (BlockAMR.cpp:247-256).
 
 Arrays are torch float64 device tensors in the reference's SoA layout, one
 (nv, h, w) tensor per patch, i fastest. A patch tensor carries the index of
@@ -14,7 +15,7 @@ from __future__ import annotations
 
 from ctypes import byref, c_double, c_int32, c_int64
 
-from ._lib import HF_BC_INFLOW, HF_BC_WEDGE, check, hf_box, hf_patch, load  # noqa: F401
+from ._lib import HF_BC_INFLOW, HF_BC_WEDGE, check, hf_box, hf_face_fluxes, hf_patch, load  # noqa: F401
 from .chemistry import Mechanism
 
 
@@ -123,3 +124,40 @@ def average_down(mechanism, fine, fine_lo, coarse, coarse_lo, cregion):
     fp, cp = _patch(fine, fine_lo), _patch(coarse, coarse_lo)
     reg = (c_int32 * 4)(ilo, jlo, ihi, jhi)
     check(m.handle, load().hf_average_down(m.handle, fine.data_ptr(), byref(fp), coarse.data_ptr(), byref(cp), reg))
+
+
+def face_fluxes(fx, fy, lo=(0, 0)):
+    """hf_face_fluxes for device tensors fx (nv, ny, nx+1) and fy (nv, ny+1, nx)
+    whose element [:, 0, 0] is face lo of their index space."""
+    for t, name in ((fx, "fx"), (fy, "fy")):
+        _device_f64(t, name)
+    return hf_face_fluxes(fx.data_ptr(), -(int(lo[1]) * fx.shape[2] + int(lo[0])), fx.shape[2],
+                          fx.shape[1] * fx.shape[2],
+                          fy.data_ptr(), -(int(lo[1]) * fy.shape[2] + int(lo[0])), fy.shape[2],
+                          fy.shape[1] * fy.shape[2])
+
+
+def reflux(mechanism, Uc, coarse_lo, coarse, fine, fine_box, sides, h, dxc, dyc):
+    """Flux-register correction of one fine box (ratio 2) on the coarse level Uc.
+
+    coarse = (fx, fy, lo) and fine = (fx, fy, lo): the face fluxes of the
+    coarse stage and of the fine box's stage (FluxSweep.stage(..., fluxes=)),
+    lo the level index of their face [:, 0, 0]. fine_box = (ilo, jlo, ihi, jhi)
+    fine cells; sides: bit0 x-lo, bit1 x-hi, bit2 y-lo, bit3 y-hi (the sides
+    that face coarse cells)."""
+    m = _mech(mechanism)
+    _device_f64(Uc, "Uc")
+    ilo, jlo, ihi, jhi = (int(v) for v in fine_box)
+    ci0, cj0, ci1, cj1 = ilo >> 1, jlo >> 1, ihi >> 1, jhi >> 1
+    cx0, cy0 = coarse_lo
+    if ci0 - 1 < cx0 or cj0 - 1 < cy0 or ci1 + 1 >= cx0 + Uc.shape[2] or cj1 + 1 >= cy0 + Uc.shape[1]:
+        raise ValueError("coarse patch must hold the cells around the fine box")
+    for (fx, fy, lo), (a0, b0, a1, b1) in ((coarse, (ci0, cj0, ci1 + 1, cj1 + 1)), (fine, (ilo, jlo, ihi + 1, jhi + 1))):
+        if a0 < lo[0] or b0 < lo[1] or a1 > lo[0] + fx.shape[2] - 1 or b1 - 1 >= lo[1] + fx.shape[1] or \
+                b1 > lo[1] + fy.shape[1] - 1 or a1 - 1 >= lo[0] + fy.shape[2]:
+            raise ValueError("face fluxes must cover the fine box outline")
+    cf, ff = face_fluxes(*coarse), face_fluxes(*fine)
+    cp = _patch(Uc, coarse_lo)
+    reg = (c_int32 * 4)(ilo, jlo, ihi, jhi)
+    check(m.handle, load().hf_reflux(m.handle, Uc.data_ptr(), byref(cp), byref(cf), byref(ff), reg, int(sides),
+                                     float(h), float(dxc), float(dyc)))

@@ -449,8 +449,9 @@ int hf_chem_advance(hf_ctx* c, double* U, int64_t ncell, int64_t ld, const doubl
     return HF_OK;
 }
 
-int hf_flux_stage(hf_ctx* c, const hf_box* boxes, int nbox, const double* Uin, double* Uout, int first_order,
-                  hf_flux_stats* stats, int64_t* first_failed) {
+namespace {
+int flux_stage_impl(hf_ctx* c, const hf_box* boxes, int nbox, const double* Uin, double* Uout, int first_order,
+                    const hf_face_fluxes* fluxes, hf_flux_stats* stats, int64_t* first_failed) {
     if (!c) return HF_EINVAL;
     BusyGuard guard(c);
     if (!guard.ok) return fail(c, HF_EBUSY, "concurrent flux stage is unsupported");
@@ -484,10 +485,18 @@ int hf_flux_stage(hf_ctx* c, const hf_box* boxes, int nbox, const double* Uin, d
                 tiles.push_back(tj);
             }
     const int ntiles = (int)(tiles.size() / 3);
-    if (!c->boxes.ensure(sizeof(hf_box) * nbox) || !c->tiles.ensure(sizeof(int32_t) * tiles.size()) ||
+    if (fluxes)
+        for (int b = 0; b < nbox; ++b)
+            if ((fluxes[b].fx && !is_device_ptr(fluxes[b].fx)) || (fluxes[b].fy && !is_device_ptr(fluxes[b].fy)))
+                return fail(c, HF_EINVAL, "face fluxes need device memory");
+    const size_t flux_table = fluxes ? sizeof(hf_face_fluxes) * nbox : 0;
+    if (!c->boxes.ensure(sizeof(hf_box) * nbox + flux_table) || !c->tiles.ensure(sizeof(int32_t) * tiles.size()) ||
         !c->scal.ensure(sizeof(unsigned long long) * 16))
         return fail(c, HF_ENOMEM, "device scratch");
     HF_HIP(c, hipMemcpyAsync(c->boxes.p, boxes, sizeof(hf_box) * nbox, hipMemcpyHostToDevice, c->stream));
+    if (fluxes)
+        HF_HIP(c, hipMemcpyAsync(static_cast<char*>(c->boxes.p) + sizeof(hf_box) * nbox, fluxes, flux_table,
+                                 hipMemcpyHostToDevice, c->stream));
     HF_HIP(c, hipMemcpyAsync(c->tiles.p, tiles.data(), sizeof(int32_t) * tiles.size(), hipMemcpyHostToDevice, c->stream));
     const bool dev_in = is_device_ptr(Uin), dev_out = is_device_ptr(Uout);
     const double* din = Uin;
@@ -517,6 +526,8 @@ int hf_flux_stage(hf_ctx* c, const hf_box* boxes, int nbox, const double* Uin, d
     L.one_d = one_d ? 1 : 0;
     L.first_failed = scal;
     L.counters = scal + 1;
+    L.fluxes = fluxes ? reinterpret_cast<const hf_face_fluxes*>(static_cast<char*>(c->boxes.p) + sizeof(hf_box) * nbox)
+                      : nullptr;
     L.stream = c->stream;
     float ms = 0;
     L.kernel_ms = &ms;
@@ -539,6 +550,18 @@ int hf_flux_stage(hf_ctx* c, const hf_box* boxes, int nbox, const double* Uin, d
                                        " cell " + std::to_string(hs[0] & 0xffffffffull));
     }
     return HF_OK;
+}
+}  // namespace
+
+int hf_flux_stage(hf_ctx* c, const hf_box* boxes, int nbox, const double* Uin, double* Uout, int first_order,
+                  hf_flux_stats* stats, int64_t* first_failed) {
+    return flux_stage_impl(c, boxes, nbox, Uin, Uout, first_order, nullptr, stats, first_failed);
+}
+
+int hf_flux_stage_fluxes(hf_ctx* c, const hf_box* boxes, int nbox, const double* Uin, double* Uout, int first_order,
+                         const hf_face_fluxes* fluxes, hf_flux_stats* stats, int64_t* first_failed) {
+    if (c && nbox > 0 && !fluxes) return fail(c, HF_EINVAL, "missing face-flux descriptors");
+    return flux_stage_impl(c, boxes, nbox, Uin, Uout, first_order, fluxes, stats, first_failed);
 }
 
 static int props_common(hf_ctx* c, const double* U, int64_t ncell, int64_t ld, double* p, double* a, double* T,
@@ -857,6 +880,36 @@ int hf_interp_coarse_to_fine(hf_ctx* c, const double* coarse, const hf_patch* cp
         if (first_failed) *first_failed = (int64_t)h[1];
         return fail(c, HF_EDOMAIN, "inadmissible coarse parent " + std::to_string(h[1]) + " in interpolation");
     }
+    return HF_OK;
+}
+
+int hf_reflux(hf_ctx* c, double* Uc, const hf_patch* cp, const hf_face_fluxes* coarse, const hf_face_fluxes* fine,
+              const int32_t fine_box[4], int32_t sides, double h, double dxc, double dyc) {
+    if (!c) return HF_EINVAL;
+    BusyGuard guard(c);
+    if (!guard.ok) return fail(c, HF_EBUSY, "concurrent call is unsupported");
+    if (!coarse || !fine || !fine_box) return fail(c, HF_EINVAL, "missing fluxes or fine box");
+    if (sides < 0 || sides > 15) return fail(c, HF_EINVAL, "invalid side mask");
+    if ((fine_box[0] & 1) || (fine_box[1] & 1) || !(fine_box[2] & 1) || !(fine_box[3] & 1))
+        return fail(c, HF_EINVAL, "the fine box must be aligned to the refinement ratio 2");
+    if (!(h > 0) || !(dxc > 0) || !(dyc > 0) || !std::isfinite(h) || !std::isfinite(dxc) || !std::isfinite(dyc))
+        return fail(c, HF_EINVAL, "invalid step or cell size");
+    for (const hf_face_fluxes* f : {coarse, fine})
+        if (!f->fx || !f->fy || !is_device_ptr(f->fx) || !is_device_ptr(f->fy))
+            return fail(c, HF_EINVAL, "face fluxes need device memory");
+    if (sides == 0) return HF_OK;
+    BoxLaunch L{};
+    const int r = patch_common(c, L, Uc, cp, Uc, cp, fine_box);
+    if (r != HF_OK) return r;
+    L.op = BoxLaunch::REFLUX;
+    L.cflux = *coarse;
+    L.fflux = *fine;
+    L.sides = sides;
+    L.h = h;
+    L.dx = dxc;
+    L.dy = dyc;
+    if (c->ops->box_ops(L) != HF_OK) return fail(c, HF_EDEVICE, "reflux launch failed");
+    HF_HIP(c, hipStreamSynchronize(c->stream));
     return HF_OK;
 }
 

@@ -46,7 +46,7 @@ __global__ void __launch_bounds__((TX + 1) * (TWO_D ? TY + 1 : TY))
 flux_stage_kernel(const MechData<T>* __restrict__ dm, const hf_box* __restrict__ boxes,
                   const int32_t* __restrict__ tiles, int ntiles, const double* __restrict__ Uin,
                   double* __restrict__ Uout, int first_order, unsigned long long* __restrict__ first_failed,
-                  unsigned long long* __restrict__ counters) {
+                  unsigned long long* __restrict__ counters, const hf_face_fluxes* __restrict__ fout) {
     using FT = FluxTile<T, TX, TY, TWO_D>;
     constexpr int NS = T::NS, NV = NS + 3, E = NS + 2;
     constexpr int GX = FT::GX, GY = FT::GY, BX = FT::BX;
@@ -147,6 +147,19 @@ flux_stage_kernel(const MechData<T>* __restrict__ dm, const hf_box* __restrict__
                 }
             }
             ++faces;
+            // face-flux output for the flux registers (a face two tiles share is
+            // computed by both from the same staged cells: identical values)
+            if (fout) {
+                const hf_face_fluxes& O = fout[bi];
+                double* dst = DIR == 0 ? O.fx : O.fy;
+                const int64_t off = DIR == 0 ? O.fx_offset : O.fy_offset;
+                const int64_t pitch = DIR == 0 ? O.fx_pitch : O.fy_pitch;
+                const int64_t plane = DIR == 0 ? O.fx_plane : O.fy_plane;
+                if (dst) {
+                    #pragma unroll
+                    for (int c = 0; c < NV; ++c) dst[off + c * plane + (int64_t)fj * pitch + fi] = f[c];
+                }
+            }
         }
         #pragma unroll
         for (int c = 0; c < NV; ++c) s_fx[c * FT::THREADS + lane] = f[c];

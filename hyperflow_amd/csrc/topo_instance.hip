@@ -535,11 +535,13 @@ int flux(const FluxLaunch& L) {
     if (L.one_d) {
         hipLaunchKernelGGL((flux_stage_kernel<Topo, TX1, 1, false>), dim3(grid),
                            dim3(FluxTile<Topo, TX1, 1, false>::THREADS), 0, L.stream, static_cast<const MD*>(L.mech),
-                           L.boxes, L.tiles, L.ntiles, L.Uin, L.Uout, L.first_order, L.first_failed, L.counters);
+                           L.boxes, L.tiles, L.ntiles, L.Uin, L.Uout, L.first_order, L.first_failed, L.counters,
+                           L.fluxes);
     } else {
         hipLaunchKernelGGL((flux_stage_kernel<Topo, TX2, TY2, true>), dim3(grid),
                            dim3(FluxTile<Topo, TX2, TY2, true>::THREADS), 0, L.stream, static_cast<const MD*>(L.mech),
-                           L.boxes, L.tiles, L.ntiles, L.Uin, L.Uout, L.first_order, L.first_failed, L.counters);
+                           L.boxes, L.tiles, L.ntiles, L.Uin, L.Uout, L.first_order, L.first_failed, L.counters,
+                           L.fluxes);
     }
     const hipError_t err = hipGetLastError();
     if (err != hipSuccess) return HF_EDEVICE;
@@ -741,6 +743,63 @@ __global__ void __launch_bounds__(256) interp_kernel(const MD* __restrict__ dm, 
     }
 }
 
+// Flux-register correction of one fine box at ratio (2, 2) (BlockAMR.cpp:
+// 247-256): one lane per (coarse face on a selected side, component). The
+// four sides' outside cells are distinct, so no two lanes write one value.
+__device__ __forceinline__ double reflux_register(double mc, double fc, double mf, double f0, double f1) {
+#pragma clang fp contract(off)
+    double reg = mc * fc;
+    reg += mf * f0;
+    reg += mf * f1;
+    return reg;
+}
+
+__global__ void __launch_bounds__(256) reflux_kernel(double* __restrict__ Uc, hf_patch cp, hf_face_fluxes C,
+                                                     hf_face_fluxes F, int4 fb, int sides, double h, double dxc,
+                                                     double dyc) {
+#pragma clang fp contract(off)
+    constexpr int NV = Topo::NS + 3;
+    const int ci0 = fb.x >> 1, cj0 = fb.y >> 1, ci1 = fb.z >> 1, cj1 = fb.w >> 1;
+    const int ny = cj1 - cj0 + 1, nx = ci1 - ci0 + 1;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long per = 2L * ny + 2L * nx;
+    if (t >= per * NV) return;
+    const int n = (int)(t / per);
+    long k = t % per;
+    int side;
+    if (k < ny) side = 0;
+    else if (k < 2L * ny) { side = 1; k -= ny; }
+    else if (k < 2L * ny + nx) { side = 2; k -= 2L * ny; }
+    else { side = 3; k -= 2L * ny + nx; }
+    if (!((sides >> side) & 1)) return;
+    const double vol = dxc * dyc;
+    double reg;
+    int oi, oj;
+    if (side < 2) {                                        // x faces: column I, coarse row j
+        const int j = cj0 + (int)k;
+        const int I = side == 0 ? ci0 : ci1 + 1;           // coarse face index
+        const int FI = side == 0 ? fb.x : fb.z + 1;        // fine face index
+        const double fc = C.fx[C.fx_offset + n * C.fx_plane + (int64_t)j * C.fx_pitch + I];
+        const double f0 = F.fx[F.fx_offset + n * F.fx_plane + (int64_t)(2 * j) * F.fx_pitch + FI];
+        const double f1 = F.fx[F.fx_offset + n * F.fx_plane + (int64_t)(2 * j + 1) * F.fx_pitch + FI];
+        reg = reflux_register(-h * dyc, fc, h * (0.5 * dyc), f0, f1);
+        oi = side == 0 ? ci0 - 1 : ci1 + 1;
+        oj = j;
+    } else {                                               // y faces: row J, coarse column i
+        const int i = ci0 + (int)k;
+        const int J = side == 2 ? cj0 : cj1 + 1;
+        const int FJ = side == 2 ? fb.y : fb.w + 1;
+        const double fc = C.fy[C.fy_offset + n * C.fy_plane + (int64_t)J * C.fy_pitch + i];
+        const double f0 = F.fy[F.fy_offset + n * F.fy_plane + (int64_t)FJ * F.fy_pitch + 2 * i];
+        const double f1 = F.fy[F.fy_offset + n * F.fy_plane + (int64_t)FJ * F.fy_pitch + 2 * i + 1];
+        reg = reflux_register(-h * dxc, fc, h * (0.5 * dxc), f0, f1);
+        oi = i;
+        oj = side == 2 ? cj0 - 1 : cj1 + 1;
+    }
+    const double sgn = (side & 1) ? 1.0 : -1.0;
+    patch_at(Uc, cp, n, oi, oj) += (sgn * reg) / vol;
+}
+
 // Clears one counter and the failure index (a kernel, not a memset, so the
 // sequence stays valid inside a captured graph).
 __global__ void counter_init_kernel(unsigned long long* __restrict__ counter, unsigned long long* __restrict__ first_failed) {
@@ -909,6 +968,15 @@ int box_ops(const BoxLaunch& L) {
         hipLaunchKernelGGL(counter_init_kernel, dim3(1), dim3(64), 0, L.stream, L.scal, L.first_failed);
         hipLaunchKernelGGL(interp_kernel, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, L.stream, dm, L.src, L.sp,
                            L.dst, L.dp, r, L.scal, L.first_failed);
+        break;
+    }
+    case BoxLaunch::REFLUX: {
+        constexpr int NV = Topo::NS + 3;
+        const int4 r = make_int4(L.region[0], L.region[1], L.region[2], L.region[3]);
+        const long per = 2L * ((r.w >> 1) - (r.y >> 1) + 1) + 2L * ((r.z >> 1) - (r.x >> 1) + 1);
+        const long t = per * NV;
+        hipLaunchKernelGGL(reflux_kernel, dim3((unsigned)((t + 255) / 256)), dim3(256), 0, L.stream, L.dst, L.dp,
+                           L.cflux, L.fflux, r, L.sides, L.h, L.dx, L.dy);
         break;
     }
     case BoxLaunch::AVGDOWN: {

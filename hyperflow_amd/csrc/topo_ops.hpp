@@ -49,6 +49,7 @@ struct FluxLaunch {
     int one_d;                      // x-only stage: tiles are flux_tx1 x 1, else flux_tx2 x flux_ty2
     unsigned long long* first_failed;  // device scalar
     unsigned long long* counters;   // device: [faces, first-order sides]
+    const hf_face_fluxes* fluxes;   // device copy of the per-box face-flux outputs, or null
     hipStream_t stream;
     float* kernel_ms;
 };
@@ -84,7 +85,7 @@ struct AdjointLaunch {
 // Single-box driver operations (ghost fill, roundoff repair, validation and
 // CFL rate) on a ghosted box in device memory.
 struct BoxLaunch {
-    enum Op { FILL, REPAIR, PROPS, INTERP, AVGDOWN };
+    enum Op { FILL, REPAIR, PROPS, INTERP, AVGDOWN, REFLUX };
     const void* mech;
     double* U;
     hf_box box;
@@ -97,7 +98,10 @@ struct BoxLaunch {
     const double* src;
     double* dst;
     hf_patch sp, dp;
-    int region[4];                  // INTERP: fine cells; AVGDOWN: coarse cells (ilo, jlo, ihi, jhi)
+    int region[4];                  // INTERP, REFLUX: fine cells; AVGDOWN: coarse cells (ilo, jlo, ihi, jhi)
+    hf_face_fluxes cflux, fflux;    // REFLUX: coarse and fine face fluxes (level index spaces)
+    int sides;                      // REFLUX: bit0 x-lo, bit1 x-hi, bit2 y-lo, bit3 y-hi
+    double h;                       // REFLUX: stage time step (dx, dy: coarse cell size)
     unsigned long long* scal;       // device scratch (repair: 3 NS + 2 words; props: rate bits;
                                     // interp: limited-cell count)
     unsigned long long* first_failed;

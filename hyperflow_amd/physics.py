@@ -32,15 +32,28 @@ class FluxSweep:
         self.mech = mechanism if isinstance(mechanism, Mechanism) else Mechanism(mechanism)
         self.last_stats = None
 
-    def stage(self, boxes, Uin, Uout, first_order=False):
-        """One Euler stage over ``boxes`` (sequence of hf_box) reading Uin, writing Uout."""
+    def stage(self, boxes, Uin, Uout, first_order=False, fluxes=None):
+        """One Euler stage over ``boxes`` (sequence of hf_box) reading Uin, writing Uout.
+
+        fluxes: optional, one (fx, fy) pair of device tensors per box, fx
+        (nv, ny, nx+1) and fy (nv, ny+1, nx), that receive the box's face
+        fluxes in box-local face indices (the flux-register input)."""
         nb = len(boxes)
         arr = (hf_box * nb)(*boxes)
         st = hf_flux_stats()
         ff = c_int64(-1)
         pin, _ = _ptr(Uin.reshape(-1) if hasattr(Uin, "reshape") else Uin)
         pout, _ = _ptr(Uout.reshape(-1) if hasattr(Uout, "reshape") else Uout)
-        r = load().hf_flux_stage(self.mech.handle, arr, nb, pin, pout, int(bool(first_order)), byref(st), byref(ff))
+        if fluxes is None:
+            r = load().hf_flux_stage(self.mech.handle, arr, nb, pin, pout, int(bool(first_order)), byref(st),
+                                     byref(ff))
+        else:
+            from .amr import face_fluxes
+            if len(fluxes) != nb:
+                raise ValueError("one (fx, fy) pair per box")
+            desc = (_lib.hf_face_fluxes * nb)(*[face_fluxes(fx, fy) for fx, fy in fluxes])
+            r = load().hf_flux_stage_fluxes(self.mech.handle, arr, nb, pin, pout, int(bool(first_order)), desc,
+                                            byref(st), byref(ff))
         self.last_stats = st.as_dict()
         check(self.mech.handle, r, ff.value)
         return self.last_stats

@@ -148,6 +148,26 @@ int hf_chem_advance(hf_ctx *ctx, double *U, int64_t ncell, int64_t ld, const dou
 int hf_flux_stage(hf_ctx *ctx, const hf_box *boxes, int nbox, const double *Uin, double *Uout, int first_order,
                   hf_flux_stats *stats, int64_t *first_failed);
 
+/* Face fluxes of one box, as the face-centred flux arrays BlockAMR::euler
+ * keeps for the flux registers (BlockAMR.cpp:208, 230): x face (i, j) lies
+ * between cells i-1 and i, 0 <= i <= nx, 0 <= j < ny, component n at
+ * fx[fx_offset + n*fx_plane + j*fx_pitch + i]; y face (i, j), 0 <= i < nx,
+ * 0 <= j <= ny, at fy[fy_offset + n*fy_plane + j*fy_pitch + i]. The values are
+ * the HLLC fluxes after wall zeroing (per unit area and time). Device memory;
+ * fy is unused by a 1-D stage. */
+typedef struct hf_face_fluxes {
+    double *fx;
+    int64_t fx_offset, fx_pitch, fx_plane;
+    double *fy;
+    int64_t fy_offset, fy_pitch, fy_plane;
+} hf_face_fluxes;
+
+/* hf_flux_stage that also stores every box's face fluxes (fluxes: nbox host
+ * descriptors, in box order), for refluxing across levels. The update of
+ * Uout is bit-identical to hf_flux_stage's. */
+int hf_flux_stage_fluxes(hf_ctx *ctx, const hf_box *boxes, int nbox, const double *Uin, double *Uout, int first_order,
+                         const hf_face_fluxes *fluxes, hf_flux_stats *stats, int64_t *first_failed);
+
 /* properties(u) (physics.cpp:4-15) on SoA planes: p, a, T per cell; status
  * per cell (HF_OK / HF_EDOMAIN). Returns HF_EDOMAIN when any cell fails
  * (BlockAMR::validate, BlockAMR.cpp:286-298), first_failed = lowest index. */
@@ -240,6 +260,24 @@ int hf_interp_coarse_to_fine(hf_ctx *ctx, const double *coarse, const hf_patch *
  * (2i, 2j+1), (2i+1, 2j+1). */
 int hf_average_down(hf_ctx *ctx, const double *fine, const hf_patch *fp, double *coarse, const hf_patch *cp,
                     const int32_t cregion[4]);
+
+/* The flux-register correction of one fine box at ratio (2, 2) (BlockAMR.cpp:
+ * 247-256: FluxRegister CrseInit with -h dy_c (x faces) / -h dx_c (y faces),
+ * FineAdd with h dy_f / h dx_f, Reflux with scale 1; Cartesian volumes).
+ * fine_box = {ilo, jlo, ihi, jhi}: the fine box's cells, inclusive, in the fine
+ * index space (ilo, jlo even, ihi, jhi odd). coarse addresses the coarse
+ * stage's face fluxes and fine the fine box's, each with its offset taken in
+ * its level's index space (face (i, j) of the coarse level at
+ * fx[fx_offset + n*fx_plane + j*fx_pitch + i]). sides (bit0 x-lo, bit1 x-hi,
+ * bit2 y-lo, bit3 y-hi) selects the box sides that face coarse cells: leave out
+ * a side on the domain boundary or shared with another fine box
+ * (ClearInternalBorders). For each coarse face on a selected side:
+ *   reg = (-h dy_c) Fc;  reg += (h dy_f) Ff(2j);  reg += (h dy_f) Ff(2j+1)
+ * (x faces; dx for y faces), and the coarse cell just outside the box gets
+ * -reg / (dx_c dy_c) on a lo side, +reg / (dx_c dy_c) on a hi side, in Uc
+ * (addressed by cp, coarse index space). No fused multiply-add. */
+int hf_reflux(hf_ctx *ctx, double *Uc, const hf_patch *cp, const hf_face_fluxes *coarse, const hf_face_fluxes *fine,
+              const int32_t fine_box[4], int32_t sides, double h, double dxc, double dyc);
 
 /* BlockAMR::repairRoundoff (BlockAMR.cpp:299-334) on the valid cells of one
  * box of equal cell volumes: negative species within 64 eps of the species'

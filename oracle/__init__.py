@@ -133,16 +133,22 @@ class Oracle:
         return w
 
     # ------------------------------------------------------------ flux stage
-    def flux_stage(self, Uin, dtdx, dtdy, first_order=False, walls=0, dims=2, wedge_i=0):
-        """Uin: (nv, ny+4, nx+4) ghosted SoA -> (status, Uout (nv, ny, nx), fallbacks, first_failed)."""
+    def flux_stage(self, Uin, dtdx, dtdy, first_order=False, walls=0, dims=2, wedge_i=0, fluxes=False):
+        """Uin: (nv, ny+4, nx+4) ghosted SoA -> (status, Uout (nv, ny, nx), fallbacks, first_failed),
+        plus (Fx (nv, ny, nx+1), Fy (nv, ny+1, nx)) face fluxes when fluxes is set."""
         Uin = np.ascontiguousarray(Uin, dtype=np.float64)
         nv, gy, gx = Uin.shape
         nx, ny = gx - 4, gy - 4
         out = np.zeros((nv, ny, nx))
         fb, ff = ctypes.c_long(0), ctypes.c_long(-1)
+        fx = np.zeros((ny, nx + 1, nv)) if fluxes else None
+        fy = np.zeros((ny + 1, nx, nv)) if fluxes else None
         r = lib().or_flux_stage(self.h, nx, ny, _d(Uin), out.ctypes.data_as(_dp), ctypes.c_double(dtdx),
                                 ctypes.c_double(dtdy), int(first_order), int(walls), int(wedge_i), int(dims),
-                                ctypes.byref(fb), ctypes.byref(ff))
+                                ctypes.byref(fb), ctypes.byref(ff),
+                                fx.ctypes.data_as(_dp) if fluxes else None, fy.ctypes.data_as(_dp) if fluxes else None)
+        if fluxes:
+            return r, out, fb.value, ff.value, fx.transpose(2, 0, 1).copy(), fy.transpose(2, 0, 1).copy()
         return r, out, fb.value, ff.value
 
     # ------------------------------------------------------------ chemistry

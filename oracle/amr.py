@@ -12,6 +12,12 @@ TEST INFRASTRUCTURE ONLY (the checker of hyperflow_amd.amr).
                 HyperFlow_V3/CMakeLists.txt:22-26, not vendored): for equal
                 cell volumes, coarse = (1/4) * sum of the 2 x 2 children,
                 accumulated from 0 with i fastest.
+  reflux        BlockAMR::euler's flux registers (HyperFlow_V3/src/BlockAMR.cpp:247-256)
+                for one fine box at ratio 2 over AMReX FluxRegister (same
+                pin): CrseInit stores (-h dy_c) Fc on the coarse faces of the
+                box outline, FineAdd adds (h dy_f) Ff of both fine faces,
+                Reflux adds -reg / V_c to the coarse cell outside a lo side
+                and +reg / V_c outside a hi side.
 
 Admissibility is the oracle's properties() (physics.cpp:4-15, 46-48).
 
@@ -21,7 +27,11 @@ slope, every parent's four children average back to it), by a closed form of
 the theta search on species-gradient fields (theta = the largest power of two
 keeping every child's partial densities non-negative), and by the known
 failure (an inadmissible parent is an error). average_down is pinned by
-exact integer-valued sums and by conservation.
+exact integer-valued sums and by conservation. reflux is pinned by
+composite conservation: one Euler stage on a periodic coarse level and an
+interior fine box (same h), reflux, average-down keeps every conserved total
+to roundoff, which it does not without the correction, and a flipped sign or
+a wrong face scale breaks it (tests/test_oracle_amr.py).
 """
 from __future__ import annotations
 
@@ -118,3 +128,42 @@ def average_down(F, flo, cregion):
         for a in (0, 1):
             s = s + F[:, fj0 + b:fj1 + 1:2, fi0 + a:fi1 + 1:2]
     return 0.25 * s
+
+
+def reflux(Uc, clo, coarse, fine, fine_box, sides, h, dxc, dyc):
+    """Apply the flux-register correction of one fine box to Uc in place.
+
+    Uc: coarse level (nv, h, w), Uc[:, j - clo[1], i - clo[0]] is cell (i, j).
+    coarse = (Fx, Fy, lo): the coarse stage's face fluxes, Fx (nv, ny, nx+1) and
+    Fy (nv, ny+1, nx), whose element [:, 0, 0] is face lo of the coarse level;
+    fine = (Fx, Fy, lo) likewise for the fine box's stage in the fine index
+    space. fine_box = (ilo, jlo, ihi, jhi) fine cells; sides bit0 x-lo, bit1
+    x-hi, bit2 y-lo, bit3 y-hi.
+    """
+    FI0, FJ0, FI1, FJ1 = fine_box
+    ci0, cj0, ci1, cj1 = FI0 >> 1, FJ0 >> 1, FI1 >> 1, FJ1 >> 1
+    cfx, cfy, cl = coarse
+    ffx, ffy, fl = fine
+    vol = dxc * dyc
+    for side in range(4):
+        if not (sides >> side) & 1:
+            continue
+        sgn = 1.0 if side & 1 else -1.0
+        if side < 2:
+            I = ci0 if side == 0 else ci1 + 1
+            FI = FI0 if side == 0 else FI1 + 1
+            oi = ci0 - 1 if side == 0 else ci1 + 1
+            for j in range(cj0, cj1 + 1):
+                reg = (-h * dyc) * cfx[:, j - cl[1], I - cl[0]]
+                reg = reg + (h * (0.5 * dyc)) * ffx[:, 2 * j - fl[1], FI - fl[0]]
+                reg = reg + (h * (0.5 * dyc)) * ffx[:, 2 * j + 1 - fl[1], FI - fl[0]]
+                Uc[:, j - clo[1], oi - clo[0]] += (sgn * reg) / vol
+        else:
+            J = cj0 if side == 2 else cj1 + 1
+            FJ = FJ0 if side == 2 else FJ1 + 1
+            oj = cj0 - 1 if side == 2 else cj1 + 1
+            for i in range(ci0, ci1 + 1):
+                reg = (-h * dxc) * cfy[:, J - cl[1], i - cl[0]]
+                reg = reg + (h * (0.5 * dxc)) * ffy[:, FJ - fl[1], 2 * i - fl[0]]
+                reg = reg + (h * (0.5 * dxc)) * ffy[:, FJ - fl[1], 2 * i + 1 - fl[0]]
+                Uc[:, oj - clo[1], i - clo[0]] += (sgn * reg) / vol

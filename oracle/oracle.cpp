@@ -844,10 +844,14 @@ void or_wdot(void* mp, double T, const double* C, double* w, const double* lnA_s
 //  wedge_i: the y-lo wall holds only for faces of columns i >= wedge_i (the
 //           wedge bottom, BlockAMR.cpp:225-226: x = lo + (i + 0.5) dx >= wedgeStart).
 //  dims: 1 -> x sweep only (1-D), 2 -> both.
+//  fx_out, fy_out (optional): the face fluxes after wall zeroing, the arrays
+//           BlockAMR::euler keeps for its flux registers (BlockAMR.cpp:230):
+//           fx[((j*(nx+1) + i)*nv + c)], fy[((j*nx + i)*nv + c)] (y: j <= ny).
 //  fallbacks: number of face sides that fell back to first order.
 // Returns 0, or EDOMAIN with *first_failed = lowest cell index whose updated state is inadmissible.
 int or_flux_stage(void* mp, int nx, int ny, const double* Uin, double* Uout, double dtdx, double dtdy,
-                  int first_order, int walls, int wedge_i, int dims, long* fallbacks, long* first_failed) {
+                  int first_order, int walls, int wedge_i, int dims, long* fallbacks, long* first_failed,
+                  double* fx_out, double* fy_out) {
     const Mech& m = *static_cast<Mech*>(mp);
     const int ns = m.ns, nv = ns + 3;
     const int gx = nx + 4, gy = ny + 4;
@@ -913,6 +917,8 @@ int or_flux_stage(void* mp, int nx, int ny, const double* Uin, double* Uout, dou
             Props p;
             if (bad < 0 && properties(m, q, p) != OK) bad = (long)j * nx + i;
         }
+    if (fx_out) std::memcpy(fx_out, fx.data(), sizeof(double) * fx.size());
+    if (fy_out && dims > 1) std::memcpy(fy_out, fy.data(), sizeof(double) * fy.size());
     if (fallbacks) *fallbacks = fb;
     if (bad >= 0) {
         if (first_failed) *first_failed = bad;

@@ -183,3 +183,62 @@ def test_average_down_exact_and_conservative():
                 s = sum(F[n, 2 * j + b - 6, 2 * i + a - 4] for b in (0, 1) for a in (0, 1))
                 assert Cd[n, j - 3, i - 2] == 0.25 * s
     assert np.array_equal(4 * Cd.sum(axis=(1, 2)), F.sum(axis=(1, 2)))
+
+
+def composite_stage(o, seed=5, nxc=16, nyc=12, fine_box=(8, 6, 15, 13), dxc=1e-4, cfl=0.3):
+    """One Euler stage of a two-level composite (the same h on both levels,
+    BlockAMR.cpp:202-256): a periodic coarse level nxc x nyc and one interior
+    fine box whose valid cells and two ghost layers come from the coarse level
+    by conservative interpolation. Returns (coarse before, coarse after the
+    stage, fine after the stage, coarse fluxes, fine fluxes, h, dxc, dyc)."""
+    from oracle.driver import PERIODIC, fill_ghosts
+    C = smooth_field(o, nxc, nyc, seed)
+    ns, nv = o.ns, o.ns + 3
+    dyc = dxc
+    # coarse stage on the periodic level
+    G = np.zeros((nv, nyc + 4, nxc + 4))
+    G[:, 2:-2, 2:-2] = C
+    fill_ghosts(G, ns, (PERIODIC,) * 4, False)
+    rho = C[:ns].sum(0)
+    _, a, _, _ = o.properties(C.reshape(nv, -1).T.copy())
+    speed = (np.abs(C[ns] / rho).max() + np.abs(C[ns + 1] / rho).max() + 2 * a.max())
+    h = cfl * 0.5 * dxc / speed
+    r, C1, _, _, cfx, cfy = o.flux_stage(G, h / dxc, h / dyc, fluxes=True)
+    assert r == 0
+    # fine box: interpolation of the coarse level over the box grown by 2 (its ghosts)
+    FI0, FJ0, FI1, FJ1 = fine_box
+    Fg, _, failed = interp(o, G, (-2, -2), (FI0 - 2, FJ0 - 2, FI1 + 2, FJ1 + 2))
+    assert failed.size == 0
+    r, F1, _, _, ffx, ffy = o.flux_stage(Fg, 2 * h / dxc, 2 * h / dyc, fluxes=True)
+    assert r == 0
+    return C, C1, F1, (cfx, cfy, (0, 0)), (ffx, ffy, (FI0, FJ0)), h, dxc, dyc
+
+
+def _composite_totals(C1, F1, fine_box, corrected, o, coarse, fine, h, dxc, dyc):
+    from oracle.amr import reflux
+    C2 = C1.copy()
+    if corrected:
+        reflux(C2, (0, 0), coarse, fine, fine_box, 15, h, dxc, dyc)
+    FI0, FJ0, FI1, FJ1 = fine_box
+    C2[:, FJ0 // 2:FJ1 // 2 + 1, FI0 // 2:FI1 // 2 + 1] = average_down(F1, (FI0, FJ0),
+                                                                      (FI0 // 2, FJ0 // 2, FI1 // 2, FJ1 // 2))
+    return C2.sum(axis=(1, 2))
+
+
+def test_reflux_composite_conservation(oracle_h2):
+    o = oracle_h2
+    box = (8, 6, 15, 13)
+    C, C1, F1, coarse, fine, h, dxc, dyc = composite_stage(o, fine_box=box)
+    before = C.sum(axis=(1, 2))
+    scale = np.abs(C).sum(axis=(1, 2))
+    with_reflux = _composite_totals(C1, F1, box, True, o, coarse, fine, h, dxc, dyc)
+    without = _composite_totals(C1, F1, box, False, o, coarse, fine, h, dxc, dyc)
+    err = np.abs(with_reflux - before) / scale
+    miss = np.abs(without - before) / scale
+    assert err.max() < 1e-14, err
+    # the correction is material: without it the composite loses conservation
+    assert miss.max() > 1e-9, miss
+    # a flipped sign (the correction applied to the wrong side) is caught too
+    ffx, ffy, fl = fine
+    flipped = _composite_totals(C1, F1, box, True, o, coarse, (-ffx, -ffy, fl), h, dxc, dyc)
+    assert (np.abs(flipped - before) / scale).max() > 1e-9
