@@ -290,6 +290,89 @@ def transfer_run(torch, hf, C, reps):
     return t_i, t_a, limited, Fd, Ad
 
 
+COMPOSITE_BOX = (128, 128, 639, 383)          # fine cells of the refined patch (coarse 64..319 x 64..191)
+
+
+def composite_stages(torch, hf, stages, h=None):
+    """Two-level Euler stages (BlockAMR::euler over two levels, BlockAMR.cpp:
+    202-256, the same h on both): the 1024 x 256 channel as the coarse level
+    (reflecting walls) and one interior fine box COMPOSITE_BOX at ratio 2,
+    initialised by conservative interpolation. Per stage: coarse ghost fill,
+    coarse stage with face fluxes, the fine box's two-cell ghost ring
+    interpolated from the coarse level (four strips), fine stage with face
+    fluxes, reflux, average-down. Returns (wall per stage, h, coarse state,
+    fine state, initial coarse state)."""
+    from hyperflow_amd.amr import ConservativeInterp, average_down, fill_ghosts, reflux
+    from hyperflow_amd.driver import channel_2d
+    from hyperflow_amd.physics import FluxSweep, box_for_arrays
+    mech = hf.Mechanism(hf.H2_LI)
+    Q, dx = channel_2d(mech, **CHANNEL)
+    nv = mech.nv
+    nxc, nyc = CHANNEL["nx"], CHANNEL["ny"]
+    FI0, FJ0, FI1, FJ1 = COMPOSITE_BOX
+    nxf, nyf = FI1 - FI0 + 1, FJ1 - FJ0 + 1
+    dev = dict(dtype=torch.float64, device="cuda")
+    G = torch.zeros((nv, nyc + 4, nxc + 4), **dev)
+    G[:, 2:-2, 2:-2] = torch.from_numpy(Q).to("cuda")
+    bc = (hf.HF_BC_REFLECTING,) * 4
+    walls = 15
+    fill_ghosts(mech, G, bc)
+    Fg = torch.zeros((nv, nyf + 4, nxf + 4), **dev)
+    ci = ConservativeInterp(mech)
+    ci.interp(G, (-2, -2), Fg, (FI0 - 2, FJ0 - 2), COMPOSITE_BOX)
+    if h is None:
+        # CFL 0.5 at the fine cell size (BlockAMR::stableDt over the finest level)
+        h = mech.stable_dt(G[:, 2:-2, 2:-2].reshape(nv, -1).contiguous(), dx / 2, dx / 2, 0.5)
+    C1 = torch.zeros((nv, nyc, nxc), **dev)
+    F1 = torch.zeros((nv, nyf, nxf), **dev)
+    cfx, cfy = torch.zeros((nv, nyc, nxc + 1), **dev), torch.zeros((nv, nyc + 1, nxc), **dev)
+    ffx, ffy = torch.zeros((nv, nyf, nxf + 1), **dev), torch.zeros((nv, nyf + 1, nxf), **dev)
+    sweep = FluxSweep(mech)
+    strips = ((FI0 - 2, FJ0 - 2, FI1 + 2, FJ0 - 1), (FI0 - 2, FJ1 + 1, FI1 + 2, FJ1 + 2),
+              (FI0 - 2, FJ0, FI0 - 1, FJ1), (FI1 + 1, FJ0, FI1 + 2, FJ1))
+    C0 = G[:, 2:-2, 2:-2].cpu().numpy()
+    cbox = box_for_arrays(G, C1, h / dx, h / dx, walls)
+    fbox = box_for_arrays(Fg, F1, 2 * h / dx, 2 * h / dx, 0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(stages):
+        fill_ghosts(mech, G, bc)
+        sweep.stage([cbox], G, C1, fluxes=[(cfx, cfy)])
+        for r in strips:
+            ci.interp(G, (-2, -2), Fg, (FI0 - 2, FJ0 - 2), r)
+        sweep.stage([fbox], Fg, F1, fluxes=[(ffx, ffy)])
+        reflux(mech, C1, (0, 0), (cfx, cfy, (0, 0)), (ffx, ffy, (FI0, FJ0)), COMPOSITE_BOX, 15, h, dx, dx)
+        average_down(mech, F1, (FI0, FJ0), C1, (0, 0), (FI0 // 2, FJ0 // 2, FI1 // 2, FJ1 // 2))
+        G[:, 2:-2, 2:-2].copy_(C1)
+        Fg[:, 2:-2, 2:-2].copy_(F1)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / max(stages, 1)
+    return wall, h, C1.cpu().numpy(), F1.cpu().numpy(), C0, Q, dx
+
+
+def composite_oracle(hf, Q, dx, h):
+    """The oracle's first composite stage on the same data (the oracle's
+    ghost fill, flux stage with face fluxes, interpolation, reflux and
+    average-down). Returns (coarse, fine, wall)."""
+    from oracle import Oracle
+    from oracle import amr as oamr
+    from oracle.driver import REFLECTING, fill_ghosts
+    o = Oracle(hf.H2_LI)
+    t = time.perf_counter()
+    nv = o.ns + 3
+    FI0, FJ0, FI1, FJ1 = COMPOSITE_BOX
+    G = np.zeros((nv, Q.shape[1] + 4, Q.shape[2] + 4))
+    G[:, 2:-2, 2:-2] = Q
+    fill_ghosts(G, o.ns, (REFLECTING,) * 4, False)
+    Fg, _, _ = oamr.interp(o, G, (-2, -2), (FI0 - 2, FJ0 - 2, FI1 + 2, FJ1 + 2))
+    r, C1, _, _, cfx, cfy = o.flux_stage(G, h / dx, h / dx, walls=15, fluxes=True)
+    r2, F1, _, _, ffx, ffy = o.flux_stage(Fg, 2 * h / dx, 2 * h / dx, fluxes=True)
+    oamr.reflux(C1, (0, 0), (cfx, cfy, (0, 0)), (ffx, ffy, (FI0, FJ0)), COMPOSITE_BOX, 15, h, dx, dx)
+    C1[:, FJ0 // 2:FJ1 // 2 + 1, FI0 // 2:FI1 // 2 + 1] = oamr.average_down(
+        F1, (FI0, FJ0), (FI0 // 2, FJ0 // 2, FI1 // 2, FJ1 // 2))
+    return C1, F1, time.perf_counter() - t, r, r2
+
+
 ADJ_H, ADJ_STEPS, ADJ_CKPT = 1e-8, 200, 20
 
 
@@ -375,6 +458,8 @@ def main():
                     help="Strang steps of the wedge case's 144 x 64 base level (0 skips it)")
     ap.add_argument("--transfer-level", type=int, default=2048,
                     help="coarse level width of the level-transfer leg (height half of it; 0 skips it)")
+    ap.add_argument("--composite-stages", type=int, default=50,
+                    help="two-level Euler stages of the channel with one refined box (0 skips it)")
     ap.add_argument("--copy-ref-mib", type=int, default=0,
                     help="also run a device-to-device copy of this many MiB (reference for HBM counter scaling)")
     args = ap.parse_args()
@@ -532,6 +617,21 @@ def main():
         xfer_coarse = Ax.cpu().numpy()
         del Fx, Ax
 
+    comp = None
+    if args.composite_stages:
+        log(f"composite: 1024 x 256 coarse + 512 x 256 fine box, {args.composite_stages} two-level stages")
+        composite_stages(torch, hf, 1)                      # warmup
+        cw, ch, _, _, _, _, _ = composite_stages(torch, hf, args.composite_stages)
+        ncells = CHANNEL["nx"] * CHANNEL["ny"] + 512 * 256
+        comp = {"workload": "two-level Euler stages (the same h on both levels, reflux and average-down each "
+                            "stage): the 1024 x 256 channel (H2_Li, reflecting walls) as the coarse level and one "
+                            "interior box of 512 x 256 fine cells at ratio 2 (fine cells 128..639 x 128..383); "
+                            "coarse ghost fill, coarse stage with face fluxes, the fine ghost ring interpolated "
+                            "from the coarse level, fine stage with face fluxes, reflux, average-down",
+                "stages": args.composite_stages, "ms_per_stage": cw * 1e3,
+                "Mcell_updates_per_s": ncells / cw / 1e6, "h_s": ch}
+        log(f"composite: {cw * 1e3:.3f} ms per two-level stage")
+
     adj = None
     if args.adjoint_cells:
         log(f"adjoint: {args.adjoint_cells} cells, {ADJ_STEPS} steps of {ADJ_H:g} s, checkpoints every {ADJ_CKPT}")
@@ -607,6 +707,8 @@ def main():
         line["wedge_2d"] = wedge
     if xfer is not None:
         line["level_transfers"] = xfer
+    if comp is not None:
+        line["composite_2level"] = comp
     if adj is not None:
         line["adjoint"] = adj
     if not args.no_cpu:
@@ -619,6 +721,32 @@ def main():
                 "sample": f"the first {nc} steps of the same run, the oracle's step loop, {cw:.1f} s"}
             line["wedge_2d"]["parity_vs_oracle"] = {
                 "steps": nc, **state_difference(gstate, cstate, hf.Mechanism(hf.H2_LI).ns)}
+        if comp is not None:
+            log("composite: oracle's first stage")
+            _, ch, Cg, Fg1, C0, Qc, cdx = composite_stages(torch, hf, 1, h=comp["h_s"])
+            Co, Fo, ow, r1, r2 = composite_oracle(hf, Qc, cdx, comp["h_s"])
+            nsx = hf.Mechanism(hf.H2_LI).ns
+            # scales: species against the total density, the momenta of both levels against
+            # the coarse level's largest momentum, the energy against its peak (radicals and
+            # the y momentum start at zero; the fine box is still at rest after one stage)
+            mom = np.abs(Co[nsx:nsx + 2]).max()           # one momentum scale for both levels
+
+            def rel(a, e):
+                sc = np.abs(e).max(axis=(1, 2), keepdims=True)
+                sc[:nsx] = np.abs(e[:nsx]).sum(axis=0).max()
+                sc[nsx:nsx + 2] = mom
+                return float((np.abs(a - e) / np.maximum(sc, 1e-300)).max())
+            tot0, tot1 = C0.sum(axis=(1, 2)), Cg.sum(axis=(1, 2))
+            line["composite_2level"]["parity_vs_oracle"] = {
+                "stages": 1, "coarse_rel": rel(Cg, Co), "fine_rel": rel(Fg1, Fo),
+                # species masses over the total mass, and the total energy (the walls' pressure
+                # flux changes the momenta)
+                "conservation_rel": max(float(np.abs(tot1[:nsx] - tot0[:nsx]).max() / tot0[:nsx].sum()),
+                                        float(abs(tot1[nsx + 2] - tot0[nsx + 2]) / abs(tot0[nsx + 2])))}
+            line["composite_2level"]["cpu_baseline"] = {
+                "value": 1.0 / ow, "unit": "two-level stages/s", "cores": 1, "kind": "port",
+                "sample": f"the first stage through the oracle (C++ flux stages, numpy interpolation and reflux), "
+                          f"{ow:.2f} s"}
         if xfer is not None:
             log("level transfers: oracle sample and CPU baseline")
             from oracle import Oracle
