@@ -9,7 +9,7 @@ from pathlib import Path
 
 from ._lib import (HF_BC_INFLOW, HF_BC_PERIODIC, HF_BC_REFLECTING, HF_BC_TRANSMISSIVE, HF_BC_WEDGE, HF_CHEM_BATCHED,
                    HF_CHEM_PLAIN, HF_PREC_FP64, HF_PREC_MIXED, HyperflowError, LIB_PATH, load)
-from .amr import ConservativeInterp, average_down, face_fluxes, fill_ghosts, reflux, wedge_column
+from .amr import ConservativeInterp, average_down, face_fluxes, fill_ghosts, reflux, tag_cells, wedge_column
 from .adjoint import ChemSensitivity, fixed_step_chemistry
 from .chemistry import ChemistryBatch, Mechanism
 from .physics import FluxSweep
@@ -21,4 +21,5 @@ DME_SK39 = MECHANISM_DIR / "dme_sk39.yaml"
 __all__ = ["ChemistryBatch", "Mechanism", "ChemSensitivity", "fixed_step_chemistry", "FluxSweep", "HyperflowError",
            "load", "LIB_PATH", "H2_LI", "DME_SK39", "HF_CHEM_PLAIN", "HF_CHEM_BATCHED", "HF_PREC_FP64", "HF_PREC_MIXED",
            "HF_BC_TRANSMISSIVE", "HF_BC_REFLECTING", "HF_BC_PERIODIC", "HF_BC_INFLOW", "HF_BC_WEDGE",
-           "ConservativeInterp", "average_down", "face_fluxes", "fill_ghosts", "reflux", "wedge_column"]
+           "ConservativeInterp", "average_down", "face_fluxes", "fill_ghosts", "reflux", "tag_cells",
+           "wedge_column"]

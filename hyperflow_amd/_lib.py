@@ -52,6 +52,13 @@ class hf_face_fluxes(Structure):
                 ("fy", c_void_p), ("fy_offset", c_int64), ("fy_pitch", c_int64), ("fy_plane", c_int64)]
 
 
+class hf_tag_params(Structure):
+    _fields_ = [("pressure_tag", c_double), ("temperature_tag", c_double), ("fuel_rate_tag", c_double),
+                ("physics", c_int32), ("chemistry", c_int32), ("fuel", c_int32), ("has_box", c_int32),
+                ("box", c_double * 4), ("velocity", c_double * 2), ("x0", c_double), ("y0", c_double),
+                ("dx", c_double), ("dy", c_double), ("time", c_double)]
+
+
 class hf_flux_stats(Structure):
     _fields_ = [("cells", c_int64), ("faces", c_int64), ("first_order_sides", c_int64), ("kernel_ms", c_double)]
 
@@ -92,6 +99,8 @@ _SIGNATURES = {
                                          POINTER(c_int32), POINTER(c_int64), POINTER(c_int64)]),
     "hf_average_down": (c_int, [c_void_p, c_void_p, POINTER(hf_patch), c_void_p, POINTER(hf_patch),
                                 POINTER(c_int32)]),
+    "hf_tag_cells": (c_int, [c_void_p, c_void_p, POINTER(hf_box), POINTER(hf_tag_params), c_void_p,
+                             POINTER(c_int64), POINTER(c_int64)]),
     "hf_repair_roundoff": (c_int, [c_void_p, c_void_p, POINTER(hf_box), POINTER(c_int64), POINTER(c_int64)]),
     "hf_validate_box": (c_int, [c_void_p, c_void_p, POINTER(hf_box), c_double, c_double, _dp, POINTER(c_int64)]),
     "hf_lincomb": (c_int, [c_void_p, c_void_p, c_void_p, c_double, c_double, c_int64]),

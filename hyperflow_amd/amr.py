@@ -15,7 +15,8 @@ from __future__ import annotations
 
 from ctypes import byref, c_double, c_int32, c_int64
 
-from ._lib import HF_BC_INFLOW, HF_BC_WEDGE, check, hf_box, hf_face_fluxes, hf_patch, load  # noqa: F401
+from ._lib import (HF_BC_INFLOW, HF_BC_WEDGE, check, hf_box, hf_face_fluxes, hf_patch, hf_tag_params,  # noqa: F401
+                   load)
 from .chemistry import Mechanism
 
 
@@ -161,3 +162,27 @@ def reflux(mechanism, Uc, coarse_lo, coarse, fine, fine_box, sides, h, dxc, dyc)
     reg = (c_int32 * 4)(ilo, jlo, ihi, jhi)
     check(m.handle, load().hf_reflux(m.handle, Uc.data_ptr(), byref(cp), byref(cf), byref(ff), reg, int(sides),
                                      float(h), float(dxc), float(dyc)))
+
+
+def tag_cells(mechanism, U, dx, dy, x0=0.0, y0=0.0, time=0.0, pressure_tag=0.08, temperature_tag=0.08,
+              fuel_rate_tag=50.0, physics=True, chemistry=True, fuel=0, box=None, velocity=(0.0, 0.0)):
+    """Refinement tags of BlockAMR::ErrorEst (BlockAMR.cpp:170-200) for the
+    ghosted box U (nv, ny+4, nx+4) whose first ghost layer is filled: a uint8
+    device tensor (ny, nx), 1 where tagged. Defaults are the reference's
+    (pressure_tag and temperature_tag 0.08, fuel_rate_tag 50 /s, species 0 the
+    fuel). An inadmissible cell or neighbour raises HyperflowError(HF_EDOMAIN)."""
+    import torch
+    m = _mech(mechanism)
+    _device_f64(U, "U")
+    nv, gy, gx = U.shape
+    nx, ny = gx - 4, gy - 4
+    b = hf_box(nx, ny, 0, 0, 0, gx, gx * gy, 0, nx, nx * ny, 0.0, 0.0)
+    P = hf_tag_params(float(pressure_tag), float(temperature_tag), float(fuel_rate_tag), int(bool(physics)),
+                      int(bool(chemistry)), int(fuel), int(box is not None),
+                      (c_double * 4)(*(box if box is not None else (0.0, 0.0, 0.0, 0.0))),
+                      (c_double * 2)(*velocity), float(x0), float(y0), float(dx), float(dy), float(time))
+    tags = torch.zeros((ny, nx), dtype=torch.uint8, device=U.device)
+    n, ff = c_int64(0), c_int64(-1)
+    r = load().hf_tag_cells(m.handle, U.data_ptr(), byref(b), byref(P), tags.data_ptr(), byref(n), byref(ff))
+    check(m.handle, r, ff.value)
+    return tags

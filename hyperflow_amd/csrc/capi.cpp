@@ -927,6 +927,37 @@ int hf_average_down(hf_ctx* c, const double* fine, const hf_patch* fp, double* c
     return HF_OK;
 }
 
+int hf_tag_cells(hf_ctx* c, const double* U, const hf_box* box, const hf_tag_params* prm, uint8_t* tags,
+                 int64_t* ntagged, int64_t* first_failed) {
+    if (!c) return HF_EINVAL;
+    BusyGuard guard(c);
+    if (!guard.ok) return fail(c, HF_EBUSY, "concurrent call is unsupported");
+    if (ntagged) *ntagged = 0;
+    if (first_failed) *first_failed = -1;
+    if (!prm || !tags) return fail(c, HF_EINVAL, "missing tagging parameters or output");
+    if (prm->fuel < 0 || prm->fuel >= c->mech.ns()) return fail(c, HF_EINVAL, "fuel species index out of range");
+    if (!(prm->dx > 0) || !(prm->dy > 0)) return fail(c, HF_EINVAL, "invalid cell size");
+    BoxLaunch L{};
+    const int r = box_common(c, L, box, U);
+    if (r != HF_OK) return r;
+    if (!is_device_ptr(tags)) return fail(c, HF_EINVAL, "tags need device memory");
+    L.U = const_cast<double*>(U);
+    L.op = BoxLaunch::TAG;
+    L.tag = *prm;
+    L.tags = tags;
+    if (c->ops->box_ops(L) != HF_OK) return fail(c, HF_EDEVICE, "tagging launch failed");
+    unsigned long long h[2] = {0, ~0ull};
+    HF_HIP(c, hipMemcpyAsync(&h[0], L.scal, sizeof(h[0]), hipMemcpyDeviceToHost, c->stream));
+    HF_HIP(c, hipMemcpyAsync(&h[1], L.first_failed, sizeof(h[1]), hipMemcpyDeviceToHost, c->stream));
+    HF_HIP(c, hipStreamSynchronize(c->stream));
+    if (ntagged) *ntagged = (int64_t)h[0];
+    if (h[1] != ~0ull) {
+        if (first_failed) *first_failed = (int64_t)h[1];
+        return fail(c, HF_EDOMAIN, "inadmissible state while tagging at cell " + std::to_string(h[1]));
+    }
+    return HF_OK;
+}
+
 int hf_repair_roundoff(hf_ctx* c, double* U, const hf_box* box, int64_t* repairs, int64_t* first_failed) {
     if (!c) return HF_EINVAL;
     BusyGuard guard(c);

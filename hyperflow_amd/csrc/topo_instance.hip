@@ -800,6 +800,74 @@ __global__ void __launch_bounds__(256) reflux_kernel(double* __restrict__ Uc, hf
     patch_at(Uc, cp, n, oi, oj) += (sgn * reg) / vol;
 }
 
+// Refinement tags (BlockAMR::ErrorEst, BlockAMR.cpp:178-198): one lane per
+// valid cell of a ghosted box. The neighbours' inversions warm-start from the
+// cell's own temperature. scal[0] counts tagged cells.
+__global__ void __launch_bounds__(256) tag_kernel(const MD* __restrict__ dm, const double* __restrict__ U, hf_box B,
+                                                  hf_tag_params P, uint8_t* __restrict__ tags,
+                                                  unsigned long long* __restrict__ scal,
+                                                  unsigned long long* __restrict__ first_failed) {
+    constexpr int NS = Topo::NS, NV = NS + 3;
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    bool marked = false;
+    if (c < (long)B.nx * B.ny) {
+        const int i = (int)(c % B.nx), j = (int)(c / B.nx);
+        double* u0 = const_cast<double*>(U);
+        const double x = P.x0 + (i + 0.5) * P.dx - P.velocity[0] * P.time;
+        const double y = P.y0 + (j + 0.5) * P.dy - P.velocity[1] * P.time;
+        marked = P.has_box && x >= P.box[0] && x <= P.box[1] && y >= P.box[2] && y <= P.box[3];
+        bool ok = true;
+        if (P.physics) {
+            double q[NV];
+            #pragma unroll
+            for (int n = 0; n < NV; ++n) q[n] = box_at(u0, B, n, i, j);
+            Props p;
+            ok = properties(*dm, q, p, 1000.0);
+            for (int dd = 0; dd < 2 && ok; ++dd) {
+                const int di = dd == 0, dj = dd == 1;
+                double ql[NV], qr[NV];
+                #pragma unroll
+                for (int n = 0; n < NV; ++n) {
+                    ql[n] = box_at(u0, B, n, i - di, j - dj);
+                    qr[n] = box_at(u0, B, n, i + di, j + dj);
+                }
+                Props pl, pr;
+                ok = properties(*dm, ql, pl, p.T) && properties(*dm, qr, pr, p.T);
+                if (!ok) break;
+                const double jump = fmax(fabs(pl.p - p.p) / fmin(pl.p, p.p), fabs(pr.p - p.p) / fmin(pr.p, p.p));
+                const double temp = fmax(fabs(pl.T - p.T) / fmin(pl.T, p.T), fabs(pr.T - p.T) / fmin(pr.T, p.T));
+                marked = marked || jump > P.pressure_tag || temp > P.temperature_tag;
+            }
+            if (ok && P.chemistry) {
+                double C[NS], w[NS], rho = 0;
+                #pragma unroll
+                for (int k = 0; k < NS; ++k) {
+                    C[k] = q[k] * dm->invW[k];
+                    rho += q[k];
+                }
+                production_rates(*dm, p.T, C, w);
+                double wf = 0, Wf = 0;
+                #pragma unroll
+                for (int k = 0; k < NS; ++k)
+                    if (k == P.fuel) {
+                        wf = w[k];
+                        Wf = dm->W[k];
+                    }
+                marked = marked || -wf * Wf / rho > P.fuel_rate_tag;
+            }
+        }
+        if (!ok) {
+            atomicMin(first_failed, (unsigned long long)c);
+            marked = false;
+        }
+        tags[c] = marked ? 1 : 0;
+    }
+    unsigned long long n = marked ? 1ull : 0ull;
+    #pragma unroll
+    for (int off = 32; off > 0; off >>= 1) n += __shfl_xor(n, off, 64);
+    if ((threadIdx.x & 63) == 0 && n) atomicAdd(&scal[0], n);
+}
+
 // Clears one counter and the failure index (a kernel, not a memset, so the
 // sequence stays valid inside a captured graph).
 __global__ void counter_init_kernel(unsigned long long* __restrict__ counter, unsigned long long* __restrict__ first_failed) {
@@ -977,6 +1045,12 @@ int box_ops(const BoxLaunch& L) {
         const long t = per * NV;
         hipLaunchKernelGGL(reflux_kernel, dim3((unsigned)((t + 255) / 256)), dim3(256), 0, L.stream, L.dst, L.dp,
                            L.cflux, L.fflux, r, L.sides, L.h, L.dx, L.dy);
+        break;
+    }
+    case BoxLaunch::TAG: {
+        hipLaunchKernelGGL(counter_init_kernel, dim3(1), dim3(64), 0, L.stream, L.scal, L.first_failed);
+        hipLaunchKernelGGL(tag_kernel, dim3(g256), dim3(256), 0, L.stream, dm, L.U, L.box, L.tag, L.tags, L.scal,
+                           L.first_failed);
         break;
     }
     case BoxLaunch::AVGDOWN: {

@@ -85,7 +85,7 @@ struct AdjointLaunch {
 // Single-box driver operations (ghost fill, roundoff repair, validation and
 // CFL rate) on a ghosted box in device memory.
 struct BoxLaunch {
-    enum Op { FILL, REPAIR, PROPS, INTERP, AVGDOWN, REFLUX };
+    enum Op { FILL, REPAIR, PROPS, INTERP, AVGDOWN, REFLUX, TAG };
     const void* mech;
     double* U;
     hf_box box;
@@ -102,6 +102,8 @@ struct BoxLaunch {
     hf_face_fluxes cflux, fflux;    // REFLUX: coarse and fine face fluxes (level index spaces)
     int sides;                      // REFLUX: bit0 x-lo, bit1 x-hi, bit2 y-lo, bit3 y-hi
     double h;                       // REFLUX: stage time step (dx, dy: coarse cell size)
+    hf_tag_params tag;              // TAG: criteria
+    uint8_t* tags;                  // TAG: device output, nx*ny
     unsigned long long* scal;       // device scratch (repair: 3 NS + 2 words; props: rate bits;
                                     // interp: limited-cell count)
     unsigned long long* first_failed;

@@ -279,6 +279,35 @@ int hf_average_down(hf_ctx *ctx, const double *fine, const hf_patch *fp, double 
 int hf_reflux(hf_ctx *ctx, double *Uc, const hf_patch *cp, const hf_face_fluxes *coarse, const hf_face_fluxes *fine,
               const int32_t fine_box[4], int32_t sides, double h, double dxc, double dyc);
 
+/* Refinement criteria of BlockAMR::ErrorEst (BlockAMR.cpp:170-200). */
+typedef struct hf_tag_params {
+    double pressure_tag;     /* relative pressure jump to a face neighbour (reference default 0.08) */
+    double temperature_tag;  /* relative temperature jump to a face neighbour (default 0.08) */
+    double fuel_rate_tag;    /* fuel consumption rate -wdot_f W_f / rho in 1/s (default 50) */
+    int32_t physics;         /* 0: geometry only (tag_mode "box") */
+    int32_t chemistry;       /* nonzero: apply the fuel-rate criterion (config "chemistry") */
+    int32_t fuel;            /* fuel species index (the reference's species 0, H2) */
+    int32_t has_box;         /* nonzero: tag_box is given */
+    double box[4];           /* tag_box: x0, x1, y0, y1 (inclusive) */
+    double velocity[2];      /* tag_velocity: the box moves with it */
+    double x0, y0;           /* coordinates of the box's lower-left corner (cell (0, 0)'s low faces) */
+    double dx, dy, time;     /* cell size and the time the tags are taken at */
+} hf_tag_params;
+
+/* Tags of the valid cells of one ghosted box (the hf_box input layout; the
+ * first ghost layer must be filled, e.g. by hf_fill_ghosts_flow): cell (i, j)
+ * is tagged when its centre, moved back by velocity * time, lies in the box;
+ * or (physics) when max(|p_n - p| / min(p_n, p)) over the two face neighbours
+ * of either direction exceeds pressure_tag, or the same for T exceeds
+ * temperature_tag; or (chemistry) when -wdot_fuel W_fuel / rho exceeds
+ * fuel_rate_tag. tags: device, nx*ny bytes, tags[j*nx + i] = 1 or 0. ntagged
+ * (optional) receives the count. An inadmissible cell or neighbour (properties
+ * throws in the reference) returns HF_EDOMAIN with the lowest such valid cell
+ * (j*nx + i) in first_failed. The tag buffer and the clustering into boxes
+ * belong to AMReX's regrid and are not part of this call. */
+int hf_tag_cells(hf_ctx *ctx, const double *U, const hf_box *box, const hf_tag_params *prm, uint8_t *tags,
+                 int64_t *ntagged, int64_t *first_failed);
+
 /* BlockAMR::repairRoundoff (BlockAMR.cpp:299-334) on the valid cells of one
  * box of equal cell volumes: negative species within 64 eps of the species'
  * peak are zeroed and their total is taken from the first cell holding the

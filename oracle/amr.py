@@ -19,6 +19,10 @@ TEST INFRASTRUCTURE ONLY (the checker of hyperflow_amd.amr).
                 Reflux adds -reg / V_c to the coarse cell outside a lo side
                 and +reg / V_c outside a hi side.
 
+  tag_cells     BlockAMR::ErrorEst (HyperFlow_V3/src/BlockAMR.cpp:170-200): the moving
+                tag box, the relative p and T jumps to the face neighbours, the
+                fuel consumption rate.
+
 Admissibility is the oracle's properties() (physics.cpp:4-15, 46-48).
 
 Parity status: interp is pinned by its invariants (a uniform field is
@@ -31,7 +35,12 @@ exact integer-valued sums and by conservation. reflux is pinned by
 composite conservation: one Euler stage on a periodic coarse level and an
 interior fine box (same h), reflux, average-down keeps every conserved total
 to roundoff, which it does not without the correction, and a flipped sign or
-a wrong face scale breaks it (tests/test_oracle_amr.py).
+a wrong face scale breaks it (tests/test_oracle_amr.py). tag_cells is
+pinned by closed forms: a uniform gas tags nothing; a 10 % pressure step tags
+exactly the two columns beside it and a 5 % one none; a temperature step at
+fixed pressure likewise; the box criterion tags exactly the cells whose moved
+centres lie in the box; the fuel-rate criterion at a threshold between two
+known rates (the oracle's kinetics, pinned on their own) splits the cells.
 """
 from __future__ import annotations
 
@@ -167,3 +176,46 @@ def reflux(Uc, clo, coarse, fine, fine_box, sides, h, dxc, dyc):
                 reg = reg + (h * (0.5 * dxc)) * ffy[:, FJ - fl[1], 2 * i - fl[0]]
                 reg = reg + (h * (0.5 * dxc)) * ffy[:, FJ - fl[1], 2 * i + 1 - fl[0]]
                 Uc[:, oj - clo[1], i - clo[0]] += (sgn * reg) / vol
+
+
+def tag_cells(o, G, dx, dy, x0=0.0, y0=0.0, time=0.0, pressure_tag=0.08, temperature_tag=0.08, fuel_rate_tag=50.0,
+              physics=True, chemistry=True, fuel=0, box=None, velocity=(0.0, 0.0)):
+    """Tags (ny, nx) of the valid cells of the ghosted box G (nv, ny+4, nx+4),
+    first ghost layer filled. Raises DomainError on an inadmissible cell or
+    neighbour (properties throws, BlockAMR.cpp:183-185)."""
+    nv, gy, gx = G.shape
+    nx, ny = gx - 4, gy - 4
+    ns = nv - 3
+    i = np.arange(nx)[None, :]
+    j = np.arange(ny)[:, None]
+    x = x0 + (i + 0.5) * dx - velocity[0] * time
+    y = y0 + (j + 0.5) * dy - velocity[1] * time
+    marked = np.zeros((ny, nx), dtype=bool)
+    if box is not None:
+        marked |= (x >= box[0]) & (x <= box[1]) & (y >= box[2]) & (y <= box[3])
+    if not physics:
+        return marked
+    R = G[:, 1:ny + 3, 1:nx + 3]                      # valid cells and the first ghost ring
+    p, _, T, st = o.properties(np.ascontiguousarray(R.reshape(nv, -1).T))
+    p, T, st = p.reshape(ny + 2, nx + 2), T.reshape(ny + 2, nx + 2), st.reshape(ny + 2, nx + 2)
+    bad = st != 0
+    inner = (slice(1, ny + 1), slice(1, nx + 1))
+    failed = bad[inner] | bad[0:ny, 1:nx + 1] | bad[2:ny + 2, 1:nx + 1] | bad[1:ny + 1, 0:nx] | bad[1:ny + 1, 2:nx + 2]
+    if failed.any():
+        raise DomainError(f"inadmissible state while tagging at cell {int(np.flatnonzero(failed.ravel())[0])}")
+    pc, Tc = p[inner], T[inner]
+    for (sl_l, sl_r) in (((slice(1, ny + 1), slice(0, nx)), (slice(1, ny + 1), slice(2, nx + 2))),
+                         ((slice(0, ny), slice(1, nx + 1)), (slice(2, ny + 2), slice(1, nx + 1)))):
+        pl, pr, Tl, Tr = p[sl_l], p[sl_r], T[sl_l], T[sl_r]
+        jump = np.maximum(np.abs(pl - pc) / np.minimum(pl, pc), np.abs(pr - pc) / np.minimum(pr, pc))
+        temp = np.maximum(np.abs(Tl - Tc) / np.minimum(Tl, Tc), np.abs(Tr - Tc) / np.minimum(Tr, Tc))
+        marked |= (jump > pressure_tag) | (temp > temperature_tag)
+    if chemistry:
+        W = o.m["W"]
+        Q = G[:, 2:ny + 2, 2:nx + 2]
+        for jj in range(ny):
+            for ii in range(nx):
+                C = Q[:ns, jj, ii] / W
+                w = o.wdot(Tc[jj, ii], C)
+                marked[jj, ii] |= -w[fuel] * W[fuel] / Q[:ns, jj, ii].sum() > fuel_rate_tag
+    return marked

@@ -295,3 +295,63 @@ def test_composite_stage_conserves_on_gpu(mech_h2, oracle_h2):
         F1o, (FI0, FJ0), (FI0 // 2, FJ0 // 2, FI1 // 2, FJ1 // 2))
     s = np.maximum(np.abs(exp).max(axis=(1, 2), keepdims=True), 1e-300)
     assert (np.abs(got - exp) / s).max() <= 1e-12
+
+
+def _tags_both(m, o, G, **kw):
+    from hyperflow_amd.amr import tag_cells
+    got = tag_cells(m, _dev(G), **kw).cpu().numpy().astype(bool)
+    exp = oamr.tag_cells(o, G, **kw)
+    np.testing.assert_array_equal(got, exp)
+    return exp
+
+
+def test_tag_cells_closed_forms_gpu(mech_h2, oracle_h2):
+    from test_oracle_amr import _ghosted, fresh_Y, state, step_field
+    print("[gpu] tagging: pressure / temperature steps, box", flush=True)
+    o = oracle_h2
+    Y = fresh_Y(o)
+    a = state(o, 300.0, 101325.0, Y)
+    for b in (state(o, 300.0, 101325.0 * 1.1, Y), state(o, 330.0, 101325.0, Y), a):
+        _tags_both(mech_h2, o, _ghosted(step_field(o, 10, 6, a, b, 4), o.ns), dx=1e-3, dy=1e-3)
+    t = _tags_both(mech_h2, o, _ghosted(step_field(o, 10, 6, a, a, 4), o.ns), dx=1.0, dy=1.0, time=0.5,
+                   physics=False, box=(2.0, 4.0, 1.0, 1.6), velocity=(1.0, 0.0))
+    assert sorted(map(tuple, np.argwhere(t))) == [(1, 2), (1, 3), (1, 4)]
+
+
+def test_tag_cells_channel_and_wedge(mech_h2, oracle_h2):
+    """Tags at the reference defaults on the channel's hot strip (p, T jumps
+    and the fuel rate) and on the wedge case after two Strang steps."""
+    from hyperflow_amd.driver import channel_2d
+    from oracle.driver import REFLECTING, TRANSMISSIVE, fill_ghosts as of
+    print("[gpu] tagging: channel 128 x 32, wedge case 72 x 32", flush=True)
+    Q, dx = channel_2d(mech_h2, nx=128, ny=32, length=0.025)
+    G = np.zeros((Q.shape[0], 36, 132))
+    G[:, 2:-2, 2:-2] = Q
+    of(G, mech_h2.ns, (REFLECTING, TRANSMISSIVE, REFLECTING, REFLECTING), False)
+    t = _tags_both(mech_h2, oracle_h2, G, dx=dx, dy=dx)
+    assert 0 < t.sum() < t.size
+    m = hf.Mechanism(hf.H2_LI, rtol=1e-12, atol=1e-20)
+    w = teng_osw_base(m, nx=72, ny=32)
+    d = BoxDriver(m, 72, 32, w["dx"], w["dy"], w["bc"], cfl=0.6, inlet=w["inlet"], wedge_i=w["wedge_i"])
+    d.set_state(w["Q"])
+    d.step()
+    d.step()
+    d._fill(d.U)
+    Gw = d.U.cpu().numpy()
+    tw = _tags_both(mech_h2, oracle_h2, Gw, dx=w["dx"], dy=w["dy"], x0=-0.000128, time=d.time,
+                    box=(-3.2e-05, 0.000128, 0.0, 6.4e-05))
+    assert tw.sum() > 0
+
+
+def test_tag_cells_inadmissible_neighbour(mech_h2, oracle_h2):
+    from hyperflow_amd.amr import tag_cells
+    from test_oracle_amr import _ghosted, fresh_Y, state, step_field
+    print("[gpu] tagging: inadmissible neighbour", flush=True)
+    o = oracle_h2
+    a = state(o, 300.0, 101325.0, fresh_Y(o))
+    G = _ghosted(step_field(o, 10, 6, a, a, 4), o.ns)
+    G[1, 2 + 3, 2 + 7] = -1.0                               # O2 of cell (7, 3) negative
+    with pytest.raises(hf.HyperflowError) as e:
+        tag_cells(mech_h2, _dev(G), 1e-3, 1e-3)
+    # the lowest valid cell whose stencil holds it: (7, 2), index 2 * 10 + 7
+    assert e.value.code == hf._lib.HF_EDOMAIN and e.value.first_failed == 27

@@ -242,3 +242,71 @@ def test_reflux_composite_conservation(oracle_h2):
     ffx, ffy, fl = fine
     flipped = _composite_totals(C1, F1, box, True, o, coarse, (-ffx, -ffy, fl), h, dxc, dyc)
     assert (np.abs(flipped - before) / scale).max() > 1e-9
+
+
+def _ghosted(Q, ns):
+    from oracle.driver import TRANSMISSIVE, fill_ghosts
+    G = np.zeros((Q.shape[0], Q.shape[1] + 4, Q.shape[2] + 4))
+    G[:, 2:-2, 2:-2] = Q
+    fill_ghosts(G, ns, (TRANSMISSIVE,) * 4, False)
+    return G
+
+
+def step_field(o, nx, ny, left, right, k):
+    """Columns i < k hold state left, the others state right."""
+    Q = np.zeros((o.ns + 3, ny, nx))
+    Q[:, :, :k] = left[:, None, None]
+    Q[:, :, k:] = right[:, None, None]
+    return Q
+
+
+def fresh_Y(o):
+    W = o.m["W"]
+    X = np.zeros(o.ns)
+    X[0], X[1], X[8] = 2.0, 1.0, 3.76                    # H2, O2, N2 (H2_Li order)
+    Y = X * W
+    return Y / Y.sum()
+
+
+def test_tag_cells_closed_forms(oracle_h2):
+    from oracle.amr import tag_cells
+    o = oracle_h2
+    Y = fresh_Y(o)
+    nx, ny, k = 10, 6, 4
+    a = state(o, 300.0, 101325.0, Y)
+    # uniform gas: nothing
+    assert not tag_cells(o, _ghosted(step_field(o, nx, ny, a, a, k), o.ns), 1e-3, 1e-3).any()
+    # 10 % pressure step at fixed T: exactly the two columns beside it; 5 %: none
+    for ratio, cols in ((1.10, {k - 1, k}), (1.05, set())):
+        b = state(o, 300.0, 101325.0 * ratio, Y)
+        t = tag_cells(o, _ghosted(step_field(o, nx, ny, a, b, k), o.ns), 1e-3, 1e-3)
+        assert set(np.flatnonzero(t.any(axis=0))) == cols and (t.all(axis=0) == t.any(axis=0)).all()
+    # temperature step at fixed p: 10 % tags the two columns, 5 % none
+    for ratio, cols in ((1.10, {k - 1, k}), (1.05, set())):
+        b = state(o, 300.0 * ratio, 101325.0, Y)
+        t = tag_cells(o, _ghosted(step_field(o, nx, ny, a, b, k), o.ns), 1e-3, 1e-3)
+        assert set(np.flatnonzero(t.any(axis=0))) == cols
+    # geometry only: box x in [2, 4], y in [1, 1.6], moving at (1, 0), at t = 0.5 with unit cells:
+    # the moved centres are (i, j + 0.5), so cells i = 2, 3, 4 of row 1
+    t = tag_cells(o, _ghosted(step_field(o, nx, ny, a, a, k), o.ns), 1.0, 1.0, time=0.5, physics=False,
+                  box=(2.0, 4.0, 1.0, 1.6), velocity=(1.0, 0.0))
+    assert sorted(map(tuple, np.argwhere(t))) == [(1, 2), (1, 3), (1, 4)]
+
+
+def test_tag_cells_fuel_rate(oracle_h2):
+    """A threshold halfway between a cold and a hot state's fuel consumption
+    rate (the oracle's kinetics) tags exactly the hot cells."""
+    from oracle.amr import tag_cells
+    o = oracle_h2
+    W = o.m["W"]
+    cold = state(o, 300.0, 101325.0, fresh_Y(o))
+    Yh = np.array([0.02, 0.18, 0.06, 1e-4, 1e-3, 4e-3, 1e-5, 1e-6, 0.0])
+    Yh[8] = 1.0 - Yh.sum()
+    hot = state(o, 1500.0, 101325.0, Yh)
+    rate = lambda q, T: -o.wdot(T, q[:o.ns] / W)[0] * W[0] / q[:o.ns].sum()
+    r_cold, r_hot = rate(cold, 300.0), rate(hot, 1500.0)
+    assert r_hot > 100 * max(r_cold, 1e-30)
+    Q = step_field(o, 8, 4, cold, hot, 5)
+    t = tag_cells(o, _ghosted(Q, o.ns), 1e-3, 1e-3, pressure_tag=1e9, temperature_tag=1e9,
+                  fuel_rate_tag=0.5 * r_hot)
+    assert set(np.flatnonzero(t.any(axis=0))) == {5, 6, 7} and (t.all(axis=0) == t.any(axis=0)).all()
