@@ -350,6 +350,27 @@ def composite_stages(torch, hf, stages, h=None):
     return wall, h, C1.cpu().numpy(), F1.cpu().numpy(), C0, Q, dx
 
 
+def tag_run(torch, hf, reps):
+    """Tags of the channel's 1024 x 256 level (reflecting ghosts) at the
+    reference defaults, reps calls. Returns (ms per call, tagged cells,
+    ghosted level as numpy, dx)."""
+    from hyperflow_amd.amr import fill_ghosts, tag_cells
+    from hyperflow_amd.driver import channel_2d
+    mech = hf.Mechanism(hf.H2_LI)
+    Q, dx = channel_2d(mech, **CHANNEL)
+    G = torch.zeros((mech.nv, Q.shape[1] + 4, Q.shape[2] + 4), dtype=torch.float64, device="cuda")
+    G[:, 2:-2, 2:-2] = torch.from_numpy(Q).to("cuda")
+    fill_ghosts(mech, G, (hf.HF_BC_REFLECTING,) * 4)
+    tag_cells(mech, G, dx, dx)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        t = tag_cells(mech, G, dx, dx)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / reps * 1e3
+    return ms, int(t.sum().item()), (G.cpu().numpy(), t.cpu().numpy()), dx
+
+
 def composite_oracle(hf, Q, dx, h):
     """The oracle's first composite stage on the same data (the oracle's
     ghost fill, flux stage with face fluxes, interpolation, reflux and
@@ -622,6 +643,7 @@ def main():
         log(f"composite: 1024 x 256 coarse + 512 x 256 fine box, {args.composite_stages} two-level stages")
         composite_stages(torch, hf, 1)                      # warmup
         cw, ch, _, _, _, _, _ = composite_stages(torch, hf, args.composite_stages)
+        tag_ms, tag_n, tagQ, tagdx = tag_run(torch, hf, 5)
         ncells = CHANNEL["nx"] * CHANNEL["ny"] + 512 * 256
         comp = {"workload": "two-level Euler stages (the same h on both levels, reflux and average-down each "
                             "stage): the 1024 x 256 channel (H2_Li, reflecting walls) as the coarse level and one "
@@ -629,7 +651,11 @@ def main():
                             "coarse ghost fill, coarse stage with face fluxes, the fine ghost ring interpolated "
                             "from the coarse level, fine stage with face fluxes, reflux, average-down",
                 "stages": args.composite_stages, "ms_per_stage": cw * 1e3,
-                "Mcell_updates_per_s": ncells / cw / 1e6, "h_s": ch}
+                "Mcell_updates_per_s": ncells / cw / 1e6, "h_s": ch,
+                "tagging": {"workload": "ErrorEst criteria at the reference defaults (p and T jumps 0.08, fuel "
+                                        "rate 50 /s) over the 1024 x 256 coarse level",
+                            "ms_per_call": tag_ms, "tagged": tag_n,
+                            "Mcells_per_s": CHANNEL["nx"] * CHANNEL["ny"] / tag_ms / 1e3}}
         log(f"composite: {cw * 1e3:.3f} ms per two-level stage")
 
     adj = None
@@ -743,6 +769,20 @@ def main():
                 # flux changes the momenta)
                 "conservation_rel": max(float(np.abs(tot1[:nsx] - tot0[:nsx]).max() / tot0[:nsx].sum()),
                                         float(abs(tot1[nsx + 2] - tot0[nsx + 2]) / abs(tot0[nsx + 2])))}
+            # tags of the first 64 rows x 256 columns through the oracle (their ghost
+            # ring is the level's own neighbours)
+            from oracle import Oracle
+            from oracle import amr as oamr
+            Gt, tg = tagQ
+            t0 = time.perf_counter()
+            to = oamr.tag_cells(Oracle(hf.H2_LI), np.ascontiguousarray(Gt[:, :68, :260]), tagdx, tagdx)
+            tw = time.perf_counter() - t0
+            line["composite_2level"]["tagging"]["parity_vs_oracle"] = {
+                "cells": 64 * 256, "identical": bool(np.array_equal(to, tg[:64, :256].astype(bool)))}
+            line["composite_2level"]["tagging"]["cpu_baseline"] = {
+                "value": 64 * 256 / tw / 1e6, "unit": "Mcells/s", "cores": 1, "kind": "port",
+                "sample": f"64 x 256 cells of the same level through the oracle (numpy, C++ properties and "
+                          f"kinetics), {tw:.2f} s"}
             line["composite_2level"]["cpu_baseline"] = {
                 "value": 1.0 / ow, "unit": "two-level stages/s", "cores": 1, "kind": "port",
                 "sample": f"the first stage through the oracle (C++ flux stages, numpy interpolation and reflux), "
