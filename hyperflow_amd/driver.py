@@ -270,3 +270,205 @@ def teng_osw_base(mech, nx=144, ny=64):
     Q = np.repeat(np.repeat(np.asarray(inlet)[:, None, None], ny, 1), nx, 2)
     return {"Q": Q, "inlet": np.asarray(inlet), "dx": dx, "dy": dy, "wedge_i": wedge_column(x0, dx, 0.0),
             "bc": (HF_BC_INFLOW, HF_BC_TRANSMISSIVE, HF_BC_WEDGE, HF_BC_INFLOW)}
+
+
+class TwoLevelDriver:
+    """Strang steps of a two-level composite: a single-box coarse level and
+    one fixed interior fine box at ratio 2 (BlockAMR::run / flow / euler /
+    react, BlockAMR.cpp:202-285, 400-425, without regrid).
+
+    euler: coarse ghost fill; the fine box's two-cell ghost ring interpolated
+    from the coarse level (ConservativeInterp, as FillPatchTwoLevels does for
+    an interior box); both stages with face fluxes; reflux; average-down;
+    roundoff repair and validation of both levels, with the first-order redo
+    of the whole composite stage. react: one all-or-nothing chemistry batch
+    over the active cells (the uncovered coarse cells and every fine cell),
+    then average-down. stableDt: the largest CFL rate over both levels.
+
+    Roundoff repair runs per level (each species' integral is kept exactly on
+    each level); the reference picks one donor over both levels.
+    fine_box = (ilo, jlo, ihi, jhi) fine cells; its coarse cover must lie at
+    least two cells inside the coarse level.
+    """
+
+    def __init__(self, mechanism, nx, ny, dx, dy, bc, fine_box, cfl=0.5, chem_mode=HF_CHEM_BATCHED,
+                 max_attempts=18, inlet=None, wedge_i=0):
+        import torch
+        from .amr import ConservativeInterp
+        from .physics import FluxSweep
+        self.torch = torch
+        self.mech = mechanism
+        self.coarse = BoxDriver(mechanism, nx, ny, dx, dy, bc, cfl=cfl, chem_mode=chem_mode,
+                                max_attempts=max_attempts, inlet=inlet, wedge_i=wedge_i)
+        FI0, FJ0, FI1, FJ1 = (int(v) for v in fine_box)
+        if FI0 % 2 or FJ0 % 2 or FI1 % 2 == 0 or FJ1 % 2 == 0:
+            raise ValueError("the fine box must be aligned to the refinement ratio 2")
+        if FI0 // 2 < 2 or FJ0 // 2 < 2 or FI1 // 2 > nx - 3 or FJ1 // 2 > ny - 3:
+            raise ValueError("the fine box must lie at least two coarse cells inside the level")
+        self.fine_box = (FI0, FJ0, FI1, FJ1)
+        self.nxf, self.nyf = FI1 - FI0 + 1, FJ1 - FJ0 + 1
+        self.dxf, self.dyf = 0.5 * float(dx), 0.5 * float(dy)
+        nv = mechanism.nv
+        dev = dict(dtype=torch.float64, device="cuda")
+        shape = (nv, self.nyf + 4, self.nxf + 4)
+        self.Uf, self.Uf1, self.Uf2 = (torch.zeros(shape, **dev) for _ in range(3))
+        self.oldf, self.backupf = torch.zeros(shape, **dev), torch.zeros(shape, **dev)
+        self.cfx = torch.zeros((nv, ny, nx + 1), **dev)
+        self.cfy = torch.zeros((nv, ny + 1, nx), **dev)
+        self.ffx = torch.zeros((nv, self.nyf, self.nxf + 1), **dev)
+        self.ffy = torch.zeros((nv, self.nyf + 1, self.nxf), **dev)
+        self.interp = ConservativeInterp(mechanism)
+        self.sweep = FluxSweep(mechanism)
+        self.strips = ((FI0 - 2, FJ0 - 2, FI1 + 2, FJ0 - 1), (FI0 - 2, FJ1 + 1, FI1 + 2, FJ1 + 2),
+                       (FI0 - 2, FJ0, FI0 - 1, FJ1), (FI1 + 1, FJ0, FI1 + 2, FJ1))
+        self.covered = (slice(FJ0 // 2, FJ1 // 2 + 1), slice(FI0 // 2, FI1 // 2 + 1))
+        mask = torch.ones((ny, nx), dtype=torch.bool, device="cuda")
+        mask[self.covered] = False
+        self.active = mask.reshape(-1)
+        self.cfl = float(cfl)
+        self.max_attempts = max_attempts
+        self.time, self.steps, self.retries, self.first_order_stages, self.repairs = 0.0, 0, 0, 0, 0
+        self._h = mechanism.handle
+
+    # ------------------------------------------------------------ state
+    def set_state(self, Q):
+        """Q: coarse conserved values (nv, ny, nx); the fine box starts as
+        their conservative interpolation."""
+        self.coarse.set_state(Q)
+        self.coarse._fill(self.coarse.U)
+        self.interp.interp(self.coarse.U, (-2, -2), self.Uf, (self.fine_box[0] - 2, self.fine_box[1] - 2),
+                           self.fine_box)
+
+    def fine_interior(self, A=None):
+        A = self.Uf if A is None else A
+        return A[:, 2:self.nyf + 2, 2:self.nxf + 2]
+
+    def state(self):
+        return self.coarse.state(), self.fine_interior().cpu().numpy().copy()
+
+    # ------------------------------------------------------------ stages
+    def _fbox(self, h=0.0):
+        gx, gy = self.nxf + 4, self.nyf + 4
+        plane = gx * gy
+        return hf_box(self.nxf, self.nyf, 0, 0, 0, gx, plane, 2 * gx + 2, gx, plane, h / self.dxf, h / self.dyf)
+
+    def _average(self, Ac, Af):
+        from .amr import average_down
+        FI0, FJ0, FI1, FJ1 = self.fine_box
+        fine = self.fine_interior(Af).contiguous()
+        Cv = self.coarse.interior(Ac).contiguous()
+        average_down(self.mech, fine, (FI0, FJ0), Cv, (0, 0), (FI0 // 2, FJ0 // 2, FI1 // 2, FJ1 // 2))
+        self.coarse.interior(Ac).copy_(Cv)
+
+    def _validate_fine(self, A):
+        box = self._fbox()
+        rate, ff = c_double(0.0), c_int64(-1)
+        r = load().hf_validate_box(self._h, A.data_ptr(), byref(box), self.dxf, self.dyf, byref(rate), byref(ff))
+        check(self._h, r, ff.value)
+        return rate.value
+
+    def _composite_once(self, src_c, src_f, dst_c, dst_f, h, first_order):
+        from .amr import reflux
+        from .physics import box_for_arrays
+        c = self.coarse
+        FI0, FJ0, _, _ = self.fine_box
+        c._fill(src_c)
+        for r in self.strips:
+            self.interp.interp(src_c, (-2, -2), src_f, (FI0 - 2, FJ0 - 2), r)
+        out_c = c.interior(dst_c)
+        oc = self.torch.empty((self.mech.nv, c.ny, c.nx), dtype=self.torch.float64, device="cuda")
+        of = self.torch.empty((self.mech.nv, self.nyf, self.nxf), dtype=self.torch.float64, device="cuda")
+        for (src, out, dx, dy, walls, wi, fl) in ((src_c, oc, c.dx, c.dy, c.walls, c.wedge_i, (self.cfx, self.cfy)),
+                                                  (src_f, of, self.dxf, self.dyf, 0, 0, (self.ffx, self.ffy))):
+            box = box_for_arrays(src, out, h / dx, h / dy, walls, wi)
+            try:
+                self.sweep.stage([box], src, out, first_order=first_order, fluxes=[fl])
+            except HyperflowError as e:
+                # an inadmissible raw update is repaired or rejected below, as in BoxDriver
+                if e.code != HF_EDOMAIN:
+                    raise
+        reflux(self.mech, oc, (0, 0), (self.cfx, self.cfy, (0, 0)), (self.ffx, self.ffy, (FI0, FJ0)),
+               self.fine_box, 15, h, c.dx, c.dy)
+        out_c.copy_(oc)
+        self.fine_interior(dst_f).copy_(of)
+        self._average(dst_c, dst_f)
+        rep, ff = c_int64(0), c_int64(-1)
+        repaired = 0
+        for A, box in ((dst_f, self._fbox()), (dst_c, c._box())):
+            r = load().hf_repair_roundoff(self._h, A.data_ptr(), byref(box), byref(rep), byref(ff))
+            if r != HF_OK:
+                return r, ff.value
+            repaired += rep.value
+        self.repairs += repaired
+        if repaired:
+            self._average(dst_c, dst_f)
+        for A, box, dx, dy in ((dst_c, c._box(), c.dx, c.dy), (dst_f, self._fbox(), self.dxf, self.dyf)):
+            r = load().hf_validate_box(self._h, A.data_ptr(), byref(box), dx, dy, None, byref(ff))
+            if r != HF_OK:
+                return r, ff.value
+        return HF_OK, -1
+
+    def euler(self, src_c, src_f, dst_c, dst_f, h):
+        r, ff = self._composite_once(src_c, src_f, dst_c, dst_f, h, False)
+        if r == HF_EDOMAIN:
+            self.first_order_stages += 1
+            r, ff = self._composite_once(src_c, src_f, dst_c, dst_f, h, True)
+        check(self._h, r, ff)
+
+    def flow(self, h):
+        c = self.coarse
+        c.old.copy_(c.U)
+        self.oldf.copy_(self.Uf)
+        self.euler(c.U, self.Uf, c.U1, self.Uf1, h)
+        self.euler(c.U1, self.Uf1, c.U2, self.Uf2, h)
+        for A, old in ((c.U2, c.old), (self.Uf2, self.oldf)):
+            check(self._h, load().hf_lincomb(self._h, A.data_ptr(), old.data_ptr(), 0.5, 0.5, A.numel()))
+        c.U, c.U2 = c.U2, c.U
+        self.Uf, self.Uf2 = self.Uf2, self.Uf
+        self._average(c.U, self.Uf)
+        c._validate(c.U)
+        self._validate_fine(self.Uf)
+
+    def react(self, h):
+        c = self.coarse
+        nv = self.mech.nv
+        Cc = c.interior().reshape(nv, -1)
+        Ff = self.fine_interior().reshape(nv, -1)
+        batch = self.torch.cat([Cc[:, self.active], Ff], dim=1).contiguous()
+        c.chem.advance(batch, h)
+        na = int(self.active.sum().item())
+        Cc2 = Cc.clone()
+        Cc2[:, self.active] = batch[:, :na]
+        c.interior().copy_(Cc2.reshape(c.interior().shape))
+        self.fine_interior().copy_(batch[:, na:].reshape(self.fine_interior().shape))
+        self._average(c.U, self.Uf)
+        c._validate(c.U)
+        self._validate_fine(self.Uf)
+
+    def stable_dt(self):
+        return self.cfl / max(self.coarse._validate(self.coarse.U), self._validate_fine(self.Uf))
+
+    def step(self, target=None):
+        """One Strang step of both levels; returns the dt taken."""
+        c = self.coarse
+        dt = self.stable_dt()
+        if target is not None:
+            dt = min(target - self.time, dt)
+        c.backup.copy_(c.U)
+        self.backupf.copy_(self.Uf)
+        for _ in range(self.max_attempts):
+            try:
+                self.flow(0.5 * dt)
+                self.react(dt)
+                self.flow(0.5 * dt)
+                break
+            except HyperflowError:
+                c.U.copy_(c.backup)
+                self.Uf.copy_(self.backupf)
+                self.retries += 1
+                dt *= 0.5
+        else:
+            raise RuntimeError("time advancement failed")
+        self.time = target if target is not None and dt >= target - self.time else self.time + dt
+        self.steps += 1
+        return dt

@@ -202,3 +202,124 @@ class OracleBoxDriver:
         self.time += dt
         self.steps += 1
         return dt
+
+
+class OracleTwoLevelDriver:
+    """The two-level composite step loop (BlockAMR.cpp:202-285, 400-425 over
+    one fixed interior fine box at ratio 2, no regrid) on the oracle's
+    functions: the checker of hyperflow_amd.driver.TwoLevelDriver. Same
+    composition: coarse fill, fine ghost ring by interpolation, both stages
+    with face fluxes, reflux, average-down, per-level roundoff repair,
+    validation, first-order redo; Heun; one chemistry batch over the active
+    cells; CFL over both levels. Parity status: pinned by the functions it
+    composes and by composite conservation (tests/test_oracle_amr.py)."""
+
+    def __init__(self, oracle, nx, ny, dx, dy, bc, fine_box, cfl=0.5, rtol=1e-12, atol=1e-20, max_attempts=18,
+                 inlet=None, wedge_i=0):
+        self.c = OracleBoxDriver(oracle, nx, ny, dx, dy, bc, cfl=cfl, rtol=rtol, atol=atol,
+                                 max_attempts=max_attempts, inlet=inlet, wedge_i=wedge_i)
+        self.o = oracle
+        self.ns, self.nv = oracle.ns, oracle.ns + 3
+        self.fine_box = tuple(fine_box)
+        FI0, FJ0, FI1, FJ1 = self.fine_box
+        self.nxf, self.nyf = FI1 - FI0 + 1, FJ1 - FJ0 + 1
+        self.dxf, self.dyf = 0.5 * dx, 0.5 * dy
+        self.cov = (slice(FJ0 // 2, FJ1 // 2 + 1), slice(FI0 // 2, FI1 // 2 + 1))
+        self.active = np.ones((ny, nx), dtype=bool)
+        self.active[self.cov] = False
+        self.cfl, self.rtol, self.atol, self.max_attempts = cfl, rtol, atol, max_attempts
+        self.Qc = np.zeros((self.nv, ny, nx))
+        self.Qf = np.zeros((self.nv, self.nyf, self.nxf))
+        self.time, self.steps, self.retries, self.first_order_stages = 0.0, 0, 0, 0
+
+    def set_state(self, Q):
+        from .amr import interp
+        self.Qc = Q.copy()
+        G = self.c._ghosted(self.Qc)
+        self.Qf, _, _ = interp(self.o, G, (-2, -2), self.fine_box)
+
+    def _average(self, Qc, Qf):
+        from .amr import average_down
+        FI0, FJ0, FI1, FJ1 = self.fine_box
+        Qc[(slice(None),) + self.cov] = average_down(Qf, (FI0, FJ0), (FI0 // 2, FJ0 // 2, FI1 // 2, FJ1 // 2))
+
+    def _validate_fine(self, Qf):
+        p, a, T, st = self.o.properties(Qf.reshape(self.nv, -1).T.copy())
+        if np.any(st != 0):
+            raise DomainError("inadmissible state")
+        rho = Qf[:self.ns].sum(0).reshape(-1)
+        return ((np.abs(Qf[self.ns].reshape(-1) / rho) + a) / self.dxf +
+                (np.abs(Qf[self.ns + 1].reshape(-1) / rho) + a) / self.dyf).max()
+
+    def euler(self, Qc, Qf, h):
+        from .amr import interp, reflux
+        FI0, FJ0, FI1, FJ1 = self.fine_box
+        G = self.c._ghosted(Qc)
+        Gf = np.zeros((self.nv, self.nyf + 4, self.nxf + 4))
+        ring, _, _ = interp(self.o, G, (-2, -2), (FI0 - 2, FJ0 - 2, FI1 + 2, FJ1 + 2))
+        Gf[:] = ring
+        Gf[:, 2:-2, 2:-2] = Qf
+        for first_order in (False, True):
+            _, oc, _, _, cfx, cfy = self.o.flux_stage(G, h / self.c.dx, h / self.c.dy, first_order=first_order,
+                                                      walls=self.c.walls, wedge_i=self.c.wedge_i, fluxes=True)
+            _, of, _, _, ffx, ffy = self.o.flux_stage(Gf, h / self.dxf, h / self.dyf, first_order=first_order,
+                                                      fluxes=True)
+            reflux(oc, (0, 0), (cfx, cfy, (0, 0)), (ffx, ffy, (FI0, FJ0)), self.fine_box, 15, h, self.c.dx, self.c.dy)
+            self._average(oc, of)
+            try:
+                n = repair_roundoff(of, self.ns) + repair_roundoff(oc, self.ns)
+                if n:
+                    self._average(oc, of)
+                self.c._validate(oc)
+                self._validate_fine(of)
+                return oc, of
+            except DomainError:
+                if first_order:
+                    raise
+                self.first_order_stages += 1
+        raise DomainError("unreachable")
+
+    def flow(self, h):
+        oc, of = self.Qc.copy(), self.Qf.copy()
+        c1, f1 = self.euler(self.Qc, self.Qf, h)
+        c2, f2 = self.euler(c1, f1, h)
+        self.Qc = 0.5 * c2 + 0.5 * oc
+        self.Qf = 0.5 * f2 + 0.5 * of
+        self._average(self.Qc, self.Qf)
+        self.c._validate(self.Qc)
+        self._validate_fine(self.Qf)
+
+    def react(self, h):
+        nv = self.nv
+        cells = np.concatenate([self.Qc.reshape(nv, -1)[:, self.active.reshape(-1)], self.Qf.reshape(nv, -1)], axis=1)
+        cells = np.ascontiguousarray(cells)
+        r, st, ff, msg = self.o.chem_advance(cells, np.full(cells.shape[1], h), rtol=self.rtol, atol=self.atol)
+        if r != 0:
+            raise DomainError(msg)
+        na = int(self.active.sum())
+        flat = self.Qc.reshape(nv, -1).copy()
+        flat[:, self.active.reshape(-1)] = cells[:, :na]
+        self.Qc = flat.reshape(self.Qc.shape)
+        self.Qf = cells[:, na:].reshape(self.Qf.shape)
+        self._average(self.Qc, self.Qf)
+        self.c._validate(self.Qc)
+        self._validate_fine(self.Qf)
+
+    def step(self):
+        dt = self.cfl / max(self.c._validate(self.Qc), self._validate_fine(self.Qf))
+        bc, bf = self.Qc.copy(), self.Qf.copy()
+        for _ in range(self.max_attempts):
+            try:
+                self.flow(0.5 * dt)
+                self.react(dt)
+                self.flow(0.5 * dt)
+                break
+            except DomainError:
+                self.Qc, self.Qf = bc.copy(), bf.copy()
+                self.retries += 1
+                dt *= 0.5
+        else:
+            raise RuntimeError("time advancement failed")
+        self.time += dt
+        self.steps += 1
+        return dt
