@@ -371,6 +371,54 @@ def tag_run(torch, hf, reps):
     return ms, int(t.sum().item()), (G.cpu().numpy(), t.cpu().numpy()), dx
 
 
+TWO_LEVEL_BOX = (64, 128, 319, 383)          # fine cells: coarse 32..159 x 64..191, across the strip's front
+
+
+def two_level_run(torch, hf, steps, warmup):
+    """Strang steps of the channel (1024 x 256, reflecting left, top and
+    bottom, transmissive right) with one fixed refined box TWO_LEVEL_BOX
+    through TwoLevelDriver, batched FP64 chemistry over the active cells.
+    Returns (wall of the timed steps, driver, (coarse, fine))."""
+    from hyperflow_amd.driver import TwoLevelDriver, channel_2d
+    mech = hf.Mechanism(hf.H2_LI, rtol=1e-8, atol=1e-12)
+    Q, dx = channel_2d(mech, **CHANNEL)
+    bc = (hf.HF_BC_REFLECTING, hf.HF_BC_TRANSMISSIVE, hf.HF_BC_REFLECTING, hf.HF_BC_REFLECTING)
+
+    def fresh():
+        d = TwoLevelDriver(mech, CHANNEL["nx"], CHANNEL["ny"], dx, dx, bc, TWO_LEVEL_BOX, cfl=0.5,
+                           chem_mode=hf.HF_CHEM_BATCHED)
+        d.set_state(Q)
+        return d
+    d = fresh()
+    for _ in range(warmup):
+        d.step()
+    d = fresh()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        d.step()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, d, d.state()
+
+
+def two_level_cpu(hf, target_s=12.0):
+    """The oracle's two-level step loop on the same case: whole steps until
+    about target_s has passed. Returns (steps, wall, (coarse, fine))."""
+    from hyperflow_amd.driver import channel_2d
+    from oracle import Oracle
+    from oracle.driver import REFLECTING, TRANSMISSIVE, OracleTwoLevelDriver
+    Q, dx = channel_2d(hf.Mechanism(hf.H2_LI, rtol=1e-8, atol=1e-12), **CHANNEL)
+    o = OracleTwoLevelDriver(Oracle(hf.H2_LI), CHANNEL["nx"], CHANNEL["ny"], dx, dx,
+                             (REFLECTING, TRANSMISSIVE, REFLECTING, REFLECTING), TWO_LEVEL_BOX, cfl=0.5,
+                             rtol=1e-8, atol=1e-12)
+    o.set_state(Q)
+    n, t0 = 0, time.perf_counter()
+    while n == 0 or time.perf_counter() - t0 < target_s:
+        o.step()
+        n += 1
+    return n, time.perf_counter() - t0, (o.Qc.copy(), o.Qf.copy())
+
+
 def composite_oracle(hf, Q, dx, h):
     """The oracle's first composite stage on the same data (the oracle's
     ghost fill, flux stage with face fluxes, interpolation, reflux and
@@ -481,6 +529,8 @@ def main():
                     help="coarse level width of the level-transfer leg (height half of it; 0 skips it)")
     ap.add_argument("--composite-stages", type=int, default=50,
                     help="two-level Euler stages of the channel with one refined box (0 skips it)")
+    ap.add_argument("--two-level-steps", type=int, default=50,
+                    help="Strang steps of the channel with one fixed refined box (0 skips it)")
     ap.add_argument("--copy-ref-mib", type=int, default=0,
                     help="also run a device-to-device copy of this many MiB (reference for HBM counter scaling)")
     args = ap.parse_args()
@@ -658,6 +708,23 @@ def main():
                             "Mcells_per_s": CHANNEL["nx"] * CHANNEL["ny"] / tag_ms / 1e3}}
         log(f"composite: {cw * 1e3:.3f} ms per two-level stage")
 
+    two = None
+    if args.two_level_steps:
+        log(f"two-level channel: 1024 x 256 + 256 x 256 fine box, {args.two_level_steps} Strang steps")
+        twall, tdrv, _ = two_level_run(torch, hf, args.two_level_steps, 1)
+        leaves = CHANNEL["nx"] * CHANNEL["ny"] - 128 * 128 + 256 * 256
+        two = {"workload": "the 2-D channel (1024 x 256 coarse cells, H2_Li, config 3's base level) with one fixed "
+                           "refined box of 256 x 256 fine cells at ratio 2 across the hot strip's front (coarse "
+                           "cells 32..159 x 64..191); two-level composite Heun stages (reflux, average-down, "
+                           "repair, validation), Strang chemistry (batched FP64 form, rtol 1e-8 atol 1e-12) over "
+                           "the 311,296 active cells, CFL 0.5 over both levels",
+               "steps": args.two_level_steps, "steps_per_s": args.two_level_steps / twall,
+               "ms_per_step": twall / args.two_level_steps * 1e3,
+               "Mleaf_updates_per_s": leaves * 4 * args.two_level_steps / twall / 1e6,
+               "simulated_time_s": tdrv.time, "first_order_stages": tdrv.first_order_stages,
+               "retries": tdrv.retries, "repairs": tdrv.repairs}
+        log(f"two-level channel: {two['steps_per_s']:.4g} steps/s")
+
     adj = None
     if args.adjoint_cells:
         log(f"adjoint: {args.adjoint_cells} cells, {ADJ_STEPS} steps of {ADJ_H:g} s, checkpoints every {ADJ_CKPT}")
@@ -735,6 +802,8 @@ def main():
         line["level_transfers"] = xfer
     if comp is not None:
         line["composite_2level"] = comp
+    if two is not None:
+        line["channel_2level"] = two
     if adj is not None:
         line["adjoint"] = adj
     if not args.no_cpu:
@@ -747,6 +816,16 @@ def main():
                 "sample": f"the first {nc} steps of the same run, the oracle's step loop, {cw:.1f} s"}
             line["wedge_2d"]["parity_vs_oracle"] = {
                 "steps": nc, **state_difference(gstate, cstate, hf.Mechanism(hf.H2_LI).ns)}
+        if two is not None:
+            log("two-level channel: CPU oracle step loop")
+            nc, cw, (oc, of) = two_level_cpu(hf)
+            _, _, (gc, gf) = two_level_run(torch, hf, nc, 0)
+            nsx = hf.Mechanism(hf.H2_LI).ns
+            line["channel_2level"]["cpu_baseline"] = {
+                "value": nc / cw, "unit": "Strang steps/s", "cores": len(os.sched_getaffinity(0)), "kind": "port",
+                "sample": f"the first {nc} steps of the same run, the oracle's two-level step loop, {cw:.1f} s"}
+            line["channel_2level"]["parity_vs_oracle"] = {
+                "steps": nc, "coarse": state_difference(gc, oc, nsx), "fine": state_difference(gf, of, nsx)}
         if comp is not None:
             log("composite: oracle's first stage")
             _, ch, Cg, Fg1, C0, Qc, cdx = composite_stages(torch, hf, 1, h=comp["h_s"])
