@@ -114,3 +114,38 @@ def test_no_gpu_means_device_error(mech_h2):
     with pytest.raises(hf.HyperflowError) as e:
         hf.ChemistryBatch(mech_h2).advance(U, dt)
     assert e.value.code == _lib.HF_EDEVICE
+
+
+def test_level_transfer_entry_points_reject_bad_arguments(mech_h2):
+    """The ghost-fill, interpolation, reflux and tagging entry points check
+    their arguments before any device work (HF_EINVAL, no kernel launched)."""
+    from ctypes import byref, c_double, c_int32, c_int64
+    L, h = _lib.load(), mech_h2.handle
+    box = _lib.hf_box(8, 4, 0, 0, 0, 12, 96, 0, 8, 32, 0.0, 0.0)
+    inlet = (c_double * mech_h2.nv)()
+    for bc, q in (((_lib.HF_BC_WEDGE, 0, 0, 0), inlet),        # wedge on x-lo
+                  ((0, 0, 0, _lib.HF_BC_WEDGE), inlet),        # wedge on y-hi
+                  ((0, 0, _lib.HF_BC_WEDGE, -1), inlet),       # wedge on a 1-D box
+                  ((_lib.HF_BC_INFLOW, 0, 0, 0), None),        # inflow without an inlet state
+                  ((0, 0, 7, 0), inlet)):                      # unknown code
+        assert L.hf_fill_ghosts_flow(h, None, byref(box), (c_int32 * 4)(*bc), q) == _lib.HF_EINVAL
+    # the plain fill refuses the flow codes
+    assert L.hf_fill_ghosts(h, None, byref(box), (c_int32 * 4)(_lib.HF_BC_INFLOW, 0, 0, 0)) == _lib.HF_EINVAL
+    p = _lib.hf_patch(0, 8, 32)
+    ff = c_int64(0)
+    assert L.hf_interp_coarse_to_fine(h, None, byref(p), None, byref(p), (c_int32 * 4)(4, 4, 3, 7), byref(ff),
+                                      byref(ff)) == _lib.HF_EINVAL           # empty region
+    assert L.hf_average_down(h, None, byref(p), None, byref(p), (c_int32 * 4)(0, 0, 1, 1)) == _lib.HF_EINVAL
+    fl = _lib.hf_face_fluxes()
+    for fb, sides, step in (((1, 0, 7, 7), 15, 1e-9),                 # not aligned to ratio 2
+                            ((0, 0, 7, 7), 16, 1e-9),                 # invalid side mask
+                            ((0, 0, 7, 7), 15, -1.0)):                # invalid step
+        assert L.hf_reflux(h, None, byref(p), byref(fl), byref(fl), (c_int32 * 4)(*fb), sides, step, 1e-3,
+                           1e-3) == _lib.HF_EINVAL
+    prm = _lib.hf_tag_params(0.08, 0.08, 50.0, 1, 1, mech_h2.ns, 0)     # fuel index out of range
+    prm.dx = prm.dy = 1e-3
+    from ctypes import c_uint8
+    tags = (c_uint8 * 32)()                      # reaches the fuel-range check, which comes first
+    assert L.hf_tag_cells(h, None, byref(box), byref(prm), tags, byref(ff), byref(ff)) == _lib.HF_EINVAL
+    assert "fuel" in L.hf_last_error(h).decode()
+    assert L.hf_flux_stage_fluxes(h, byref(box), 1, None, None, 0, None, None, None) == _lib.HF_EINVAL
