@@ -809,10 +809,9 @@ int hf_fill_ghosts(hf_ctx* c, double* U, const hf_box* box, const int32_t bc[4])
     return hf_fill_ghosts_flow(c, U, box, bc, nullptr);
 }
 
-int hf_fill_ghosts_flow(hf_ctx* c, double* U, const hf_box* box, const int32_t bc[4], const double* inlet) {
-    if (!c) return HF_EINVAL;
-    BusyGuard guard(c);
-    if (!guard.ok) return fail(c, HF_EBUSY, "concurrent call is unsupported");
+namespace {
+// Boundary codes of a ghost fill (hf_fill_ghosts_flow's rules).
+int check_bc(hf_ctx* c, const hf_box* box, const int32_t bc[4], const double* inlet) {
     if (!bc) return fail(c, HF_EINVAL, "missing boundary codes");
     const bool two_d = bc[2] >= 0 && bc[3] >= 0;
     for (int k = 0; k < 4; ++k) {
@@ -823,16 +822,128 @@ int hf_fill_ghosts_flow(hf_ctx* c, double* U, const hf_box* box, const int32_t b
     }
     if (bc[2] == HF_BC_WEDGE && box && box->ny < 2)
         return fail(c, HF_EINVAL, "a wedge bottom needs at least two rows");
-    BoxLaunch L{};
-    const int r = box_common(c, L, box, U);
-    if (r != HF_OK) return r;
+    return HF_OK;
+}
+
+void set_fill(BoxLaunch& L, double* U, const int32_t bc[4], const double* inlet) {
     L.U = U;
     L.op = BoxLaunch::FILL;
     for (int k = 0; k < 4; ++k) L.bc[k] = bc[k];
-    L.two_d = two_d;
+    L.two_d = bc[2] >= 0 && bc[3] >= 0;
     L.inlet = inlet;
+}
+}  // namespace
+
+int hf_fill_ghosts_flow(hf_ctx* c, double* U, const hf_box* box, const int32_t bc[4], const double* inlet) {
+    if (!c) return HF_EINVAL;
+    BusyGuard guard(c);
+    if (!guard.ok) return fail(c, HF_EBUSY, "concurrent call is unsupported");
+    const int rb = check_bc(c, box, bc, inlet);
+    if (rb != HF_OK) return rb;
+    BoxLaunch L{};
+    const int r = box_common(c, L, box, U);
+    if (r != HF_OK) return r;
+    set_fill(L, U, bc, inlet);
     if (c->ops->box_ops(L) != HF_OK) return fail(c, HF_EDEVICE, "ghost fill launch failed");
     HF_HIP(c, hipStreamSynchronize(c->stream));
+    return HF_OK;
+}
+
+int hf_euler_stage(hf_ctx* c, double* Uin, double* Uout, const hf_box* box, const int32_t bc[4], const double* inlet,
+                   double dx, double dy, int64_t* repairs, int32_t* first_order, int64_t* first_failed) {
+    if (!c) return HF_EINVAL;
+    BusyGuard guard(c);
+    if (!guard.ok) return fail(c, HF_EBUSY, "concurrent call is unsupported");
+    if (repairs) *repairs = 0;
+    if (first_order) *first_order = 0;
+    if (first_failed) *first_failed = -1;
+    const int rb = check_bc(c, box, bc, inlet);
+    if (rb != HF_OK) return rb;
+    if (!(dx > 0) || !(dy >= 0)) return fail(c, HF_EINVAL, "invalid cell size");
+    BoxLaunch L{};
+    const int r = box_common(c, L, box, Uin);
+    if (r != HF_OK) return r;
+    const hf_box& B = *box;
+    if (!Uout || !is_device_ptr(Uout)) return fail(c, HF_EINVAL, "the stage needs device memory");
+    if (B.out_pitch < B.nx || B.out_plane < B.out_pitch * B.ny || B.out_offset < 0 || !std::isfinite(B.dtdx) ||
+        !std::isfinite(B.dtdy))
+        return fail(c, HF_EINVAL, "invalid output extent or step");
+    const int ns = c->ops->ns;
+    // ghost fill of Uin
+    set_fill(L, Uin, bc, inlet);
+    if (c->ops->box_ops(L) != HF_OK) return fail(c, HF_EDEVICE, "ghost fill launch failed");
+    // flux tiles of the box (device copies shared by both attempts)
+    const bool one_d = B.ny == 1 && B.dtdy == 0;
+    const int TX = one_d ? c->ops->flux_tx1 : c->ops->flux_tx2, TY = one_d ? 1 : c->ops->flux_ty2;
+    std::vector<int32_t> tiles;
+    for (int tj = 0; tj < (B.ny + TY - 1) / TY; ++tj)
+        for (int ti = 0; ti < (B.nx + TX - 1) / TX; ++ti) {
+            tiles.push_back(0);
+            tiles.push_back(ti);
+            tiles.push_back(tj);
+        }
+    if (!c->boxes.ensure(sizeof(hf_box)) || !c->tiles.ensure(sizeof(int32_t) * tiles.size()))
+        return fail(c, HF_ENOMEM, "device scratch");
+    HF_HIP(c, hipMemcpyAsync(c->boxes.p, &B, sizeof(hf_box), hipMemcpyHostToDevice, c->stream));
+    HF_HIP(c, hipMemcpyAsync(c->tiles.p, tiles.data(), sizeof(int32_t) * tiles.size(), hipMemcpyHostToDevice,
+                             c->stream));
+    unsigned long long* scal = c->scal.as<unsigned long long>();
+    // status block read back once per attempt: [0] the failure index shared by
+    // the repair's material-negative test and the validation, [8 + 3 ns] the
+    // repair count, [8 + 3 ns + 1] the repair's donor failure
+    std::vector<unsigned long long> hv(8 + 3 * ns + 2);
+    for (int fo = 0; fo < 2; ++fo) {
+        // flux stage: its status words are not read here (the repair and the
+        // validation judge the raw update), so they need no reset
+        FluxLaunch F{};
+        F.mech = c->d_mech;
+        F.boxes = c->boxes.as<hf_box>();
+        F.nbox = 1;
+        F.tiles = c->tiles.as<int32_t>();
+        F.ntiles = (int)(tiles.size() / 3);
+        F.Uin = Uin;
+        F.Uout = Uout;
+        F.first_order = fo;
+        F.one_d = one_d ? 1 : 0;
+        F.first_failed = scal;
+        F.counters = scal + 1;
+        F.fluxes = nullptr;
+        F.stream = c->stream;
+        F.kernel_ms = nullptr;
+        const int rf = c->ops->flux(F);
+        if (rf != HF_OK) return fail(c, rf, "flux kernel launch failed");
+        // repair (its init kernel resets the failure index), then the validation
+        // on the same index; the validation's CFL rate lands in a slot the
+        // repair no longer needs
+        BoxLaunch R = L;
+        R.U = Uout;
+        R.op = BoxLaunch::REPAIR;
+        if (c->ops->box_ops(R) != HF_OK) return fail(c, HF_EDEVICE, "roundoff repair launch failed");
+        BoxLaunch V = L;
+        V.U = Uout;
+        V.op = BoxLaunch::PROPS;
+        V.dx = dx;
+        V.dy = dy;
+        if (c->ops->box_ops(V) != HF_OK) return fail(c, HF_EDEVICE, "validation launch failed");
+        HF_HIP(c, hipMemcpyAsync(hv.data(), scal, sizeof(unsigned long long) * hv.size(), hipMemcpyDeviceToHost,
+                                 c->stream));
+        HF_HIP(c, hipStreamSynchronize(c->stream));
+        const unsigned long long bad = hv[0];
+        const bool donor_failed = hv[8 + 3 * ns + 1] != 0;
+        if (bad == ~0ull && !donor_failed) {
+            if (repairs) *repairs = (int64_t)hv[8 + 3 * ns];
+            return HF_OK;
+        }
+        if (first_order) *first_order = 1;
+        if (fo == 1) {
+            const unsigned long long at = bad == ~0ull ? 0ull : bad;
+            if (first_failed) *first_failed = (int64_t)at;
+            return fail(c, HF_EDOMAIN, donor_failed ? std::string("insufficient positive donor for roundoff "
+                                                                  "redistribution after the first-order stage")
+                                                    : "inadmissible state after the first-order stage at cell " +
+                                                          std::to_string(at));
+        }
+    }
     return HF_OK;
 }
 

@@ -11,9 +11,9 @@ Mirrors the reference's step loop for a single level
                      on domain_error redo the stage with first-order faces
   react    :271-285  ChemistryBatch::advance over every valid cell
   stableDt :335-346  cfl / max((|u|+a)/dx + (|v|+a)/dy)
-Every arithmetic step runs in libhyperflow_hip.so (hf_fill_ghosts_flow,
-hf_flux_stage, hf_repair_roundoff, hf_validate_box, hf_lincomb,
-hf_chem_advance); torch only holds the device arrays.
+Every arithmetic step runs in libhyperflow_hip.so (hf_euler_stage: ghost
+fill, flux stage, repair and validation in one call; hf_validate_box,
+hf_lincomb, hf_chem_advance); torch only holds the device arrays.
 
 A box with ny = 1 and dy = 0 is one-dimensional: x faces only and an x-only
 CFL rate (the reference itself needs ny >= 2 with reflecting y walls).
@@ -113,31 +113,16 @@ class BoxDriver:
         check(self._h, r, ff.value)
         return rate.value
 
-    def _stage_once(self, src, dst, h, first_order):
-        box = self._box(h)
-        st, ff = hf_flux_stats(), c_int64(-1)
-        r = load().hf_flux_stage(self._h, byref(box), 1, src.data_ptr(), dst.data_ptr(), int(first_order), byref(st),
-                                 byref(ff))
-        # HF_EDOMAIN here only says an updated cell is inadmissible as written:
-        # the reference repairs roundoff before validating, so continue.
-        if r not in (HF_OK, HF_EDOMAIN):
-            check(self._h, r, ff.value)
-        rep, ff2 = c_int64(0), c_int64(-1)
-        r = load().hf_repair_roundoff(self._h, dst.data_ptr(), byref(box), byref(rep), byref(ff2))
-        if r == HF_OK:
-            self.repairs += rep.value
-            r = load().hf_validate_box(self._h, dst.data_ptr(), byref(box), self.dx, 0.0 if self.one_d else self.dy,
-                                       None, byref(ff2))
-        return r, ff2.value
-
     def euler(self, src, dst, h):
-        """One Euler stage src -> dst (interior), with the first-order redo."""
-        self._fill(src)
-        r, ff = self._stage_once(src, dst, h, False)
-        if r == HF_EDOMAIN:
-            self.first_order_stages += 1
-            r, ff = self._stage_once(src, dst, h, True)
-        check(self._h, r, ff)
+        """One Euler stage src -> dst (interior), with the first-order redo:
+        ghost fill, flux stage, repair and validation in one library call."""
+        box = self._box(h)
+        rep, fo, ff = c_int64(0), c_int32(0), c_int64(-1)
+        r = load().hf_euler_stage(self._h, src.data_ptr(), dst.data_ptr(), byref(box), self.bc, self.inlet, self.dx,
+                                  0.0 if self.one_d else self.dy, byref(rep), byref(fo), byref(ff))
+        self.repairs += rep.value
+        self.first_order_stages += fo.value
+        check(self._h, r, ff.value)
 
     def flow(self, h):
         self.old.copy_(self.U)

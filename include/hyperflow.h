@@ -308,6 +308,20 @@ typedef struct hf_tag_params {
 int hf_tag_cells(hf_ctx *ctx, const double *U, const hf_box *box, const hf_tag_params *prm, uint8_t *tags,
                  int64_t *ntagged, int64_t *first_failed);
 
+/* One Euler stage of BlockAMR::euler (BlockAMR.cpp:202-265) on one
+ * single-level ghosted box in a single call: fill Uin's ghosts
+ * (hf_fill_ghosts_flow), the flux stage into Uout's valid cells (both arrays
+ * in the box's input layout; out_* must address Uout's valid cells), roundoff
+ * repair and validation of Uout; when the repair or the validation rejects
+ * the stage, the same with first-order face states (BlockAMR.cpp:257-263).
+ * One host synchronisation per attempt. repairs (optional) receives the
+ * values the accepted attempt zeroed, first_order (optional) 1 when the
+ * first-order redo ran. HF_EDOMAIN with first_failed = the lowest cell
+ * (j*nx + i) that the repair or the validation rejected, when the
+ * first-order stage is rejected too. Device pointers only. */
+int hf_euler_stage(hf_ctx *ctx, double *Uin, double *Uout, const hf_box *box, const int32_t bc[4], const double *inlet,
+                   double dx, double dy, int64_t *repairs, int32_t *first_order, int64_t *first_failed);
+
 /* BlockAMR::repairRoundoff (BlockAMR.cpp:299-334) on the valid cells of one
  * box of equal cell volumes: negative species within 64 eps of the species'
  * peak are zeroed and their total is taken from the first cell holding the

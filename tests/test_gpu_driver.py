@@ -110,3 +110,48 @@ def test_box_2d_matches_oracle(mech_h2, oracle_h2):
         dtg, dto = g.step(), o.step()
         assert dtg == pytest.approx(dto, rel=1e-12)
     _compare(g.state(), o.Q, 1e-8, mech_h2.ns)
+
+
+@pytest.mark.parametrize("one_d", [True, False])
+def test_fused_euler_stage_equals_separate_calls(mech_h2, one_d):
+    """hf_euler_stage (fill, flux stage, repair, validation in one call) gives
+    the same bits as the four calls made one by one."""
+    from ctypes import byref, c_int32, c_int64
+    from hyperflow_amd._lib import load
+    print(f"[gpu] fused Euler stage, one_d={one_d}", flush=True)
+    if one_d:
+        Q, dx = shock_tube_1d(mech_h2, nx=300, length=0.015)
+        nx, ny, dy, bc = 300, 1, 0.0, (HF_BC_REFLECTING, HF_BC_TRANSMISSIVE, -1, -1)
+    else:
+        Q1, dx = shock_tube_1d(mech_h2, nx=4, length=0.04)
+        cold, hot = Q1[:, 0, 3], Q1[:, 0, 0]
+        nx, ny, dy = 30, 20, dx
+        Q = np.repeat(np.repeat(cold[:, None, None], ny, 1), nx, 2)
+        Q[:, 5:12, 8:19] = hot[:, None, None]
+        bc = (HF_BC_TRANSMISSIVE, HF_BC_REFLECTING, HF_BC_REFLECTING, HF_BC_PERIODIC)
+    d = BoxDriver(mech_h2, nx, ny, dx, dy, bc)
+    d.set_state(Q)
+    h = 0.4 * d.stable_dt() / d.cfl
+    A = d.U.clone()
+    d.euler(d.U, d.U1, h)                                  # fused
+    fused = d.U1.clone()
+    # the same stage call by call
+    box = d._box(h)
+    L, hdl = load(), d._h
+    assert L.hf_fill_ghosts(hdl, A.data_ptr(), byref(box), d.bc) == 0
+    out = torch.zeros_like(A)
+    from hyperflow_amd._lib import hf_flux_stats
+    st, ff = hf_flux_stats(), c_int64(-1)
+    L.hf_flux_stage(hdl, byref(box), 1, A.data_ptr(), out.data_ptr(), 0, byref(st), byref(ff))
+    rep = c_int64(0)
+    assert L.hf_repair_roundoff(hdl, out.data_ptr(), byref(box), byref(rep), byref(ff)) == 0
+    assert L.hf_validate_box(hdl, out.data_ptr(), byref(box), dx, dy, None, byref(ff)) == 0
+    assert torch.equal(fused[:, 2:ny + 2, 2:nx + 2], out[:, 2:ny + 2, 2:nx + 2])
+    # a materially negative species: both attempts are rejected, the cell is named
+    B = d.U.clone()
+    B[1, 2 + (ny // 2 if not one_d else 0), 2 + 7] = -1e-3 * float(B[:mech_h2.ns, 2, 2].sum())
+    fo = c_int32(0)
+    r = L.hf_euler_stage(hdl, B.data_ptr(), d.U2.data_ptr(), byref(box), d.bc, None, dx, dy, byref(rep), byref(fo),
+                         byref(ff))
+    assert r == hf._lib.HF_EDOMAIN and fo.value == 1
+    assert ff.value == (ny // 2 if not one_d else 0) * nx + 7
