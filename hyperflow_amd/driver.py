@@ -73,6 +73,7 @@ class BoxDriver:
             self.inlet = (c_double * nv)(*vals)
         self.walls = walls
         self.chem = ChemistryBatch(mechanism, mode=chem_mode)
+        self._rate = None                  # CFL rate of the current state, when known
         self.time = 0.0
         self.steps = 0
         self.retries = 0
@@ -90,6 +91,7 @@ class BoxDriver:
         t = self.torch
         Qt = Q if isinstance(Q, t.Tensor) else t.from_numpy(Q)
         self.interior().copy_(Qt.reshape(self.interior().shape))
+        self._rate = None
 
     def state(self):
         return self.interior().cpu().numpy().copy()
@@ -131,9 +133,10 @@ class BoxDriver:
         n = self.U.numel()
         check(self._h, load().hf_lincomb(self._h, self.U2.data_ptr(), self.old.data_ptr(), 0.5, 0.5, n))
         self.U, self.U2 = self.U2, self.U
-        self._validate(self.U)
+        self._rate = self._validate(self.U)
 
     def react(self, h):
+        self._rate = None
         if self.one_d:
             view = self.U[:, 2, 2:self.nx + 2]          # (nv, nx), row-contiguous, ld = plane
             self.chem.advance(view, h)
@@ -143,7 +146,10 @@ class BoxDriver:
             self.interior().copy_(cells.reshape(self.interior().shape))
 
     def stable_dt(self):
-        return self.cfl / self._validate(self.U)
+        """cfl / max rate (BlockAMR::stableDt); the rate of the state a step
+        ended on is the one its final validation computed."""
+        rate = self._rate if self._rate is not None else self._validate(self.U)
+        return self.cfl / rate
 
     def step(self, target=None):
         """One Strang step; returns the dt taken."""
@@ -159,6 +165,7 @@ class BoxDriver:
                 break
             except HyperflowError:
                 self.U.copy_(self.backup)
+                self._rate = None
                 self.retries += 1
                 dt *= 0.5
         else:
