@@ -384,3 +384,40 @@ def test_two_level_driver_matches_oracle(mech_h2, oracle_h2):
           f"retries {g.retries}/{o.retries}", flush=True)
     _compare(Cg, o.Qc, 1e-8, mech_h2.ns)
     _compare(Fg, o.Qf, 1e-8, mech_h2.ns)
+
+
+def test_snapshot_files_from_drivers(mech_h2, oracle_h2, tmp_path):
+    """state/boxes/history/summary files from a single-level and a two-level
+    run: one row per active cell, p and T equal to the oracle's properties
+    (1e-12), history integrals equal to the state's."""
+    import csv
+    from hyperflow_amd import output as O
+    from hyperflow_amd.driver import TwoLevelDriver
+    from test_oracle_amr import two_level_case
+    print("[gpu] snapshot files from BoxDriver and TwoLevelDriver", flush=True)
+    Q, dx, box = two_level_case(oracle_h2)
+    ny, nx = Q.shape[1:]
+    for two in (False, True):
+        out = tmp_path / ("two" if two else "one")
+        d = (TwoLevelDriver(mech_h2, nx, ny, dx, dx, (T, R_, R_, P), box) if two
+             else BoxDriver(mech_h2, nx, ny, dx, dx, (T, R_, R_, P)))
+        d.set_state(Q)
+        h = O.History(out)
+        dt = d.step()
+        levels = O.snapshot_levels(d)
+        O.write_snapshot(out, 1, d.time, levels, mech_h2.species)
+        h.log(1, d.time, dt, levels, mech_h2.ns, retries=d.retries)
+        h.close()
+        rows = list(csv.DictReader(open(out / "state_1.csv")))
+        n_active = nx * ny - (16 * 12 if two else 0) + (32 * 24 if two else 0)
+        assert len(rows) == n_active
+        assert len(list(csv.DictReader(open(out / "boxes_1.csv")))) == (2 if two else 1)
+        for L in levels:
+            flat = L.Q.reshape(L.Q.shape[0], -1).T.copy()
+            p, _, Tt, st = oracle_h2.properties(flat)
+            assert (st == 0).all()
+            assert np.allclose(L.p.reshape(-1), p, rtol=1e-12, atol=0)
+            assert np.allclose(L.T.reshape(-1), Tt, rtol=1e-12, atol=0)
+        hist = list(csv.DictReader(open(out / "history.csv")))[0]
+        mass = sum(float(r["rho"]) * float(r["dx"]) * float(r["dy"]) for r in rows)
+        assert float(hist["mass"]) == pytest.approx(mass, rel=1e-13)
