@@ -344,13 +344,23 @@ class TwoLevelDriver:
         plane = gx * gy
         return hf_box(self.nxf, self.nyf, 0, 0, 0, gx, plane, 2 * gx + 2, gx, plane, h / self.dxf, h / self.dyf)
 
+    @staticmethod
+    def _interior_patch(nx, ny, lo=(0, 0)):
+        """hf_patch of a ghosted (nv, ny+4, nx+4) array's valid cells, cell
+        (lo) at its first valid element."""
+        from ._lib import hf_patch
+        gx, gy = nx + 4, ny + 4
+        return hf_patch(2 * gx + 2 - (int(lo[1]) * gx + int(lo[0])), gx, gx * gy)
+
     def _average(self, Ac, Af):
-        from .amr import average_down
+        """Average-down of the fine box into the coarse cells it covers, in place
+        on the ghosted arrays."""
         FI0, FJ0, FI1, FJ1 = self.fine_box
-        fine = self.fine_interior(Af).contiguous()
-        Cv = self.coarse.interior(Ac).contiguous()
-        average_down(self.mech, fine, (FI0, FJ0), Cv, (0, 0), (FI0 // 2, FJ0 // 2, FI1 // 2, FJ1 // 2))
-        self.coarse.interior(Ac).copy_(Cv)
+        c = self.coarse
+        fp = self._interior_patch(self.nxf, self.nyf, (FI0, FJ0))
+        cp = self._interior_patch(c.nx, c.ny)
+        reg = (c_int32 * 4)(FI0 // 2, FJ0 // 2, FI1 // 2, FJ1 // 2)
+        check(self._h, load().hf_average_down(self._h, Af.data_ptr(), byref(fp), Ac.data_ptr(), byref(cp), reg))
 
     def _validate_fine(self, A):
         box = self._fbox()
@@ -360,29 +370,26 @@ class TwoLevelDriver:
         return rate.value
 
     def _composite_once(self, src_c, src_f, dst_c, dst_f, h, first_order):
-        from .amr import reflux
-        from .physics import box_for_arrays
+        from .amr import face_fluxes
         c = self.coarse
         FI0, FJ0, _, _ = self.fine_box
         c._fill(src_c)
         for r in self.strips:
             self.interp.interp(src_c, (-2, -2), src_f, (FI0 - 2, FJ0 - 2), r)
-        out_c = c.interior(dst_c)
-        oc = self.torch.empty((self.mech.nv, c.ny, c.nx), dtype=self.torch.float64, device="cuda")
-        of = self.torch.empty((self.mech.nv, self.nyf, self.nxf), dtype=self.torch.float64, device="cuda")
-        for (src, out, dx, dy, walls, wi, fl) in ((src_c, oc, c.dx, c.dy, c.walls, c.wedge_i, (self.cfx, self.cfy)),
-                                                  (src_f, of, self.dxf, self.dyf, 0, 0, (self.ffx, self.ffy))):
-            box = box_for_arrays(src, out, h / dx, h / dy, walls, wi)
+        # both stages write straight into the ghosted outputs' valid cells
+        for (src, dst, box, fl) in ((src_c, dst_c, c._box(h), (self.cfx, self.cfy)),
+                                    (src_f, dst_f, self._fbox(h), (self.ffx, self.ffy))):
             try:
-                self.sweep.stage([box], src, out, first_order=first_order, fluxes=[fl])
+                self.sweep.stage([box], src, dst, first_order=first_order, fluxes=[fl])
             except HyperflowError as e:
                 # an inadmissible raw update is repaired or rejected below, as in BoxDriver
                 if e.code != HF_EDOMAIN:
                     raise
-        reflux(self.mech, oc, (0, 0), (self.cfx, self.cfy, (0, 0)), (self.ffx, self.ffy, (FI0, FJ0)),
-               self.fine_box, 15, h, c.dx, c.dy)
-        out_c.copy_(oc)
-        self.fine_interior(dst_f).copy_(of)
+        cf, ff_ = face_fluxes(self.cfx, self.cfy), face_fluxes(self.ffx, self.ffy, (FI0, FJ0))
+        cp = self._interior_patch(c.nx, c.ny)
+        fb = (c_int32 * 4)(*self.fine_box)
+        check(self._h, load().hf_reflux(self._h, dst_c.data_ptr(), byref(cp), byref(cf), byref(ff_), fb, 15, h, c.dx,
+                                        c.dy))
         self._average(dst_c, dst_f)
         rep, ff = c_int64(0), c_int64(-1)
         repaired = 0
