@@ -31,7 +31,9 @@ slope, every parent's four children average back to it), by a closed form of
 the theta search on species-gradient fields (theta = the largest power of two
 keeping every child's partial densities non-negative), and by the known
 failure (an inadmissible parent is an error). average_down is pinned by
-exact integer-valued sums and by conservation. reflux is pinned by
+exact integer-valued sums and by conservation (its summation order follows
+AMReX's loop; AMReX is not vendored, so against AMReX's own rounding it and
+reflux are parity unpinned). reflux is pinned by
 composite conservation: one Euler stage on a periodic coarse level and an
 interior fine box (same h), reflux, average-down keeps every conserved total
 to roundoff, which it does not without the correction, and a flipped sign or
