@@ -713,7 +713,7 @@ def main():
         log(f"two-level channel: 1024 x 256 + 256 x 256 fine box, {args.two_level_steps} Strang steps")
         twall, tdrv, _ = two_level_run(torch, hf, args.two_level_steps, 1)
         leaves = CHANNEL["nx"] * CHANNEL["ny"] - 128 * 128 + 256 * 256
-        two = {"workload": "the 2-D channel (1024 x 256 coarse cells, H2_Li, config 3's base level) with one fixed "
+        two = {"workload": "the 2-D channel (1024 x 256 coarse cells over 0.2 x 0.05 m, H2_Li) with one fixed "
                            "refined box of 256 x 256 fine cells at ratio 2 across the hot strip's front (coarse "
                            "cells 32..159 x 64..191); two-level composite Heun stages (reflux, average-down, "
                            "repair, validation), Strang chemistry (batched FP64 form, rtol 1e-8 atol 1e-12) over "
