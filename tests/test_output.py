@@ -82,3 +82,23 @@ def test_summary_keys(tmp_path):
                           "final_conserved_integrals", "time_subcycling", "refinement_ratio"] and data == r
     with pytest.raises(ValueError):
         O.write_summary(tmp_path, not_a_key=1)
+
+
+def test_wedge_case_setup_matches_the_reference_case_file(mech_h2, oracle_h2):
+    """teng_osw_base against HyperFlow_V3/cases/teng_case1's stored values:
+    u0 = 2588.5462044182495, v0 = -891.3079364573576 (the reference computes
+    them from 2737.7 m/s at 19 degrees, BlockAMR.cpp:61-66), the wedge tip at
+    x = 0 on the 144-cell base grid from x = -0.128 mm (column 8), inflow left
+    and top, wedge bottom, outflow right (the case's boundary_lo / _hi)."""
+    import hyperflow_amd as hf
+    from hyperflow_amd.driver import teng_osw_base
+    w = teng_osw_base(mech_h2)
+    ns = mech_h2.ns
+    q = w["inlet"]
+    rho = q[:ns].sum()
+    assert q[ns] / rho == pytest.approx(2588.5462044182495, rel=1e-15)
+    assert q[ns + 1] / rho == pytest.approx(-891.3079364573576, rel=1e-15)
+    assert w["wedge_i"] == 8 and w["dx"] == pytest.approx(1.6e-5, rel=1e-14)
+    assert w["bc"] == (hf.HF_BC_INFLOW, hf.HF_BC_TRANSMISSIVE, hf.HF_BC_WEDGE, hf.HF_BC_INFLOW)
+    p, _, T, st = oracle_h2.properties(q.reshape(1, -1).copy())
+    assert st[0] == 0 and T[0] == pytest.approx(958.1, rel=1e-10) and p[0] == pytest.approx(1202900.0, rel=1e-10)
