@@ -186,3 +186,27 @@ def tag_cells(mechanism, U, dx, dy, x0=0.0, y0=0.0, time=0.0, pressure_tag=0.08,
     r = load().hf_tag_cells(m.handle, U.data_ptr(), byref(b), byref(P), tags.data_ptr(), byref(n), byref(ff))
     check(m.handle, r, ff.value)
     return tags
+
+
+def choose_fine_box(tags, buffer, blocking, nx, ny, margin=2):
+    """One refined box for the tags of a coarse level (ny, nx bool): the
+    tags grown by ``buffer`` cells (AMReX's n_error_buf, HyperFlow_V3/src/
+    main.cpp:35), their bounding box widened to multiples of ``blocking``
+    coarse cells (blocking_factor) and kept ``margin`` cells inside the
+    level. Returns fine cells (ilo, jlo, ihi, jhi) at ratio 2, or None when
+    nothing is tagged. AMReX clusters the tags into several boxes; a single
+    bounding box is the one-box driver's form of it."""
+    import numpy as np
+    t = np.asarray(tags, dtype=bool)
+    if not t.any():
+        return None
+    js, is_ = np.nonzero(t)
+    i0, i1 = int(is_.min()) - buffer, int(is_.max()) + buffer
+    j0, j1 = int(js.min()) - buffer, int(js.max()) + buffer
+    i0, j0 = (i0 // blocking) * blocking, (j0 // blocking) * blocking
+    i1, j1 = (i1 // blocking + 1) * blocking - 1, (j1 // blocking + 1) * blocking - 1
+    i0, j0 = max(i0, margin), max(j0, margin)
+    i1, j1 = min(i1, nx - 1 - margin), min(j1, ny - 1 - margin)
+    if i1 < i0 or j1 < j0:
+        return None
+    return (2 * i0, 2 * j0, 2 * i1 + 1, 2 * j1 + 1)

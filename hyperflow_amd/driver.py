@@ -22,6 +22,8 @@ from __future__ import annotations
 
 from ctypes import byref, c_double, c_int32, c_int64
 
+import numpy as np
+
 from ._lib import (HF_BC_INFLOW, HF_BC_PERIODIC, HF_BC_REFLECTING, HF_BC_TRANSMISSIVE, HF_BC_WEDGE,  # noqa: F401
                    HF_CHEM_BATCHED, HF_EDOMAIN, HF_OK,
                    HyperflowError, check, hf_box, hf_flux_stats, load)
@@ -297,30 +299,40 @@ class TwoLevelDriver:
             raise ValueError("the fine box must be aligned to the refinement ratio 2")
         if FI0 // 2 < 2 or FJ0 // 2 < 2 or FI1 // 2 > nx - 3 or FJ1 // 2 > ny - 3:
             raise ValueError("the fine box must lie at least two coarse cells inside the level")
-        self.fine_box = (FI0, FJ0, FI1, FJ1)
-        self.nxf, self.nyf = FI1 - FI0 + 1, FJ1 - FJ0 + 1
         self.dxf, self.dyf = 0.5 * float(dx), 0.5 * float(dy)
         nv = mechanism.nv
         dev = dict(dtype=torch.float64, device="cuda")
-        shape = (nv, self.nyf + 4, self.nxf + 4)
-        self.Uf, self.Uf1, self.Uf2 = (torch.zeros(shape, **dev) for _ in range(3))
-        self.oldf, self.backupf = torch.zeros(shape, **dev), torch.zeros(shape, **dev)
         self.cfx = torch.zeros((nv, ny, nx + 1), **dev)
         self.cfy = torch.zeros((nv, ny + 1, nx), **dev)
-        self.ffx = torch.zeros((nv, self.nyf, self.nxf + 1), **dev)
-        self.ffy = torch.zeros((nv, self.nyf + 1, self.nxf), **dev)
         self.interp = ConservativeInterp(mechanism)
         self.sweep = FluxSweep(mechanism)
-        self.strips = ((FI0 - 2, FJ0 - 2, FI1 + 2, FJ0 - 1), (FI0 - 2, FJ1 + 1, FI1 + 2, FJ1 + 2),
-                       (FI0 - 2, FJ0, FI0 - 1, FJ1), (FI1 + 1, FJ0, FI1 + 2, FJ1))
-        self.covered = (slice(FJ0 // 2, FJ1 // 2 + 1), slice(FI0 // 2, FI1 // 2 + 1))
-        mask = torch.ones((ny, nx), dtype=torch.bool, device="cuda")
-        mask[self.covered] = False
-        self.active = mask.reshape(-1)
+        self._set_box((FI0, FJ0, FI1, FJ1))
+        self.regrids = 0
         self.cfl = float(cfl)
         self.max_attempts = max_attempts
         self.time, self.steps, self.retries, self.first_order_stages, self.repairs = 0.0, 0, 0, 0, 0
         self._h = mechanism.handle
+
+    def _set_box(self, box):
+        """Fine-level arrays, ghost strips and the active mask for box."""
+        torch = self.torch
+        FI0, FJ0, FI1, FJ1 = box
+        c = self.coarse
+        self.fine_box = tuple(box)
+        self.nxf, self.nyf = FI1 - FI0 + 1, FJ1 - FJ0 + 1
+        nv = self.mech.nv
+        dev = dict(dtype=torch.float64, device="cuda")
+        shape = (nv, self.nyf + 4, self.nxf + 4)
+        self.Uf, self.Uf1, self.Uf2 = (torch.zeros(shape, **dev) for _ in range(3))
+        self.oldf, self.backupf = torch.zeros(shape, **dev), torch.zeros(shape, **dev)
+        self.ffx = torch.zeros((nv, self.nyf, self.nxf + 1), **dev)
+        self.ffy = torch.zeros((nv, self.nyf + 1, self.nxf), **dev)
+        self.strips = ((FI0 - 2, FJ0 - 2, FI1 + 2, FJ0 - 1), (FI0 - 2, FJ1 + 1, FI1 + 2, FJ1 + 2),
+                       (FI0 - 2, FJ0, FI0 - 1, FJ1), (FI1 + 1, FJ0, FI1 + 2, FJ1))
+        self.covered = (slice(FJ0 // 2, FJ1 // 2 + 1), slice(FI0 // 2, FI1 // 2 + 1))
+        mask = torch.ones((c.ny, c.nx), dtype=torch.bool, device="cuda")
+        mask[self.covered] = False
+        self.active = mask.reshape(-1)
 
     # ------------------------------------------------------------ state
     def set_state(self, Q):
@@ -446,6 +458,54 @@ class TwoLevelDriver:
 
     def stable_dt(self):
         return self.cfl / max(self.coarse._validate(self.coarse.U), self._validate_fine(self.Uf))
+
+    def _levels_totals(self):
+        from .output import Level, totals
+        c = self.coarse
+        Qc = c.interior().cpu().numpy()
+        Qf = self.fine_interior().cpu().numpy()
+        act = self.active.reshape(c.ny, c.nx).cpu().numpy()
+        z = lambda Q: np.zeros(Q.shape[1:])
+        return totals([Level(0, Qc, z(Qc), z(Qc), (0, 0), c.dx, c.dy, (0, 0), act),
+                       Level(1, Qf, z(Qf), z(Qf), self.fine_box[:2], self.dxf, self.dyf, (0, 0))]), \
+            totals([Level(0, Qc, z(Qc), z(Qc), (0, 0), c.dx, c.dy, (0, 0), act),
+                    Level(1, Qf, z(Qf), z(Qf), self.fine_box[:2], self.dxf, self.dyf, (0, 0))], absolute=True)
+
+    def regrid(self, buffer=2, blocking=4, **tag_kw):
+        """Move the refined box to the tagged region (BlockAMR::regridChecked,
+        BlockAMR.cpp:367-373, with ErrorEst's criteria, :170-200): tags of the
+        coarse level, one box around them (choose_fine_box), the new fine data
+        by conservative interpolation with the old fine cells kept where the
+        boxes overlap, average-down, and the remap defect check (relative
+        change of any conserved integral above 1e-8 raises). Returns the new
+        box (unchanged when nothing is tagged) and the defect."""
+        from .amr import choose_fine_box, tag_cells
+        c = self.coarse
+        self._average(c.U, self.Uf)
+        before, scale = self._levels_totals()
+        c._fill(c.U)
+        tags = tag_cells(self.mech, c.U, c.dx, c.dy, time=self.time, **tag_kw).cpu().numpy().astype(bool)
+        box = choose_fine_box(tags, buffer, blocking, c.nx, c.ny)
+        if box is None or box == self.fine_box:
+            return self.fine_box, 0.0
+        old_box, old_f = self.fine_box, self.fine_interior().clone()
+        self._set_box(box)
+        self.interp.interp(c.U, (-2, -2), self.Uf, (box[0] - 2, box[1] - 2), box)
+        # keep the old fine cells where the boxes overlap
+        a0, b0 = max(box[0], old_box[0]), max(box[1], old_box[1])
+        a1, b1 = min(box[2], old_box[2]), min(box[3], old_box[3])
+        if a0 <= a1 and b0 <= b1:
+            self.Uf[:, 2 + b0 - box[1]:3 + b1 - box[1], 2 + a0 - box[0]:3 + a1 - box[0]] = \
+                old_f[:, b0 - old_box[1]:b1 - old_box[1] + 1, a0 - old_box[0]:a1 - old_box[0] + 1]
+        self._average(c.U, self.Uf)
+        after, _ = self._levels_totals()
+        defect = float(np.max(np.abs(after - before) / np.maximum(scale, 1e-300)))
+        if defect > 1e-8:
+            raise RuntimeError("nonconservative remap")
+        c._validate(c.U)
+        self._validate_fine(self.Uf)
+        self.regrids += 1
+        return self.fine_box, defect
 
     def step(self, target=None):
         """One Strang step of both levels; returns the dt taken."""

@@ -323,3 +323,64 @@ class OracleTwoLevelDriver:
         self.time += dt
         self.steps += 1
         return dt
+
+
+def fine_box_for_tags(tags, buffer, blocking, nx, ny, margin=2):
+    """One refined box around the tags (oracle restatement of the one-box
+    regrid): tags grown by buffer cells (n_error_buf, main.cpp:35), the
+    bounding box widened to whole blocks of `blocking` coarse cells, kept
+    `margin` cells inside the level; fine cells at ratio 2, or None."""
+    t = np.asarray(tags, dtype=bool)
+    if not t.any():
+        return None
+    rows = np.flatnonzero(t.any(axis=1))
+    cols = np.flatnonzero(t.any(axis=0))
+    lo_i = (cols[0] - buffer) - (cols[0] - buffer) % blocking
+    lo_j = (rows[0] - buffer) - (rows[0] - buffer) % blocking
+    hi_i = (cols[-1] + buffer) - (cols[-1] + buffer) % blocking + blocking - 1
+    hi_j = (rows[-1] + buffer) - (rows[-1] + buffer) % blocking + blocking - 1
+    lo_i, lo_j = max(int(lo_i), margin), max(int(lo_j), margin)
+    hi_i, hi_j = min(int(hi_i), nx - 1 - margin), min(int(hi_j), ny - 1 - margin)
+    if hi_i < lo_i or hi_j < lo_j:
+        return None
+    return (2 * lo_i, 2 * lo_j, 2 * hi_i + 1, 2 * hi_j + 1)
+
+
+def _regrid(self, buffer=2, blocking=4, **tag_kw):
+    """OracleTwoLevelDriver.regrid: tags, one box, remap by interpolation with
+    the old fine cells kept on the overlap, average-down, defect check
+    (BlockAMR::regridChecked, BlockAMR.cpp:367-373)."""
+    from .amr import interp, tag_cells
+    self._average(self.Qc, self.Qf)
+
+    def tot(absolute=False):
+        f = np.abs if absolute else (lambda a: a)
+        a = (f(self.Qc)[:, self.active]).sum(axis=1) * self.c.dx * self.c.dy
+        return a + f(self.Qf).sum(axis=(1, 2)) * self.dxf * self.dyf
+    before, scale = tot(), tot(True)
+    G = self.c._ghosted(self.Qc)
+    tags = tag_cells(self.o, G, self.c.dx, self.c.dy, time=self.time, **tag_kw)
+    box = fine_box_for_tags(tags, buffer, blocking, self.c.nx, self.c.ny)
+    if box is None or tuple(box) == self.fine_box:
+        return self.fine_box, 0.0
+    old_box, old_f = self.fine_box, self.Qf.copy()
+    FI0, FJ0, FI1, FJ1 = box
+    self.fine_box = tuple(box)
+    self.nxf, self.nyf = FI1 - FI0 + 1, FJ1 - FJ0 + 1
+    self.cov = (slice(FJ0 // 2, FJ1 // 2 + 1), slice(FI0 // 2, FI1 // 2 + 1))
+    self.active = np.ones((self.c.ny, self.c.nx), dtype=bool)
+    self.active[self.cov] = False
+    self.Qf, _, _ = interp(self.o, G, (-2, -2), box)
+    a0, b0 = max(box[0], old_box[0]), max(box[1], old_box[1])
+    a1, b1 = min(box[2], old_box[2]), min(box[3], old_box[3])
+    if a0 <= a1 and b0 <= b1:
+        self.Qf[:, b0 - FJ0:b1 - FJ0 + 1, a0 - FI0:a1 - FI0 + 1] = \
+            old_f[:, b0 - old_box[1]:b1 - old_box[1] + 1, a0 - old_box[0]:a1 - old_box[0] + 1]
+    self._average(self.Qc, self.Qf)
+    defect = float(np.max(np.abs(tot() - before) / np.maximum(scale, 1e-300)))
+    if defect > 1e-8:
+        raise RuntimeError("nonconservative remap")
+    return self.fine_box, defect
+
+
+OracleTwoLevelDriver.regrid = _regrid

@@ -421,3 +421,29 @@ def test_snapshot_files_from_drivers(mech_h2, oracle_h2, tmp_path):
         hist = list(csv.DictReader(open(out / "history.csv")))[0]
         mass = sum(float(r["rho"]) * float(r["dx"]) * float(r["dy"]) for r in rows)
         assert float(hist["mass"]) == pytest.approx(mass, rel=1e-13)
+
+
+def test_two_level_regrid_matches_oracle(mech_h2, oracle_h2):
+    """Step, regrid (tags -> one box -> remap -> defect check), step: the
+    same new box as the oracle's, remap defect at roundoff, states to 1e-8."""
+    from hyperflow_amd.driver import TwoLevelDriver
+    from oracle.driver import OracleTwoLevelDriver
+    from test_oracle_amr import two_level_case
+    print("[gpu] two-level regrid: 40 x 24 coarse, step / regrid / step", flush=True)
+    Q, dx, box = two_level_case(oracle_h2)
+    bc = (T, R_, R_, P)
+    m = hf.Mechanism(hf.H2_LI, rtol=1e-12, atol=1e-20)
+    g = TwoLevelDriver(m, Q.shape[2], Q.shape[1], dx, dx, bc, box, cfl=0.5)
+    o = OracleTwoLevelDriver(oracle_h2, Q.shape[2], Q.shape[1], dx, dx, bc, box, cfl=0.5)
+    g.set_state(Q)
+    o.set_state(Q)
+    g.step()
+    o.step()
+    gb, gd = g.regrid()
+    ob, od = o.regrid()
+    assert gb == ob and gb != box and gd < 1e-13 and od < 1e-13
+    g.step()
+    o.step()
+    Cg, Fg = g.state()
+    _compare(Cg, o.Qc, 1e-8, mech_h2.ns)
+    _compare(Fg, o.Qf, 1e-8, mech_h2.ns)

@@ -343,3 +343,28 @@ def test_two_level_step_conserves(oracle_h2):
     assert d.time > 0 and (np.abs(after - before) / scale).max() < 1e-13
     # the fine box carries its own solution: it differs from the interpolated start
     assert not np.allclose(d.Qf[ns + 2], Q[ns + 2, 6:18, 8:24].repeat(2, 0).repeat(2, 1))
+
+
+def test_fine_box_for_tags_closed_forms():
+    from oracle.driver import fine_box_for_tags
+    t = np.zeros((24, 40), dtype=bool)
+    assert fine_box_for_tags(t, 2, 4, 40, 24) is None
+    t[10, 17] = True
+    # grown to 15..19 x 8..12, blocks of 4: 12..19 x 8..15 -> fine 24..39 x 16..31
+    assert fine_box_for_tags(t, 2, 4, 40, 24) == (24, 16, 39, 31)
+    t[1, 39] = True                                        # near the corner: clamped 2 cells inside
+    assert fine_box_for_tags(t, 2, 4, 40, 24) == (24, 4, 75, 31)
+
+
+def test_two_level_regrid_conserves(oracle_h2):
+    """Steps and regrids of the oracle's two-level driver keep the conserved
+    integrals to roundoff (the remap defect check) and follow the hot band."""
+    from oracle.driver import PERIODIC, OracleTwoLevelDriver
+    o = oracle_h2
+    Q, dx, box = two_level_case(o)
+    d = OracleTwoLevelDriver(o, Q.shape[2], Q.shape[1], dx, dx, (PERIODIC,) * 4, box, cfl=0.5)
+    d.set_state(Q)
+    d.step()
+    new_box, defect = d.regrid()
+    assert new_box != box and defect < 1e-13
+    d.step()
