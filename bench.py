@@ -374,11 +374,17 @@ def tag_run(torch, hf, reps):
 TWO_LEVEL_BOX = (64, 128, 319, 383)          # fine cells: coarse 32..159 x 64..191, across the strip's front
 
 
+REGRID_INTERVAL = 8                          # the reference's default regrid_interval (main.cpp, BlockAMR.cpp:420)
+
+
 def two_level_run(torch, hf, steps, warmup):
     """Strang steps of the channel (1024 x 256, reflecting left, top and
-    bottom, transmissive right) with one fixed refined box TWO_LEVEL_BOX
-    through TwoLevelDriver, batched FP64 chemistry over the active cells.
-    Returns (wall of the timed steps, driver, (coarse, fine))."""
+    bottom, transmissive right) with one refined box through TwoLevelDriver,
+    batched FP64 chemistry over the active cells. The box starts at
+    TWO_LEVEL_BOX, is re-chosen from the tags of the initial state, and is
+    regridded every REGRID_INTERVAL steps (ErrorEst's default criteria, a
+    2-cell buffer, blocks of 4 coarse cells). Returns (wall of the timed
+    steps, driver, (coarse, fine))."""
     from hyperflow_amd.driver import TwoLevelDriver, channel_2d
     mech = hf.Mechanism(hf.H2_LI, rtol=1e-8, atol=1e-12)
     Q, dx = channel_2d(mech, **CHANNEL)
@@ -388,15 +394,20 @@ def two_level_run(torch, hf, steps, warmup):
         d = TwoLevelDriver(mech, CHANNEL["nx"], CHANNEL["ny"], dx, dx, bc, TWO_LEVEL_BOX, cfl=0.5,
                            chem_mode=hf.HF_CHEM_BATCHED)
         d.set_state(Q)
+        d.regrid()
         return d
+
+    def advance(d, n):
+        for _ in range(n):
+            d.step()
+            if d.steps % REGRID_INTERVAL == 0:
+                d.regrid()
     d = fresh()
-    for _ in range(warmup):
-        d.step()
+    advance(d, warmup)
     d = fresh()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(steps):
-        d.step()
+    advance(d, steps)
     torch.cuda.synchronize()
     return time.perf_counter() - t0, d, d.state()
 
@@ -412,10 +423,13 @@ def two_level_cpu(hf, target_s=12.0):
                              (REFLECTING, TRANSMISSIVE, REFLECTING, REFLECTING), TWO_LEVEL_BOX, cfl=0.5,
                              rtol=1e-8, atol=1e-12)
     o.set_state(Q)
+    o.regrid()
     n, t0 = 0, time.perf_counter()
     while n == 0 or time.perf_counter() - t0 < target_s:
         o.step()
         n += 1
+        if o.steps % REGRID_INTERVAL == 0:
+            o.regrid()
     return n, time.perf_counter() - t0, (o.Qc.copy(), o.Qf.copy())
 
 
@@ -712,12 +726,16 @@ def main():
     if args.two_level_steps:
         log(f"two-level channel: 1024 x 256 + 256 x 256 fine box, {args.two_level_steps} Strang steps")
         twall, tdrv, _ = two_level_run(torch, hf, args.two_level_steps, 1)
-        leaves = CHANNEL["nx"] * CHANNEL["ny"] - 128 * 128 + 256 * 256
-        two = {"workload": "the 2-D channel (1024 x 256 coarse cells over 0.2 x 0.05 m, H2_Li) with one fixed "
-                           "refined box of 256 x 256 fine cells at ratio 2 across the hot strip's front (coarse "
-                           "cells 32..159 x 64..191); two-level composite Heun stages (reflux, average-down, "
-                           "repair, validation), Strang chemistry (batched FP64 form, rtol 1e-8 atol 1e-12) over "
-                           "the 311,296 active cells, CFL 0.5 over both levels",
+        fb = tdrv.fine_box
+        nf = (fb[2] - fb[0] + 1) * (fb[3] - fb[1] + 1)
+        leaves = CHANNEL["nx"] * CHANNEL["ny"] - nf // 4 + nf
+        two = {"workload": "the 2-D channel (1024 x 256 coarse cells over 0.2 x 0.05 m, H2_Li) with one refined "
+                           "box at ratio 2 chosen from ErrorEst's tags at the reference defaults (p and T jumps "
+                           "0.08, fuel rate 50 /s; 2-cell buffer, blocks of 4 coarse cells) and regridded every "
+                           f"{REGRID_INTERVAL} steps with the remap defect check; two-level composite Heun stages "
+                           "(reflux, average-down, repair, validation), Strang chemistry (batched FP64 form, rtol "
+                           "1e-8 atol 1e-12) over the active cells, CFL 0.5 over both levels",
+               "fine_box_end": list(fb), "regrids": tdrv.regrids,
                "steps": args.two_level_steps, "steps_per_s": args.two_level_steps / twall,
                "ms_per_step": twall / args.two_level_steps * 1e3,
                "Mleaf_updates_per_s": leaves * 4 * args.two_level_steps / twall / 1e6,
