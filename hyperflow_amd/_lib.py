@@ -14,7 +14,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libhyperflow_hip.so"
 
 HF_OK, HF_EINVAL, HF_EDOMAIN, HF_ECONV, HF_ENOMEM, HF_EDEVICE, HF_EBUSY = range(7)
 HF_CHEM_PLAIN, HF_CHEM_BATCHED = 0, 1
-HF_BC_TRANSMISSIVE, HF_BC_REFLECTING, HF_BC_PERIODIC, HF_BC_INFLOW, HF_BC_WEDGE = 0, 1, 2, 3, 4
+HF_BC_TRANSMISSIVE, HF_BC_REFLECTING, HF_BC_PERIODIC, HF_BC_INFLOW, HF_BC_WEDGE, HF_BC_KEEP = 0, 1, 2, 3, 4, 5
 HF_PREC_FP64, HF_PREC_MIXED = 0, 1
 
 ERROR_NAMES = {0: "HF_OK", 1: "HF_EINVAL", 2: "HF_EDOMAIN", 3: "HF_ECONV", 4: "HF_ENOMEM", 5: "HF_EDEVICE",

@@ -193,9 +193,10 @@ def choose_fine_box(tags, buffer, blocking, nx, ny, margin=2):
     tags grown by ``buffer`` cells (AMReX's n_error_buf, HyperFlow_V3/src/
     main.cpp:35), their bounding box widened to multiples of ``blocking``
     coarse cells (blocking_factor) and kept ``margin`` cells inside the
-    level. Returns fine cells (ilo, jlo, ihi, jhi) at ratio 2, or None when
-    nothing is tagged. AMReX clusters the tags into several boxes; a single
-    bounding box is the one-box driver's form of it."""
+    level (one number, or one per side: x-lo, x-hi, y-lo, y-hi). Returns fine
+    cells (ilo, jlo, ihi, jhi) at ratio 2, or None when nothing is tagged.
+    AMReX clusters the tags into several boxes; a single bounding box is the
+    one-box driver's form of it."""
     import numpy as np
     t = np.asarray(tags, dtype=bool)
     if not t.any():
@@ -205,8 +206,9 @@ def choose_fine_box(tags, buffer, blocking, nx, ny, margin=2):
     j0, j1 = int(js.min()) - buffer, int(js.max()) + buffer
     i0, j0 = (i0 // blocking) * blocking, (j0 // blocking) * blocking
     i1, j1 = (i1 // blocking + 1) * blocking - 1, (j1 // blocking + 1) * blocking - 1
-    i0, j0 = max(i0, margin), max(j0, margin)
-    i1, j1 = min(i1, nx - 1 - margin), min(j1, ny - 1 - margin)
+    m = (margin,) * 4 if np.isscalar(margin) else tuple(margin)
+    i0, j0 = max(i0, m[0]), max(j0, m[2])
+    i1, j1 = min(i1, nx - 1 - m[1]), min(j1, ny - 1 - m[3])
     if i1 < i0 or j1 < j0:
         return None
     return (2 * i0, 2 * j0, 2 * i1 + 1, 2 * j1 + 1)

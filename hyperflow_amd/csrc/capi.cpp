@@ -804,8 +804,8 @@ int hf_fill_ghosts(hf_ctx* c, double* U, const hf_box* box, const int32_t bc[4])
     if (!c) return HF_EINVAL;
     if (bc)
         for (int k = 0; k < 4; ++k)
-            if (bc[k] == HF_BC_INFLOW || bc[k] == HF_BC_WEDGE)
-                return fail(c, HF_EINVAL, "inflow and wedge boundaries need hf_fill_ghosts_flow");
+            if (bc[k] == HF_BC_INFLOW || bc[k] == HF_BC_WEDGE || bc[k] == HF_BC_KEEP)
+                return fail(c, HF_EINVAL, "inflow, wedge and kept sides need hf_fill_ghosts_flow");
     return hf_fill_ghosts_flow(c, U, box, bc, nullptr);
 }
 
@@ -815,7 +815,7 @@ int check_bc(hf_ctx* c, const hf_box* box, const int32_t bc[4], const double* in
     if (!bc) return fail(c, HF_EINVAL, "missing boundary codes");
     const bool two_d = bc[2] >= 0 && bc[3] >= 0;
     for (int k = 0; k < 4; ++k) {
-        if (bc[k] > HF_BC_WEDGE || (k < 2 && bc[k] < 0)) return fail(c, HF_EINVAL, "unknown boundary code");
+        if (bc[k] > HF_BC_KEEP || (k < 2 && bc[k] < 0)) return fail(c, HF_EINVAL, "unknown boundary code");
         if (bc[k] == HF_BC_WEDGE && (k != 2 || !two_d))
             return fail(c, HF_EINVAL, "the wedge boundary exists only on the y-lo side of a 2-D box");
         if (bc[k] == HF_BC_INFLOW && !inlet) return fail(c, HF_EINVAL, "an inflow boundary needs the inlet state");

@@ -637,6 +637,7 @@ __global__ void __launch_bounds__(256) ghost_kernel(double* __restrict__ U, hf_b
     const int g = g4 < 2 ? g4 - 2 : len + g4 - 2;          // -2, -1, len, len+1
     const int code = g < 0 ? (pass == 0 ? bc.x : bc.z) : (pass == 0 ? bc.y : bc.w);
     if (pass == 0) {
+        if (code == HF_BC_KEEP) return;
         if (code == HF_BC_INFLOW) {
             box_at(U, B, n, g, a) = inlet.q[n];
             return;
@@ -648,6 +649,20 @@ __global__ void __launch_bounds__(256) ghost_kernel(double* __restrict__ U, hf_b
         return;
     }
     const int i = a - 2;
+    if (code == HF_BC_KEEP) {
+        // an in-domain y ghost row: only its cells beyond a physical x side
+        // are filled, by that side's rule from the kept cell beside them
+        const int xcode = i < 0 ? bc.x : (i >= B.nx ? bc.y : HF_BC_KEEP);
+        if (xcode == HF_BC_KEEP) return;
+        if (xcode == HF_BC_INFLOW) {
+            box_at(U, B, n, i, g) = inlet.q[n];
+            return;
+        }
+        const int s = ghost_source(i, B.nx, xcode);
+        const double v = box_at(U, B, n, s, g);
+        box_at(U, B, n, i, g) = xcode == HF_BC_REFLECTING && n == Topo::NS ? -v : v;
+        return;
+    }
     const bool beyond_x_inflow = (i < 0 && bc.x == HF_BC_INFLOW) || (i >= B.nx && bc.y == HF_BC_INFLOW);
     if (beyond_x_inflow || code == HF_BC_INFLOW) {
         box_at(U, B, n, i, g) = inlet.q[n];
@@ -655,7 +670,8 @@ __global__ void __launch_bounds__(256) ghost_kernel(double* __restrict__ U, hf_b
     }
     if (code == HF_BC_WEDGE) {                             // g < 0 here: y-lo only
         const bool wall = i >= B.wedge_i;
-        const int ci = min(max(i, 0), B.nx - 1);
+        // the column is clamped to the domain: only at a physical x side
+        const int ci = (i < 0 && bc.x != HF_BC_KEEP) ? 0 : ((i >= B.nx && bc.y != HF_BC_KEEP) ? B.nx - 1 : i);
         const double v = box_at(U, B, n, ci, wall ? -g - 1 : 0);
         box_at(U, B, n, i, g) = wall && n == Topo::NS + 1 ? -v : v;
         return;

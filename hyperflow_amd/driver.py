@@ -24,7 +24,8 @@ from ctypes import byref, c_double, c_int32, c_int64
 
 import numpy as np
 
-from ._lib import (HF_BC_INFLOW, HF_BC_PERIODIC, HF_BC_REFLECTING, HF_BC_TRANSMISSIVE, HF_BC_WEDGE,  # noqa: F401
+from ._lib import (HF_BC_INFLOW, HF_BC_KEEP, HF_BC_PERIODIC, HF_BC_REFLECTING, HF_BC_TRANSMISSIVE,  # noqa: F401
+                   HF_BC_WEDGE,
                    HF_CHEM_BATCHED, HF_EDOMAIN, HF_OK,
                    HyperflowError, check, hf_box, hf_flux_stats, load)
 from .chemistry import ChemistryBatch
@@ -281,8 +282,10 @@ class TwoLevelDriver:
 
     Roundoff repair runs per level (each species' integral is kept exactly on
     each level); the reference picks one donor over both levels.
-    fine_box = (ilo, jlo, ihi, jhi) fine cells; its coarse cover must lie at
-    least two cells inside the coarse level.
+    fine_box = (ilo, jlo, ihi, jhi) fine cells. A side of the box either lies
+    on a non-periodic domain side (its ghosts then take that side's physical
+    rule on the fine level, its faces are walls where the domain's are, and it
+    has no flux register) or at least one coarse cell inside the level.
     """
 
     def __init__(self, mechanism, nx, ny, dx, dy, bc, fine_box, cfl=0.5, chem_mode=HF_CHEM_BATCHED,
@@ -295,10 +298,7 @@ class TwoLevelDriver:
         self.coarse = BoxDriver(mechanism, nx, ny, dx, dy, bc, cfl=cfl, chem_mode=chem_mode,
                                 max_attempts=max_attempts, inlet=inlet, wedge_i=wedge_i)
         FI0, FJ0, FI1, FJ1 = (int(v) for v in fine_box)
-        if FI0 % 2 or FJ0 % 2 or FI1 % 2 == 0 or FJ1 % 2 == 0:
-            raise ValueError("the fine box must be aligned to the refinement ratio 2")
-        if FI0 // 2 < 2 or FJ0 // 2 < 2 or FI1 // 2 > nx - 3 or FJ1 // 2 > ny - 3:
-            raise ValueError("the fine box must lie at least two coarse cells inside the level")
+        self._check_box((FI0, FJ0, FI1, FJ1))
         self.dxf, self.dyf = 0.5 * float(dx), 0.5 * float(dy)
         nv = mechanism.nv
         dev = dict(dtype=torch.float64, device="cuda")
@@ -313,8 +313,24 @@ class TwoLevelDriver:
         self.time, self.steps, self.retries, self.first_order_stages, self.repairs = 0.0, 0, 0, 0, 0
         self._h = mechanism.handle
 
+    def _check_box(self, box):
+        c = self.coarse
+        FI0, FJ0, FI1, FJ1 = box
+        if FI0 % 2 or FJ0 % 2 or FI1 % 2 == 0 or FJ1 % 2 == 0:
+            raise ValueError("the fine box must be aligned to the refinement ratio 2")
+        lo_hi = ((FI0 // 2, 0), (c.nx - 1 - FI1 // 2, 1), (FJ0 // 2, 2), (c.ny - 1 - FJ1 // 2, 3))
+        for gap, k in lo_hi:
+            if gap < 0 or (gap == 0 and c.bc[k] in (HF_BC_PERIODIC, -1)):
+                raise ValueError("a fine box side must lie on a non-periodic domain side or inside the level")
+
+    def margins(self):
+        """Per side (x-lo, x-hi, y-lo, y-hi): 0 where a fine box may touch the
+        domain side, 1 where it must stay a cell inside (periodic sides)."""
+        return tuple(1 if self.coarse.bc[k] == HF_BC_PERIODIC else 0 for k in range(4))
+
     def _set_box(self, box):
-        """Fine-level arrays, ghost strips and the active mask for box."""
+        """Fine-level arrays, ghost strips, boundary handling and the active
+        mask for box."""
         torch = self.torch
         FI0, FJ0, FI1, FJ1 = box
         c = self.coarse
@@ -333,6 +349,12 @@ class TwoLevelDriver:
         mask = torch.ones((c.ny, c.nx), dtype=torch.bool, device="cuda")
         mask[self.covered] = False
         self.active = mask.reshape(-1)
+        touch = (FI0 == 0, FI1 == 2 * c.nx - 1, FJ0 == 0, FJ1 == 2 * c.ny - 1)
+        self.fine_bc = (c_int32 * 4)(*[c.bc[k] if touch[k] else HF_BC_KEEP for k in range(4)])
+        self.fine_walls = sum(1 << k for k in range(4)
+                              if touch[k] and (c.bc[k] == HF_BC_REFLECTING or (k == 2 and c.bc[k] == HF_BC_WEDGE)))
+        self.fine_wedge_i = 2 * c.wedge_i - FI0
+        self.reflux_sides = sum(1 << k for k in range(4) if not touch[k])
 
     # ------------------------------------------------------------ state
     def set_state(self, Q):
@@ -354,7 +376,8 @@ class TwoLevelDriver:
     def _fbox(self, h=0.0):
         gx, gy = self.nxf + 4, self.nyf + 4
         plane = gx * gy
-        return hf_box(self.nxf, self.nyf, 0, 0, 0, gx, plane, 2 * gx + 2, gx, plane, h / self.dxf, h / self.dyf)
+        return hf_box(self.nxf, self.nyf, self.fine_walls, self.fine_wedge_i, 0, gx, plane, 2 * gx + 2, gx, plane,
+                      h / self.dxf, h / self.dyf)
 
     @staticmethod
     def _interior_patch(nx, ny, lo=(0, 0)):
@@ -388,6 +411,10 @@ class TwoLevelDriver:
         c._fill(src_c)
         for r in self.strips:
             self.interp.interp(src_c, (-2, -2), src_f, (FI0 - 2, FJ0 - 2), r)
+        if self.reflux_sides != 15:
+            # ghosts beyond the domain: the physical rule on the fine level
+            fb = self._fbox()
+            check(self._h, load().hf_fill_ghosts_flow(self._h, src_f.data_ptr(), byref(fb), self.fine_bc, c.inlet))
         # both stages write straight into the ghosted outputs' valid cells
         for (src, dst, box, fl) in ((src_c, dst_c, c._box(h), (self.cfx, self.cfy)),
                                     (src_f, dst_f, self._fbox(h), (self.ffx, self.ffy))):
@@ -400,8 +427,8 @@ class TwoLevelDriver:
         cf, ff_ = face_fluxes(self.cfx, self.cfy), face_fluxes(self.ffx, self.ffy, (FI0, FJ0))
         cp = self._interior_patch(c.nx, c.ny)
         fb = (c_int32 * 4)(*self.fine_box)
-        check(self._h, load().hf_reflux(self._h, dst_c.data_ptr(), byref(cp), byref(cf), byref(ff_), fb, 15, h, c.dx,
-                                        c.dy))
+        check(self._h, load().hf_reflux(self._h, dst_c.data_ptr(), byref(cp), byref(cf), byref(ff_), fb,
+                                        self.reflux_sides, h, c.dx, c.dy))
         self._average(dst_c, dst_f)
         rep, ff = c_int64(0), c_int64(-1)
         repaired = 0
@@ -485,7 +512,7 @@ class TwoLevelDriver:
         before, scale = self._levels_totals()
         c._fill(c.U)
         tags = tag_cells(self.mech, c.U, c.dx, c.dy, time=self.time, **tag_kw).cpu().numpy().astype(bool)
-        box = choose_fine_box(tags, buffer, blocking, c.nx, c.ny)
+        box = choose_fine_box(tags, buffer, blocking, c.nx, c.ny, self.margins())
         if box is None or box == self.fine_box:
             return self.fine_box, 0.0
         old_box, old_f = self.fine_box, self.fine_interior().clone()

@@ -208,7 +208,8 @@ int hf_chem_tangent(hf_ctx *ctx, int64_t ncell, const double *rho, const double 
  * Device-memory helpers for a driver that advances one single-level ghosted
  * box (the hf_box input layout; the output fields of the box are unused).
  * All pointers are device pointers (HF_EINVAL otherwise). */
-enum { HF_BC_TRANSMISSIVE = 0, HF_BC_REFLECTING = 1, HF_BC_PERIODIC = 2, HF_BC_INFLOW = 3, HF_BC_WEDGE = 4 };
+enum { HF_BC_TRANSMISSIVE = 0, HF_BC_REFLECTING = 1, HF_BC_PERIODIC = 2, HF_BC_INFLOW = 3, HF_BC_WEDGE = 4,
+       HF_BC_KEEP = 5 };
 
 /* Fill the two ghost layers of U (BlockAMR::fill, BlockAMR.cpp:137-152, with
  * the boundary types of BlockAMR.cpp:48-59): transmissive copies the nearest
@@ -228,7 +229,11 @@ int hf_fill_ghosts(hf_ctx *ctx, double *U, const hf_box *box, const int32_t bc[4
  * tip (copy of row 0) and a slip wall from column box->wedge_i on (mirror,
  * negated y momentum), reading the valid cell whose column is the ghost's
  * clamped to [0, nx). A corner beyond an inflow side is inlet even on a wedge
- * bottom. inlet may be NULL when no side is HF_BC_INFLOW. */
+ * bottom. inlet may be NULL when no side is HF_BC_INFLOW. HF_BC_KEEP marks a
+ * side inside the domain (a fine box's side whose ghosts came from the coarse
+ * level): its ghosts are kept, except that a ghost beyond a physical x side
+ * but beside a kept y side takes that x side's rule (FillPatchTwoLevels fills
+ * the in-domain ghosts first, then the physical ones). */
 int hf_fill_ghosts_flow(hf_ctx *ctx, double *U, const hf_box *box, const int32_t bc[4], const double *inlet);
 
 /* One SoA array of a level patch: element (component n, cell i, j), i and j

@@ -24,7 +24,7 @@ from __future__ import annotations
 
 import numpy as np
 
-TRANSMISSIVE, REFLECTING, PERIODIC, INFLOW, WEDGE = 0, 1, 2, 3, 4
+TRANSMISSIVE, REFLECTING, PERIODIC, INFLOW, WEDGE, KEEP = 0, 1, 2, 3, 4, 5
 EPS = np.finfo(float).eps
 
 
@@ -55,8 +55,8 @@ def fill_ghosts(G, ns, bc, one_d, inlet=None, wedge_i=0):
     nx, ny = gx - 4, gy - 4
     for g in (-2, -1, nx, nx + 1):
         code = bc[0] if g < 0 else bc[1]
-        if code == INFLOW:
-            continue                      # the functor below sets it
+        if code in (INFLOW, KEEP):
+            continue                      # the functor below sets it / an in-domain side
         s = _src(g, nx, code)
         G[:, 2:ny + 2, g + 2] = G[:, 2:ny + 2, s + 2]
         if code == REFLECTING:
@@ -64,6 +64,17 @@ def fill_ghosts(G, ns, bc, one_d, inlet=None, wedge_i=0):
     if not one_d:
         for g in (-2, -1, ny, ny + 1):
             code = bc[2] if g < 0 else bc[3]
+            if code == KEEP:
+                # in-domain rows: only the cells beyond a physical x side, by its rule
+                for i in list(range(-2, 0)) + list(range(nx, nx + 2)):
+                    xcode = bc[0] if i < 0 else bc[1]
+                    if xcode in (KEEP, INFLOW):
+                        continue
+                    s = _src(i, nx, xcode)
+                    G[:, g + 2, i + 2] = G[:, g + 2, s + 2]
+                    if xcode == REFLECTING:
+                        G[ns, g + 2, i + 2] *= -1.0
+                continue
             if code in (INFLOW, WEDGE):
                 continue
             s = _src(g, ny, code)
@@ -85,7 +96,7 @@ def fill_ghosts(G, ns, bc, one_d, inlet=None, wedge_i=0):
             elif below and bc[2] == WEDGE:
                 wall = i >= wedge_i
                 sj = -j - 1 if wall else 0
-                si = min(max(i, 0), nx - 1)
+                si = 0 if (i < 0 and bc[0] != KEEP) else (nx - 1 if (i >= nx and bc[1] != KEEP) else i)
                 G[:, j + 2, i + 2] = G[:, sj + 2, si + 2]
                 if wall:
                     G[ns + 1, j + 2, i + 2] *= -1.0
@@ -220,17 +231,31 @@ class OracleTwoLevelDriver:
                                  max_attempts=max_attempts, inlet=inlet, wedge_i=wedge_i)
         self.o = oracle
         self.ns, self.nv = oracle.ns, oracle.ns + 3
-        self.fine_box = tuple(fine_box)
-        FI0, FJ0, FI1, FJ1 = self.fine_box
-        self.nxf, self.nyf = FI1 - FI0 + 1, FJ1 - FJ0 + 1
         self.dxf, self.dyf = 0.5 * dx, 0.5 * dy
-        self.cov = (slice(FJ0 // 2, FJ1 // 2 + 1), slice(FI0 // 2, FI1 // 2 + 1))
-        self.active = np.ones((ny, nx), dtype=bool)
-        self.active[self.cov] = False
+        self._set_box(tuple(fine_box))
         self.cfl, self.rtol, self.atol, self.max_attempts = cfl, rtol, atol, max_attempts
         self.Qc = np.zeros((self.nv, ny, nx))
         self.Qf = np.zeros((self.nv, self.nyf, self.nxf))
         self.time, self.steps, self.retries, self.first_order_stages = 0.0, 0, 0, 0
+
+    def _set_box(self, box):
+        """Box geometry, the active mask and the fine level's boundary handling:
+        a side on a (non-periodic) domain side takes its physical rule and
+        walls and has no flux register; the others are kept."""
+        FI0, FJ0, FI1, FJ1 = box
+        c = self.c
+        self.fine_box = tuple(box)
+        self.nxf, self.nyf = FI1 - FI0 + 1, FJ1 - FJ0 + 1
+        self.cov = (slice(FJ0 // 2, FJ1 // 2 + 1), slice(FI0 // 2, FI1 // 2 + 1))
+        self.active = np.ones((c.ny, c.nx), dtype=bool)
+        self.active[self.cov] = False
+        touch = (FI0 == 0, FI1 == 2 * c.nx - 1, FJ0 == 0, FJ1 == 2 * c.ny - 1)
+        self.fine_bc = [c.bc[k] if touch[k] else KEEP for k in range(4)]
+        self.fine_walls = sum(1 << k for k in range(4)
+                              if touch[k] and (c.bc[k] == REFLECTING or (k == 2 and c.bc[k] == WEDGE)))
+        self.fine_wedge_i = 2 * c.wedge_i - FI0
+        self.reflux_sides = sum(1 << k for k in range(4) if not touch[k])
+        self.margins = tuple(1 if c.bc[k] == PERIODIC else 0 for k in range(4))
 
     def set_state(self, Q):
         from .amr import interp
@@ -259,12 +284,15 @@ class OracleTwoLevelDriver:
         ring, _, _ = interp(self.o, G, (-2, -2), (FI0 - 2, FJ0 - 2, FI1 + 2, FJ1 + 2))
         Gf[:] = ring
         Gf[:, 2:-2, 2:-2] = Qf
+        if self.reflux_sides != 15:
+            fill_ghosts(Gf, self.ns, self.fine_bc, False, self.c.inlet, self.fine_wedge_i)
         for first_order in (False, True):
             _, oc, _, _, cfx, cfy = self.o.flux_stage(G, h / self.c.dx, h / self.c.dy, first_order=first_order,
                                                       walls=self.c.walls, wedge_i=self.c.wedge_i, fluxes=True)
             _, of, _, _, ffx, ffy = self.o.flux_stage(Gf, h / self.dxf, h / self.dyf, first_order=first_order,
-                                                      fluxes=True)
-            reflux(oc, (0, 0), (cfx, cfy, (0, 0)), (ffx, ffy, (FI0, FJ0)), self.fine_box, 15, h, self.c.dx, self.c.dy)
+                                                      walls=self.fine_walls, wedge_i=self.fine_wedge_i, fluxes=True)
+            reflux(oc, (0, 0), (cfx, cfy, (0, 0)), (ffx, ffy, (FI0, FJ0)), self.fine_box, self.reflux_sides, h,
+                   self.c.dx, self.c.dy)
             self._average(oc, of)
             try:
                 n = repair_roundoff(of, self.ns) + repair_roundoff(oc, self.ns)
@@ -339,8 +367,9 @@ def fine_box_for_tags(tags, buffer, blocking, nx, ny, margin=2):
     lo_j = (rows[0] - buffer) - (rows[0] - buffer) % blocking
     hi_i = (cols[-1] + buffer) - (cols[-1] + buffer) % blocking + blocking - 1
     hi_j = (rows[-1] + buffer) - (rows[-1] + buffer) % blocking + blocking - 1
-    lo_i, lo_j = max(int(lo_i), margin), max(int(lo_j), margin)
-    hi_i, hi_j = min(int(hi_i), nx - 1 - margin), min(int(hi_j), ny - 1 - margin)
+    m = (margin,) * 4 if np.isscalar(margin) else tuple(margin)
+    lo_i, lo_j = max(int(lo_i), m[0]), max(int(lo_j), m[2])
+    hi_i, hi_j = min(int(hi_i), nx - 1 - m[1]), min(int(hi_j), ny - 1 - m[3])
     if hi_i < lo_i or hi_j < lo_j:
         return None
     return (2 * lo_i, 2 * lo_j, 2 * hi_i + 1, 2 * hi_j + 1)
@@ -360,16 +389,12 @@ def _regrid(self, buffer=2, blocking=4, **tag_kw):
     before, scale = tot(), tot(True)
     G = self.c._ghosted(self.Qc)
     tags = tag_cells(self.o, G, self.c.dx, self.c.dy, time=self.time, **tag_kw)
-    box = fine_box_for_tags(tags, buffer, blocking, self.c.nx, self.c.ny)
+    box = fine_box_for_tags(tags, buffer, blocking, self.c.nx, self.c.ny, self.margins)
     if box is None or tuple(box) == self.fine_box:
         return self.fine_box, 0.0
     old_box, old_f = self.fine_box, self.Qf.copy()
     FI0, FJ0, FI1, FJ1 = box
-    self.fine_box = tuple(box)
-    self.nxf, self.nyf = FI1 - FI0 + 1, FJ1 - FJ0 + 1
-    self.cov = (slice(FJ0 // 2, FJ1 // 2 + 1), slice(FI0 // 2, FI1 // 2 + 1))
-    self.active = np.ones((self.c.ny, self.c.nx), dtype=bool)
-    self.active[self.cov] = False
+    self._set_box(tuple(box))
     self.Qf, _, _ = interp(self.o, G, (-2, -2), box)
     a0, b0 = max(box[0], old_box[0]), max(box[1], old_box[1])
     a1, b1 = min(box[2], old_box[2]), min(box[3], old_box[3])

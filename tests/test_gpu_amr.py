@@ -25,6 +25,7 @@ pytestmark = pytest.mark.gpu
 
 I, T, R_, P, W = hf.HF_BC_INFLOW, hf._lib.HF_BC_TRANSMISSIVE, hf._lib.HF_BC_REFLECTING, hf._lib.HF_BC_PERIODIC, \
     hf.HF_BC_WEDGE
+K = hf._lib.HF_BC_KEEP
 
 
 def _dev(a):
@@ -58,7 +59,9 @@ def test_boundary_check_known_answers(mech_h2):
 
 @pytest.mark.parametrize("mech", ["h2", "dme"])
 @pytest.mark.parametrize("bc,wedge_i", [((I, T, W, I), 3), ((R_, I, W, T), -1), ((T, T, W, R_), 99),
-                                        ((P, P, W, I), 2), ((I, I, I, I), 0), ((R_, T, I, P), 0)])
+                                        ((P, P, W, I), 2), ((I, I, I, I), 0), ((R_, T, I, P), 0),
+                                        ((K, R_, K, T), 0), ((I, K, W, K), 4), ((K, K, W, K), 2),
+                                        ((R_, K, K, I), 0), ((K, K, K, K), 0)])
 def test_flow_fill_matches_oracle(mech, bc, wedge_i, mech_h2, mech_dme):
     m = mech_h2 if mech == "h2" else mech_dme
     print(f"[gpu] flow fill {mech} bc={bc} wedge_i={wedge_i}", flush=True)
@@ -74,7 +77,7 @@ def test_flow_fill_matches_oracle(mech, bc, wedge_i, mech_h2, mech_dme):
 
 def test_flow_fill_rejects_bad_codes(mech_h2):
     U = _dev(np.zeros((mech_h2.nv, 8, 9)))
-    for bc, inlet in (((W, T, T, T), None), ((T, T, T, W), None), ((I, T, T, T), None), ((T, T, 5, T), None)):
+    for bc, inlet in (((W, T, T, T), None), ((T, T, T, W), None), ((I, T, T, T), None), ((T, T, 7, T), None)):
         with pytest.raises(hf.HyperflowError):
             fill_ghosts(mech_h2, U, bc, inlet)
     with pytest.raises(hf.HyperflowError):       # plain fill refuses the flow codes
@@ -444,6 +447,32 @@ def test_two_level_regrid_matches_oracle(mech_h2, oracle_h2):
     assert gb == ob and gb != box and gd < 1e-13 and od < 1e-13
     g.step()
     o.step()
+    Cg, Fg = g.state()
+    _compare(Cg, o.Qc, 1e-8, mech_h2.ns)
+    _compare(Fg, o.Qf, 1e-8, mech_h2.ns)
+
+
+def test_two_level_box_at_a_wall_matches_oracle(mech_h2, oracle_h2):
+    """A fine box on the x-lo reflecting wall (its ghosts there by the
+    physical rule on the fine level, wall faces, no flux register on that
+    side): 2 Strang steps against the oracle, 1e-8."""
+    from hyperflow_amd.driver import TwoLevelDriver
+    from oracle.driver import OracleTwoLevelDriver
+    from test_oracle_amr import two_level_case
+    print("[gpu] two-level driver: fine box on the x-lo wall, 2 steps", flush=True)
+    Q, dx, _ = two_level_case(oracle_h2)
+    Q[:, 10:15, 0:6] = Q[:, 10:15, 14:15]
+    bc = (R_, T, R_, P)
+    box = (0, 12, 47, 35)
+    m = hf.Mechanism(hf.H2_LI, rtol=1e-12, atol=1e-20)
+    g = TwoLevelDriver(m, Q.shape[2], Q.shape[1], dx, dx, bc, box, cfl=0.5)
+    o = OracleTwoLevelDriver(oracle_h2, Q.shape[2], Q.shape[1], dx, dx, bc, box, cfl=0.5)
+    assert g.reflux_sides == o.reflux_sides == 14 and g.fine_walls == o.fine_walls == 1
+    g.set_state(Q)
+    o.set_state(Q)
+    for _ in range(2):
+        dtg, dto = g.step(), o.step()
+        assert dtg == pytest.approx(dto, rel=1e-12)
     Cg, Fg = g.state()
     _compare(Cg, o.Qc, 1e-8, mech_h2.ns)
     _compare(Fg, o.Qf, 1e-8, mech_h2.ns)

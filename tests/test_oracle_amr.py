@@ -368,3 +368,45 @@ def test_two_level_regrid_conserves(oracle_h2):
     new_box, defect = d.regrid()
     assert new_box != box and defect < 1e-13
     d.step()
+
+
+def test_keep_sides_closed_forms():
+    """HF_BC_KEEP sides keep their (interpolated) ghosts; a ghost beyond a
+    physical x side beside a kept y side takes the x rule; a physical y side
+    applies its rule on every column, kept x ghosts included."""
+    from oracle.driver import KEEP
+    rng = np.random.default_rng(9)
+    ns, nx, ny = 9, 6, 5
+    G = rng.standard_normal((ns + 3, ny + 4, nx + 4))
+    G0 = G.copy()
+    fill_ghosts(G, ns, (KEEP, REFLECTING, KEEP, TRANSMISSIVE), False)
+    a = lambda A, i, j: A[:, j + 2, i + 2]
+    for j in range(-2, ny):                                # x-lo kept on every row below y-hi
+        for i in (-2, -1):
+            assert np.array_equal(a(G, i, j), a(G0, i, j))
+    for j in (-2, -1):                                     # y-lo kept inside x; reflecting beyond x-hi
+        for i in range(-2, nx):
+            assert np.array_equal(a(G, i, j), a(G0, i, j))
+        w = a(G0, nx - 1, j).copy()
+        w[ns] *= -1
+        assert np.array_equal(a(G, nx, j), w)
+    for i in range(-2, nx + 2):                            # y-hi transmissive on every column
+        assert np.array_equal(a(G, i, ny), a(G, i, ny - 1)) and np.array_equal(a(G, i, ny + 1), a(G, i, ny - 1))
+
+
+def test_two_level_box_at_a_wall_conserves(oracle_h2):
+    """A fine box on the x-lo wall of a closed (all-reflecting) box: two
+    two-level Strang steps keep mass and energy to roundoff."""
+    from oracle.driver import OracleTwoLevelDriver
+    o = oracle_h2
+    Q, dx, _ = two_level_case(o)
+    Q[:, 10:15, 0:6] = Q[:, 10:15, 14:15]                 # hot gas against the wall too
+    d = OracleTwoLevelDriver(o, Q.shape[2], Q.shape[1], dx, dx, (REFLECTING,) * 4, (0, 12, 47, 35), cfl=0.5)
+    assert d.reflux_sides == 14 and d.fine_walls == 1
+    d.set_state(Q)
+    ns = o.ns
+    before = np.array([d.Qc[:ns].sum(), d.Qc[ns + 2].sum()])
+    d.step()
+    d.step()
+    after = np.array([d.Qc[:ns].sum(), d.Qc[ns + 2].sum()])
+    assert (np.abs(after - before) / np.abs(before)).max() < 1e-13
