@@ -724,7 +724,7 @@ def main():
 
     two = None
     if args.two_level_steps:
-        log(f"two-level channel: 1024 x 256 + 256 x 256 fine box, {args.two_level_steps} Strang steps")
+        log(f"two-level channel: 1024 x 256 + one adaptive refined box, {args.two_level_steps} Strang steps")
         twall, tdrv, _ = two_level_run(torch, hf, args.two_level_steps, 1)
         fb = tdrv.fine_box
         nf = (fb[2] - fb[0] + 1) * (fb[3] - fb[1] + 1)
