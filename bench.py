@@ -522,6 +522,24 @@ def parity(U0, dt, out, n=256, mech_path=None):
             "Y_rel_floor_1e-10": float(np.max(np.abs(Yg - Yr) / np.maximum(np.abs(Yr), 1e-10)))}
 
 
+HBM_SUMMARY = Path(__file__).resolve().parent / "profiles" / "hbm_counters_chem_h2_1048576_dme_65536_flux_4096x1024.txt"
+
+
+def headline_traffic(cells):
+    """HBM bytes per launch of the headline kernel (H2 batched form, FP32
+    Newton matrix) from the committed rocprofv3 PMC summary of the same
+    2^20-cell workload: FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by
+    what the counter reads for a device-to-device copy of known size. None
+    when the summary is absent or was taken at another size."""
+    if cells != 1 << 20 or not HBM_SUMMARY.exists():
+        return None
+    for line in HBM_SUMMARY.read_text().splitlines():
+        if line.startswith("hf::chem_lane_kernel<hf_gen::h2_li, float, false>"):
+            f = line.split()
+            return (float(f[-2]) + float(f[-1])) * 1e6        # scaled MB in + MB out
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -783,11 +801,13 @@ def main():
                                "dt log-uniform [1e-8,1e-6] s, rtol 1e-8 atol 1e-15",
                    "cells": args.cells, "form": "batched"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": headline_traffic(args.cells),
                      "note": "FP64 algorithmic operations of the chemistry kernel (hyperflow_amd/flops.py: "
                              "the kernel's operation counters x per-operation counts, exp/log/log10/division "
                              "at the FP64 arithmetic of their gfx950 code) over its event-timed duration; "
-                             "peak is the FP64 rate; HBM traffic per launch is in profiles/"},
+                             "peak is the FP64 rate; traffic is the kernel's HBM bytes per launch (FETCH_SIZE + "
+                             "WRITE_SIZE, copy-scaled) from the committed rocprofv3 PMC summary of this workload "
+                             "(profiles/hbm_counters_*), against 192 algorithmic bytes per cell"},
     }
     line["config"]["precision"] = "FP64 state, residuals, rates and error control; FP32 Newton matrix and LU"
     for name in ("plain", "batched_fp64"):
