@@ -476,3 +476,30 @@ def test_two_level_box_at_a_wall_matches_oracle(mech_h2, oracle_h2):
     Cg, Fg = g.state()
     _compare(Cg, o.Qc, 1e-8, mech_h2.ns)
     _compare(Fg, o.Qf, 1e-8, mech_h2.ns)
+
+
+def test_two_level_wedge_box_matches_oracle(mech_h2, oracle_h2):
+    """The wedge case (72 x 32) with a fine box over the wedge tip, on the
+    inflow side and the wedge bottom: the fine level's inflow ghosts, its
+    wedge bottom with the tip column mapped into the box, its wall faces.
+    3 Strang steps against the oracle, 1e-8."""
+    from hyperflow_amd.driver import TwoLevelDriver
+    from oracle.driver import OracleTwoLevelDriver
+    print("[gpu] two-level wedge case: 72 x 32 coarse, fine box over the tip, 3 steps", flush=True)
+    m = hf.Mechanism(hf.H2_LI, rtol=1e-12, atol=1e-20)
+    w = teng_osw_base(m, nx=72, ny=32)
+    box = (0, 0, 31, 15)                                   # coarse 0..15 x 0..7; the tip at coarse column 4
+    kw = dict(cfl=0.6, inlet=w["inlet"], wedge_i=w["wedge_i"])
+    g = TwoLevelDriver(m, 72, 32, w["dx"], w["dy"], w["bc"], box, **kw)
+    o = OracleTwoLevelDriver(oracle_h2, 72, 32, w["dx"], w["dy"], w["bc"], box, **kw)
+    assert g.reflux_sides == o.reflux_sides == 2 | 8 and g.fine_wedge_i == o.fine_wedge_i == 2 * w["wedge_i"]
+    assert g.fine_walls == o.fine_walls == 4
+    g.set_state(w["Q"])
+    o.set_state(w["Q"])
+    for _ in range(3):
+        dtg, dto = g.step(), o.step()
+        assert dtg == pytest.approx(dto, rel=1e-12)
+    Cg, Fg = g.state()
+    assert not np.allclose(Fg, Fg[:, :1, :1])             # the wall turned the flow in the box
+    _compare(Cg, o.Qc, 1e-8, mech_h2.ns)
+    _compare(Fg, o.Qf, 1e-8, mech_h2.ns)
