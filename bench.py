@@ -212,6 +212,55 @@ def wedge_run(torch, hf, steps, warmup):
     return time.perf_counter() - t0, d, d.state()
 
 
+def wedge_amr_run(torch, hf, steps, warmup):
+    """The wedge case's 144 x 64 base level with one refined box regridded
+    every REGRID_INTERVAL steps from ErrorEst's tags (the case's own
+    pressure_tag 0.08, temperature_tag 0.06, fuel_rate_tag 50 /s;
+    cases/teng_case1), through TwoLevelDriver. Returns (wall, driver, states)."""
+    from hyperflow_amd.driver import TwoLevelDriver, teng_osw_base
+    mech = hf.Mechanism(hf.H2_LI, rtol=1e-8, atol=1e-12)
+    w = teng_osw_base(mech)
+    tag = dict(temperature_tag=0.06, x0=-0.000128)
+
+    def fresh():
+        d = TwoLevelDriver(mech, 144, 64, w["dx"], w["dy"], w["bc"], (0, 0, 31, 15), cfl=0.6,
+                           chem_mode=hf.HF_CHEM_BATCHED, inlet=w["inlet"], wedge_i=w["wedge_i"])
+        d.set_state(w["Q"])
+        return d
+
+    def advance(d, n):
+        for _ in range(n):
+            d.step()
+            if d.steps % REGRID_INTERVAL == 0:
+                d.regrid(**tag)
+    d = fresh()
+    advance(d, warmup)
+    d = fresh()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    advance(d, steps)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, d, d.state()
+
+
+def wedge_amr_cpu(hf, target_s=12.0):
+    """The oracle's two-level loop on the same adaptive wedge case."""
+    from hyperflow_amd.driver import teng_osw_base
+    from oracle import Oracle
+    from oracle.driver import OracleTwoLevelDriver
+    w = teng_osw_base(hf.Mechanism(hf.H2_LI))
+    o = OracleTwoLevelDriver(Oracle(hf.H2_LI), 144, 64, w["dx"], w["dy"], w["bc"], (0, 0, 31, 15), cfl=0.6,
+                             rtol=1e-8, atol=1e-12, inlet=w["inlet"], wedge_i=w["wedge_i"])
+    o.set_state(w["Q"])
+    n, t0 = 0, time.perf_counter()
+    while n == 0 or time.perf_counter() - t0 < target_s:
+        o.step()
+        n += 1
+        if o.steps % REGRID_INTERVAL == 0:
+            o.regrid(temperature_tag=0.06, x0=-0.000128)
+    return n, time.perf_counter() - t0, (o.Qc.copy(), o.Qf.copy())
+
+
 def wedge_cpu(hf, target_s=12.0):
     """The oracle's step loop on the same wedge case: whole steps until about
     target_s has passed. Returns (steps, wall, state)."""
@@ -557,6 +606,8 @@ def main():
                     help="cells of the adjoint workload (200 fixed steps, checkpoints every 20; 0 skips it)")
     ap.add_argument("--wedge-steps", type=int, default=200,
                     help="Strang steps of the wedge case's 144 x 64 base level (0 skips it)")
+    ap.add_argument("--wedge-amr-steps", type=int, default=200,
+                    help="Strang steps of the wedge case with one adaptive refined box (0 skips it)")
     ap.add_argument("--transfer-level", type=int, default=2048,
                     help="coarse level width of the level-transfer leg (height half of it; 0 skips it)")
     ap.add_argument("--composite-stages", type=int, default=50,
@@ -692,6 +743,21 @@ def main():
                  "ms_per_step": wwall / args.wedge_steps * 1e3, "simulated_time_s": wdrv.time,
                  "first_order_stages": wdrv.first_order_stages, "retries": wdrv.retries, "repairs": wdrv.repairs}
         log(f"wedge case: {wedge['steps_per_s']:.4g} steps/s")
+
+    wamr = None
+    if args.wedge_amr_steps:
+        log(f"wedge case with one adaptive refined box: {args.wedge_amr_steps} Strang steps")
+        wawall, wadrv, _ = wedge_amr_run(torch, hf, args.wedge_amr_steps, 1)
+        wamr = {"workload": "the wedge case (144 x 64 base level, cases/teng_case1) with one refined box at ratio 2, "
+                            "starting over the wedge tip and regridded every "
+                            f"{REGRID_INTERVAL} steps from ErrorEst's tags at the case's criteria (p jump 0.08, T "
+                            "jump 0.06, fuel rate 50 /s; 2-cell buffer, blocks of 4); inflow and wedge bottom on "
+                            "both levels; CFL 0.6; Strang chemistry (batched FP64 form, rtol 1e-8 atol 1e-12)",
+                "steps": args.wedge_amr_steps, "steps_per_s": args.wedge_amr_steps / wawall,
+                "ms_per_step": wawall / args.wedge_amr_steps * 1e3, "simulated_time_s": wadrv.time,
+                "fine_box_end": list(wadrv.fine_box), "regrids": wadrv.regrids, "retries": wadrv.retries,
+                "first_order_stages": wadrv.first_order_stages}
+        log(f"adaptive wedge case: {wamr['steps_per_s']:.4g} steps/s, box {wadrv.fine_box}")
 
     xfer = None
     if args.transfer_level:
@@ -836,6 +902,8 @@ def main():
         line["channel_2d"] = chan
     if wedge is not None:
         line["wedge_2d"] = wedge
+    if wamr is not None:
+        line["wedge_2level"] = wamr
     if xfer is not None:
         line["level_transfers"] = xfer
     if comp is not None:
@@ -854,6 +922,17 @@ def main():
                 "sample": f"the first {nc} steps of the same run, the oracle's step loop, {cw:.1f} s"}
             line["wedge_2d"]["parity_vs_oracle"] = {
                 "steps": nc, **state_difference(gstate, cstate, hf.Mechanism(hf.H2_LI).ns)}
+        if wamr is not None:
+            log("adaptive wedge case: CPU oracle step loop")
+            nc, cw, (oc, of) = wedge_amr_cpu(hf)
+            _, _, (gc, gf) = wedge_amr_run(torch, hf, nc, 0)
+            nsx = hf.Mechanism(hf.H2_LI).ns
+            line["wedge_2level"]["cpu_baseline"] = {
+                "value": nc / cw, "unit": "Strang steps/s", "cores": len(os.sched_getaffinity(0)), "kind": "port",
+                "sample": f"the first {nc} steps of the same run, the oracle's two-level step loop, {cw:.1f} s"}
+            line["wedge_2level"]["parity_vs_oracle"] = {
+                "steps": nc, "coarse": state_difference(gc, oc, nsx),
+                "fine": state_difference(gf, of, nsx) if gf.shape == of.shape else "boxes differ"}
         if two is not None:
             log("two-level channel: CPU oracle step loop")
             nc, cw, (oc, of) = two_level_cpu(hf)
