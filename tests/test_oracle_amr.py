@@ -410,3 +410,17 @@ def test_two_level_box_at_a_wall_conserves(oracle_h2):
     d.step()
     after = np.array([d.Qc[:ns].sum(), d.Qc[ns + 2].sum()])
     assert (np.abs(after - before) / np.abs(before)).max() < 1e-13
+
+
+def test_box_choice_matches_the_oracle_restatement():
+    """hyperflow_amd.amr.choose_fine_box against oracle.driver.fine_box_for_tags
+    on random tag patterns, buffers, block sizes and per-side margins."""
+    from hyperflow_amd.amr import choose_fine_box
+    from oracle.driver import fine_box_for_tags
+    rng = np.random.default_rng(13)
+    for _ in range(300):
+        nx, ny = rng.integers(8, 60, size=2)
+        t = rng.random((ny, nx)) < rng.choice([0.0, 0.002, 0.02, 0.2])
+        buf, blk = int(rng.integers(0, 4)), int(rng.choice([1, 2, 4, 8]))
+        m = tuple(int(v) for v in rng.integers(0, 3, size=4))
+        assert choose_fine_box(t, buf, blk, nx, ny, m) == fine_box_for_tags(t, buf, blk, nx, ny, m)
