@@ -23,7 +23,8 @@ ERROR_NAMES = {0: "HF_OK", 1: "HF_EINVAL", 2: "HF_EDOMAIN", 3: "HF_ECONV", 4: "H
 
 class hf_options(Structure):
     _fields_ = [("rtol", c_double), ("atol", c_double), ("max_steps", c_int), ("precision", c_int),
-                ("device", c_int), ("waves_per_cu", c_int), ("chunk", c_int), ("nbins", c_int), ("eager", c_int)]
+                ("device", c_int), ("waves_per_cu", c_int), ("chunk", c_int), ("nbins", c_int), ("eager", c_int),
+                ("jac_wait", c_int), ("setup_wait", c_int)]
 
 
 class hf_chem_stats(Structure):

@@ -44,9 +44,10 @@ class Mechanism:
     """A loaded mechanism bound to one HIP device (owns an hf_ctx)."""
 
     def __init__(self, path, rtol=1e-8, atol=1e-15, max_steps=5000, precision=HF_PREC_FP64, device=0,
-                 waves_per_cu=0, chunk=0, nbins=0, eager=False):
+                 waves_per_cu=0, chunk=0, nbins=0, eager=False, jac_wait=0, setup_wait=0):
         L = load()
-        opt = hf_options(rtol, atol, max_steps, precision, device, waves_per_cu, chunk, nbins, int(bool(eager)))
+        opt = hf_options(rtol, atol, max_steps, precision, device, waves_per_cu, chunk, nbins, int(bool(eager)),
+                         jac_wait, setup_wait)
         h = c_void_p()
         check(None, L.hf_create(str(path).encode(), byref(opt), byref(h)))
         self._h = h
@@ -126,6 +127,26 @@ class Mechanism:
         return dt.value
 
 
+def _interval(dt, n):
+    """(array, per_cell) for hf_chem_advance: a Python number, a 0-d or
+    one-element array/tensor is one interval for every cell; otherwise one
+    float64 value per cell is required, at least n of them."""
+    if np.isscalar(dt):
+        return np.array([float(dt)]), 0
+    size = dt.numel() if hasattr(dt, "numel") else np.size(dt)
+    if size == 1:
+        if isinstance(dt, np.ndarray):
+            return np.array([float(dt.reshape(-1)[0])]), 0
+        return dt.reshape(1), 0
+    if dt.ndim != 1 or size < n:
+        raise ValueError(f"per-cell intervals: need a 1-D array of at least {n} values, got shape {tuple(dt.shape)}")
+    if isinstance(dt, np.ndarray) and not dt.flags.c_contiguous:
+        raise ValueError("per-cell intervals must be contiguous")
+    if hasattr(dt, "is_contiguous") and not dt.is_contiguous():
+        raise ValueError("per-cell intervals must be contiguous")
+    return dt, 1
+
+
 class ChemistryBatch:
     """ChemistryBatch of HyperFlow_V3/src/chemistry_batch.hpp:8-17 on one GPU.
 
@@ -157,12 +178,7 @@ class ChemistryBatch:
         if n > field.shape[1] or n < 0:
             raise ValueError("invalid chemistry batch extent or interval")
         ptr, ld = _ptr(field)
-        if np.isscalar(dt):
-            dta = np.array([float(dt)])
-            per = 0
-        else:
-            dta = dt
-            per = 1
+        dta, per = _interval(dt, n)
         dptr = _ptr(dta)[0]
         st = hf_chem_stats()
         ff = c_int64(-1)

@@ -60,8 +60,7 @@ bool is_device_ptr(const void* ptr) {
 // Online tuning of the batched form's refill group (hf_options.chunk = 0):
 // the first calls of a size class (cell counts within a factor of two) try
 // each candidate once and the fastest per cell is kept for the following
-// calls. Results depend on the choice only within the integration tolerance
-// (the wave-synchronised Jacobian refresh groups lanes differently).
+// calls. Each cell's result is independent of the choice.
 struct ChemTuner {
     static constexpr int kN = 3;
     static constexpr int kCand[kN] = {8, 4, 16};
@@ -72,6 +71,8 @@ struct ChemTuner {
         if (size == 0 || ncell > 2 * size || 2 * ncell < size) {
             size = ncell;
             calls = 0;
+            best = 0;
+            for (double& t : per_cell) t = 0;
         }
         return kCand[calls < kN ? calls : best];
     }
@@ -289,7 +290,8 @@ int chem_graph_launch(hf_ctx* c, ChemLaunch L, float* ms, int* how) {
         (uint64_t)(uintptr_t)L.dt, (uint64_t)L.dt_per_cell, (uint64_t)(uintptr_t)L.out, (uint64_t)L.ldo,
         (uint64_t)(uintptr_t)L.status, (uint64_t)(uintptr_t)L.first_failed, (uint64_t)(uintptr_t)L.counters,
         bits(L.rtol), bits(L.atol), (uint64_t)L.maxsteps, (uint64_t)L.mode, (uint64_t)L.precision,
-        (uint64_t)L.num_cus, (uint64_t)L.waves_per_cu, (uint64_t)L.chunk, (uint64_t)L.nbins,
+        (uint64_t)L.num_cus, (uint64_t)L.waves_per_cu, (uint64_t)L.chunk, (uint64_t)L.nbins, (uint64_t)L.jac_wait,
+        (uint64_t)L.setup_wait,
         (uint64_t)(uintptr_t)L.workspace, (uint64_t)L.workspace_bytes, (uint64_t)(uintptr_t)L.commit,
         (uint64_t)L.commit_ld};
     ChemGraph& g = c->graph;
@@ -389,6 +391,8 @@ int hf_chem_advance(hf_ctx* c, double* U, int64_t ncell, int64_t ld, const doubl
     const bool tune = mode == HF_CHEM_BATCHED && c->opt.chunk == 0;
     L.chunk = tune ? c->tuner.pick(ncell) : c->opt.chunk;
     L.nbins = c->opt.nbins;
+    L.jac_wait = c->opt.jac_wait;
+    L.setup_wait = c->opt.setup_wait;
     const size_t wbytes = c->ops->chem_workspace(ncell, c->opt.nbins, c->num_cus, c->opt.waves_per_cu);
     if (wbytes) {
         if (!c->work.ensure(wbytes)) return fail(c, HF_ENOMEM, "device workspace");

@@ -15,10 +15,13 @@
 // reloaded to re-form I - gamma J when gamma moves. GPU-specific choices: J is
 // analytic (not a difference quotient), it and the factored Newton matrix
 // are kept lane-contiguously in global memory (so they occupy registers only
-// while they are formed or applied, not across rate evaluations), the batched
-// form refreshes J for a whole wave at once, and I - gamma J is factored
-// without pivoting (a zero or non-finite pivot is a recoverable failure that
-// reduces the step, where I - gamma J tends to I).
+// while they are formed or applied, not across rate evaluations), and
+// I - gamma J is factored without pivoting (a zero or non-finite pivot is a
+// recoverable failure that reduces the step, where I - gamma J tends to I).
+// Every decision of a lane depends on its own cell only: the batched form may
+// hold a lane back for some trips so that Jacobian evaluations of several
+// lanes fall into one trip (pre() / wants_jac()), which changes when a trip
+// runs, never what it computes, so results do not depend on the grouping.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -49,7 +52,6 @@ struct Bdf {
     double rtol, atol;
     int q, qprime, L, qwait, nst, nstlp, nstlj;
     bool force_setup, jfresh;
-    bool sync_jac = false;      // refresh Jacobians wave-wide (batched form)
     // Saved Jacobian and factored Newton matrix: element k at [k * ZSTRIDE],
     // the callers point them at this lane's column of a per-wave
     // [element][64 lanes] tile, so the offsets are compile-time constants.
@@ -443,7 +445,7 @@ struct Bdf {
     // therefore shares one evaluation per trip instead of serialising each
     // lane's whole step. The algorithm per lane is CVODE's: the order of
     // evaluations and decisions is unchanged, only their grouping into calls.
-    enum { PH_F0 = 0, PH_HIN = 1, PH_START = 2, PH_NEWTON = 3, PH_ERR_RHS = 4 };
+    enum { PH_F0 = 0, PH_HIN = 1, PH_START = 2, PH_NEWTON = 3, PH_ERR_RHS = 4, PH_DEAD = 5 };
     int phase, m, convfail, ncf, nef, hin_count, hin_tries;
     bool retry, callsetup, jbad;
     double saved_t, delp, hg, hub, hlb;
@@ -694,18 +696,20 @@ struct Bdf {
         phase = PH_START;
     }
 
+    // The part of a trip before its rate evaluation that needs none: the
+    // start of a step attempt (predictor, coefficients, the setup and
+    // Jacobian decisions). After it, wants_jac() tells whether the coming
+    // trip evaluates a Jacobian. A lane may then sit out trips: its state
+    // does not change until it runs one.
+    __device__ __forceinline__ void pre() {
+        if (phase == PH_START && !start_attempt()) phase = PH_DEAD;
+    }
+    __device__ __forceinline__ bool wants_setup() const { return phase == PH_NEWTON && m == 0 && callsetup; }
+    __device__ __forceinline__ bool wants_jac() const { return wants_setup() && jbad; }
+
     __device__ __forceinline__ int trip(Sys& sys, double* yout) {
-        if (phase == PH_START && !start_attempt()) return BDF_FAIL;
-        // Wave-synchronised Jacobian refresh: when any lane starting a Newton
-        // solve needs a new Jacobian, every lane starting one takes it, so the
-        // expensive evaluation falls in few trips of the wave.
-        if (sync_jac) {
-            const bool at0 = phase == PH_NEWTON && m == 0;
-            if (__any(at0 && jbad) && at0) {
-                jbad = true;
-                callsetup = true;
-            }
-        }
+        pre();
+        if (phase == PH_DEAD) return BDF_FAIL;
         double yv[N], f[N];
         #pragma unroll
         for (int i = 0; i < N; ++i) {

@@ -83,16 +83,14 @@ struct CellRun {
     double y[N];
 
     // jsave/lu: this lane's saved-Jacobian and LU-factor slots (element k at
-    // [k * 64], a per-wave [element][lane] tile); sync_jac: wave-wide refresh.
-    __device__ __forceinline__ CellRun(const MechData<T>& d, double* zs, const ChemParams& p, MT* jsave, MT* lu,
-                                       bool sync_jac)
+    // [k * 64], a per-wave [element][lane] tile).
+    __device__ __forceinline__ CellRun(const MechData<T>& d, double* zs, const ChemParams& p, MT* jsave, MT* lu)
         : sys{d, 1.0, 0.0, 1000.0, nullptr} {
         bdf.zs = zs;
         bdf.rtol = p.rtol;
         bdf.atol = p.atol;
         bdf.jsave = jsave;
         bdf.lu = lu;
-        bdf.sync_jac = sync_jac;
     }
 
     // HF_OK when the integration is running, else the cell's error code.

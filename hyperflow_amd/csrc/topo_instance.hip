@@ -264,7 +264,7 @@ __global__ void __launch_bounds__(64)
 chem_lane_kernel(const MechData<TT>* __restrict__ dm, const double* __restrict__ U, long ncell, long ld,
                  const double* __restrict__ dt, int per_cell, double* __restrict__ out, long ldo,
                  int8_t* __restrict__ status, unsigned long long* __restrict__ first_failed, ChemParams p,
-                 unsigned long long* __restrict__ counters, Workspace w, int refill) {
+                 unsigned long long* __restrict__ counters, Workspace w, int refill, int jwait, int swait) {
     constexpr int N = TT::NS - 1;
     double* zbase;
     if constexpr (kSlabLds) {
@@ -276,7 +276,7 @@ chem_lane_kernel(const MechData<TT>* __restrict__ dm, const double* __restrict__
     const int lane = threadIdx.x;
     const long tile = (long)blockIdx.x * N * N * 64;
     CellRun<TT, MT, kSlabLds ? 3 : 1> run(*dm, zbase + lane, p, static_cast<MT*>(w.jsave) + tile + lane,
-                                        static_cast<MT*>(w.lu) + tile + lane, !PLAIN);
+                                        static_cast<MT*>(w.lu) + tile + lane);
     BdfCounters tot;
     int max_steps_cell = 0;
     long cell = -1;
@@ -286,6 +286,15 @@ chem_lane_kernel(const MechData<TT>* __restrict__ dm, const double* __restrict__
     // idle), so the cost of starting a cell (initial rate and step-size
     // estimate) is paid by many lanes in the same trip, not one at a time.
     bool exhausted = PLAIN;
+    // Setup grouping (wave-uniform): a lane whose next trip evaluates a
+    // Jacobian waits until a quarter of the active lanes want one too, every
+    // active lane does, or jwait trips have passed since the last trip that
+    // evaluated Jacobians; a lane whose next trip only re-forms and factors
+    // its Newton matrix does the same with swait (and goes along with any
+    // Jacobian trip). Waiting shifts a lane's trips in time and changes none
+    // of its arithmetic, so each cell's result is independent of its
+    // wave-mates. A wait of 0 never holds a lane back.
+    int since_jac = jwait, since_setup = swait;
     if constexpr (PLAIN) {
         cell = (long)blockIdx.x * 64 + lane;
         if (cell < ncell) {
@@ -326,7 +335,32 @@ chem_lane_kernel(const MechData<TT>* __restrict__ dm, const double* __restrict__
             if (exhausted) break;
             continue;
         }
-        if (active) {
+        bool go = active;
+        if (!PLAIN && (jwait > 0 || swait > 0)) {
+            bool wj = false, ws = false;
+            if (active) {
+                run.bdf.pre();
+                ws = run.bdf.wants_setup();
+                wj = ws && run.bdf.wants_jac();
+            }
+            const unsigned long long wmask = __ballot(wj), smask = __ballot(ws), amask = __ballot(active);
+            const int na = __popcll(amask);
+            bool jtrip = true;
+            if (jwait > 0 && wmask) {
+                jtrip = since_jac >= jwait || 4 * __popcll(wmask) >= na || wmask == amask;
+                if (!jtrip) go = go && !wj;
+            }
+            if (wmask && jtrip) since_jac = 0;
+            if (swait > 0 && smask) {
+                const bool strip = (wmask && jtrip) || since_setup >= swait || 4 * __popcll(smask) >= na ||
+                                   smask == amask;
+                if (strip) since_setup = 0;
+                else go = go && !ws;
+            }
+            ++since_jac;
+            ++since_setup;
+        }
+        if (go) {
             const int r = run.step();
             if (r != BDF_RUNNING) {
                 int st = r == BDF_OK ? run.finish(U, cell, ld, out, ldo) : HF_ECONV;
@@ -467,6 +501,8 @@ int chem(const ChemLaunch& L) {
             const int wpc = L.waves_per_cu > 0 ? L.waves_per_cu : 4;
             // queue chunking: idle lanes refilled together
             const int chunk = L.chunk > 0 ? min(L.chunk, 64) : 8;
+            const int jwait = L.jac_wait > 0 ? L.jac_wait : (L.jac_wait == 0 ? 16 : 0);
+            const int swait = L.setup_wait > 0 ? L.setup_wait : (L.setup_wait == 0 ? 2 : 0);
             const unsigned g256 = (unsigned)((L.ncell + 255) / 256);
             hipLaunchKernelGGL(chem_init_kernel, dim3(1), dim3(256), 0, L.stream, w, L.first_failed, L.counters);
             hipLaunchKernelGGL(chem_estimate_kernel<Topo>, dim3(g256), dim3(256), 0, L.stream, dm, L.U,
@@ -492,11 +528,11 @@ int chem(const ChemLaunch& L) {
                 if (L.precision == HF_PREC_MIXED)
                     hipLaunchKernelGGL((chem_lane_kernel<Topo, float, false>), dim3((unsigned)(L.num_cus * wpc)),
                                        dim3(64), 0, L.stream, dm, L.U, (long)L.ncell, (long)L.ld, L.dt, L.dt_per_cell,
-                                       L.out, (long)L.ldo, L.status, L.first_failed, p, L.counters, w, chunk);
+                                       L.out, (long)L.ldo, L.status, L.first_failed, p, L.counters, w, chunk, jwait, swait);
                 else
                     hipLaunchKernelGGL((chem_lane_kernel<Topo, double, false>), dim3((unsigned)(L.num_cus * wpc)),
                                        dim3(64), 0, L.stream, dm, L.U, (long)L.ncell, (long)L.ld, L.dt, L.dt_per_cell,
-                                       L.out, (long)L.ldo, L.status, L.first_failed, p, L.counters, w, chunk);
+                                       L.out, (long)L.ldo, L.status, L.first_failed, p, L.counters, w, chunk, jwait, swait);
                 if (hipGetLastError() != hipSuccess) return HF_EDEVICE;
                 timer.stop();
                 return chem_commit(L);
@@ -509,7 +545,7 @@ int chem(const ChemLaunch& L) {
         const unsigned grid = (unsigned)((L.ncell + 63) / 64);
         hipLaunchKernelGGL((chem_lane_kernel<Topo, double, true>), dim3(grid), dim3(64), 0, L.stream, dm, L.U,
                            (long)L.ncell, (long)L.ld, L.dt, L.dt_per_cell, L.out, (long)L.ldo, L.status,
-                           L.first_failed, p, L.counters, none, 64);
+                           L.first_failed, p, L.counters, none, 64, 0, 0);
         if (hipGetLastError() != hipSuccess) return HF_EDEVICE;
         timer.stop();
         return chem_commit(L);

@@ -29,6 +29,8 @@ struct ChemLaunch {
     int precision;
     // batched form
     int num_cus, waves_per_cu, chunk, nbins;
+    int jac_wait;                   // lane form: trips a Jacobian request may wait for others (0: none)
+    int setup_wait;                 // lane form: the same for a Newton-matrix setup without a Jacobian
     void* workspace;                // device scratch for the batched form
     size_t workspace_bytes;
     hipStream_t stream;

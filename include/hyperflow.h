@@ -72,6 +72,15 @@ typedef struct hf_options {
      * replayed while calls repeat their arguments and buffers; 1: launch the
      * kernels one by one */
     int eager;
+    /* batched lane form: a lane whose next trip evaluates a Jacobian may wait
+     * up to this many trips for other lanes of its wave to need one too, so
+     * that the evaluations fall into few trips; waiting never changes a
+     * cell's arithmetic. 0 selects the default (16), a negative value no
+     * waiting. */
+    int jac_wait;
+    /* the same for a lane whose next trip re-forms and factors its Newton
+     * matrix from its saved Jacobian (0: default 2; negative: no waiting) */
+    int setup_wait;
 } hf_options;
 
 typedef struct hf_chem_stats {

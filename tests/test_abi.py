@@ -149,3 +149,27 @@ def test_level_transfer_entry_points_reject_bad_arguments(mech_h2):
     assert L.hf_tag_cells(h, None, byref(box), byref(prm), tags, byref(ff), byref(ff)) == _lib.HF_EINVAL
     assert "fuel" in L.hf_last_error(h).decode()
     assert L.hf_flux_stage_fluxes(h, byref(box), 1, None, None, 0, None, None, None) == _lib.HF_EINVAL
+
+
+def test_chemistry_interval_argument_checks():
+    """One interval for every cell from a number, a 0-d or a one-element
+    array or tensor; per-cell intervals must be 1-D, contiguous and hold at
+    least as many values as cells (a shorter buffer would be read past its
+    end by the library)."""
+    import torch
+
+    from hyperflow_amd.chemistry import _interval
+    a, per = _interval(2e-7, 10)
+    assert per == 0 and a.shape == (1,) and a[0] == 2e-7
+    a, per = _interval(np.array(3e-7), 10)
+    assert per == 0 and a[0] == 3e-7
+    a, per = _interval(np.array([4e-7]), 10)
+    assert per == 0 and a[0] == 4e-7
+    a, per = _interval(torch.tensor(5e-7, dtype=torch.float64), 10)
+    assert per == 0 and a.numel() == 1 and float(a[0]) == 5e-7
+    a, per = _interval(np.full(10, 1e-7), 10)
+    assert per == 1
+    for bad in (np.full(9, 1e-7), torch.full((9,), 1e-7, dtype=torch.float64), np.full((2, 5), 1e-7),
+                np.full(20, 1e-7)[::2]):
+        with pytest.raises(ValueError):
+            _interval(bad, 10)
