@@ -85,6 +85,8 @@ _SIGNATURES = {
                               POINTER(hf_flux_stats), POINTER(c_int64)]),
     "hf_flux_stage_fluxes": (c_int, [c_void_p, POINTER(hf_box), c_int, c_void_p, c_void_p, c_int,
                                      POINTER(hf_face_fluxes), POINTER(hf_flux_stats), POINTER(c_int64)]),
+    "hf_flux_stage_noalloc": (c_int, [c_void_p, POINTER(hf_box), c_int, c_void_p, c_void_p, c_int,
+                                      POINTER(hf_face_fluxes), POINTER(hf_flux_stats), POINTER(c_int64)]),
     "hf_reflux": (c_int, [c_void_p, c_void_p, POINTER(hf_patch), POINTER(hf_face_fluxes), POINTER(hf_face_fluxes),
                           POINTER(c_int32), c_int32, c_double, c_double, c_double]),
     "hf_properties": (c_int, [c_void_p, c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_void_p,

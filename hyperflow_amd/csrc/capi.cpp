@@ -18,9 +18,40 @@
 
 using namespace hf;
 
+// Scratch of hf_flux_stage_noalloc: the failure word and counters, the box
+// table, the face-flux descriptors and the tile table of one call, carved at
+// fixed offsets. Every GPU the library runs on holds one copy (1.4 MB) until
+// the process ends; calls that overlap on one GPU share it and race.
+constexpr int kNoAllocMaxBoxes = 4096;
+constexpr int kNoAllocMaxTiles = 65536;
+constexpr size_t kNoAllocBoxOff = 256;
+constexpr size_t kNoAllocFluxOff = kNoAllocBoxOff + sizeof(hf_box) * kNoAllocMaxBoxes;
+constexpr size_t kNoAllocTileOff = kNoAllocFluxOff + sizeof(hf_face_fluxes) * kNoAllocMaxBoxes;
+constexpr size_t kNoAllocBytes = kNoAllocTileOff + sizeof(int32_t) * 3 * kNoAllocMaxTiles;
+static_assert(kNoAllocFluxOff % 256 == 0 && kNoAllocTileOff % 256 == 0, "scratch regions stay 256-byte aligned");
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
 namespace {
 
 thread_local std::string g_create_error;
+
+// Device address of `scratch` per device, taken once with that device
+// current (each device holds its own copy at its own address). Host code
+// uses these addresses, never the array's name.
+constexpr int kMaxDevices = 64;
+std::atomic<void*> g_scratch_ptr[kMaxDevices];
+
+int scratch_setup(int device) {
+    if (device < 0 || device >= kMaxDevices) return HF_EDEVICE;
+    if (g_scratch_ptr[device].load()) return HF_OK;
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+        (void)hipGetLastError();
+        return HF_EDEVICE;
+    }
+    g_scratch_ptr[device].store(p);
+    return HF_OK;
+}
 
 struct DevBuf {
     void* p = nullptr;
@@ -160,6 +191,8 @@ int ensure_device(hf_ctx* c) {
         return fail(c, HF_EDEVICE, "cannot upload mechanism tables");
     }
     c->d_mech = d;
+    // the static scratch of the no-allocation entry point, once per device
+    if (scratch_setup(c->opt.device) != HF_OK) return fail(c, HF_EDEVICE, "hipGetSymbolAddress(scratch) failed");
     return HF_OK;
 }
 
@@ -230,8 +263,19 @@ int hf_create(const char* mechanism_yaml, const hf_options* opt, hf_ctx** out) {
         g_create_error = "invalid device ordinal";
         return HF_EINVAL;
     }
-    // The device is bound on the first call that needs it (ensure_device), so
-    // host-only set-up helpers work on a machine without a GPU.
+    // The device is bound here when it exists (stream, mechanism tables, the
+    // no-allocation scratch address), else on the first call that needs it,
+    // so host-only set-up helpers work on a machine without a GPU.
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) == hipSuccess && ndev > c->opt.device) {
+        if (ensure_device(c.get()) != HF_OK) {
+            g_create_error = c->err;
+            hf_destroy(c.release());
+            return HF_EDEVICE;
+        }
+    } else {
+        (void)hipGetLastError();
+    }
     *out = c.release();
     return HF_OK;
 }
@@ -454,8 +498,10 @@ int hf_chem_advance(hf_ctx* c, double* U, int64_t ncell, int64_t ld, const doubl
 }
 
 namespace {
+// noalloc: the tables and counters live in the device's static scratch (no
+// device allocation, no timing events; device-resident arrays only).
 int flux_stage_impl(hf_ctx* c, const hf_box* boxes, int nbox, const double* Uin, double* Uout, int first_order,
-                    const hf_face_fluxes* fluxes, hf_flux_stats* stats, int64_t* first_failed) {
+                    const hf_face_fluxes* fluxes, hf_flux_stats* stats, int64_t* first_failed, bool noalloc = false) {
     if (!c) return HF_EINVAL;
     BusyGuard guard(c);
     if (!guard.ok) return fail(c, HF_EBUSY, "concurrent flux stage is unsupported");
@@ -463,6 +509,18 @@ int flux_stage_impl(hf_ctx* c, const hf_box* boxes, int nbox, const double* Uin,
     if (stats) std::memset(stats, 0, sizeof(*stats));
     if (nbox < 0 || (nbox > 0 && (!boxes || !Uin || !Uout))) return fail(c, HF_EINVAL, "invalid flux-stage arguments");
     if (nbox == 0) return HF_OK;
+    char* fixed = nullptr;
+    if (noalloc) {
+        // the device was bound when the context was created: binding it now would allocate
+        if (!c->d_mech) return fail(c, HF_EDEVICE, "no device was bound when the context was created");
+        if (hipSetDevice(c->opt.device) != hipSuccess) return fail(c, HF_EDEVICE, "hipSetDevice failed");
+        fixed = static_cast<char*>(g_scratch_ptr[c->opt.device].load());
+        if (!fixed) return fail(c, HF_EDEVICE, "the static scratch has no device address");
+        if (nbox > kNoAllocMaxBoxes)
+            return fail(c, HF_EINVAL, "more than " + std::to_string(kNoAllocMaxBoxes) + " boxes for the no-allocation stage");
+        if (!is_device_ptr(Uin) || !is_device_ptr(Uout))
+            return fail(c, HF_EINVAL, "the no-allocation stage needs device arrays");
+    }
     HF_DEVICE(c);
     const int nv = c->mech.ns() + 3;
     // all boxes 1-D (x only) or all 2-D
@@ -489,19 +547,33 @@ int flux_stage_impl(hf_ctx* c, const hf_box* boxes, int nbox, const double* Uin,
                 tiles.push_back(tj);
             }
     const int ntiles = (int)(tiles.size() / 3);
+    if (noalloc && ntiles > kNoAllocMaxTiles)
+        return fail(c, HF_EINVAL, std::to_string(ntiles) + " tiles exceed the no-allocation limit of " +
+                                      std::to_string(kNoAllocMaxTiles));
     if (fluxes)
         for (int b = 0; b < nbox; ++b)
             if ((fluxes[b].fx && !is_device_ptr(fluxes[b].fx)) || (fluxes[b].fy && !is_device_ptr(fluxes[b].fy)))
                 return fail(c, HF_EINVAL, "face fluxes need device memory");
     const size_t flux_table = fluxes ? sizeof(hf_face_fluxes) * nbox : 0;
-    if (!c->boxes.ensure(sizeof(hf_box) * nbox + flux_table) || !c->tiles.ensure(sizeof(int32_t) * tiles.size()) ||
-        !c->scal.ensure(sizeof(unsigned long long) * 16))
-        return fail(c, HF_ENOMEM, "device scratch");
-    HF_HIP(c, hipMemcpyAsync(c->boxes.p, boxes, sizeof(hf_box) * nbox, hipMemcpyHostToDevice, c->stream));
-    if (fluxes)
-        HF_HIP(c, hipMemcpyAsync(static_cast<char*>(c->boxes.p) + sizeof(hf_box) * nbox, fluxes, flux_table,
-                                 hipMemcpyHostToDevice, c->stream));
-    HF_HIP(c, hipMemcpyAsync(c->tiles.p, tiles.data(), sizeof(int32_t) * tiles.size(), hipMemcpyHostToDevice, c->stream));
+    char* box_dev;
+    int32_t* tile_dev;
+    unsigned long long* scal;
+    if (noalloc) {
+        scal = reinterpret_cast<unsigned long long*>(fixed);
+        box_dev = fixed + kNoAllocBoxOff;
+        tile_dev = reinterpret_cast<int32_t*>(fixed + kNoAllocTileOff);
+    } else {
+        if (!c->boxes.ensure(sizeof(hf_box) * nbox + flux_table) || !c->tiles.ensure(sizeof(int32_t) * tiles.size()) ||
+            !c->scal.ensure(sizeof(unsigned long long) * 16))
+            return fail(c, HF_ENOMEM, "device scratch");
+        scal = c->scal.as<unsigned long long>();
+        box_dev = static_cast<char*>(c->boxes.p);
+        tile_dev = c->tiles.as<int32_t>();
+    }
+    char* flux_dev = noalloc ? fixed + kNoAllocFluxOff : box_dev + sizeof(hf_box) * nbox;
+    HF_HIP(c, hipMemcpyAsync(box_dev, boxes, sizeof(hf_box) * nbox, hipMemcpyHostToDevice, c->stream));
+    if (fluxes) HF_HIP(c, hipMemcpyAsync(flux_dev, fluxes, flux_table, hipMemcpyHostToDevice, c->stream));
+    HF_HIP(c, hipMemcpyAsync(tile_dev, tiles.data(), sizeof(int32_t) * tiles.size(), hipMemcpyHostToDevice, c->stream));
     const bool dev_in = is_device_ptr(Uin), dev_out = is_device_ptr(Uout);
     const double* din = Uin;
     double* dout = Uout;
@@ -515,14 +587,15 @@ int flux_stage_impl(hf_ctx* c, const hf_box* boxes, int nbox, const double* Uin,
         HF_HIP(c, hipMemcpyAsync(c->out.p, Uout, sizeof(double) * out_extent, hipMemcpyHostToDevice, c->stream));
         dout = c->out.as<double>();
     }
-    unsigned long long* scal = c->scal.as<unsigned long long>();
+    // the failure word and counters start fresh on every call (the static
+    // scratch keeps the previous call's values)
     unsigned long long init[3] = {~0ull, 0, 0};
     HF_HIP(c, hipMemcpyAsync(scal, init, sizeof(init), hipMemcpyHostToDevice, c->stream));
     FluxLaunch L{};
     L.mech = c->d_mech;
-    L.boxes = c->boxes.as<hf_box>();
+    L.boxes = reinterpret_cast<const hf_box*>(box_dev);
     L.nbox = nbox;
-    L.tiles = c->tiles.as<int32_t>();
+    L.tiles = tile_dev;
     L.ntiles = ntiles;
     L.Uin = din;
     L.Uout = dout;
@@ -530,11 +603,10 @@ int flux_stage_impl(hf_ctx* c, const hf_box* boxes, int nbox, const double* Uin,
     L.one_d = one_d ? 1 : 0;
     L.first_failed = scal;
     L.counters = scal + 1;
-    L.fluxes = fluxes ? reinterpret_cast<const hf_face_fluxes*>(static_cast<char*>(c->boxes.p) + sizeof(hf_box) * nbox)
-                      : nullptr;
+    L.fluxes = fluxes ? reinterpret_cast<const hf_face_fluxes*>(flux_dev) : nullptr;
     L.stream = c->stream;
     float ms = 0;
-    L.kernel_ms = &ms;
+    L.kernel_ms = noalloc ? nullptr : &ms;
     const int r = c->ops->flux(L);
     if (r != HF_OK) return fail(c, r, "flux kernel launch failed");
     unsigned long long hs[3];
@@ -566,6 +638,11 @@ int hf_flux_stage_fluxes(hf_ctx* c, const hf_box* boxes, int nbox, const double*
                          const hf_face_fluxes* fluxes, hf_flux_stats* stats, int64_t* first_failed) {
     if (c && nbox > 0 && !fluxes) return fail(c, HF_EINVAL, "missing face-flux descriptors");
     return flux_stage_impl(c, boxes, nbox, Uin, Uout, first_order, fluxes, stats, first_failed);
+}
+
+int hf_flux_stage_noalloc(hf_ctx* c, const hf_box* boxes, int nbox, const double* Uin, double* Uout, int first_order,
+                          const hf_face_fluxes* fluxes, hf_flux_stats* stats, int64_t* first_failed) {
+    return flux_stage_impl(c, boxes, nbox, Uin, Uout, first_order, fluxes, stats, first_failed, true);
 }
 
 static int props_common(hf_ctx* c, const double* U, int64_t ncell, int64_t ld, double* p, double* a, double* T,

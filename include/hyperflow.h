@@ -177,6 +177,22 @@ typedef struct hf_face_fluxes {
 int hf_flux_stage_fluxes(hf_ctx *ctx, const hf_box *boxes, int nbox, const double *Uin, double *Uout, int first_order,
                          const hf_face_fluxes *fluxes, hf_flux_stats *stats, int64_t *first_failed);
 
+/* hf_flux_stage_fluxes (fluxes may be NULL: then hf_flux_stage) that
+ * allocates nothing. No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). The box table, face-flux
+ * descriptors, tile table and counters live in a static device array of
+ * 1.4 MB that each GPU holds until the process ends; at most 4096 boxes and
+ * 65536 tiles per call (15 x 15 cells per 2-D tile, 255 per 1-D tile), else
+ * HF_EINVAL. Uin, Uout and the face fluxes must be device memory (HF_EINVAL
+ * otherwise), and the context's device must have been bound by hf_create
+ * (a GPU was visible then; HF_EDEVICE otherwise). stats->kernel_ms is 0 (no
+ * timing events are created). Every call on a GPU shares that GPU's array:
+ * calls that overlap on one GPU race. Results are bit-identical to
+ * hf_flux_stage_fluxes. */
+int hf_flux_stage_noalloc(hf_ctx *ctx, const hf_box *boxes, int nbox, const double *Uin, double *Uout,
+                          int first_order, const hf_face_fluxes *fluxes, hf_flux_stats *stats, int64_t *first_failed);
+
 /* properties(u) (physics.cpp:4-15) on SoA planes: p, a, T per cell; status
  * per cell (HF_OK / HF_EDOMAIN). Returns HF_EDOMAIN when any cell fails
  * (BlockAMR::validate, BlockAMR.cpp:286-298), first_failed = lowest index. */
