@@ -53,6 +53,11 @@ class hf_face_fluxes(Structure):
                 ("fy", c_void_p), ("fy_offset", c_int64), ("fy_pitch", c_int64), ("fy_plane", c_int64)]
 
 
+class hf_cells(Structure):
+    _fields_ = [("U", c_void_p), ("offset", c_int64), ("pitch", c_int64), ("plane", c_int64), ("nx", c_int32),
+                ("ny", c_int32), ("mask", c_void_p), ("mask_pitch", c_int64), ("dx", c_double), ("dy", c_double)]
+
+
 class hf_tag_params(Structure):
     _fields_ = [("pressure_tag", c_double), ("temperature_tag", c_double), ("fuel_rate_tag", c_double),
                 ("physics", c_int32), ("chemistry", c_int32), ("fuel", c_int32), ("has_box", c_int32),
@@ -85,6 +90,18 @@ _SIGNATURES = {
                               POINTER(hf_flux_stats), POINTER(c_int64)]),
     "hf_flux_stage_fluxes": (c_int, [c_void_p, POINTER(hf_box), c_int, c_void_p, c_void_p, c_int,
                                      POINTER(hf_face_fluxes), POINTER(hf_flux_stats), POINTER(c_int64)]),
+    "hf_chem_advance_cells": (c_int, [c_void_p, POINTER(hf_cells), c_int, c_void_p, c_int, POINTER(hf_chem_stats),
+                                      POINTER(c_int64)]),
+    "hf_validate_cells": (c_int, [c_void_p, POINTER(hf_cells), c_int, _dp, POINTER(c_int64)]),
+    "hf_repair_cells": (c_int, [c_void_p, POINTER(hf_cells), c_int, POINTER(c_int64), POINTER(c_int64)]),
+    "hf_interp_regions": (c_int, [c_void_p, c_void_p, POINTER(hf_patch), c_void_p, POINTER(hf_patch),
+                                  POINTER(c_int32), c_int, c_void_p, c_int64, POINTER(c_int32), POINTER(c_int64),
+                                  POINTER(c_int64)]),
+    "hf_average_down_masked": (c_int, [c_void_p, c_void_p, POINTER(hf_patch), c_void_p, POINTER(hf_patch), c_void_p,
+                                       c_int64, POINTER(c_int32)]),
+    "hf_reflux_masked": (c_int, [c_void_p, c_void_p, POINTER(hf_patch), POINTER(hf_face_fluxes),
+                                 POINTER(hf_face_fluxes), c_void_p, c_int64, POINTER(c_int32), c_double, c_double,
+                                 c_double]),
     "hf_flux_stage_noalloc": (c_int, [c_void_p, POINTER(hf_box), c_int, c_void_p, c_void_p, c_int,
                                       POINTER(hf_face_fluxes), POINTER(hf_flux_stats), POINTER(c_int64)]),
     "hf_reflux": (c_int, [c_void_p, c_void_p, POINTER(hf_patch), POINTER(hf_face_fluxes), POINTER(hf_face_fluxes),

@@ -76,6 +76,21 @@ struct DevBuf {
     T* as() const { return static_cast<T*>(p); }
 };
 
+// True when doubles [lo, hi] relative to p (element indices, inclusive) lie
+// inside the device allocation that holds p (hipMemGetAddressRange). A
+// pointer the runtime cannot place passes: host memory is staged and
+// checked by the copies themselves.
+bool within_allocation(const double* p, int64_t lo, int64_t hi) {
+    hipDeviceptr_t base = nullptr;
+    size_t size = 0;
+    if (hipMemGetAddressRange(&base, &size, const_cast<double*>(p)) != hipSuccess || !base) {
+        (void)hipGetLastError();
+        return true;
+    }
+    const intptr_t b = (intptr_t)base, a0 = (intptr_t)p + (intptr_t)lo * 8, a1 = (intptr_t)p + (intptr_t)hi * 8 + 8;
+    return a0 >= b && a1 <= b + (intptr_t)size;
+}
+
 bool is_device_ptr(const void* ptr) {
     if (!ptr) return false;
     hipPointerAttribute_t a;
@@ -148,7 +163,7 @@ struct hf_ctx {
     void* d_mech = nullptr;
     std::string err;
     std::atomic<bool> busy{false};
-    DevBuf U, out, dt, status, scal, work, boxes, tiles, aux[6];
+    DevBuf U, out, dt, status, scal, work, boxes, tiles, blocks, aux[6];
 };
 
 namespace {
@@ -337,7 +352,7 @@ int chem_graph_launch(hf_ctx* c, ChemLaunch L, float* ms, int* how) {
         (uint64_t)L.num_cus, (uint64_t)L.waves_per_cu, (uint64_t)L.chunk, (uint64_t)L.nbins, (uint64_t)L.jac_wait,
         (uint64_t)L.setup_wait,
         (uint64_t)(uintptr_t)L.workspace, (uint64_t)L.workspace_bytes, (uint64_t)(uintptr_t)L.commit,
-        (uint64_t)L.commit_ld};
+        (uint64_t)L.commit_ld, (uint64_t)(uintptr_t)L.blocks, (uint64_t)L.nblock};
     ChemGraph& g = c->graph;
     if (!g.t0 && (hipEventCreate(&g.t0) != hipSuccess || hipEventCreate(&g.t1) != hipSuccess)) return HF_EDEVICE;
     *how = 1;
@@ -369,14 +384,34 @@ int chem_graph_launch(hf_ctx* c, ChemLaunch L, float* ms, int* how) {
 
 }  // namespace
 
-int hf_chem_advance(hf_ctx* c, double* U, int64_t ncell, int64_t ld, const double* dt, int dt_per_cell, int mode,
-                    hf_chem_stats* stats, int64_t* first_failed) {
+namespace {
+// hf_chem_advance over flat planes (blocks null) or over the leaf cells of a
+// block table (hf_chem_advance_cells: ncell = the blocks' cell count, U and
+// ld unused, the cells addressed through the uploaded table).
+int chem_impl(hf_ctx* c, double* U, int64_t ncell, int64_t ld, const double* dt, int dt_per_cell, int mode,
+              hf_chem_stats* stats, int64_t* first_failed, const hf_cells* blocks, int nblock) {
     if (!c) return HF_EINVAL;
     BusyGuard guard(c);
     if (!guard.ok) return fail(c, HF_EBUSY, "nested or concurrent chemistry batch is unsupported");
     if (first_failed) *first_failed = -1;
     if (stats) std::memset(stats, 0, sizeof(*stats));
-    if (ncell < 0 || ld < ncell || !dt || (ncell > 0 && !U) || (mode != HF_CHEM_PLAIN && mode != HF_CHEM_BATCHED))
+    std::vector<int64_t> bstart;
+    if (blocks) {
+        // every block in device memory, one interval for the whole batch (BlockAMR::react)
+        bstart.assign(1, 0);
+        for (int b = 0; b < nblock; ++b) {
+            const hf_cells& B = blocks[b];
+            if (B.nx < 1 || B.ny < 1 || !B.U || B.pitch < B.nx || B.plane < B.pitch * B.ny || B.offset < 0 ||
+                (B.mask && B.mask_pitch < B.nx))
+                return fail(c, HF_EINVAL, "block " + std::to_string(b) + " has an invalid extent or stride");
+            bstart.push_back(bstart.back() + (int64_t)B.nx * B.ny);
+        }
+        ncell = ld = bstart.back();
+        U = nullptr;
+        if (dt_per_cell) return fail(c, HF_EINVAL, "a block batch takes one interval");
+    }
+    if (ncell < 0 || ld < ncell || !dt || (ncell > 0 && !U && !blocks) ||
+        (mode != HF_CHEM_PLAIN && mode != HF_CHEM_BATCHED))
         return fail(c, HF_EINVAL, "invalid chemistry batch extent or interval");
     const bool dev_dt = is_device_ptr(dt);
     if (!dt_per_cell) {
@@ -388,7 +423,17 @@ int hf_chem_advance(hf_ctx* c, double* U, int64_t ncell, int64_t ld, const doubl
     if (ncell == 0) return HF_OK;
     HF_DEVICE(c);
     const int ns = c->mech.ns(), nv = ns + 3;
-    const bool dev_U = is_device_ptr(U);
+    if (blocks) {
+        for (int b = 0; b < nblock; ++b)
+            if (!is_device_ptr(blocks[b].U) || (blocks[b].mask && !is_device_ptr(blocks[b].mask)))
+                return fail(c, HF_EINVAL, "block " + std::to_string(b) + " is not in device memory");
+        const size_t tb = sizeof(hf_cells) * nblock, sb = sizeof(int64_t) * (nblock + 1);
+        if (!c->blocks.ensure(tb + sb)) return fail(c, HF_ENOMEM, "device scratch");
+        HF_HIP(c, hipMemcpyAsync(c->blocks.p, blocks, tb, hipMemcpyHostToDevice, c->stream));
+        HF_HIP(c, hipMemcpyAsync(static_cast<char*>(c->blocks.p) + tb, bstart.data(), sb, hipMemcpyHostToDevice,
+                                 c->stream));
+    }
+    const bool dev_U = blocks || is_device_ptr(U);
     const double* dU = U;
     int64_t dld = ld;
     if (!dev_U) {
@@ -418,6 +463,11 @@ int hf_chem_advance(hf_ctx* c, double* U, int64_t ncell, int64_t ld, const doubl
     L.U = dU;
     L.ncell = ncell;
     L.ld = dld;
+    if (blocks) {
+        L.blocks = c->blocks.as<hf_cells>();
+        L.block_start = reinterpret_cast<const int64_t*>(c->blocks.as<char>() + sizeof(hf_cells) * nblock);
+        L.nblock = nblock;
+    }
     L.dt = ddt;
     L.dt_per_cell = dt_per_cell;
     L.out = c->out.as<double>();
@@ -449,7 +499,8 @@ int hf_chem_advance(hf_ctx* c, double* U, int64_t ncell, int64_t ld, const doubl
     if (dev_U) {
         // device-resident field: the commit is a kernel that runs only when
         // no cell failed, so the whole call is one capturable sequence
-        L.commit = U;
+        // (block form: non-null to enable it; the kernel writes through the block table)
+        L.commit = blocks ? blocks[0].U : U;
         L.commit_ld = ld;
     }
     int r, how = 0;
@@ -482,10 +533,19 @@ int hf_chem_advance(hf_ctx* c, double* U, int64_t ncell, int64_t ld, const doubl
         const int64_t k = (int64_t)host_scal[0];
         int8_t code = 0;
         HF_HIP(c, hipMemcpy(&code, c->status.as<int8_t>() + k, 1, hipMemcpyDeviceToHost));
-        if (first_failed) *first_failed = k;
         const char* what = code == HF_EINVAL ? "invalid chemistry interval"
                            : code == HF_EDOMAIN ? "invalid accepted species or inadmissible state"
                                                 : "integrator failed (CVode-equivalent error)";
+        if (blocks) {
+            // (block << 32) | (j*nx + i), the flux stage's convention
+            const int b = (int)(std::upper_bound(bstart.begin(), bstart.end(), k) - bstart.begin()) - 1;
+            const int64_t local = k - bstart[b];
+            if (first_failed) *first_failed = ((int64_t)b << 32) | local;
+            return fail(c, code ? code : HF_ECONV,
+                        "chemistry block " + std::to_string(b) + " cell (" + std::to_string(local % blocks[b].nx) +
+                            ", " + std::to_string(local / blocks[b].nx) + "): " + what);
+        }
+        if (first_failed) *first_failed = k;
         return fail(c, code ? code : HF_ECONV, "chemistry cell " + std::to_string(k) + ": " + what);
     }
     // commit the species planes (device-resident fields were committed by the kernels)
@@ -495,6 +555,23 @@ int hf_chem_advance(hf_ctx* c, double* U, int64_t ncell, int64_t ld, const doubl
         HF_HIP(c, hipStreamSynchronize(c->stream));
     }
     return HF_OK;
+}
+}  // namespace
+
+int hf_chem_advance(hf_ctx* c, double* U, int64_t ncell, int64_t ld, const double* dt, int dt_per_cell, int mode,
+                    hf_chem_stats* stats, int64_t* first_failed) {
+    return chem_impl(c, U, ncell, ld, dt, dt_per_cell, mode, stats, first_failed, nullptr, 0);
+}
+
+int hf_chem_advance_cells(hf_ctx* c, const hf_cells* blocks, int nblock, const double* dt, int mode,
+                          hf_chem_stats* stats, int64_t* first_failed) {
+    if (c && (nblock < 0 || (nblock > 0 && !blocks))) return fail(c, HF_EINVAL, "invalid block table");
+    if (c && nblock == 0) {
+        if (first_failed) *first_failed = -1;
+        if (stats) std::memset(stats, 0, sizeof(*stats));
+        return HF_OK;
+    }
+    return chem_impl(c, nullptr, 0, 0, dt, 0, mode, stats, first_failed, blocks, nblock);
 }
 
 namespace {
@@ -551,9 +628,27 @@ int flux_stage_impl(hf_ctx* c, const hf_box* boxes, int nbox, const double* Uin,
         return fail(c, HF_EINVAL, std::to_string(ntiles) + " tiles exceed the no-allocation limit of " +
                                       std::to_string(kNoAllocMaxTiles));
     if (fluxes)
-        for (int b = 0; b < nbox; ++b)
-            if ((fluxes[b].fx && !is_device_ptr(fluxes[b].fx)) || (fluxes[b].fy && !is_device_ptr(fluxes[b].fy)))
+        for (int b = 0; b < nbox; ++b) {
+            const hf_face_fluxes& F = fluxes[b];
+            const hf_box& B = boxes[b];
+            if ((F.fx && !is_device_ptr(F.fx)) || (F.fy && !is_device_ptr(F.fy)))
                 return fail(c, HF_EINVAL, "face fluxes need device memory");
+            // the faces the stage writes: x (0..nx, 0..ny-1), y (0..nx-1, 0..ny)
+            if ((F.fx && !within_allocation(F.fx, F.fx_offset,
+                                            F.fx_offset + (nv - 1) * F.fx_plane + (B.ny - 1) * F.fx_pitch + B.nx)) ||
+                (F.fy && !one_d &&
+                 !within_allocation(F.fy, F.fy_offset, F.fy_offset + (nv - 1) * F.fy_plane + B.ny * F.fy_pitch + B.nx - 1)))
+                return fail(c, HF_EINVAL, "face fluxes of box " + std::to_string(b) + " lie outside their allocation");
+        }
+    if (is_device_ptr(Uin) && !within_allocation(Uin, 0, in_extent - 1))
+        return fail(c, HF_EINVAL, "a box reads outside the input allocation");
+    if (is_device_ptr(Uout)) {
+        for (int b = 0; b < nbox; ++b)
+            if (!within_allocation(Uout, boxes[b].out_offset, boxes[b].out_offset + (nv - 1) * boxes[b].out_plane +
+                                                                   (boxes[b].ny - 1) * boxes[b].out_pitch +
+                                                                   boxes[b].nx - 1))
+                return fail(c, HF_EINVAL, "box " + std::to_string(b) + " writes outside the output allocation");
+    }
     const size_t flux_table = fluxes ? sizeof(hf_face_fluxes) * nbox : 0;
     char* box_dev;
     int32_t* tile_dev;
@@ -1029,6 +1124,39 @@ int hf_euler_stage(hf_ctx* c, double* Uin, double* Uout, const hf_box* box, cons
 }
 
 namespace {
+// Cells (i0..i1, j0..j1) of every component of a patch lie inside the
+// device allocation of p.
+bool patch_in_allocation(const double* p, const hf_patch& P, int nv, int64_t i0, int64_t j0, int64_t i1, int64_t j1) {
+    return within_allocation(p, P.offset + j0 * P.pitch + i0, P.offset + (nv - 1) * P.plane + j1 * P.pitch + i1);
+}
+
+// The cells an operation reads and writes, against their allocations:
+// INTERP (coarse: grow(coarsen(region), 1), fine: region), AVGDOWN (fine:
+// refine(region), coarse: region), REFLUX (coarse: region grown by 1; the
+// face fluxes of its faces and of their fine faces).
+int check_extents(hf_ctx* c, int op, const double* src, const hf_patch& sp, const double* dst, const hf_patch& dp,
+                  const int32_t* g, const hf_face_fluxes* cf = nullptr, const hf_face_fluxes* ff = nullptr) {
+    const int nv = c->mech.ns() + 3;
+    bool ok = true;
+    if (op == BoxLaunch::INTERP) {
+        ok = patch_in_allocation(src, sp, nv, (g[0] >> 1) - 1, (g[1] >> 1) - 1, (g[2] >> 1) + 1, (g[3] >> 1) + 1) &&
+             patch_in_allocation(dst, dp, nv, g[0], g[1], g[2], g[3]);
+    } else if (op == BoxLaunch::AVGDOWN) {
+        ok = patch_in_allocation(src, sp, nv, 2 * g[0], 2 * g[1], 2 * g[2] + 1, 2 * g[3] + 1) &&
+             patch_in_allocation(dst, dp, nv, g[0], g[1], g[2], g[3]);
+    } else if (op == BoxLaunch::REFLUX_CELLS) {
+        auto faces = [&](const hf_face_fluxes& F, int64_t i0, int64_t j0, int64_t i1, int64_t j1) {
+            return within_allocation(F.fx, F.fx_offset + j0 * F.fx_pitch + i0,
+                                     F.fx_offset + (nv - 1) * F.fx_plane + j1 * F.fx_pitch + i1 + 1) &&
+                   within_allocation(F.fy, F.fy_offset + j0 * F.fy_pitch + i0,
+                                     F.fy_offset + (nv - 1) * F.fy_plane + (j1 + 1) * F.fy_pitch + i1);
+        };
+        ok = patch_in_allocation(dst, dp, nv, g[0], g[1], g[2], g[3]) && faces(*cf, g[0], g[1], g[2], g[3]) &&
+             faces(*ff, 2 * g[0], 2 * g[1], 2 * g[2] + 1, 2 * g[3] + 1);
+    }
+    return ok ? HF_OK : fail(c, HF_EINVAL, "a patch, region or face-flux array lies outside its allocation");
+}
+
 int patch_common(hf_ctx* c, BoxLaunch& L, const double* src, const hf_patch* sp, double* dst, const hf_patch* dp,
                  const int32_t region[4]) {
     if (!src || !sp || !dst || !dp || !region) return fail(c, HF_EINVAL, "missing patch, array or region");
@@ -1061,6 +1189,7 @@ int hf_interp_coarse_to_fine(hf_ctx* c, const double* coarse, const hf_patch* cp
     BoxLaunch L{};
     const int r = patch_common(c, L, coarse, cp, fine, fp, region);
     if (r != HF_OK) return r;
+    if (check_extents(c, BoxLaunch::INTERP, coarse, *cp, fine, *fp, region) != HF_OK) return HF_EINVAL;
     L.op = BoxLaunch::INTERP;
     if (c->ops->box_ops(L) != HF_OK) return fail(c, HF_EDEVICE, "interpolation launch failed");
     unsigned long long h[2] = {0, ~0ull};
@@ -1113,8 +1242,208 @@ int hf_average_down(hf_ctx* c, const double* fine, const hf_patch* fp, double* c
     BoxLaunch L{};
     const int r = patch_common(c, L, fine, fp, coarse, cp, cregion);
     if (r != HF_OK) return r;
+    if (check_extents(c, BoxLaunch::AVGDOWN, fine, *fp, coarse, *cp, cregion) != HF_OK) return HF_EINVAL;
     L.op = BoxLaunch::AVGDOWN;
     if (c->ops->box_ops(L) != HF_OK) return fail(c, HF_EDEVICE, "average-down launch failed");
+    HF_HIP(c, hipStreamSynchronize(c->stream));
+    return HF_OK;
+}
+
+namespace {
+// A device copy of a block table and its cell-id prefix sums in L (the
+// context's block buffer), after checking every block.
+int cells_common(hf_ctx* c, BoxLaunch& L, const hf_cells* blocks, int nblock, std::vector<int64_t>& start) {
+    if (nblock < 1 || !blocks) return fail(c, HF_EINVAL, "invalid block table");
+    HF_DEVICE(c);
+    start.assign(1, 0);
+    for (int b = 0; b < nblock; ++b) {
+        const hf_cells& B = blocks[b];
+        if (B.nx < 1 || B.ny < 1 || !B.U || B.pitch < B.nx || B.plane < B.pitch * B.ny || B.offset < 0 ||
+            (B.mask && B.mask_pitch < B.nx) || !(B.dx > 0) || !(B.dy >= 0))
+            return fail(c, HF_EINVAL, "block " + std::to_string(b) + " has an invalid extent, stride or cell size");
+        if (!is_device_ptr(B.U) || (B.mask && !is_device_ptr(B.mask)))
+            return fail(c, HF_EINVAL, "block " + std::to_string(b) + " is not in device memory");
+        start.push_back(start.back() + (int64_t)B.nx * B.ny);
+    }
+    const size_t tb = sizeof(hf_cells) * nblock, sb = sizeof(int64_t) * (nblock + 1);
+    if (!c->blocks.ensure(tb + sb) || !c->scal.ensure(sizeof(unsigned long long) * 256))
+        return fail(c, HF_ENOMEM, "device scratch");
+    HF_HIP(c, hipMemcpyAsync(c->blocks.p, blocks, tb, hipMemcpyHostToDevice, c->stream));
+    HF_HIP(c, hipMemcpyAsync(c->blocks.as<char>() + tb, start.data(), sb, hipMemcpyHostToDevice, c->stream));
+    L.mech = c->d_mech;
+    L.blocks = c->blocks.as<hf_cells>();
+    L.block_start = reinterpret_cast<const int64_t*>(c->blocks.as<char>() + tb);
+    L.nblock = nblock;
+    L.total = start.back();
+    L.scal = c->scal.as<unsigned long long>() + 8;
+    L.first_failed = c->scal.as<unsigned long long>();
+    L.stream = c->stream;
+    return HF_OK;
+}
+
+std::string cell_name(const std::vector<int64_t>& start, const hf_cells* blocks, unsigned long long code) {
+    const int b = (int)(code >> 32);
+    const int64_t k = (int64_t)(code & 0xffffffffull);
+    (void)start;
+    return "block " + std::to_string(b) + " cell (" + std::to_string(k % blocks[b].nx) + ", " +
+           std::to_string(k / blocks[b].nx) + ")";
+}
+}  // namespace
+
+int hf_validate_cells(hf_ctx* c, const hf_cells* blocks, int nblock, double* rate, int64_t* first_failed) {
+    if (!c) return HF_EINVAL;
+    BusyGuard guard(c);
+    if (!guard.ok) return fail(c, HF_EBUSY, "concurrent call is unsupported");
+    if (first_failed) *first_failed = -1;
+    if (rate) *rate = 0;
+    BoxLaunch L{};
+    std::vector<int64_t> start;
+    const int r = cells_common(c, L, blocks, nblock, start);
+    if (r != HF_OK) return r;
+    L.op = BoxLaunch::CELLS_PROPS;
+    unsigned long long init[2] = {~0ull, 0};
+    HF_HIP(c, hipMemcpyAsync(L.first_failed, &init[0], sizeof(init[0]), hipMemcpyHostToDevice, c->stream));
+    HF_HIP(c, hipMemcpyAsync(L.scal, &init[1], sizeof(init[1]), hipMemcpyHostToDevice, c->stream));
+    if (c->ops->box_ops(L) != HF_OK) return fail(c, HF_EDEVICE, "validation launch failed");
+    unsigned long long out[2];
+    HF_HIP(c, hipMemcpyAsync(&out[0], L.first_failed, sizeof(out[0]), hipMemcpyDeviceToHost, c->stream));
+    HF_HIP(c, hipMemcpyAsync(&out[1], L.scal, sizeof(out[1]), hipMemcpyDeviceToHost, c->stream));
+    HF_HIP(c, hipStreamSynchronize(c->stream));
+    if (out[0] != ~0ull) {
+        if (first_failed) *first_failed = (int64_t)out[0];
+        return fail(c, HF_EDOMAIN, "inadmissible state at " + cell_name(start, blocks, out[0]));
+    }
+    if (rate) std::memcpy(rate, &out[1], sizeof(double));
+    return HF_OK;
+}
+
+int hf_repair_cells(hf_ctx* c, const hf_cells* blocks, int nblock, int64_t* repairs, int64_t* first_failed) {
+    if (!c) return HF_EINVAL;
+    BusyGuard guard(c);
+    if (!guard.ok) return fail(c, HF_EBUSY, "concurrent call is unsupported");
+    if (first_failed) *first_failed = -1;
+    if (repairs) *repairs = 0;
+    BoxLaunch L{};
+    std::vector<int64_t> start;
+    const int r = cells_common(c, L, blocks, nblock, start);
+    if (r != HF_OK) return r;
+    const int ns = c->ops->ns;
+    const int64_t groups = (L.total + 255) / 256;
+    if (!c->aux[5].ensure(sizeof(double) * ns * groups)) return fail(c, HF_ENOMEM, "device scratch");
+    L.partials = c->aux[5].as<double>();
+    L.op = BoxLaunch::CELLS_REPAIR;
+    if (c->ops->box_ops(L) != HF_OK) return fail(c, HF_EDEVICE, "roundoff repair launch failed");
+    std::vector<unsigned long long> h(3 * ns + 2);
+    unsigned long long ff = ~0ull;
+    HF_HIP(c, hipMemcpyAsync(h.data(), L.scal, sizeof(unsigned long long) * h.size(), hipMemcpyDeviceToHost, c->stream));
+    HF_HIP(c, hipMemcpyAsync(&ff, L.first_failed, sizeof(ff), hipMemcpyDeviceToHost, c->stream));
+    HF_HIP(c, hipStreamSynchronize(c->stream));
+    if (repairs) *repairs = (int64_t)h[3 * ns];
+    if (ff != ~0ull) {
+        if (first_failed) *first_failed = (int64_t)ff;
+        return fail(c, HF_EDOMAIN, "materially negative species after composite flow stage at " +
+                                       cell_name(start, blocks, ff));
+    }
+    if (h[3 * ns + 1]) return fail(c, HF_EDOMAIN, "insufficient positive donor for roundoff redistribution");
+    return HF_OK;
+}
+
+int hf_interp_regions(hf_ctx* c, const double* coarse, const hf_patch* cp, double* fine, const hf_patch* fp,
+                      const int32_t* regions, int nregion, const uint8_t* fine_mask, int64_t fine_mask_pitch,
+                      const int32_t fine_extent[2], int64_t* limited, int64_t* first_failed) {
+    if (!c) return HF_EINVAL;
+    BusyGuard guard(c);
+    if (!guard.ok) return fail(c, HF_EBUSY, "concurrent call is unsupported");
+    if (limited) *limited = 0;
+    if (first_failed) *first_failed = -1;
+    if (nregion < 0 || (nregion > 0 && !regions)) return fail(c, HF_EINVAL, "invalid region list");
+    if (nregion == 0) return HF_OK;
+    if (fine_mask && (!fine_extent || fine_extent[0] < 1 || fine_extent[1] < 1 || fine_mask_pitch < fine_extent[0]))
+        return fail(c, HF_EINVAL, "invalid fine mask extent");
+    int64_t most = 0;
+    for (int k = 0; k < nregion; ++k) {
+        const int32_t* g = regions + 4 * k;
+        if (g[2] < g[0] || g[3] < g[1]) return fail(c, HF_EINVAL, "empty region " + std::to_string(k));
+        most = std::max<int64_t>(most, (int64_t)((g[2] >> 1) - (g[0] >> 1) + 1) * ((g[3] >> 1) - (g[1] >> 1) + 1));
+    }
+    BoxLaunch L{};
+    const int r = patch_common(c, L, coarse, cp, fine, fp, regions);
+    if (r != HF_OK) return r;
+    for (int k = 0; k < nregion; ++k)
+        if (check_extents(c, BoxLaunch::INTERP, coarse, *cp, fine, *fp, regions + 4 * k) != HF_OK) return HF_EINVAL;
+    if (fine_mask && !is_device_ptr(fine_mask)) return fail(c, HF_EINVAL, "the fine mask needs device memory");
+    if (!c->tiles.ensure(sizeof(int32_t) * 4 * nregion)) return fail(c, HF_ENOMEM, "device scratch");
+    HF_HIP(c, hipMemcpyAsync(c->tiles.p, regions, sizeof(int32_t) * 4 * nregion, hipMemcpyHostToDevice, c->stream));
+    L.op = BoxLaunch::INTERP;
+    L.regions = c->tiles.as<int32_t>();
+    L.nregion = nregion;
+    L.total = most;
+    L.mask = fine_mask;
+    L.mask_pitch = fine_mask_pitch;
+    L.mask_nx = fine_mask ? fine_extent[0] : 0;
+    L.mask_ny = fine_mask ? fine_extent[1] : 0;
+    if (c->ops->box_ops(L) != HF_OK) return fail(c, HF_EDEVICE, "interpolation launch failed");
+    unsigned long long h[2] = {0, ~0ull};
+    HF_HIP(c, hipMemcpyAsync(&h[0], L.scal, sizeof(h[0]), hipMemcpyDeviceToHost, c->stream));
+    HF_HIP(c, hipMemcpyAsync(&h[1], L.first_failed, sizeof(h[1]), hipMemcpyDeviceToHost, c->stream));
+    HF_HIP(c, hipStreamSynchronize(c->stream));
+    if (limited) *limited = (int64_t)h[0];
+    if (h[1] != ~0ull) {
+        if (first_failed) *first_failed = (int64_t)h[1];
+        return fail(c, HF_EDOMAIN, "inadmissible coarse parent " + std::to_string(h[1] & 0xffffffffull) +
+                                       " of region " + std::to_string(h[1] >> 32) + " in interpolation");
+    }
+    return HF_OK;
+}
+
+int hf_average_down_masked(hf_ctx* c, const double* fine, const hf_patch* fp, double* coarse, const hf_patch* cp,
+                           const uint8_t* coarse_mask, int64_t coarse_mask_pitch, const int32_t cregion[4]) {
+    if (!c) return HF_EINVAL;
+    BusyGuard guard(c);
+    if (!guard.ok) return fail(c, HF_EBUSY, "concurrent call is unsupported");
+    if (!coarse_mask || !cregion || cregion[0] < 0 || cregion[1] < 0 || coarse_mask_pitch <= cregion[2])
+        return fail(c, HF_EINVAL, "invalid coarse mask or region");
+    BoxLaunch L{};
+    const int r = patch_common(c, L, fine, fp, coarse, cp, cregion);
+    if (r != HF_OK) return r;
+    if (check_extents(c, BoxLaunch::AVGDOWN, fine, *fp, coarse, *cp, cregion) != HF_OK) return HF_EINVAL;
+    if (!is_device_ptr(coarse_mask)) return fail(c, HF_EINVAL, "the coarse mask needs device memory");
+    L.op = BoxLaunch::AVGDOWN;
+    L.mask = coarse_mask;
+    L.mask_pitch = coarse_mask_pitch;
+    if (c->ops->box_ops(L) != HF_OK) return fail(c, HF_EDEVICE, "average-down launch failed");
+    HF_HIP(c, hipStreamSynchronize(c->stream));
+    return HF_OK;
+}
+
+int hf_reflux_masked(hf_ctx* c, double* Uc, const hf_patch* cp, const hf_face_fluxes* coarse,
+                     const hf_face_fluxes* fine, const uint8_t* coarse_mask, int64_t coarse_mask_pitch,
+                     const int32_t cregion[4], double h, double dxc, double dyc) {
+    if (!c) return HF_EINVAL;
+    BusyGuard guard(c);
+    if (!guard.ok) return fail(c, HF_EBUSY, "concurrent call is unsupported");
+    if (!coarse || !fine || !coarse_mask || !cregion) return fail(c, HF_EINVAL, "missing fluxes, mask or region");
+    if (cregion[0] < 0 || cregion[1] < 0 || coarse_mask_pitch <= cregion[2])
+        return fail(c, HF_EINVAL, "invalid coarse mask or region");
+    if (!(h > 0) || !(dxc > 0) || !(dyc > 0) || !std::isfinite(h) || !std::isfinite(dxc) || !std::isfinite(dyc))
+        return fail(c, HF_EINVAL, "invalid step or cell size");
+    for (const hf_face_fluxes* f : {coarse, fine})
+        if (!f->fx || !f->fy || !is_device_ptr(f->fx) || !is_device_ptr(f->fy))
+            return fail(c, HF_EINVAL, "face fluxes need device memory");
+    BoxLaunch L{};
+    const int r = patch_common(c, L, Uc, cp, Uc, cp, cregion);
+    if (r != HF_OK) return r;
+    if (check_extents(c, BoxLaunch::REFLUX_CELLS, Uc, *cp, Uc, *cp, cregion, coarse, fine) != HF_OK) return HF_EINVAL;
+    if (!is_device_ptr(coarse_mask)) return fail(c, HF_EINVAL, "the coarse mask needs device memory");
+    L.op = BoxLaunch::REFLUX_CELLS;
+    L.cflux = *coarse;
+    L.fflux = *fine;
+    L.mask = coarse_mask;
+    L.mask_pitch = coarse_mask_pitch;
+    L.h = h;
+    L.dx = dxc;
+    L.dy = dyc;
+    if (c->ops->box_ops(L) != HF_OK) return fail(c, HF_EDEVICE, "reflux launch failed");
     HF_HIP(c, hipStreamSynchronize(c->stream));
     return HF_OK;
 }

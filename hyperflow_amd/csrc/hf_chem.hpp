@@ -94,12 +94,13 @@ struct CellRun {
     }
 
     // HF_OK when the integration is running, else the cell's error code.
-    __device__ __forceinline__ int start(const double* __restrict__ U, long i, long ld, double dt, int maxsteps) {
+    // q: component 0 of the cell, component n at q[n * plane].
+    __device__ __forceinline__ int start(const double* __restrict__ q, long plane, double dt, int maxsteps) {
         bdf.c = BdfCounters{};
         if (!(dt > 0) || !finite(dt)) return HF_EINVAL;
         double u[NV];
         #pragma unroll
-        for (int n = 0; n < NV; ++n) u[n] = U[n * ld + i];
+        for (int n = 0; n < NV; ++n) u[n] = q[n * plane];
         Props pr;
         if (!properties(sys.d, u, pr, 1000.0)) return HF_EDOMAIN;
         double rho = 0;
@@ -117,7 +118,7 @@ struct CellRun {
     __device__ __forceinline__ int step() { return bdf.trip(sys, y); }
 
     // Post-step checks of chemistry.cpp:68-75 and the write of the species planes.
-    __device__ __forceinline__ int finish(const double* __restrict__ U, long i, long ld, double* __restrict__ out,
+    __device__ __forceinline__ int finish(const double* __restrict__ q, long plane, double* __restrict__ out, long i,
                                           long ldo) {
         double u[NV];
         double sum = 0;
@@ -130,9 +131,9 @@ struct CellRun {
         }
         if (!ok) return HF_EDOMAIN;
         u[N] = sys.rho - sum;
-        u[NS] = U[NS * ld + i];
-        u[NS + 1] = U[(NS + 1) * ld + i];
-        u[NS + 2] = U[(NS + 2) * ld + i];
+        u[NS] = q[NS * plane];
+        u[NS + 1] = q[(NS + 1) * plane];
+        u[NS + 2] = q[(NS + 2) * plane];
         Props pr;
         if (!properties(sys.d, u, pr, sys.t)) return HF_EDOMAIN;
         #pragma unroll

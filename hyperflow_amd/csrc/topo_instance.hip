@@ -182,18 +182,23 @@ __global__ void chem_init_kernel(Workspace w, unsigned long long* first_failed, 
 // binned on a log10 scale over [1e-2, 1e10].
 template <class TT>
 __global__ void __launch_bounds__(256)
-chem_estimate_kernel(const MechData<TT>* __restrict__ dm, const double* __restrict__ U, long ncell, long ld,
-                     const double* __restrict__ dt, int per_cell, ChemParams p, int nbins, Workspace w) {
+chem_estimate_kernel(const MechData<TT>* __restrict__ dm, CellMap cm, long ncell, const double* __restrict__ dt,
+                     int per_cell, ChemParams p, int nbins, Workspace w) {
     constexpr int NS = TT::NS, NV = NS + 3, N = NS - 1;
     __shared__ unsigned lh[kMaxBins];
     for (int b = threadIdx.x; b < nbins; b += blockDim.x) lh[b] = 0;
     __syncthreads();
     const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (i < ncell) {
+    const double* q = nullptr;
+    int64_t plane = 0;
+    if (i < ncell && !cm.locate(i, q, plane)) {
+        w.bins[i] = 0;                             // not in the batch: queued last, skipped at once
+        atomicAdd(&lh[0], 1u);
+    } else if (i < ncell) {
         const MechData<TT>& d = *dm;
         double u[NV];
         #pragma unroll
-        for (int n = 0; n < NV; ++n) u[n] = U[n * ld + i];
+        for (int n = 0; n < NV; ++n) u[n] = q[n * plane];
         const double h = per_cell ? dt[i] : dt[0];
         int bin = nbins - 1;   // unusable cells fail fast: queue them first
         Props pr;
@@ -261,7 +266,7 @@ __global__ void __launch_bounds__(256) chem_scatter_kernel(long ncell, int nbins
 // Otherwise the block is a persistent wave draining the binned queue.
 template <class TT, class MT, bool PLAIN>
 __global__ void __launch_bounds__(64)
-chem_lane_kernel(const MechData<TT>* __restrict__ dm, const double* __restrict__ U, long ncell, long ld,
+chem_lane_kernel(const MechData<TT>* __restrict__ dm, CellMap cm, long ncell,
                  const double* __restrict__ dt, int per_cell, double* __restrict__ out, long ldo,
                  int8_t* __restrict__ status, unsigned long long* __restrict__ first_failed, ChemParams p,
                  unsigned long long* __restrict__ counters, Workspace w, int refill, int jwait, int swait) {
@@ -297,8 +302,12 @@ chem_lane_kernel(const MechData<TT>* __restrict__ dm, const double* __restrict__
     int since_jac = jwait, since_setup = swait;
     if constexpr (PLAIN) {
         cell = (long)blockIdx.x * 64 + lane;
-        if (cell < ncell) {
-            const int st = run.start(U, cell, ld, per_cell ? dt[cell] : dt[0], p.maxsteps);
+        const double* q;
+        int64_t plane;
+        if (cell < ncell && !cm.locate(cell, q, plane)) {
+            status[cell] = HF_OK;                   // not a leaf: no part of the batch
+        } else if (cell < ncell) {
+            const int st = run.start(q, plane, per_cell ? dt[cell] : dt[0], p.maxsteps);
             if (st == HF_OK) {
                 active = true;
             } else {
@@ -321,8 +330,12 @@ chem_lane_kernel(const MechData<TT>* __restrict__ dm, const double* __restrict__
                 if (slot < (unsigned)ncell) {
                     cell = w.order[slot];
                     const double h = per_cell ? dt[cell] : dt[0];
-                    const int st = run.start(U, cell, ld, h, p.maxsteps);
-                    if (st == HF_OK) {
+                    const double* q;
+                    int64_t plane;
+                    const int st = cm.locate(cell, q, plane) ? run.start(q, plane, h, p.maxsteps) : -1;
+                    if (st == -1) {
+                        status[cell] = HF_OK;       // not a leaf: no part of the batch
+                    } else if (st == HF_OK) {
                         active = true;
                     } else {
                         status[cell] = (int8_t)st;
@@ -363,7 +376,10 @@ chem_lane_kernel(const MechData<TT>* __restrict__ dm, const double* __restrict__
         if (go) {
             const int r = run.step();
             if (r != BDF_RUNNING) {
-                int st = r == BDF_OK ? run.finish(U, cell, ld, out, ldo) : HF_ECONV;
+                const double* q;
+                int64_t plane;
+                cm.locate(cell, q, plane);
+                int st = r == BDF_OK ? run.finish(q, plane, out, cell, ldo) : HF_ECONV;
                 status[cell] = (int8_t)st;
                 if (st != HF_OK) atomicMin(first_failed, (unsigned long long)cell);
                 tot.steps += run.bdf.c.steps;
@@ -392,7 +408,7 @@ constexpr bool kWaveChem = Topo::NS > 16;
 // faster than three waves per SIMD, where the spills grow.
 template <class TT, class MT>
 __global__ void __launch_bounds__(64, 2)
-chem_wave_kernel(const MechData<TT>* __restrict__ dm, const double* __restrict__ U, long ncell, long ld,
+chem_wave_kernel(const MechData<TT>* __restrict__ dm, CellMap cm, long ncell,
                  const double* __restrict__ dt, int per_cell, double* __restrict__ out, long ldo,
                  int8_t* __restrict__ status, unsigned long long* __restrict__ first_failed, ChemParams p,
                  unsigned long long* __restrict__ counters, Workspace w) {
@@ -416,8 +432,14 @@ chem_wave_kernel(const MechData<TT>* __restrict__ dm, const double* __restrict__
         slot = (unsigned)__builtin_amdgcn_readfirstlane((int)slot);
         if (slot >= (unsigned)ncell) break;
         const long cell = w.order[slot];
-        const double rhoY = lane < NS ? U[lane * ld + cell] : 0.0;
-        const double mx = U[NS * ld + cell], my = U[(NS + 1) * ld + cell], E = U[(NS + 2) * ld + cell];
+        const double* q;
+        int64_t plane;
+        if (!cm.locate(cell, q, plane)) {
+            if (lane == 0) status[cell] = HF_OK;      // not a leaf: no part of the batch
+            continue;
+        }
+        const double rhoY = lane < NS ? q[lane * plane] : 0.0;
+        const double mx = q[NS * plane], my = q[(NS + 1) * plane], E = q[(NS + 2) * plane];
         const double h = per_cell ? dt[cell] : dt[0];
         int st = HF_OK;
         double tt = 1000.0;
@@ -469,21 +491,28 @@ chem_wave_kernel(const MechData<TT>* __restrict__ dm, const double* __restrict__
 
 // All-or-nothing commit on the device: the advanced species planes replace
 // the caller's only when no cell failed (chemistry_batch.cpp:40-49).
-__global__ void __launch_bounds__(256) chem_commit_kernel(const double* __restrict__ out, long ldo,
-                                                          double* __restrict__ U, long ld, long ncell,
+__global__ void __launch_bounds__(256) chem_commit_kernel(const double* __restrict__ out, long ldo, CellMap cm,
+                                                          long ncell,
                                                           const unsigned long long* __restrict__ first_failed) {
     if (*first_failed != ~0ull) return;
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ncell * Topo::NS) return;
-    const long k = t / ncell, i = t - k * ncell;
-    U[k * ld + i] = out[k * ldo + i];
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const double* q;
+    int64_t plane;
+    if (i >= ncell || !cm.locate(i, q, plane)) return;
+    double* u = const_cast<double*>(q);
+    #pragma unroll
+    for (int k = 0; k < Topo::NS; ++k) u[k * plane] = out[k * ldo + i];
+}
+
+// The batch's cells: the block table, or the flat planes at U (stride ld).
+CellMap cell_map(const ChemLaunch& L, const double* U, int64_t ld) {
+    return CellMap{U, ld, L.blocks, L.block_start, L.nblock};
 }
 
 int chem_commit(const ChemLaunch& L) {
     if (!L.commit) return HF_OK;
-    const long n = (long)L.ncell * Topo::NS;
-    hipLaunchKernelGGL(chem_commit_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, L.stream, L.out,
-                       (long)L.ldo, L.commit, (long)L.commit_ld, (long)L.ncell, L.first_failed);
+    hipLaunchKernelGGL(chem_commit_kernel, dim3((unsigned)((L.ncell + 255) / 256)), dim3(256), 0, L.stream, L.out,
+                       (long)L.ldo, cell_map(L, L.commit, L.commit_ld), (long)L.ncell, L.first_failed);
     return hipGetLastError() == hipSuccess ? HF_OK : HF_EDEVICE;
 }
 
@@ -492,6 +521,7 @@ int chem(const ChemLaunch& L) {
     {
         const ChemParams p{L.rtol, L.atol, L.maxsteps};
         const MD* dm = static_cast<const MD*>(L.mech);
+        const CellMap cm = cell_map(L, L.U, L.ld);
         const int64_t lanes = slot_lanes(L.ncell, L.num_cus, L.waves_per_cu);
         if (!L.workspace || L.workspace_bytes < chem_workspace(L.ncell, L.nbins, L.num_cus, L.waves_per_cu))
             return HF_EINVAL;
@@ -505,8 +535,8 @@ int chem(const ChemLaunch& L) {
             const int swait = L.setup_wait > 0 ? L.setup_wait : (L.setup_wait == 0 ? 2 : 0);
             const unsigned g256 = (unsigned)((L.ncell + 255) / 256);
             hipLaunchKernelGGL(chem_init_kernel, dim3(1), dim3(256), 0, L.stream, w, L.first_failed, L.counters);
-            hipLaunchKernelGGL(chem_estimate_kernel<Topo>, dim3(g256), dim3(256), 0, L.stream, dm, L.U,
-                               (long)L.ncell, (long)L.ld, L.dt, L.dt_per_cell, p, nbins, w);
+            hipLaunchKernelGGL(chem_estimate_kernel<Topo>, dim3(g256), dim3(256), 0, L.stream, dm, cm,
+                               (long)L.ncell, L.dt, L.dt_per_cell, p, nbins, w);
             hipLaunchKernelGGL(chem_scan_kernel, dim3(1), dim3(64), 0, L.stream, w, nbins);
             hipLaunchKernelGGL(chem_scatter_kernel, dim3(g256), dim3(256), 0, L.stream, (long)L.ncell, nbins, w);
             StreamTimer timer(L.kernel_ms, L.stream);
@@ -515,11 +545,11 @@ int chem(const ChemLaunch& L) {
                 const int wwpc = L.waves_per_cu > 0 ? L.waves_per_cu : 8;
                 if (L.precision == HF_PREC_MIXED)
                     hipLaunchKernelGGL((chem_wave_kernel<Topo, float>), dim3((unsigned)(L.num_cus * wwpc)), dim3(64), 0,
-                                       L.stream, dm, L.U, (long)L.ncell, (long)L.ld, L.dt, L.dt_per_cell, L.out,
+                                       L.stream, dm, cm, (long)L.ncell, L.dt, L.dt_per_cell, L.out,
                                        (long)L.ldo, L.status, L.first_failed, p, L.counters, w);
                 else
                     hipLaunchKernelGGL((chem_wave_kernel<Topo, double>), dim3((unsigned)(L.num_cus * wwpc)), dim3(64), 0,
-                                       L.stream, dm, L.U, (long)L.ncell, (long)L.ld, L.dt, L.dt_per_cell, L.out,
+                                       L.stream, dm, cm, (long)L.ncell, L.dt, L.dt_per_cell, L.out,
                                        (long)L.ldo, L.status, L.first_failed, p, L.counters, w);
                 if (hipGetLastError() != hipSuccess) return HF_EDEVICE;
                 timer.stop();
@@ -527,11 +557,11 @@ int chem(const ChemLaunch& L) {
             } else {
                 if (L.precision == HF_PREC_MIXED)
                     hipLaunchKernelGGL((chem_lane_kernel<Topo, float, false>), dim3((unsigned)(L.num_cus * wpc)),
-                                       dim3(64), 0, L.stream, dm, L.U, (long)L.ncell, (long)L.ld, L.dt, L.dt_per_cell,
+                                       dim3(64), 0, L.stream, dm, cm, (long)L.ncell, L.dt, L.dt_per_cell,
                                        L.out, (long)L.ldo, L.status, L.first_failed, p, L.counters, w, chunk, jwait, swait);
                 else
                     hipLaunchKernelGGL((chem_lane_kernel<Topo, double, false>), dim3((unsigned)(L.num_cus * wpc)),
-                                       dim3(64), 0, L.stream, dm, L.U, (long)L.ncell, (long)L.ld, L.dt, L.dt_per_cell,
+                                       dim3(64), 0, L.stream, dm, cm, (long)L.ncell, L.dt, L.dt_per_cell,
                                        L.out, (long)L.ldo, L.status, L.first_failed, p, L.counters, w, chunk, jwait, swait);
                 if (hipGetLastError() != hipSuccess) return HF_EDEVICE;
                 timer.stop();
@@ -543,8 +573,8 @@ int chem(const ChemLaunch& L) {
         hipLaunchKernelGGL(chem_zero_kernel, dim3(1), dim3(64), 0, L.stream, L.first_failed, L.counters);
         StreamTimer timer(L.kernel_ms, L.stream);
         const unsigned grid = (unsigned)((L.ncell + 63) / 64);
-        hipLaunchKernelGGL((chem_lane_kernel<Topo, double, true>), dim3(grid), dim3(64), 0, L.stream, dm, L.U,
-                           (long)L.ncell, (long)L.ld, L.dt, L.dt_per_cell, L.out, (long)L.ldo, L.status,
+        hipLaunchKernelGGL((chem_lane_kernel<Topo, double, true>), dim3(grid), dim3(64), 0, L.stream, dm, cm,
+                           (long)L.ncell, L.dt, L.dt_per_cell, L.out, (long)L.ldo, L.status,
                            L.first_failed, p, L.counters, none, 64, 0, 0);
         if (hipGetLastError() != hipSuccess) return HF_EDEVICE;
         timer.stop();
@@ -738,16 +768,33 @@ __device__ __forceinline__ double minmod1(double x, double y) {
 // (ConservativeInterp::interp, ConservativeInterp.cpp:27-50): one lane per
 // coarse cell of coarsen(region), the theta search over all four children,
 // then the children inside region. scal[0] counts limited cells.
+// A list of regions (regions non-null: region blockIdx.y) shares the
+// patches; with a fine mask, a child is written only inside the mask's
+// extent where the mask is 0 (not a valid fine cell), and a parent with no
+// such child does nothing.
 __global__ void __launch_bounds__(256) interp_kernel(const MD* __restrict__ dm, const double* __restrict__ C,
-                                                     hf_patch cp, double* __restrict__ F, hf_patch fp, int4 region,
-                                                     unsigned long long* __restrict__ scal,
+                                                     hf_patch cp, double* __restrict__ F, hf_patch fp, int4 region0,
+                                                     const int4* __restrict__ regions,
+                                                     const uint8_t* __restrict__ fmask, long fmask_pitch, int fnx,
+                                                     int fny, unsigned long long* __restrict__ scal,
                                                      unsigned long long* __restrict__ first_failed) {
     constexpr int NV = Topo::NS + 3;
+    const int4 region = regions ? regions[blockIdx.y] : region0;
     const int cilo = region.x >> 1, cjlo = region.y >> 1;  // floor division (coarsen)
     const int cnx = (region.z >> 1) - cilo + 1, cny = (region.w >> 1) - cjlo + 1;
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long)cnx * cny) return;
     const int i = cilo + (int)(t % cnx), j = cjlo + (int)(t / cnx);
+    auto writable = [&](int fi, int fj) {
+        if (fi < region.x || fi > region.z || fj < region.y || fj > region.w) return false;
+        if (!fmask) return true;
+        return fi >= 0 && fi < fnx && fj >= 0 && fj < fny && fmask[(long)fj * fmask_pitch + fi] == 0;
+    };
+    if (fmask) {
+        bool any = false;
+        for (int k = 0; k < 4; ++k) any = any || writable(2 * i + (k & 1), 2 * j + (k >> 1));
+        if (!any) return;
+    }
     double* c = const_cast<double*>(C);
     double q[NV], sx[NV], sy[NV];
     #pragma unroll
@@ -781,14 +828,15 @@ __global__ void __launch_bounds__(256) interp_kernel(const MD* __restrict__ dm, 
         theta = 0.0;
         Props p;
         if (!properties(*dm, q, p, 1000.0)) {
-            atomicMin(first_failed, (unsigned long long)t);
+            atomicMin(first_failed, regions ? ((unsigned long long)blockIdx.y << 32) | (unsigned long long)t
+                                            : (unsigned long long)t);
             return;
         }
     }
     if (theta < 1.0) atomicAdd(&scal[0], 1ull);
     for (int k = 0; k < 4; ++k) {
         const int fi = 2 * i + (k & 1), fj = 2 * j + (k >> 1);
-        if (fi < region.x || fi > region.z || fj < region.y || fj > region.w) continue;
+        if (!writable(fi, fj)) continue;
         const double cx = (k & 1) ? 0.25 : -0.25, cy = (k & 2) ? 0.25 : -0.25;
         #pragma unroll
         for (int n = 0; n < NV; ++n) patch_at(F, fp, n, fi, fj) = interp_child(q[n], theta, cx, sx[n], cy, sy[n]);
@@ -931,8 +979,10 @@ __global__ void counter_init_kernel(unsigned long long* __restrict__ counter, un
 
 // Average-down at ratio (2, 2): coarse = 0.25 * (((0 + f00) + f10) + f01) + f11),
 // one lane per (component, coarse cell); i fastest for coalesced rows.
+// With a coarse mask, only coarse cells marked 2 (covered by the fine level) are written.
 __global__ void __launch_bounds__(256) avgdown_kernel(const double* __restrict__ F, hf_patch fp, double* __restrict__ C,
-                                                      hf_patch cp, int4 region) {
+                                                      hf_patch cp, int4 region, const uint8_t* __restrict__ cmask,
+                                                      long cmask_pitch) {
     constexpr int NV = Topo::NS + 3;
     const int cnx = region.z - region.x + 1, cny = region.w - region.y + 1;
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -940,6 +990,7 @@ __global__ void __launch_bounds__(256) avgdown_kernel(const double* __restrict__
     const int i = region.x + (int)(t % cnx);
     const int j = region.y + (int)((t / cnx) % cny);
     const int n = (int)(t / ((long)cnx * cny));
+    if (cmask && cmask[(long)j * cmask_pitch + i] != 2) return;
     double* f = const_cast<double*>(F);
     double sum = 0.0;
     sum += patch_at(f, fp, n, 2 * i, 2 * j);
@@ -1061,6 +1112,229 @@ __global__ void __launch_bounds__(256) box_props_kernel(const MD* __restrict__ d
     if ((threadIdx.x & 63) == 0 && rate > 0) atomicMax(rate_max, (unsigned long long)__double_as_longlong(rate));
 }
 
+// ------------------------------------------------------------------ operations over cell blocks
+// The valid cells of every level of an AMR hierarchy (hf_cells blocks with
+// masks: 0 not valid, 1 leaf, 2 covered), addressed through CellMap's block
+// form; cell ids run over the blocks in order, then j, then i.
+__device__ __forceinline__ double& cell_at(const hf_cells& B, int n, int i, int j) {
+    return B.U[B.offset + n * B.plane + (int64_t)j * B.pitch + i];
+}
+__device__ __forceinline__ int cell_kind(const hf_cells& B, int i, int j) {
+    return B.mask ? B.mask[(int64_t)j * B.mask_pitch + i] : 1;
+}
+
+// BlockAMR::validate over every valid cell (BlockAMR.cpp:286-298: all
+// levels, no mask) and stableDt's rate over the leaf cells (:335-346).
+__global__ void __launch_bounds__(256) cells_props_kernel(const MD* __restrict__ dm, CellMap S, long total,
+                                                          unsigned long long* __restrict__ first_failed,
+                                                          unsigned long long* __restrict__ rate_max) {
+    constexpr int NS = Topo::NS, NV = NS + 3;
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double rate = 0;
+    if (c < total) {
+        int b, i, j;
+        const hf_cells& B = S.find(c, b, i, j);
+        const int kind = cell_kind(B, i, j);
+        if (kind != 0) {
+            double u[NV];
+            #pragma unroll
+            for (int n = 0; n < NV; ++n) u[n] = cell_at(B, n, i, j);
+            Props p;
+            if (!properties(*dm, u, p, 1000.0)) {
+                atomicMin(first_failed, ((unsigned long long)b << 32) | (unsigned long long)((long)j * B.nx + i));
+            } else if (kind == 1) {
+                double rho = 0;
+                #pragma unroll
+                for (int k = 0; k < NS; ++k) rho += u[k];
+                rate = (fabs(u[NS] / rho) + p.a) / B.dx + (B.dy > 0 ? (fabs(u[NS + 1] / rho) + p.a) / B.dy : 0.0);
+            }
+        }
+    }
+    #pragma unroll
+    for (int off = 32; off > 0; off >>= 1) rate = fmax(rate, __shfl_xor(rate, off, 64));
+    if ((threadIdx.x & 63) == 0 && rate > 0) atomicMax(rate_max, (unsigned long long)__double_as_longlong(rate));
+}
+
+// BlockAMR::repairRoundoff (BlockAMR.cpp:299-334) over the leaf cells of all
+// blocks, cell volume dx*dy per block. scal: [0, NS) peak value bits,
+// [NS, 2NS) largest mass (value * volume) bits, [2NS, 3NS) donor cell id (the
+// first cell holding the largest mass), 3NS repairs, 3NS+1 donor failure.
+// The deficits are summed per workgroup in a fixed order (partials) and then
+// over the workgroups in order, so the result does not depend on timing.
+__global__ void cells_repair_init_kernel(unsigned long long* __restrict__ scal,
+                                         unsigned long long* __restrict__ first_failed) {
+    constexpr int NS = Topo::NS;
+    const int t = threadIdx.x;
+    if (t < NS) {
+        scal[t] = 0;
+        scal[NS + t] = 0;
+        scal[2 * NS + t] = ~0ull;
+    }
+    if (t == 0) {
+        scal[3 * NS] = 0;
+        scal[3 * NS + 1] = 0;
+        *first_failed = ~0ull;
+    }
+}
+__global__ void __launch_bounds__(256) cells_repair_peak_kernel(CellMap S, long total,
+                                                                unsigned long long* __restrict__ scal) {
+    constexpr int NS = Topo::NS;
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= total) return;
+    int b, i, j;
+    const hf_cells& B = S.find(c, b, i, j);
+    if (cell_kind(B, i, j) != 1) return;
+    const double vol = B.dy > 0 ? B.dx * B.dy : B.dx;
+    #pragma unroll
+    for (int n = 0; n < NS; ++n) {
+        const double v = cell_at(B, n, i, j);
+        if (v > 0) {
+            atomicMax(&scal[n], (unsigned long long)__double_as_longlong(v));
+            atomicMax(&scal[NS + n], (unsigned long long)__double_as_longlong(v * vol));
+        }
+    }
+}
+__global__ void __launch_bounds__(256) cells_repair_donor_kernel(CellMap S, long total,
+                                                                 unsigned long long* __restrict__ scal) {
+    constexpr int NS = Topo::NS;
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= total) return;
+    int b, i, j;
+    const hf_cells& B = S.find(c, b, i, j);
+    if (cell_kind(B, i, j) != 1) return;
+    const double vol = B.dy > 0 ? B.dx * B.dy : B.dx;
+    #pragma unroll
+    for (int n = 0; n < NS; ++n) {
+        const double v = cell_at(B, n, i, j);
+        if (v > 0 && (unsigned long long)__double_as_longlong(v * vol) == scal[NS + n])
+            atomicMin(&scal[2 * NS + n], (unsigned long long)c);
+    }
+}
+__global__ void __launch_bounds__(256) cells_repair_zero_kernel(CellMap S, long total,
+                                                                unsigned long long* __restrict__ scal,
+                                                                double* __restrict__ partials,
+                                                                unsigned long long* __restrict__ first_failed) {
+    constexpr int NS = Topo::NS;
+    __shared__ double part[4][NS];
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double d[NS];
+    #pragma unroll
+    for (int n = 0; n < NS; ++n) d[n] = 0.0;
+    unsigned long long repairs = 0;
+    if (c < total) {
+        int b, i, j;
+        const hf_cells& B = S.find(c, b, i, j);
+        if (cell_kind(B, i, j) == 1) {
+            const double vol = B.dy > 0 ? B.dx * B.dy : B.dx;
+            #pragma unroll
+            for (int n = 0; n < NS; ++n) {
+                double& v = cell_at(B, n, i, j);
+                if (v >= 0) continue;
+                const double peak = __longlong_as_double((long long)scal[n]);
+                if (!finite(v) || -v > 64.0 * 2.220446049250313e-16 * peak) {
+                    atomicMin(first_failed, ((unsigned long long)b << 32) | (unsigned long long)((long)j * B.nx + i));
+                    continue;
+                }
+                d[n] = -v * vol;
+                v = 0.0;
+                ++repairs;
+            }
+        }
+    }
+    // fixed-order sums: butterfly within the wave, then the four waves in order
+    #pragma unroll
+    for (int n = 0; n < NS; ++n) {
+        double x = d[n];
+        #pragma unroll
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+        if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6][n] = x;
+    }
+    #pragma unroll
+    for (int off = 32; off > 0; off >>= 1) repairs += __shfl_xor(repairs, off, 64);
+    if ((threadIdx.x & 63) == 0 && repairs) atomicAdd(&scal[3 * NS], repairs);
+    __syncthreads();
+    if (threadIdx.x < NS) {
+        const int n = threadIdx.x;
+        partials[(long)blockIdx.x * NS + n] = ((part[0][n] + part[1][n]) + part[2][n]) + part[3][n];
+    }
+}
+__global__ void cells_repair_donate_kernel(CellMap S, int nparts, unsigned long long* __restrict__ scal,
+                                           const double* __restrict__ partials,
+                                           unsigned long long* __restrict__ first_failed) {
+    constexpr int NS = Topo::NS;
+    const int n = threadIdx.x;
+    if (n >= NS || *first_failed != ~0ull) return;
+    double deficit = 0.0;
+    for (int g = 0; g < nparts; ++g) deficit += partials[(long)g * NS + n];
+    if (!(deficit > 0)) return;
+    const double mass = __longlong_as_double((long long)scal[NS + n]);
+    const unsigned long long d = scal[2 * NS + n];
+    if (!(mass > deficit) || d == ~0ull) {
+        scal[3 * NS + 1] = 1;
+        return;
+    }
+    int b, i, j;
+    const hf_cells& B = S.find((int64_t)d, b, i, j);
+    const double vol = B.dy > 0 ? B.dx * B.dy : B.dx;
+    cell_at(B, n, i, j) -= deficit / vol;
+}
+
+// Flux-register corrections of one level from its finer level (BlockAMR.cpp:
+// 247-256), one lane per coarse cell of region: a leaf cell (mask 1) takes,
+// for each face whose other side is covered (mask 2), in the order x-lo,
+// x-hi, y-lo, y-hi, the register (-h dy_c) Fc + (h dy_f) Ff0 + (h dy_f) Ff1
+// (dx for y faces) over its volume: + on the cell's lo faces (the fine box
+// lies below it), - on its hi faces. Coarse and fine face fluxes are
+// addressed in their levels' index spaces. No fused multiply-add.
+__global__ void __launch_bounds__(256) reflux_cells_kernel(double* __restrict__ Uc, hf_patch cp, hf_face_fluxes C,
+                                                           hf_face_fluxes F, int4 region,
+                                                           const uint8_t* __restrict__ cmask, long cmask_pitch,
+                                                           double h, double dxc, double dyc) {
+#pragma clang fp contract(off)
+    constexpr int NV = Topo::NS + 3;
+    const int nx = region.z - region.x + 1;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nx * (region.w - region.y + 1)) return;
+    const int i = region.x + (int)(t % nx), j = region.y + (int)(t / nx);
+    auto kind = [&](int a, int b) -> int {
+        if (a < region.x || a > region.z || b < region.y || b > region.w) return 0;
+        return cmask[(long)b * cmask_pitch + a];
+    };
+    if (kind(i, j) != 1) return;
+    const bool xlo = kind(i - 1, j) == 2, xhi = kind(i + 1, j) == 2;
+    const bool ylo = kind(i, j - 1) == 2, yhi = kind(i, j + 1) == 2;
+    if (!(xlo || xhi || ylo || yhi)) return;
+    const double vol = dxc * dyc;
+    for (int n = 0; n < NV; ++n) {
+        double u = patch_at(Uc, cp, n, i, j);
+        if (xlo) {          // the fine box's x-hi side at coarse face i, fine face 2i
+            const double fc = C.fx[C.fx_offset + n * C.fx_plane + (int64_t)j * C.fx_pitch + i];
+            const double f0 = F.fx[F.fx_offset + n * F.fx_plane + (int64_t)(2 * j) * F.fx_pitch + 2 * i];
+            const double f1 = F.fx[F.fx_offset + n * F.fx_plane + (int64_t)(2 * j + 1) * F.fx_pitch + 2 * i];
+            u += reflux_register(-h * dyc, fc, h * (0.5 * dyc), f0, f1) / vol;
+        }
+        if (xhi) {          // the fine box's x-lo side at coarse face i+1
+            const double fc = C.fx[C.fx_offset + n * C.fx_plane + (int64_t)j * C.fx_pitch + i + 1];
+            const double f0 = F.fx[F.fx_offset + n * F.fx_plane + (int64_t)(2 * j) * F.fx_pitch + 2 * i + 2];
+            const double f1 = F.fx[F.fx_offset + n * F.fx_plane + (int64_t)(2 * j + 1) * F.fx_pitch + 2 * i + 2];
+            u += (-reflux_register(-h * dyc, fc, h * (0.5 * dyc), f0, f1)) / vol;
+        }
+        if (ylo) {
+            const double fc = C.fy[C.fy_offset + n * C.fy_plane + (int64_t)j * C.fy_pitch + i];
+            const double f0 = F.fy[F.fy_offset + n * F.fy_plane + (int64_t)(2 * j) * F.fy_pitch + 2 * i];
+            const double f1 = F.fy[F.fy_offset + n * F.fy_plane + (int64_t)(2 * j) * F.fy_pitch + 2 * i + 1];
+            u += reflux_register(-h * dxc, fc, h * (0.5 * dxc), f0, f1) / vol;
+        }
+        if (yhi) {
+            const double fc = C.fy[C.fy_offset + n * C.fy_plane + (int64_t)(j + 1) * C.fy_pitch + i];
+            const double f0 = F.fy[F.fy_offset + n * F.fy_plane + (int64_t)(2 * j + 2) * F.fy_pitch + 2 * i];
+            const double f1 = F.fy[F.fy_offset + n * F.fy_plane + (int64_t)(2 * j + 2) * F.fy_pitch + 2 * i + 1];
+            u += (-reflux_register(-h * dxc, fc, h * (0.5 * dxc), f0, f1)) / vol;
+        }
+        patch_at(Uc, cp, n, i, j) = u;
+    }
+}
+
 int box_ops(const BoxLaunch& L) {
     const long cells = (long)L.box.nx * L.box.ny;
     const unsigned g256 = (unsigned)((cells + 255) / 256);
@@ -1084,10 +1358,40 @@ int box_ops(const BoxLaunch& L) {
     }
     case BoxLaunch::INTERP: {
         const int4 r = make_int4(L.region[0], L.region[1], L.region[2], L.region[3]);
-        const long cells = (long)((r.z >> 1) - (r.x >> 1) + 1) * ((r.w >> 1) - (r.y >> 1) + 1);
+        // a list: L.total is the largest region's parent count
+        const long cells = L.regions ? (long)L.total
+                                     : (long)((r.z >> 1) - (r.x >> 1) + 1) * ((r.w >> 1) - (r.y >> 1) + 1);
         hipLaunchKernelGGL(counter_init_kernel, dim3(1), dim3(64), 0, L.stream, L.scal, L.first_failed);
-        hipLaunchKernelGGL(interp_kernel, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, L.stream, dm, L.src, L.sp,
-                           L.dst, L.dp, r, L.scal, L.first_failed);
+        hipLaunchKernelGGL(interp_kernel, dim3((unsigned)((cells + 255) / 256), L.regions ? (unsigned)L.nregion : 1u),
+                           dim3(256), 0, L.stream, dm, L.src, L.sp, L.dst, L.dp, r,
+                           reinterpret_cast<const int4*>(L.regions), L.mask, (long)L.mask_pitch, L.mask_nx, L.mask_ny,
+                           L.scal, L.first_failed);
+        break;
+    }
+    case BoxLaunch::REFLUX_CELLS: {
+        const int4 r = make_int4(L.region[0], L.region[1], L.region[2], L.region[3]);
+        const long t = (long)(r.z - r.x + 1) * (r.w - r.y + 1);
+        hipLaunchKernelGGL(reflux_cells_kernel, dim3((unsigned)((t + 255) / 256)), dim3(256), 0, L.stream, L.dst, L.dp,
+                           L.cflux, L.fflux, r, L.mask, (long)L.mask_pitch, L.h, L.dx, L.dy);
+        break;
+    }
+    case BoxLaunch::CELLS_PROPS: {
+        const CellMap S{nullptr, 0, L.blocks, L.block_start, L.nblock};
+        const unsigned g = (unsigned)((L.total + 255) / 256);
+        hipLaunchKernelGGL(cells_props_kernel, dim3(g), dim3(256), 0, L.stream, dm, S, (long)L.total, L.first_failed,
+                           L.scal);
+        break;
+    }
+    case BoxLaunch::CELLS_REPAIR: {
+        const CellMap S{nullptr, 0, L.blocks, L.block_start, L.nblock};
+        const unsigned g = (unsigned)((L.total + 255) / 256);
+        hipLaunchKernelGGL(cells_repair_init_kernel, dim3(1), dim3(64), 0, L.stream, L.scal, L.first_failed);
+        hipLaunchKernelGGL(cells_repair_peak_kernel, dim3(g), dim3(256), 0, L.stream, S, (long)L.total, L.scal);
+        hipLaunchKernelGGL(cells_repair_donor_kernel, dim3(g), dim3(256), 0, L.stream, S, (long)L.total, L.scal);
+        hipLaunchKernelGGL(cells_repair_zero_kernel, dim3(g), dim3(256), 0, L.stream, S, (long)L.total, L.scal,
+                           L.partials, L.first_failed);
+        hipLaunchKernelGGL(cells_repair_donate_kernel, dim3(1), dim3(64), 0, L.stream, S, (int)g, L.scal, L.partials,
+                           L.first_failed);
         break;
     }
     case BoxLaunch::REFLUX: {
@@ -1110,7 +1414,7 @@ int box_ops(const BoxLaunch& L) {
         const int4 r = make_int4(L.region[0], L.region[1], L.region[2], L.region[3]);
         const long t = (long)NV * (r.z - r.x + 1) * (r.w - r.y + 1);
         hipLaunchKernelGGL(avgdown_kernel, dim3((unsigned)((t + 255) / 256)), dim3(256), 0, L.stream, L.src, L.sp, L.dst,
-                           L.dp, r);
+                           L.dp, r, L.mask, (long)L.mask_pitch);
         break;
     }
     case BoxLaunch::REPAIR:

@@ -12,10 +12,63 @@
 
 namespace hf {
 
+// Where cell c of a chemistry batch lives. Flat form (blocks null): cell c
+// of SoA planes U[n*ld + c]. Block form: the cells of blocks[0..nblock) in
+// block order, then j, then i (the order BlockAMR::react gathers them,
+// BlockAMR.cpp:271-281); start[b] is the first cell id of block b (device
+// prefix sums, start[nblock] = total). A block cell whose mask is not 1 (not
+// a leaf) is no part of the batch.
+struct CellMap {
+    const double* U;
+    int64_t ld;
+    const hf_cells* blocks;
+    const int64_t* start;
+    int nblock;
+    // component 0 of cell c and the plane stride; false for a non-leaf cell
+    __device__ __forceinline__ bool locate(int64_t c, const double*& q, int64_t& plane) const {
+        if (!blocks) {
+            q = U + c;
+            plane = ld;
+            return true;
+        }
+        int lo = 0, hi = nblock - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (start[mid] <= c) lo = mid;
+            else hi = mid - 1;
+        }
+        const hf_cells& B = blocks[lo];
+        const int64_t k = c - start[lo];
+        const int64_t j = k / B.nx, i = k - j * B.nx;
+        if (B.mask && B.mask[j * B.mask_pitch + i] != 1) return false;
+        q = B.U + B.offset + j * B.pitch + i;
+        plane = B.plane;
+        return true;
+    }
+    // block form: the block of cell c and its position in it
+    __device__ __forceinline__ const hf_cells& find(int64_t c, int& b, int& i, int& j) const {
+        int lo = 0, hi = nblock - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (start[mid] <= c) lo = mid;
+            else hi = mid - 1;
+        }
+        const hf_cells& B = blocks[lo];
+        const int64_t k = c - start[lo];
+        b = lo;
+        j = (int)(k / B.nx);
+        i = (int)(k - (int64_t)j * B.nx);
+        return B;
+    }
+};
+
 struct ChemLaunch {
     const void* mech;               // device MechData<T>
     const double* U;                // device, nv planes, stride ld
     int64_t ncell, ld;
+    const hf_cells* blocks;         // optional device block table (CellMap); U, ld unused then
+    const int64_t* block_start;
+    int nblock;
     const double* dt;               // device: 1 or ncell values
     int dt_per_cell;
     double* out;                    // device, ns planes, stride ldo
@@ -87,7 +140,7 @@ struct AdjointLaunch {
 // Single-box driver operations (ghost fill, roundoff repair, validation and
 // CFL rate) on a ghosted box in device memory.
 struct BoxLaunch {
-    enum Op { FILL, REPAIR, PROPS, INTERP, AVGDOWN, REFLUX, TAG };
+    enum Op { FILL, REPAIR, PROPS, INTERP, AVGDOWN, REFLUX, TAG, CELLS_PROPS, CELLS_REPAIR, REFLUX_CELLS };
     const void* mech;
     double* U;
     hf_box box;
@@ -106,6 +159,19 @@ struct BoxLaunch {
     double h;                       // REFLUX: stage time step (dx, dy: coarse cell size)
     hf_tag_params tag;              // TAG: criteria
     uint8_t* tags;                  // TAG: device output, nx*ny
+    // CELLS_PROPS, CELLS_REPAIR: a device block table (CellMap's block form), total cells
+    const hf_cells* blocks;
+    const int64_t* block_start;
+    int nblock;
+    int64_t total;
+    double* partials;               // CELLS_REPAIR: device, one NS-vector per workgroup
+    // AVGDOWN, REFLUX_CELLS: the coarse level's mask (2: covered by the fine level);
+    // INTERP: the fine level's mask (children are written only where it is 0)
+    const uint8_t* mask;
+    int64_t mask_pitch;
+    int mask_nx, mask_ny;           // the mask's extent (cells outside it are not written / not scanned)
+    const int32_t* regions;         // INTERP: device list of nregion fine regions (ilo, jlo, ihi, jhi), or null
+    int nregion;
     unsigned long long* scal;       // device scratch (repair: 3 NS + 2 words; props: rate bits;
                                     // interp: limited-cell count)
     unsigned long long* first_failed;

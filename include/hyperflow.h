@@ -117,6 +117,23 @@ typedef struct hf_flux_stats {
     double kernel_ms;
 } hf_flux_stats;
 
+/* A rectangle of cells in device memory: one box of a level, or a whole
+ * level array. Cell (i, j), 0 <= i < nx, 0 <= j < ny, component n is at
+ * U[offset + n*plane + j*pitch + i]. mask (optional, device, one byte per
+ * cell at mask[j*mask_pitch + i]) says what the cell is in the AMR
+ * hierarchy: 0 not a valid cell of its level, 1 a leaf (valid and not
+ * covered by a finer level: BlockAMR::mask, BlockAMR.cpp:165-168), 2 valid
+ * and covered. NULL: every cell is a leaf. dx, dy: the level's cell size
+ * (the cell volume is dx*dy; dy = 0 for a 1-D block). */
+typedef struct hf_cells {
+    double *U;
+    int64_t offset, pitch, plane;
+    int32_t nx, ny;
+    const uint8_t *mask;
+    int64_t mask_pitch;
+    double dx, dy;
+} hf_cells;
+
 /* Create a context: load the Cantera-YAML mechanism (replaces
  * Cantera::newSolution(mechanism, "", "none"), chemistry_batch.cpp:17-18) and
  * upload its tables. opt may be NULL for the reference defaults.
@@ -308,6 +325,68 @@ int hf_average_down(hf_ctx *ctx, const double *fine, const hf_patch *fp, double 
  * (addressed by cp, coarse index space). No fused multiply-add. */
 int hf_reflux(hf_ctx *ctx, double *Uc, const hf_patch *cp, const hf_face_fluxes *coarse, const hf_face_fluxes *fine,
               const int32_t fine_box[4], int32_t sides, double h, double dxc, double dyc);
+
+/* ------------------------------------------------------------------ AMR hierarchy operations
+ * A level of an AMR hierarchy may be held as one array over its whole
+ * domain (plus two ghost layers) in which its boxes are index rectangles;
+ * a per-level hf_cells mask then says which cells are valid (in a box),
+ * which of those are leaves and which are covered by the next finer level.
+ * Boxes of one level read each other's valid cells as ghosts directly. */
+
+/* BlockAMR::react's chemistry (BlockAMR.cpp:271-285): ChemistryBatch::advance
+ * (chemistry_batch.cpp:21-50) over every leaf cell (mask 1, or every cell of
+ * a block without mask) of every block, in block order, then j, then i, all
+ * in one batch read and written in place through the block table (no gather
+ * or scatter). dt: one interval (host or device). All-or-nothing over the
+ * whole batch; first_failed = (block << 32) | (j*nx + i) of the lowest
+ * failing cell. stats->cells counts every block cell, leaves or not. Device
+ * memory only. */
+int hf_chem_advance_cells(hf_ctx *ctx, const hf_cells *blocks, int nblock, const double *dt, int mode,
+                          hf_chem_stats *stats, int64_t *first_failed);
+
+/* BlockAMR::validate over every valid cell of every block (mask != 0;
+ * BlockAMR.cpp:286-298) and stableDt's rate max((|u|+a)/dx + (|v|+a)/dy)
+ * over the leaf cells (:335-346) into *rate. HF_EDOMAIN with first_failed =
+ * (block << 32) | (j*nx + i) of the lowest inadmissible cell. */
+int hf_validate_cells(hf_ctx *ctx, const hf_cells *blocks, int nblock, double *rate, int64_t *first_failed);
+
+/* BlockAMR::repairRoundoff (BlockAMR.cpp:299-334) over the leaf cells of all
+ * blocks, cell volume dx*dy: per species the peak over all leaves and the
+ * donor (the first leaf, in block/j/i order, holding the largest value *
+ * volume); negative values within 64 eps of the peak are zeroed and their
+ * volume-weighted total is taken from the donor. HF_EDOMAIN with the lowest
+ * materially negative cell ((block << 32) | (j*nx + i)), or when the donor's
+ * mass does not exceed the deficit. repairs receives the zeroed count. */
+int hf_repair_cells(hf_ctx *ctx, const hf_cells *blocks, int nblock, int64_t *repairs, int64_t *first_failed);
+
+/* hf_interp_coarse_to_fine over nregion fine regions (regions: host, 4
+ * values each, ilo jlo ihi jhi) in one launch. With fine_mask (device, the
+ * fine level's hf_cells mask, extent fine_extent = {nx, ny} of the fine
+ * level, row pitch fine_mask_pitch), a child is written only inside that
+ * extent where the mask is 0: the ghost cells of a level's boxes that no box
+ * of the level holds and that lie in the domain (FillPatchTwoLevels'
+ * coarse-filled part). first_failed = (region << 32) | parent index. */
+int hf_interp_regions(hf_ctx *ctx, const double *coarse, const hf_patch *cp, double *fine, const hf_patch *fp,
+                      const int32_t *regions, int nregion, const uint8_t *fine_mask, int64_t fine_mask_pitch,
+                      const int32_t fine_extent[2], int64_t *limited, int64_t *first_failed);
+
+/* hf_average_down restricted to the coarse cells of cregion whose coarse
+ * mask (device, row pitch coarse_mask_pitch, coarse level index space) is 2
+ * (covered by the fine level): BlockAMR::average over a whole level pair. */
+int hf_average_down_masked(hf_ctx *ctx, const double *fine, const hf_patch *fp, double *coarse, const hf_patch *cp,
+                           const uint8_t *coarse_mask, int64_t coarse_mask_pitch, const int32_t cregion[4]);
+
+/* The flux-register correction of a level from every box of its finer level
+ * (BlockAMR.cpp:247-256, ClearInternalBorders and Reflux over a whole
+ * BoxArray): each leaf coarse cell of cregion (mask 1) with a covered face
+ * neighbour (mask 2; neighbours outside cregion count as neither) takes, per
+ * such face in the order x-lo, x-hi, y-lo, y-hi, the register of hf_reflux
+ * over its volume, + on its lo faces and - on its hi faces. Faces shared by
+ * two fine boxes, and faces on the domain boundary, carry no correction.
+ * No fused multiply-add; bit-identical to hf_reflux for one box. */
+int hf_reflux_masked(hf_ctx *ctx, double *Uc, const hf_patch *cp, const hf_face_fluxes *coarse,
+                     const hf_face_fluxes *fine, const uint8_t *coarse_mask, int64_t coarse_mask_pitch,
+                     const int32_t cregion[4], double h, double dxc, double dyc);
 
 /* Refinement criteria of BlockAMR::ErrorEst (BlockAMR.cpp:170-200). */
 typedef struct hf_tag_params {

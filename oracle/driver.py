@@ -81,7 +81,10 @@ def fill_ghosts(G, ns, bc, one_d, inlet=None, wedge_i=0):
             G[:, g + 2, :] = G[:, s + 2, :]
             if code == REFLECTING:
                 G[ns + 1, g + 2, :] *= -1.0
-    # FlowBoundary over every ghost cell, reading valid cells only
+    # FlowBoundary over every ghost cell, reading valid cells only (it
+    # changes nothing without an inflow side or a wedge bottom)
+    if INFLOW not in bc and WEDGE not in bc:
+        return
     ys = range(-2, ny + 2) if not one_d else range(0, ny)
     for j in ys:
         for i in range(-2, nx + 2):
