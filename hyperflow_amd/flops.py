@@ -121,8 +121,18 @@ def topology_summary(topo_inc_path):
     return ns, reactions
 
 
-def cell_flops(stats, ns, reactions):
-    """Total algorithmic operations of a chemistry launch from its counters."""
+def dq_jac_flops(n):
+    """Difference-quotient Jacobian beyond its n rate evaluations (which the
+    kernels count as right-hand sides): the increments (a WRMS norm, n
+    maxima and reciprocals) and n columns of n scaled differences."""
+    return 2 * n + SQRT + n * (4 + DIV) + 2 * n * n
+
+
+def cell_flops(stats, ns, reactions, analytic_jac=True):
+    """Total algorithmic operations of a chemistry launch from its counters.
+    analytic_jac False: the Jacobian is CVODE's difference quotient (the
+    large-mechanism kernels), whose rate evaluations are in stats["rhs"]."""
     n = ns - 1
-    return (stats["rhs"] * rhs_flops(ns, reactions) + stats["jac"] * jac_flops(ns, reactions)
+    jac = jac_flops(ns, reactions) if analytic_jac else dq_jac_flops(n)
+    return (stats["rhs"] * rhs_flops(ns, reactions) + stats["jac"] * jac
             + stats["lu"] * lu_flops(n) + stats["newton"] * newton_flops(n) + stats["steps"] * step_flops(n))
