@@ -123,6 +123,9 @@ _SIGNATURES = {
                              POINTER(c_int64), POINTER(c_int64)]),
     "hf_euler_stage": (c_int, [c_void_p, c_void_p, c_void_p, POINTER(hf_box), POINTER(c_int32), _dp, c_double,
                                c_double, POINTER(c_int64), POINTER(c_int32), POINTER(c_int64)]),
+    "hf_box_strang_step": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, POINTER(hf_box),
+                                   POINTER(c_int32), _dp, c_double, c_double, c_double, c_int, c_double, _dp,
+                                   POINTER(c_int32), POINTER(c_int64)]),
     "hf_repair_roundoff": (c_int, [c_void_p, c_void_p, POINTER(hf_box), POINTER(c_int64), POINTER(c_int64)]),
     "hf_validate_box": (c_int, [c_void_p, c_void_p, POINTER(hf_box), c_double, c_double, _dp, POINTER(c_int64)]),
     "hf_lincomb": (c_int, [c_void_p, c_void_p, c_void_p, c_double, c_double, c_int64]),

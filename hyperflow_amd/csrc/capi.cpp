@@ -164,6 +164,9 @@ struct hf_ctx {
     std::string err;
     std::atomic<bool> busy{false};
     DevBuf U, out, dt, status, scal, work, boxes, tiles, blocks, aux[6];
+    // hf_box_strang_step: its captured graph and the tables and scalars it reads
+    ChemGraph step_graph;
+    DevBuf sbox, stiles, scells, sscal;
 };
 
 namespace {
@@ -301,6 +304,7 @@ void hf_destroy(hf_ctx* c) {
         (void)hipSetDevice(c->opt.device);
         if (c->stream) (void)hipStreamSynchronize(c->stream);
         c->graph.reset();
+        c->step_graph.reset();
         if (c->d_mech) (void)hipFree(c->d_mech);
         if (c->stream) (void)hipStreamDestroy(c->stream);
     }
@@ -1476,6 +1480,323 @@ int hf_tag_cells(hf_ctx* c, const double* U, const hf_box* box, const hf_tag_par
         if (first_failed) *first_failed = (int64_t)h[1];
         return fail(c, HF_EDOMAIN, "inadmissible state while tagging at cell " + std::to_string(h[1]));
     }
+    return HF_OK;
+}
+
+// ------------------------------------------------------------------ a whole Strang step as one graph
+namespace {
+
+// Step scalars (device, 64-bit words): [0] CFL rate bits, [1] dt, [2] h =
+// dt/2, [3] time left to the target (<= 0: none), [4] status (bit 0 a flow
+// stage or validation rejected, bit 1 the chemistry failed), [5] repairs,
+// [8] chemistry failure index, [9..16] chemistry counters.
+enum { SS_RATE = 0, SS_DT = 1, SS_H = 2, SS_LEFT = 3, SS_STATUS = 4, SS_REPAIRS = 5, SS_CHEM = 8, SS_WORDS = 32 };
+
+// validation reset: the shared failure index and the CFL-rate slot
+__global__ void step_props_reset(unsigned long long* ff, unsigned long long* rate) {
+    if (threadIdx.x == 0) {
+        *ff = ~0ull;
+        *rate = 0;
+    }
+}
+
+// dt = cfl / rate (min with the time left), h = dt / 2 (BlockAMR::run and
+// stableDt, BlockAMR.cpp:335-346, 409); the status starts from the initial
+// validation
+__global__ void step_dt(unsigned long long* s, const unsigned long long* ff, const unsigned long long* rate,
+                        double cfl) {
+    if (threadIdx.x != 0) return;
+    double* d = reinterpret_cast<double*>(s);
+    double dt = cfl / __longlong_as_double((long long)*rate);
+    const double left = d[SS_LEFT];
+    if (left > 0) dt = fmin(left, dt);
+    d[SS_DT] = dt;
+    d[SS_H] = 0.5 * dt;
+    s[SS_STATUS] = *ff != ~0ull ? 1ull : 0ull;
+    s[SS_REPAIRS] = 0;
+}
+
+// the flux stage's h/dx and h/dy from the device h, in the box table
+__global__ void step_set_h(hf_box* box, const unsigned long long* s, double dx, double dy, int one_d) {
+    if (threadIdx.x != 0) return;
+    const double h = reinterpret_cast<const double*>(s)[SS_H];
+    box->dtdx = h / dx;
+    box->dtdy = one_d ? 0.0 : h / dy;
+}
+
+// fold a repair + validation (or the chemistry) into the step status
+__global__ void step_fold(unsigned long long* s, const unsigned long long* ff, const unsigned long long* donor_fail,
+                          const unsigned long long* repairs, unsigned long long bit) {
+    if (threadIdx.x != 0) return;
+    if (*ff != ~0ull || (donor_fail && *donor_fail)) s[SS_STATUS] |= bit;
+    if (repairs) s[SS_REPAIRS] += *repairs;
+}
+
+__global__ void __launch_bounds__(256) step_copy(double* __restrict__ dst, const double* __restrict__ src, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+// out = a y + b x, the arithmetic of hf_lincomb
+__global__ void __launch_bounds__(256) step_lincomb(double* __restrict__ out, const double* __restrict__ y,
+                                                    const double* __restrict__ x, double a, double b, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = fma(a, y[i], b * x[i]);
+}
+
+struct StepPlan {
+    double *U, *U1, *U2, *old, *backup;
+    hf_box box;
+    int32_t bc[4];
+    double inlet[64];
+    int has_inlet;
+    double dx, dy, cfl;
+    int chem_mode, ntiles;
+    int64_t n;                      // doubles per array
+};
+
+// Enqueue one Strang step on the context's stream: kernels only (no copies
+// from the host, no synchronisation), so the sequence can be captured.
+int step_enqueue(hf_ctx* c, const StepPlan& P) {
+    const int ns = c->ops->ns;
+    const bool one_d = P.box.ny == 1 && P.bc[2] < 0;
+    auto* s = c->sscal.as<unsigned long long>();
+    unsigned long long* scal = c->scal.as<unsigned long long>();     // box operations' words
+    unsigned long long *ff = scal, *rate = scal + 8;
+    const unsigned gcopy = (unsigned)std::min<int64_t>((P.n + 255) / 256, 8192);
+    BoxLaunch L{};
+    L.mech = c->d_mech;
+    L.box = P.box;
+    L.scal = scal + 8;
+    L.first_failed = scal;
+    L.stream = c->stream;
+    L.dx = P.dx;
+    L.dy = one_d ? 0.0 : P.dy;
+    auto validate = [&](double* A) {
+        hipLaunchKernelGGL(step_props_reset, dim3(1), dim3(64), 0, c->stream, ff, rate);
+        BoxLaunch V = L;
+        V.U = A;
+        V.op = BoxLaunch::PROPS;
+        return c->ops->box_ops(V);
+    };
+    auto stage = [&](double* src, double* dst) {
+        BoxLaunch Fl = L;
+        set_fill(Fl, src, P.bc, P.has_inlet ? P.inlet : nullptr);
+        if (c->ops->box_ops(Fl) != HF_OK) return HF_EDEVICE;
+        FluxLaunch F{};
+        F.mech = c->d_mech;
+        F.boxes = c->sbox.as<hf_box>();
+        F.nbox = 1;
+        F.tiles = c->stiles.as<int32_t>();
+        F.ntiles = P.ntiles;
+        F.Uin = src;
+        F.Uout = dst;
+        F.first_order = 0;
+        F.one_d = one_d ? 1 : 0;
+        F.first_failed = scal + 2;           // the raw update is judged by the repair and validation
+        F.counters = scal + 3;
+        F.stream = c->stream;
+        if (c->ops->flux(F) != HF_OK) return HF_EDEVICE;
+        BoxLaunch R = L;
+        R.U = dst;
+        R.op = BoxLaunch::REPAIR;
+        if (c->ops->box_ops(R) != HF_OK) return HF_EDEVICE;
+        BoxLaunch V = L;
+        V.U = dst;
+        V.op = BoxLaunch::PROPS;
+        if (c->ops->box_ops(V) != HF_OK) return HF_EDEVICE;
+        hipLaunchKernelGGL(step_fold, dim3(1), dim3(64), 0, c->stream, s, ff, scal + 8 + 3 * ns + 1, scal + 8 + 3 * ns,
+                           1ull);
+        return HF_OK;
+    };
+    // dt from the CFL rate of the state
+    if (validate(P.U) != HF_OK) return HF_EDEVICE;
+    hipLaunchKernelGGL(step_dt, dim3(1), dim3(64), 0, c->stream, s, ff, rate, P.cfl);
+    hipLaunchKernelGGL(step_copy, dim3(gcopy), dim3(256), 0, c->stream, P.backup, P.U, P.n);
+    for (int half = 0; half < 2; ++half) {
+        // flow(dt/2): Heun, U <- U/2 + E(E(U))/2, validated
+        hipLaunchKernelGGL(step_copy, dim3(gcopy), dim3(256), 0, c->stream, P.old, P.U, P.n);
+        hipLaunchKernelGGL(step_set_h, dim3(1), dim3(64), 0, c->stream, c->sbox.as<hf_box>(), s, P.dx, P.dy,
+                           one_d ? 1 : 0);
+        if (stage(P.U, P.U1) != HF_OK || stage(P.U1, P.U2) != HF_OK) return HF_EDEVICE;
+        hipLaunchKernelGGL(step_lincomb, dim3(gcopy), dim3(256), 0, c->stream, P.U, P.U2, P.old, 0.5, 0.5, P.n);
+        if (validate(P.U) != HF_OK) return HF_EDEVICE;
+        hipLaunchKernelGGL(step_fold, dim3(1), dim3(64), 0, c->stream, s, ff, nullptr, nullptr, 1ull);
+        if (half == 1) break;
+        // react(dt): one batch over the box's cells, in place
+        ChemLaunch C{};
+        const int64_t ncell = (int64_t)P.box.nx * P.box.ny;
+        C.mech = c->d_mech;
+        C.ncell = ncell;
+        C.ld = ncell;
+        C.blocks = c->scells.as<hf_cells>();
+        C.block_start = reinterpret_cast<const int64_t*>(c->scells.as<char>() + sizeof(hf_cells));
+        C.nblock = 1;
+        C.dt = reinterpret_cast<const double*>(s + SS_DT);
+        C.dt_per_cell = 0;
+        C.out = c->out.as<double>();
+        C.ldo = ncell;
+        C.status = c->status.as<int8_t>();
+        C.first_failed = s + SS_CHEM;
+        C.counters = s + SS_CHEM + 1;
+        C.rtol = c->opt.rtol;
+        C.atol = c->opt.atol;
+        C.maxsteps = c->opt.max_steps;
+        C.mode = P.chem_mode;
+        C.precision = c->opt.precision;
+        C.num_cus = c->num_cus;
+        C.waves_per_cu = c->opt.waves_per_cu;
+        C.chunk = c->opt.chunk > 0 ? c->opt.chunk : 8;
+        C.nbins = c->opt.nbins;
+        C.jac_wait = c->opt.jac_wait;
+        C.setup_wait = c->opt.setup_wait;
+        C.workspace = c->work.p;
+        C.workspace_bytes = c->work.n;
+        C.stream = c->stream;
+        C.commit = P.U;
+        C.commit_ld = ncell;
+        if (c->ops->chem(C) != HF_OK) return HF_EDEVICE;
+        hipLaunchKernelGGL(step_fold, dim3(1), dim3(64), 0, c->stream, s, s + SS_CHEM, nullptr, nullptr, 2ull);
+    }
+    return hipGetLastError() == hipSuccess ? HF_OK : HF_EDEVICE;
+}
+
+}  // namespace
+
+int hf_box_strang_step(hf_ctx* c, double* U, double* U1, double* U2, double* old, double* backup, const hf_box* box,
+                       const int32_t bc[4], const double* inlet, double dx, double dy, double cfl, int chem_mode,
+                       double time_left, double* dt_taken, int32_t* taken, int64_t* repairs) {
+    if (!c) return HF_EINVAL;
+    BusyGuard guard(c);
+    if (!guard.ok) return fail(c, HF_EBUSY, "concurrent call is unsupported");
+    if (taken) *taken = 0;
+    if (dt_taken) *dt_taken = 0;
+    if (repairs) *repairs = 0;
+    const int rb = check_bc(c, box, bc, inlet);
+    if (rb != HF_OK) return rb;
+    if (!(dx > 0) || !(dy >= 0) || !(cfl > 0) || (chem_mode != HF_CHEM_PLAIN && chem_mode != HF_CHEM_BATCHED))
+        return fail(c, HF_EINVAL, "invalid cell size, CFL number or chemistry form");
+    BoxLaunch L0{};
+    const int r = box_common(c, L0, box, U);
+    if (r != HF_OK) return r;
+    for (double* p : {U1, U2, old, backup})
+        if (!p || !is_device_ptr(p)) return fail(c, HF_EINVAL, "the step's arrays need device memory");
+    const int nv = c->mech.ns() + 3;
+    if (nv > 64) return fail(c, HF_EINVAL, "mechanism too large for the step graph's inlet state");
+    StepPlan P{};
+    P.U = U;
+    P.U1 = U1;
+    P.U2 = U2;
+    P.old = old;
+    P.backup = backup;
+    P.box = *box;
+    P.box.out_offset = box->in_offset + 2 * box->in_pitch + 2;     // the stage writes the ghosted output's cells
+    P.box.out_pitch = box->in_pitch;
+    P.box.out_plane = box->in_plane;
+    for (int k = 0; k < 4; ++k) P.bc[k] = bc[k];
+    if (inlet) {
+        P.has_inlet = 1;
+        for (int n = 0; n < nv; ++n) P.inlet[n] = inlet[n];
+    }
+    P.dx = dx;
+    P.dy = dy;
+    P.cfl = cfl;
+    P.chem_mode = chem_mode;
+    P.n = box->in_offset + (int64_t)nv * box->in_plane;
+    for (double* p : {U, U1, U2, old, backup})
+        if (!within_allocation(p, 0, P.n - 1)) return fail(c, HF_EINVAL, "a step array is smaller than the box");
+    const bool one_d = box->ny == 1 && bc[2] < 0;
+    const int TX = one_d ? c->ops->flux_tx1 : c->ops->flux_tx2, TY = one_d ? 1 : c->ops->flux_ty2;
+    std::vector<int32_t> tiles;
+    for (int tj = 0; tj < (box->ny + TY - 1) / TY; ++tj)
+        for (int ti = 0; ti < (box->nx + TX - 1) / TX; ++ti) {
+            tiles.push_back(0);
+            tiles.push_back(ti);
+            tiles.push_back(tj);
+        }
+    P.ntiles = (int)(tiles.size() / 3);
+    const int64_t ncell = (int64_t)box->nx * box->ny;
+    const int ns = c->mech.ns();
+    const size_t wbytes = c->ops->chem_workspace(ncell, c->opt.nbins, c->num_cus, c->opt.waves_per_cu);
+    hf_cells cells{U, box->in_offset + 2 * box->in_pitch + 2, box->in_pitch, box->in_plane, box->nx, box->ny,
+                   nullptr, 0, dx, dy};
+    const int64_t start[2] = {0, ncell};
+    if (!c->sbox.ensure(sizeof(hf_box)) || !c->stiles.ensure(sizeof(int32_t) * tiles.size()) ||
+        !c->scells.ensure(sizeof(hf_cells) + sizeof(start)) || !c->sscal.ensure(8 * SS_WORDS) ||
+        !c->scal.ensure(sizeof(unsigned long long) * 256) || !c->out.ensure(sizeof(double) * ns * ncell) ||
+        !c->status.ensure(ncell) || (wbytes && !c->work.ensure(wbytes)))
+        return fail(c, HF_ENOMEM, "device scratch");
+    // a new key (arrays, box, conditions, options, and the context buffers the
+    // kernels address, which other calls may have grown) captures a new graph
+    std::vector<uint64_t> key(sizeof(StepPlan) / 8 + 1, 0);
+    std::memcpy(key.data(), &P, sizeof(StepPlan));
+    auto bits = [](double x) {
+        uint64_t u;
+        std::memcpy(&u, &x, sizeof(u));
+        return u;
+    };
+    for (uint64_t v : {bits(c->opt.rtol), bits(c->opt.atol), (uint64_t)c->opt.max_steps, (uint64_t)c->opt.precision,
+                       (uint64_t)c->opt.waves_per_cu, (uint64_t)c->opt.chunk, (uint64_t)c->opt.nbins,
+                       (uint64_t)c->opt.jac_wait, (uint64_t)c->opt.setup_wait, (uint64_t)c->opt.eager})
+        key.push_back(v);
+    for (const DevBuf* b : {&c->sbox, &c->stiles, &c->scells, &c->sscal, &c->scal, &c->out, &c->status, &c->work}) {
+        key.push_back((uint64_t)(uintptr_t)b->p);
+        key.push_back((uint64_t)b->n);
+    }
+    ChemGraph& g = c->step_graph;
+    const bool fresh = !g.exec || g.key != key;
+    if (fresh) {
+        g.reset();
+        HF_HIP(c, hipMemcpyAsync(c->sbox.p, &P.box, sizeof(hf_box), hipMemcpyHostToDevice, c->stream));
+        HF_HIP(c, hipMemcpyAsync(c->stiles.p, tiles.data(), sizeof(int32_t) * tiles.size(), hipMemcpyHostToDevice,
+                                 c->stream));
+        HF_HIP(c, hipMemcpyAsync(c->scells.p, &cells, sizeof(cells), hipMemcpyHostToDevice, c->stream));
+        HF_HIP(c, hipMemcpyAsync(c->scells.as<char>() + sizeof(hf_cells), start, sizeof(start), hipMemcpyHostToDevice,
+                                 c->stream));
+    }
+    double left = time_left;
+    HF_HIP(c, hipMemcpyAsync(c->sscal.as<char>() + 8 * SS_LEFT, &left, 8, hipMemcpyHostToDevice, c->stream));
+    int how = 0;
+    if (c->opt.eager) {
+        if (step_enqueue(c, P) != HF_OK) return fail(c, HF_EDEVICE, "step launch failed");
+    } else {
+        if (fresh) {
+            if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+                return fail(c, HF_EDEVICE, "graph capture failed");
+            const int rs = step_enqueue(c, P);
+            hipGraph_t graph = nullptr;
+            const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+            if (rs != HF_OK || e != hipSuccess) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return fail(c, HF_EDEVICE, "step capture failed");
+            }
+            g.graph = graph;
+            if (hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0) != hipSuccess) {
+                g.reset();
+                return fail(c, HF_EDEVICE, "step graph instantiation failed");
+            }
+            g.key = key;
+            how = 2;
+        } else {
+            how = 1;
+        }
+        HF_HIP(c, hipGraphLaunch(g.exec, c->stream));
+    }
+    unsigned long long hs[SS_REPAIRS + 1];
+    HF_HIP(c, hipMemcpyAsync(hs, c->sscal.p, sizeof(hs), hipMemcpyDeviceToHost, c->stream));
+    HF_HIP(c, hipStreamSynchronize(c->stream));
+    if (hs[SS_STATUS] != 0) {
+        // a stage needed its first-order redo, a validation or the chemistry
+        // failed: the state goes back to the step's start for the caller's
+        // step with host decisions
+        const unsigned gcopy = (unsigned)std::min<int64_t>((P.n + 255) / 256, 8192);
+        hipLaunchKernelGGL(step_copy, dim3(gcopy), dim3(256), 0, c->stream, U, backup, P.n);
+        HF_HIP(c, hipStreamSynchronize(c->stream));
+        return HF_OK;
+    }
+    if (taken) *taken = how == 0 ? 3 : how;
+    if (dt_taken) std::memcpy(dt_taken, &hs[SS_DT], sizeof(double));
+    if (repairs) *repairs = (int64_t)hs[SS_REPAIRS];
     return HF_OK;
 }
 

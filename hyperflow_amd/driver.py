@@ -44,7 +44,7 @@ class BoxDriver:
     """
 
     def __init__(self, mechanism, nx, ny, dx, dy, bc, cfl=0.5, chem_mode=HF_CHEM_BATCHED, max_attempts=18,
-                 inlet=None, wedge_i=0):
+                 inlet=None, wedge_i=0, graph=True):
         import torch
         self.torch = torch
         self.mech = mechanism
@@ -83,6 +83,11 @@ class BoxDriver:
         self.first_order_stages = 0
         self.repairs = 0
         self._h = mechanism.handle
+        # whole steps as one replayed HIP graph (hf_box_strang_step); a step the
+        # graph hands back (a stage to redo first-order, a failure) is taken here
+        self.graph = bool(graph)
+        self.graph_steps = 0
+        self.host_steps = 0
 
     # ------------------------------------------------------------ state
     def interior(self, A=None):
@@ -154,8 +159,35 @@ class BoxDriver:
         rate = self._rate if self._rate is not None else self._validate(self.U)
         return self.cfl / rate
 
+    def _graph_step(self, target):
+        """hf_box_strang_step: (dt, taken) with taken 0 when the step must be
+        taken with host decisions (the state is back at the step's start)."""
+        box = self._box()
+        dt, taken, rep = c_double(0.0), c_int32(0), c_int64(0)
+        left = (target - self.time) if target is not None else -1.0
+        r = load().hf_box_strang_step(self._h, self.U.data_ptr(), self.U1.data_ptr(), self.U2.data_ptr(),
+                                      self.old.data_ptr(), self.backup.data_ptr(), byref(box), self.bc, self.inlet,
+                                      self.dx, 0.0 if self.one_d else self.dy, self.cfl, self.chem.mode, left,
+                                      byref(dt), byref(taken), byref(rep))
+        check(self._h, r)
+        if taken.value:
+            self.repairs += rep.value
+        return dt.value, taken.value
+
     def step(self, target=None):
         """One Strang step; returns the dt taken."""
+        if self.graph:
+            torch = self.torch
+            torch.cuda.current_stream().synchronize()     # torch's writes to the arrays are done
+            dt, taken = self._graph_step(target)
+            if taken:
+                self._rate = None
+                self.last_graph = taken
+                self.graph_steps += 1
+                self.time = target if target is not None and dt >= target - self.time else self.time + dt
+                self.steps += 1
+                return dt
+        self.host_steps += 1
         dt = self.stable_dt()
         if target is not None:
             dt = min(target - self.time, dt)

@@ -431,6 +431,28 @@ int hf_tag_cells(hf_ctx *ctx, const double *U, const hf_box *box, const hf_tag_p
 int hf_euler_stage(hf_ctx *ctx, double *Uin, double *Uout, const hf_box *box, const int32_t bc[4], const double *inlet,
                    double dx, double dy, int64_t *repairs, int32_t *first_order, int64_t *first_failed);
 
+/* One whole Strang step of a single-level box (BlockAMR::run's loop body,
+ * BlockAMR.cpp:400-425) as one HIP graph, captured on the first call and
+ * replayed while the arrays, box, boundary, options and context buffers
+ * repeat: dt = cfl / (the CFL rate of U) (capped at time_left when > 0),
+ * flow(dt/2), react(dt), flow(dt/2), where flow is the Heun average of two
+ * hf_euler_stage stages (ghost fill, flux stage, roundoff repair,
+ * validation) written into U, and react one chemistry batch over the box's
+ * cells (chem_mode) in place. U, U1, U2, old, backup: device arrays in the
+ * box's input layout (the box's out_* fields are ignored); dt and h live in
+ * device memory, so no host value enters the graph. Every decision of the
+ * step stays on the device except one: when any stage is rejected by the
+ * repair or the validation (the reference then redoes it first-order), or
+ * the chemistry fails, U is restored to the step's start and *taken is 0, so
+ * that the caller takes the step with host decisions (hf_euler_stage, the
+ * dt halving of BlockAMR.cpp:416). Otherwise *taken is 1 (replayed), 2
+ * (captured and launched) or 3 (hf_options.eager: kernels launched one by
+ * one), *dt_taken the step and *repairs the zeroed values. The results are
+ * bit-identical to the same step taken call by call. */
+int hf_box_strang_step(hf_ctx *ctx, double *U, double *U1, double *U2, double *old, double *backup,
+                       const hf_box *box, const int32_t bc[4], const double *inlet, double dx, double dy, double cfl,
+                       int chem_mode, double time_left, double *dt_taken, int32_t *taken, int64_t *repairs);
+
 /* BlockAMR::repairRoundoff (BlockAMR.cpp:299-334) on the valid cells of one
  * box of equal cell volumes: negative species within 64 eps of the species'
  * peak are zeroed and their total is taken from the first cell holding the

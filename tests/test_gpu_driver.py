@@ -155,3 +155,32 @@ def test_fused_euler_stage_equals_separate_calls(mech_h2, one_d):
                          byref(ff))
     assert r == hf._lib.HF_EDOMAIN and fo.value == 1
     assert ff.value == (ny // 2 if not one_d else 0) * nx + 7
+
+
+@pytest.mark.parametrize("case", ["shock_tube", "channel"])
+def test_graph_steps_equal_host_steps(mech_h2, case):
+    """Whole Strang steps replayed as one HIP graph (hf_box_strang_step) give
+    the same bits as the same steps taken call by call, including a step to a
+    target time; after the first capture every step is a replay."""
+    print(f"[gpu] graph step vs host step, {case}", flush=True)
+    if case == "shock_tube":
+        Q, dx = shock_tube_1d(mech_h2, nx=400, length=0.02, hot_x=0.002)
+        args = (Q.shape[2], 1, dx, 0.0, (HF_BC_REFLECTING, HF_BC_TRANSMISSIVE, -1, -1))
+    else:
+        from hyperflow_amd.driver import channel_2d
+        Q, dx = channel_2d(mech_h2, nx=96, ny=32, length=96 * 0.2 / 1024)
+        args = (96, 32, dx, dx, (HF_BC_REFLECTING, HF_BC_TRANSMISSIVE, HF_BC_REFLECTING, HF_BC_REFLECTING))
+    runs = []
+    for graph in (False, True):
+        d = BoxDriver(mech_h2, *args, graph=graph)
+        d.set_state(Q)
+        dts = [d.step() for _ in range(6)]
+        dts.append(d.step(target=d.time + 0.25 * dts[-1]))
+        runs.append((d.state(), dts, d))
+    (a, da, ha), (b, db, g) = runs
+    print(f"[gpu]   graph steps {g.graph_steps}, host steps {g.host_steps}, last {getattr(g, 'last_graph', None)}",
+          flush=True)
+    assert da == db
+    assert np.array_equal(a, b)
+    assert g.graph_steps == 7 and g.host_steps == 0 and g.last_graph == 1
+    assert ha.graph_steps == 0
