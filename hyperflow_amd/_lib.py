@@ -31,7 +31,8 @@ class hf_chem_stats(Structure):
     _fields_ = [("cells", c_int64), ("steps", c_int64), ("rhs", c_int64), ("jac", c_int64), ("lu", c_int64),
                 ("newton", c_int64), ("conv_failures", c_int64), ("err_failures", c_int64),
                 ("max_cell_steps", c_int64), ("team", c_int), ("kernel_ms", c_double), ("chunk", c_int),
-                ("graph", c_int)]
+                ("graph", c_int), ("waves_per_cu", c_int), ("nbins", c_int), ("jac_wait", c_int),
+                ("setup_wait", c_int), ("tuned", c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -66,7 +67,8 @@ class hf_tag_params(Structure):
 
 
 class hf_flux_stats(Structure):
-    _fields_ = [("cells", c_int64), ("faces", c_int64), ("first_order_sides", c_int64), ("kernel_ms", c_double)]
+    _fields_ = [("cells", c_int64), ("faces", c_int64), ("first_order_sides", c_int64), ("kernel_ms", c_double),
+                ("tile_x", c_int), ("tile_y", c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

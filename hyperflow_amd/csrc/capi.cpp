@@ -103,31 +103,93 @@ bool is_device_ptr(const void* ptr) {
 
 }  // namespace
 
-// Online tuning of the batched form's refill group (hf_options.chunk = 0):
-// the first calls of a size class (cell counts within a factor of two) try
-// each candidate once and the fastest per cell is kept for the following
-// calls. Each cell's result is independent of the choice.
+// Online tuning of the batched form (every hf_options tuning value left at
+// 0): coordinate descent over the refill group, the resident waves per CU,
+// the stiffness bins and the Jacobian / setup grouping waits. The first call
+// of a size class (cell counts within a factor of two) runs the defaults;
+// each following call tries one candidate of one parameter, and a candidate
+// faster per cell than the best so far is kept. After every candidate has
+// run, the best configuration serves the size class. Each cell's result is
+// independent of every one of these values, so tuning never changes a
+// result.
 struct ChemTuner {
-    static constexpr int kN = 3;
-    static constexpr int kCand[kN] = {8, 4, 16};
+    enum { CHUNK, WAVES, BINS, JWAIT, SWAIT, NPARAM };
+    static constexpr int kCand = 3;
+    // candidate values per parameter, the default first
+    static constexpr int kLane[NPARAM][kCand] = {{8, 4, 16}, {4, 2, 8}, {32, 16, 64}, {16, 8, 32}, {2, 1, 4}};
+    static constexpr int kWave[NPARAM][kCand] = {{8, 8, 8}, {8, 4, 16}, {32, 16, 64}, {16, 16, 16}, {2, 2, 2}};
     int64_t size = 0;
-    int calls = 0, best = 0;
-    double per_cell[kN] = {};
-    int pick(int64_t ncell) {
-        if (size == 0 || ncell > 2 * size || 2 * ncell < size) {
+    bool wave = false;               // one cell per wave (large mechanisms): only waves and bins matter
+    int best[NPARAM] = {};
+    double best_t = 0;
+    int param = 0, cand = 0;         // the trial this call runs (cand 0: the defaults, first call)
+    bool done = false;
+    const int (*table())[kCand] { return wave ? kWave : kLane; }
+    bool tunable(int p) const { return !wave || p == WAVES || p == BINS; }
+    // the configuration of this call
+    void pick(int64_t ncell, bool wave_form, int* cfg) {
+        if (size == 0 || ncell > 2 * size || 2 * ncell < size || wave_form != wave) {
             size = ncell;
-            calls = 0;
-            best = 0;
-            for (double& t : per_cell) t = 0;
+            wave = wave_form;
+            for (int p = 0; p < NPARAM; ++p) best[p] = table()[p][0];
+            best_t = 0;
+            param = 0;
+            cand = 0;
+            done = false;
         }
-        return kCand[calls < kN ? calls : best];
+        for (int p = 0; p < NPARAM; ++p) cfg[p] = best[p];
+        if (!done && cand > 0) cfg[param] = table()[param][cand];
+    }
+    void advance() {
+        for (;;) {
+            if (++cand >= kCand) {
+                cand = 1;
+                if (++param >= NPARAM) {
+                    done = true;
+                    return;
+                }
+            }
+            if (tunable(param)) return;
+        }
     }
     void record(double ms, int64_t ncell) {
-        if (calls >= kN || ncell <= 0) return;
-        per_cell[calls] = ms / (double)ncell;
-        if (++calls == kN)
-            for (int k = 1; k < kN; ++k)
+        if (done || ncell <= 0) return;
+        const double t = ms / (double)ncell;
+        if (cand == 0) {
+            best_t = t;
+            param = 0;
+        } else if (t < best_t) {
+            best_t = t;
+            best[param] = table()[param][cand];
+        }
+        advance();
+    }
+};
+
+// Online choice of the 2-D flux tile shape: the first calls of a size class
+// (cell counts within a factor of two) run each shape once and the fastest
+// per cell serves the following calls.
+struct FluxTuner {
+    int64_t size = 0;
+    int calls = 0, best = 0, n = 1;
+    double per_cell[3] = {};
+    int pick(int64_t cells, int nshape) {
+        if (size == 0 || cells > 2 * size || 2 * cells < size || nshape != n) {
+            size = cells;
+            n = nshape;
+            calls = 0;
+            best = 0;
+        }
+        return calls < n ? calls : best;
+    }
+    void record(double ms, int64_t cells) {
+        if (calls >= n || cells <= 0 || !(ms > 0)) return;
+        per_cell[calls] = ms / (double)cells;
+        if (++calls == n) {
+            best = 0;
+            for (int k = 1; k < n; ++k)
                 if (per_cell[k] < per_cell[best]) best = k;
+        }
     }
 };
 
@@ -155,6 +217,7 @@ struct ChemGraph {
 struct hf_ctx {
     Mechanism mech;
     ChemTuner tuner;
+    FluxTuner ftuner;
     ChemGraph graph;
     const TopoOps* ops = nullptr;
     hf_options opt{};
@@ -486,12 +549,23 @@ int chem_impl(hf_ctx* c, double* U, int64_t ncell, int64_t ld, const double* dt,
     L.precision = c->opt.precision;
     L.num_cus = c->num_cus;
     L.waves_per_cu = c->opt.waves_per_cu;
-    const bool tune = mode == HF_CHEM_BATCHED && c->opt.chunk == 0;
-    L.chunk = tune ? c->tuner.pick(ncell) : c->opt.chunk;
+    L.chunk = c->opt.chunk;
     L.nbins = c->opt.nbins;
     L.jac_wait = c->opt.jac_wait;
     L.setup_wait = c->opt.setup_wait;
-    const size_t wbytes = c->ops->chem_workspace(ncell, c->opt.nbins, c->num_cus, c->opt.waves_per_cu);
+    // online tuning when every tuning value is left to the library
+    const bool tune = mode == HF_CHEM_BATCHED && !c->opt.chunk && !c->opt.waves_per_cu && !c->opt.nbins &&
+                      !c->opt.jac_wait && !c->opt.setup_wait;
+    if (tune) {
+        int cfg[ChemTuner::NPARAM];
+        c->tuner.pick(ncell, c->ops->ns > 16, cfg);
+        L.chunk = cfg[ChemTuner::CHUNK];
+        L.waves_per_cu = cfg[ChemTuner::WAVES];
+        L.nbins = cfg[ChemTuner::BINS];
+        L.jac_wait = cfg[ChemTuner::JWAIT];
+        L.setup_wait = cfg[ChemTuner::SWAIT];
+    }
+    const size_t wbytes = c->ops->chem_workspace(ncell, L.nbins, c->num_cus, L.waves_per_cu);
     if (wbytes) {
         if (!c->work.ensure(wbytes)) return fail(c, HF_ENOMEM, "device workspace");
         L.workspace = c->work.p;
@@ -531,6 +605,11 @@ int chem_impl(hf_ctx* c, double* U, int64_t ncell, int64_t ld, const double* dt,
         stats->kernel_ms = ms;
         stats->chunk = mode == HF_CHEM_BATCHED ? L.chunk : 0;
         stats->graph = how;
+        stats->waves_per_cu = mode == HF_CHEM_BATCHED ? L.waves_per_cu : 0;
+        stats->nbins = mode == HF_CHEM_BATCHED ? L.nbins : 0;
+        stats->jac_wait = mode == HF_CHEM_BATCHED ? L.jac_wait : 0;
+        stats->setup_wait = mode == HF_CHEM_BATCHED ? L.setup_wait : 0;
+        stats->tuned = tune ? (c->tuner.done ? 2 : 1) : 0;
     }
     if (tune && host_scal[0] == ~0ull) c->tuner.record(ms, ncell);
     if (host_scal[0] != ~0ull) {
@@ -618,7 +697,11 @@ int flux_stage_impl(hf_ctx* c, const hf_box* boxes, int nbox, const double* Uin,
         out_extent = std::max<int64_t>(out_extent, B.out_offset + (nv - 1) * B.out_plane + (B.ny - 1) * B.out_pitch + B.nx);
         cells += (int64_t)B.nx * B.ny;
     }
-    const int TX = one_d ? c->ops->flux_tx1 : c->ops->flux_tx2, TY = one_d ? 1 : c->ops->flux_ty2;
+    // 2-D tile shape: tuned online per size class (results do not depend on it)
+    const bool tune_shape = !one_d && !noalloc && c->ops->flux_nshape > 1;
+    const int shape = tune_shape ? c->ftuner.pick(cells, c->ops->flux_nshape) : 0;
+    const int TX = one_d ? c->ops->flux_tx1 : c->ops->flux_shape_tx[shape];
+    const int TY = one_d ? 1 : c->ops->flux_shape_ty[shape];
     std::vector<int32_t> tiles;
     for (int b = 0; b < nbox; ++b)
         for (int tj = 0; tj < (boxes[b].ny + TY - 1) / TY; ++tj)
@@ -700,6 +783,7 @@ int flux_stage_impl(hf_ctx* c, const hf_box* boxes, int nbox, const double* Uin,
     L.Uout = dout;
     L.first_order = first_order ? 1 : 0;
     L.one_d = one_d ? 1 : 0;
+    L.shape = shape;
     L.first_failed = scal;
     L.counters = scal + 1;
     L.fluxes = fluxes ? reinterpret_cast<const hf_face_fluxes*>(flux_dev) : nullptr;
@@ -718,7 +802,10 @@ int flux_stage_impl(hf_ctx* c, const hf_box* boxes, int nbox, const double* Uin,
         stats->faces = (int64_t)hs[1];
         stats->first_order_sides = (int64_t)hs[2];
         stats->kernel_ms = ms;
+        stats->tile_x = TX;
+        stats->tile_y = TY;
     }
+    if (tune_shape) c->ftuner.record(ms, cells);
     if (hs[0] != ~0ull) {
         if (first_failed) *first_failed = (int64_t)hs[0];
         return fail(c, HF_EDOMAIN, "inadmissible state after flux stage at box " + std::to_string(hs[0] >> 32) +

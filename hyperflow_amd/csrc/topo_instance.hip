@@ -593,21 +593,36 @@ static_assert(sizeof(double) * (Topo::NS + 3) * ((TX2 + 4) * (TY2 + 4) + (TX2 + 
                   sizeof(double) * (TX2 + 4) * (TY2 + 4) <= 65536,
               "flux tile exceeds 64 KB of LDS");
 
+// Other 2-D tile shapes of 256 lanes for the tuner (small mechanisms only):
+// 31 x 7 (wide rows) and 7 x 31 (tall columns); every shape computes each
+// face and cell update from the same staged values, so the results do not
+// depend on the shape.
+constexpr int NSHAPE = kLargeState ? 1 : 3;
+constexpr int SHAPE_TX[3] = {TX2, 31, 7}, SHAPE_TY[3] = {TY2, 7, 31};
+static_assert(kLargeState || sizeof(double) * (Topo::NS + 3) * (35 * 11 + 32 * 8) + sizeof(double) * 35 * 11 <= 65536,
+              "flux tile exceeds 64 KB of LDS");
+
+template <int TX, int TY, bool TWO_D>
+void launch_flux(const FluxLaunch& L, unsigned grid) {
+    hipLaunchKernelGGL((flux_stage_kernel<Topo, TX, TY, TWO_D>), dim3(grid),
+                       dim3(FluxTile<Topo, TX, TY, TWO_D>::THREADS), 0, L.stream, static_cast<const MD*>(L.mech),
+                       L.boxes, L.tiles, L.ntiles, L.Uin, L.Uout, L.first_order, L.first_failed, L.counters,
+                       L.fluxes);
+}
+
 int flux(const FluxLaunch& L) {
     if (L.ntiles == 0) return HF_OK;
     {
     StreamTimer timer(L.kernel_ms, L.stream);
     const unsigned grid = (unsigned)((L.ntiles + 7) / 8) * 8;   // XCD remap needs a multiple of 8
     if (L.one_d) {
-        hipLaunchKernelGGL((flux_stage_kernel<Topo, TX1, 1, false>), dim3(grid),
-                           dim3(FluxTile<Topo, TX1, 1, false>::THREADS), 0, L.stream, static_cast<const MD*>(L.mech),
-                           L.boxes, L.tiles, L.ntiles, L.Uin, L.Uout, L.first_order, L.first_failed, L.counters,
-                           L.fluxes);
+        launch_flux<TX1, 1, false>(L, grid);
+    } else if constexpr (!kLargeState) {
+        if (L.shape == 1) launch_flux<31, 7, true>(L, grid);
+        else if (L.shape == 2) launch_flux<7, 31, true>(L, grid);
+        else launch_flux<TX2, TY2, true>(L, grid);
     } else {
-        hipLaunchKernelGGL((flux_stage_kernel<Topo, TX2, TY2, true>), dim3(grid),
-                           dim3(FluxTile<Topo, TX2, TY2, true>::THREADS), 0, L.stream, static_cast<const MD*>(L.mech),
-                           L.boxes, L.tiles, L.ntiles, L.Uin, L.Uout, L.first_order, L.first_failed, L.counters,
-                           L.fluxes);
+        launch_flux<TX2, TY2, true>(L, grid);
     }
     const hipError_t err = hipGetLastError();
     if (err != hipSuccess) return HF_EDEVICE;
@@ -1694,7 +1709,8 @@ int tangent(const AdjointLaunch& L) { return sens(L, true); }
 extern const TopoOps HF_OPS_NAME(HF_TOPO);
 const TopoOps HF_OPS_NAME(HF_TOPO) = {HF_STR(HF_TOPO), Topo::topology, Topo::NS, Topo::NR, sizeof(MD), fill, chem,
                                       chem_workspace, flux, props, adjoint, tangent, sens_workspace,
-                                      box_ops, TX2, TY2, TX1};
+                                      box_ops, TX2, TY2, TX1, NSHAPE, {SHAPE_TX[0], SHAPE_TX[1], SHAPE_TX[2]},
+                                      {SHAPE_TY[0], SHAPE_TY[1], SHAPE_TY[2]}};
 #endif
 
 }  // namespace hf

@@ -102,6 +102,7 @@ struct FluxLaunch {
     double* Uout;
     int first_order;
     int one_d;                      // x-only stage: tiles are flux_tx1 x 1, else flux_tx2 x flux_ty2
+    int shape;                      // 2-D tile shape index (flux_shape_tx/ty), 0 the default
     unsigned long long* first_failed;  // device scalar
     unsigned long long* counters;   // device: [faces, first-order sides]
     const hf_face_fluxes* fluxes;   // device copy of the per-box face-flux outputs, or null
@@ -193,6 +194,8 @@ struct TopoOps {
     size_t (*sens_workspace)(int64_t ncell, int nsteps, int ckpt_every);
     int (*box_ops)(const BoxLaunch& L);
     int flux_tx2, flux_ty2, flux_tx1;               // flux tile shapes: 2-D TX x TY, 1-D TX x 1
+    int flux_nshape;                                // 2-D tile shapes available to the tuner (shape 0 the default)
+    int flux_shape_tx[3], flux_shape_ty[3];
 };
 
 // All compiled topologies (null-terminated).

@@ -63,8 +63,12 @@ typedef struct hf_options {
     int max_steps;      /* per cell; reference: 5000 (chemistry.cpp:25) */
     int precision;      /* HF_PREC_FP64 or HF_PREC_MIXED */
     int device;         /* HIP device ordinal */
-    /* batched-form tuning; 0 selects the default (waves_per_cu 4, nbins 32) or,
-     * for chunk, online tuning over the first calls of a size class */
+    /* batched-form tuning. When all five tuning values (waves_per_cu, chunk,
+     * nbins, jac_wait, setup_wait) are 0 the library tunes them online per
+     * size class (coordinate descent over candidates on the first calls;
+     * hf_chem_stats reports the values used); otherwise a 0 selects that
+     * value's default (waves_per_cu 4 lanes / 8 waves, chunk 8, nbins 32,
+     * jac_wait 16, setup_wait 2). Results do not depend on any of them. */
     int waves_per_cu;   /* resident waves per CU of the persistent kernel */
     int chunk;          /* idle lanes of a wave refilled together from the queue */
     int nbins;          /* stiffness bins */
@@ -89,10 +93,13 @@ typedef struct hf_chem_stats {
     int64_t max_cell_steps; /* largest step count of any cell */
     int team;               /* lanes of work (ChemistryBatch::lastTeamSize analogue) */
     double kernel_ms;       /* device time of the integration kernel(s) */
-    int chunk;              /* batched form: idle lanes refilled per queue claim in this call
-                               (the online-tuned value when hf_options.chunk is 0) */
+    int chunk;              /* batched form: idle lanes refilled per queue claim in this call */
     int graph;              /* 0 kernels launched one by one, 1 cached HIP graph replayed,
                                2 graph captured by this call and launched */
+    /* batched form: the values this call ran with (the online tuner's when
+     * every hf_options tuning value is 0; no value changes any result) */
+    int waves_per_cu, nbins, jac_wait, setup_wait;
+    int tuned;              /* 0 fixed by the options, 1 a tuning trial, 2 the tuned configuration */
 } hf_chem_stats;
 
 /* One ghosted SoA box for a flux stage. Element (component n, cell i, j) of
@@ -115,6 +122,9 @@ typedef struct hf_flux_stats {
     int64_t cells, faces;
     int64_t first_order_sides;  /* face sides that fell back to first order (BlockAMR.cpp:221-222) */
     double kernel_ms;
+    int tile_x, tile_y;         /* the tile shape this call ran with (2-D: chosen online per size
+                                   class among 15 x 15, 31 x 7 and 7 x 31 for small mechanisms;
+                                   the results do not depend on it) */
 } hf_flux_stats;
 
 /* A rectangle of cells in device memory: one box of a level, or a whole

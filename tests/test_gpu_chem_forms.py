@@ -356,3 +356,27 @@ def test_concurrent_call_is_ebusy():
     assert seen is not None and seen.code == _lib.HF_EBUSY, seen
     assert "nested or concurrent" in seen.message
     assert np.array_equal(small.cpu().numpy(), small0)
+
+
+def test_online_tuner_keeps_every_result():
+    """With every tuning value left to the library, the first calls of a size
+    class try candidate configurations (refill group, resident waves, bins,
+    Jacobian and setup waits) one at a time; every call gives the same bits,
+    and the tuner settles on one configuration."""
+    print("[gpu] online tuner, 12 calls of 16384 cells", flush=True)
+    mech = hf.Mechanism(H2, precision=hf.HF_PREC_MIXED)
+    U0, dt = mech.synth_cells(0, 2, 16384)
+    U0d, dtd = dev(U0), dev(dt)
+    b = hf.ChemistryBatch(mech, mode=hf.HF_CHEM_BATCHED)
+    first, seen, tuned = None, set(), []
+    for _ in range(12):
+        Ud = U0d.clone()
+        st = b.advance(Ud, dtd)
+        cfg = (st["chunk"], st["waves_per_cu"], st["nbins"], st["jac_wait"], st["setup_wait"])
+        seen.add(cfg)
+        tuned.append(st["tuned"])
+        out = Ud.cpu().numpy()
+        first = out if first is None else first
+        assert np.array_equal(out, first)
+    print(f"[gpu]   {len(seen)} configurations tried, states {tuned}, final {cfg}", flush=True)
+    assert len(seen) >= 6 and tuned[0] == 1 and tuned[-1] == 2

@@ -145,3 +145,22 @@ def test_dme_flux_stage_one_dimensional(oracle_dme):
     out = torch.zeros((42, 1, nx), dtype=torch.float64, device="cuda")
     FluxSweep(mech).stage([box_for_arrays(G, out, 3e-4, 0.0, WALL_XLO)], dev(G), out)
     assert normalised_error(out.cpu().numpy(), ref) < TOL
+
+
+def test_tile_shapes_give_the_same_stage(oracle_h2, mech_h2):
+    """The flux stage's online tile-shape choice (15 x 15, 31 x 7, 7 x 31 on
+    the first calls of a size class) never changes the result."""
+    print("[gpu] flux tile shapes 70x45", flush=True)
+    G = smooth_box(oracle_h2, 70, 45, seed=77)
+    Gd = dev(G)
+    sweep = FluxSweep(hf.Mechanism(H2))
+    ref, shapes = None, []
+    for _ in range(4):
+        out = torch.zeros((G.shape[0], 45, 70), dtype=torch.float64, device="cuda")
+        st = sweep.stage([box_for_arrays(G, out, 2e-4, 1.5e-4)], Gd, out)
+        shapes.append((st["tile_x"], st["tile_y"]))
+        o = out.cpu().numpy()
+        ref = o if ref is None else ref
+        assert np.array_equal(o, ref)
+    print(f"[gpu]   shapes {shapes}", flush=True)
+    assert shapes[:3] == [(15, 15), (31, 7), (7, 31)] and shapes[3] in shapes[:3]

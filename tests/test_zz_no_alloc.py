@@ -66,7 +66,11 @@ def test_noalloc_matches_allocating_stage_twice(oracle_h2, sizes):
         assert np.array_equal(got, ref)
         for (a, b), (c, d) in zip(got_fl, ref_fl):
             assert np.array_equal(a, c) and np.array_equal(b, d)
-        assert (st_b.cells, st_b.faces, st_b.first_order_sides) == (st_a.cells, st_a.faces, st_a.first_order_sides)
+        assert st_b.cells == st_a.cells
+        # faces computed count a face two tiles share twice: equal for the same tile shape
+        # (the allocating stage may be trying another shape for its tuner)
+        if (st_a.tile_x, st_a.tile_y) == (st_b.tile_x, st_b.tile_y):
+            assert (st_b.faces, st_b.first_order_sides) == (st_a.faces, st_a.first_order_sides)
         print(f"[gpu]   call {call}: {len(sz)} boxes, {st_b.faces} faces, identical", flush=True)
 
 
