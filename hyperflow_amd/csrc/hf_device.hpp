@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 #include <utility>
 
 // Lambdas that carry per-reaction or per-face code must be inlined: a lambda

@@ -13,8 +13,9 @@
 // convergence test and Jacobian reuse, difference-quotient Jacobian
 // (cvLsDenseDQJac, the reference's configuration). The Jacobian is kept in
 // a per-wave global slot (row = lane, one entry per column); I - gamma J is
-// formed and factored in LDS, each lane updating its own row against the
-// broadcast pivot row.
+// formed and factored in LDS: in FP32 by a blocked LU whose trailing updates
+// run on the matrix cores, in FP64 row by row against the broadcast pivot
+// row.
 #pragma once
 #include "hf_device.hpp"
 
@@ -444,16 +445,27 @@ struct WaveBdf {
         }
         return true;
     }
+    // lu = I - gamma J from the wave's global Jacobian slot; rows N..63 are
+    // zero. (Loading 16 columns at a time measured slower than this rolled
+    // loop: the wider batch spills.)
+    __device__ __forceinline__ void form_newton() {
+        const MT gm = (MT)gamma;
+        #pragma unroll 2
+        for (int j = 0; j < N; ++j)
+            lu[j * 64 + lane] = lane < N ? (lane == j ? MT(1) : MT(0)) - gm * jg[j * 64 + lane] : MT(0);
+    }
+    __device__ __forceinline__ bool factor() {
+        if constexpr (std::is_same<MT, float>::value) return factor_blocked();
+        else return factor_rows();
+    }
     // I - gamma J factored in LDS without pivoting (reciprocal pivots on the
     // diagonal), one row per lane. Step k reads pivot row k as broadcasts
     // (one address for the whole wave) and each lane below it updates its own
     // row; rolled loops keep the register footprint small, which the
     // unrolled register form (38 values per lane and ~700 lane broadcasts
-    // into scalar registers) did not.
-    __device__ __forceinline__ bool factor() {
-        const MT gm = (MT)gamma;
-        #pragma unroll 2
-        for (int j = 0; j < N; ++j) lu[j * 64 + lane] = (lane == j ? MT(1) : MT(0)) - gm * jg[j * 64 + lane];
+    // into scalar registers) did not. Used for the FP64 Newton matrix.
+    __device__ __forceinline__ bool factor_rows() {
+        form_newton();
         __syncthreads();
         bool ok = true;
         #pragma unroll 1
@@ -477,19 +489,135 @@ struct WaveBdf {
         }
         return ok;
     }
+    // The FP32 Newton matrix: blocked right-looking LU without pivoting, in
+    // panels of 16 columns. For the panel starting at column k0:
+    //  (a) rows k0.. of its 16 columns are factored in registers, one row per
+    //      lane, each pivot row broadcast with readlane (no barrier per pivot);
+    //  (b) the row block right of it, U12 = L11^-1 A12, is solved one column
+    //      per lane, L11 read from the panel registers;
+    //  (c) the trailing matrix A22 -= L21 U12 is updated on the matrix cores,
+    //      16 x 16 tiles by v_mfma_f32_16x16x4f32 (lane l holds A[l%16][l/16],
+    //      B[l/16][l%16] and D[4(l/16)+v][l%16]; the four D rows of a lane are
+    //      contiguous in the column-major LDS matrix).
+    // Rows N..63 of every column are zero, so the rows of a tile past the
+    // matrix contribute nothing; its columns past the matrix are masked per
+    // lane and never stored. The factors are those of factor_rows up to FP32
+    // summation order: unit L below the diagonal, U above, reciprocal pivots
+    // on it. lu must be 16-byte aligned.
+    __device__ __forceinline__ bool factor_blocked() {
+        typedef float f4 __attribute__((ext_vector_type(4)));
+        form_newton();
+        __syncthreads();
+        bool ok = true;
+        static_for<0, (N + 15) / 16>([&](auto kbc) HF_AI {
+            constexpr int k0 = 16 * decltype(kbc)::value, R0 = k0 + 16;
+            // (a) the panel
+            float p[16];
+            #pragma unroll
+            for (int c = 0; c < 16; ++c) p[c] = k0 + c < N ? (float)lu[(k0 + c) * 64 + lane] : 0.0f;
+            #pragma unroll
+            for (int c = 0; c < 16; ++c) {
+                if (k0 + c < N) {
+                    const int k = k0 + c;
+                    const float piv = lane_of(p[c], k);
+                    ok = ok && fabsf(piv) > __FLT_MIN__ && __builtin_isfinite(piv);
+                    const float inv = 1.0f / piv;
+                    const float l = p[c] * inv;
+                    const bool below = lane > k;
+                    #pragma unroll
+                    for (int j = c + 1; j < 16; ++j) {
+                        if (k0 + j < N) {
+                            const float pkj = lane_of(p[j], k);
+                            if (below) p[j] = fmaf(-l, pkj, p[j]);
+                        }
+                    }
+                    p[c] = below ? l : (lane == k ? inv : p[c]);
+                }
+            }
+            if (lane >= k0 && lane < N) {
+                #pragma unroll
+                for (int c = 0; c < 16; ++c)
+                    if (k0 + c < N) lu[(k0 + c) * 64 + lane] = p[c];
+            }
+            if constexpr (R0 < N) {
+                // (b) U12: lane j holds column R0 + j (lanes past the matrix
+                // work on a copy of column R0 and store nothing)
+                constexpr int NC = N - R0;
+                const int col = lane < NC ? R0 + lane : R0;
+                float x[16];
+                #pragma unroll
+                for (int r = 0; r < 16; ++r) x[r] = lu[col * 64 + k0 + r];
+                #pragma unroll
+                for (int c = 0; c < 16; ++c)
+                    #pragma unroll
+                    for (int r = c + 1; r < 16; ++r) x[r] = fmaf(-lane_of(p[c], k0 + r), x[c], x[r]);
+                if (lane < NC) {
+                    #pragma unroll
+                    for (int r = 0; r < 16; ++r) lu[col * 64 + k0 + r] = x[r];
+                }
+                __syncthreads();
+                // (c) trailing tiles; every MFMA runs on the whole wave
+                constexpr int NT = (NC + 15) / 16;
+                const int i = lane & 15, q = lane >> 4;
+                #pragma unroll
+                for (int rb = 0; rb < NT; ++rb) {
+                    #pragma unroll
+                    for (int cb = 0; cb < NT; ++cb) {
+                        const int r0 = R0 + 16 * rb, cc = R0 + 16 * cb + i;
+                        const bool cv = cc < N;
+                        f4* dst = reinterpret_cast<f4*>(&lu[(cv ? cc : R0) * 64 + r0 + 4 * q]);
+                        f4 acc = cv ? *dst : f4{0.0f, 0.0f, 0.0f, 0.0f};
+                        #pragma unroll
+                        for (int ks = 0; ks < 4; ++ks) {
+                            const int kk = k0 + 4 * ks + q;
+                            const float a = -(float)lu[kk * 64 + r0 + i];
+                            const float b = cv ? (float)lu[cc * 64 + kk] : 0.0f;
+                            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+                        }
+                        if (cv) *dst = acc;
+                    }
+                }
+                __syncthreads();
+            }
+        });
+        return ok;
+    }
     // Solve (I - gamma J) x = b with b, x one component per lane.
+    // Each substitution step depends on the previous one through lane k's b,
+    // so the lane's row entries are loaded from LDS 16 at a time ahead of the
+    // steps that use them (loaded per step, every step waited out an LDS
+    // round trip). The arithmetic is the plain row-by-row substitution.
     __device__ __forceinline__ double solve(double bd) const {
         MT b = (MT)bd;
-        #pragma unroll 2
-        for (int k = 0; k < N; ++k) {
-            const MT bk = lane_of(b, k);
-            if (lane > k) b = fma(-lu[k * 64 + lane], bk, b);
+        #pragma unroll
+        for (int k0 = 0; k0 < N; k0 += 16) {
+            MT r[16];
+            #pragma unroll
+            for (int c = 0; c < 16; ++c)
+                if (k0 + c < N) r[c] = lu[(k0 + c) * 64 + lane];
+            #pragma unroll
+            for (int c = 0; c < 16; ++c) {
+                if (k0 + c < N) {
+                    const MT bk = lane_of(b, k0 + c);
+                    if (lane > k0 + c) b = fma(-r[c], bk, b);
+                }
+            }
         }
-        #pragma unroll 2
-        for (int k = N - 1; k >= 0; --k) {
-            if (lane == k) b *= lu[k * 64 + k];
-            const MT bk = lane_of(b, k);
-            if (lane < k) b = fma(-lu[k * 64 + lane], bk, b);
+        #pragma unroll
+        for (int k1 = ((N - 1) / 16) * 16; k1 >= 0; k1 -= 16) {
+            MT r[16];
+            #pragma unroll
+            for (int c = 0; c < 16; ++c)
+                if (k1 + c < N) r[c] = lu[(k1 + c) * 64 + lane];
+            #pragma unroll
+            for (int c = 15; c >= 0; --c) {
+                const int k = k1 + c;
+                if (k < N) {
+                    if (lane == k) b *= r[c];
+                    const MT bk = lane_of(b, k);
+                    if (lane < k) b = fma(-r[c], bk, b);
+                }
+            }
         }
         return own ? (double)b : 0.0;
     }

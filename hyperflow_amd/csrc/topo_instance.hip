@@ -415,7 +415,7 @@ chem_wave_kernel(const MechData<TT>* __restrict__ dm, CellMap cm, long ncell,
     constexpr int NS = TT::NS, N = NS - 1, NR = TT::NR;
     __shared__ double scratch[2 * NS + NR];
     __shared__ double coef[13];
-    __shared__ MT lu[N * 64];
+    __shared__ __attribute__((aligned(16))) MT lu[N * 64];   // 16-byte aligned for the blocked LU
     __shared__ int inc[2 * NR * TT::NRMAX];
     const int lane = threadIdx.x;
     // this wave's Jacobian slot: N x 64 entries inside the per-lane Jacobian
