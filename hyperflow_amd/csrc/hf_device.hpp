@@ -50,6 +50,11 @@ struct MechData {
     // Runtime copy of the topology, for kernels that assign reactions and
     // species to lanes (indexed by a lane-dependent reaction or species):
     int8_t rk_kind[NR], rk_troe[NR], rk_rev[NR];
+    // rate_slot[s]: the reaction evaluated in slot s of the one-cell-per-wave
+    // rate loop (lane s % 64, round s / 64). Falloff reactions come first,
+    // then three-body ones, so that the rounds after the first hold only
+    // elementary reactions and skip those code paths on every lane.
+    int16_t rate_slot[NR];
     int16_t rk_reac[NR][T::NRMAX], rk_prod[NR][T::NRMAX], rk_eff[NR][NEFF > 0 ? NEFF : 1];
     // species k appears in reactions inc_rxn[inc_start[k] .. inc_start[k+1]) with net
     // stoichiometric coefficient inc_nu (products minus reactants)

@@ -3,8 +3,9 @@
 // value, Nordsieck history, error weight and correction live in that lane's
 // registers; norms are wave reductions), lane N holds the closing species
 // for the thermodynamics, and the reactions are spread over the lanes (lane
-// l evaluates reactions l, l + 64, ...; production rates are then gathered
-// per species in a fixed order). Every step-control decision is uniform
+// l evaluates rate slots l, l + 64, ..., falloff and three-body reactions in
+// the first round; production rates are then gathered per species in a fixed
+// order). Every step-control decision is uniform
 // across the wave, so the wave never diverges on control flow.
 //
 // The integrator is the same CVODE-equivalent BDF as hf_bdf.hpp (the
@@ -205,13 +206,28 @@ struct WaveGas {
         }
         const double ctot = wave_sum(C);
         __syncthreads();
-        for (int i = lane; i < NR; i += 64) lds[2 * NS + i] = rate(i, logt, it, logc0, ctot);
+        for (int sl = lane; sl < NR; sl += 64) {
+            const int i = d.rate_slot[sl];
+            lds[2 * NS + i] = rate(i, logt, it, logc0, ctot);
+        }
         __syncthreads();
         double w = 0;
         if (lane < N) {
-            for (int p = inc_b; p < inc_e; ++p) {
+            // four incidences per trip, their loads issued together; the sum
+            // runs in list order
+            const double* q = lds + 2 * NS;
+            int p = inc_b;
+            for (; p + 4 <= inc_e; p += 4) {
+                const int v0 = inc[p], v1 = inc[p + 1], v2 = inc[p + 2], v3 = inc[p + 3];
+                const double q0 = q[v0 & 0xffff], q1 = q[v1 & 0xffff], q2 = q[v2 & 0xffff], q3 = q[v3 & 0xffff];
+                w = fma((double)(v0 >> 16), q0, w);
+                w = fma((double)(v1 >> 16), q1, w);
+                w = fma((double)(v2 >> 16), q2, w);
+                w = fma((double)(v3 >> 16), q3, w);
+            }
+            for (; p < inc_e; ++p) {
                 const int v = inc[p];
-                w = fma((double)(v >> 16), lds[2 * NS + (v & 0xffff)], w);
+                w = fma((double)(v >> 16), q[v & 0xffff], w);
             }
             f = w * W / rho;
         } else {

@@ -70,6 +70,12 @@ void fill(const Mechanism& m, void* host) {
         }
         for (int j = 0; j < NEFF; ++j) d.rk_eff[i][j] = Topo::NEFF > 0 ? (int16_t)Topo::effsp[i][j] : (int16_t)-1;
     }
+    {
+        int s = 0;
+        for (int kind = 2; kind >= 0; --kind)
+            for (int i = 0; i < nr; ++i)
+                if (Topo::kind[i] == kind) d.rate_slot[s++] = (int16_t)i;
+    }
     int n = 0;
     for (int k = 0; k < ns; ++k) {
         d.inc_start[k] = (int16_t)n;
