@@ -69,16 +69,26 @@ flux_stage_kernel(const MechData<T>* __restrict__ dm, const hf_box* __restrict__
     const int lx = lane % BX, ly = lane / BX;
 
     // ---- stage the ghosted tile: rows j0-2 .. j0+TY+1 (2-D) or j0 .. j0+TY-1 (1-D)
+    // Eight loads per lane are issued before their LDS stores: one load per
+    // trip left every trip waiting out a full HBM round trip.
     const int jlo = TWO_D ? j0 - 2 : j0;
-    for (int e = lane; e < NV * GX * GY; e += FT::THREADS) {
-        const int n = e / (GX * GY);
-        const int r = e - n * GX * GY;
-        const int gj = r / GX, gi = r - gj * GX;
-        const int ci = i0 - 2 + gi, cj = jlo + gj;
-        double v = 0;
-        if (ci < B.nx + 2 && cj < B.ny + 2)
-            v = Uin[B.in_offset + n * B.in_plane + (int64_t)(cj + 2) * B.in_pitch + (ci + 2)];
-        s_in[e] = v;
+    constexpr int SB = 8;
+    for (int e0 = lane; e0 < NV * GX * GY; e0 += SB * FT::THREADS) {
+        double v[SB];
+        #pragma unroll
+        for (int k = 0; k < SB; ++k) {
+            const int e = e0 + k * FT::THREADS;
+            const int n = e / (GX * GY);
+            const int r = e - n * GX * GY;
+            const int gj = r / GX, gi = r - gj * GX;
+            const int ci = i0 - 2 + gi, cj = jlo + gj;
+            v[k] = 0;
+            if (e < NV * GX * GY && ci < B.nx + 2 && cj < B.ny + 2)
+                v[k] = Uin[B.in_offset + n * B.in_plane + (int64_t)(cj + 2) * B.in_pitch + (ci + 2)];
+        }
+        #pragma unroll
+        for (int k = 0; k < SB; ++k)
+            if (e0 + k * FT::THREADS < NV * GX * GY) s_in[e0 + k * FT::THREADS] = v[k];
     }
     __syncthreads();
 
