@@ -85,11 +85,15 @@ class AmrDriver:
     column wedge_i on) are walls. max_level, blocking, max_grid_size,
     grid_eff, n_error_buf, n_proper: the regrid parameters of main.cpp:33-36
     (n_proper 2 is the reference's). tag: ErrorEst's criteria for
-    hf_tag_cells (pressure_tag, temperature_tag, fuel_rate_tag, chemistry)."""
+    hf_tag_cells (pressure_tag, temperature_tag, fuel_rate_tag; box, a tag
+    rectangle (x0, x1, y0, y1) moving with velocity (u, v); mode "physics"
+    (the criteria, plus the box if given) or "box" (the box alone);
+    BlockAMR.cpp:170-200). chemistry False skips react and the fuel-rate
+    criterion (the configuration's "chemistry", BlockAMR.cpp:271-272)."""
 
     def __init__(self, mechanism, nx0, ny0, dx0, dy0, bc, max_level=2, cfl=0.5, chem_mode=HF_CHEM_BATCHED,
                  blocking=16, max_grid_size=64, grid_eff=0.7, n_error_buf=2, n_proper=2, regrid_interval=8,
-                 max_attempts=18, inlet=None, wedge_i=0, tag=None, x0=0.0, y0=0.0):
+                 max_attempts=18, inlet=None, wedge_i=0, tag=None, x0=0.0, y0=0.0, chemistry=True):
         import torch
         self.torch = torch
         self.mech = mechanism
@@ -105,8 +109,14 @@ class AmrDriver:
         self.n_error_buf, self.n_proper, self.regrid_interval = int(n_error_buf), int(n_proper), int(regrid_interval)
         self.wedge_i0 = int(wedge_i)
         self.x0, self.y0 = float(x0), float(y0)
-        self.tag = dict(pressure_tag=0.08, temperature_tag=0.08, fuel_rate_tag=50.0, chemistry=True)
+        self.chemistry = bool(chemistry)
+        self.tag = dict(pressure_tag=0.08, temperature_tag=0.08, fuel_rate_tag=50.0, chemistry=self.chemistry,
+                        box=None, velocity=(0.0, 0.0), mode="physics")
         self.tag.update(tag or {})
+        if self.tag["mode"] not in ("physics", "box") or (self.tag["mode"] == "box" and self.tag["box"] is None):
+            raise ValueError("invalid tagging geometry")
+        if (self.tag["box"] is not None and len(self.tag["box"]) != 4) or len(self.tag["velocity"]) != 2:
+            raise ValueError("invalid tagging geometry")
         self.inlet = None
         if inlet is not None:
             vals = [float(v) for v in inlet]
@@ -286,6 +296,8 @@ class AmrDriver:
         self._rate = self._validate("U")
 
     def react(self, h):
+        if not self.chemistry:
+            return
         self._rate = None
         blocks, n = self._cells("U")
         from ._lib import hf_chem_stats
@@ -356,9 +368,12 @@ class AmrDriver:
         for (i0, j0, i1, j1) in L.boxes:
             nx, ny = i1 - i0 + 1, j1 - j0 + 1
             box = hf_box(nx, ny, 0, 0, j0 * L.gx + i0, L.gx, L.gx * L.gy, 0, nx, nx * ny, 0.0, 0.0)
-            P = hf_tag_params(float(k["pressure_tag"]), float(k["temperature_tag"]), float(k["fuel_rate_tag"]), 1,
-                              int(bool(k["chemistry"])), 0, 0, (c_double * 4)(), (c_double * 2)(),
-                              self.x0 + i0 * L.dx, self.y0 + j0 * L.dy, L.dx, L.dy, self.time)
+            box_ = k["box"]
+            P = hf_tag_params(float(k["pressure_tag"]), float(k["temperature_tag"]), float(k["fuel_rate_tag"]),
+                              int(k["mode"] == "physics"), int(bool(k["chemistry"])), 0, int(box_ is not None),
+                              (c_double * 4)(*(box_ if box_ is not None else (0.0,) * 4)),
+                              (c_double * 2)(*k["velocity"]), self.x0 + i0 * L.dx, self.y0 + j0 * L.dy, L.dx, L.dy,
+                              self.time)
             tg = self.torch.zeros((ny, nx), dtype=self.torch.uint8, device="cuda")
             n, ff = c_int64(0), c_int64(-1)
             r = load().hf_tag_cells(self._h, L.A["U"].data_ptr(), byref(box), byref(P), tg.data_ptr(), byref(n),
