@@ -104,6 +104,8 @@ _SIGNATURES = {
     "hf_reflux_masked": (c_int, [c_void_p, c_void_p, POINTER(hf_patch), POINTER(hf_face_fluxes),
                                  POINTER(hf_face_fluxes), c_void_p, c_int64, POINTER(c_int32), c_double, c_double,
                                  c_double]),
+    "hf_flux_stage_boxes": (c_int, [c_void_p, POINTER(hf_box), c_int, POINTER(c_void_p), POINTER(c_void_p), c_int,
+                                    POINTER(hf_face_fluxes), POINTER(hf_flux_stats), POINTER(c_int64)]),
     "hf_flux_stage_noalloc": (c_int, [c_void_p, POINTER(hf_box), c_int, c_void_p, c_void_p, c_int,
                                       POINTER(hf_face_fluxes), POINTER(hf_flux_stats), POINTER(c_int64)]),
     "hf_reflux": (c_int, [c_void_p, c_void_p, POINTER(hf_patch), POINTER(hf_face_fluxes), POINTER(hf_face_fluxes),

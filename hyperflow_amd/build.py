@@ -84,7 +84,24 @@ def build(jobs: int | None = None, verbose: bool = False) -> Path:
     tmp = LIB.with_suffix(".so.tmp")
     _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", tmp, *objs, "-lpthread"], verbose)
     os.replace(tmp, LIB)
+    build_consumer(verbose)
     return LIB
+
+
+CONSUMER_SRC = PKG.parent / "tests" / "abi_consumer.cpp"
+CONSUMER = BUILD / "abi_consumer"
+
+
+def build_consumer(verbose=False):
+    """The C++ consumer of the ABI (tests/abi_consumer.cpp), built with the
+    host compiler against the header and the library, as a C++ driver links
+    it. Skipped when the tests are not present."""
+    if not CONSUMER_SRC.exists():
+        return None
+    _run(["g++", "-std=c++17", "-O2", "-Wall", "-D__HIP_PLATFORM_AMD__", "-I", INCLUDE, "-I", "/opt/rocm/include",
+          CONSUMER_SRC, "-o", CONSUMER, "-L", PKG, "-lhyperflow_hip", "-L", "/opt/rocm/lib", "-lamdhip64",
+          "-Wl,-rpath,$ORIGIN/..:/opt/rocm/lib"], verbose)
+    return CONSUMER
 
 
 def clean():

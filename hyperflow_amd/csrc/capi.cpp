@@ -660,14 +660,18 @@ int hf_chem_advance_cells(hf_ctx* c, const hf_cells* blocks, int nblock, const d
 namespace {
 // noalloc: the tables and counters live in the device's static scratch (no
 // device allocation, no timing events; device-resident arrays only).
+// in_ptrs/out_ptrs: one device base pointer per box (then Uin/Uout are unused).
 int flux_stage_impl(hf_ctx* c, const hf_box* boxes, int nbox, const double* Uin, double* Uout, int first_order,
-                    const hf_face_fluxes* fluxes, hf_flux_stats* stats, int64_t* first_failed, bool noalloc = false) {
+                    const hf_face_fluxes* fluxes, hf_flux_stats* stats, int64_t* first_failed, bool noalloc = false,
+                    const double* const* in_ptrs = nullptr, double* const* out_ptrs = nullptr) {
     if (!c) return HF_EINVAL;
     BusyGuard guard(c);
     if (!guard.ok) return fail(c, HF_EBUSY, "concurrent flux stage is unsupported");
     if (first_failed) *first_failed = -1;
     if (stats) std::memset(stats, 0, sizeof(*stats));
-    if (nbox < 0 || (nbox > 0 && (!boxes || !Uin || !Uout))) return fail(c, HF_EINVAL, "invalid flux-stage arguments");
+    const bool per_box = in_ptrs || out_ptrs;
+    if (nbox < 0 || (nbox > 0 && (!boxes || (per_box ? !(in_ptrs && out_ptrs) : (!Uin || !Uout)))))
+        return fail(c, HF_EINVAL, "invalid flux-stage arguments");
     if (nbox == 0) return HF_OK;
     char* fixed = nullptr;
     if (noalloc) {
@@ -727,9 +731,22 @@ int flux_stage_impl(hf_ctx* c, const hf_box* boxes, int nbox, const double* Uin,
                  !within_allocation(F.fy, F.fy_offset, F.fy_offset + (nv - 1) * F.fy_plane + B.ny * F.fy_pitch + B.nx - 1)))
                 return fail(c, HF_EINVAL, "face fluxes of box " + std::to_string(b) + " lie outside their allocation");
         }
-    if (is_device_ptr(Uin) && !within_allocation(Uin, 0, in_extent - 1))
+    if (per_box) {
+        // every box inside its own input and output allocations
+        for (int b = 0; b < nbox; ++b) {
+            const hf_box& B = boxes[b];
+            if (!is_device_ptr(in_ptrs[b]) || !is_device_ptr(out_ptrs[b]))
+                return fail(c, HF_EINVAL, "box " + std::to_string(b) + ": per-box arrays need device memory");
+            if (!within_allocation(in_ptrs[b], B.in_offset,
+                                   B.in_offset + (nv - 1) * B.in_plane + (B.ny + 3) * B.in_pitch + B.nx + 3))
+                return fail(c, HF_EINVAL, "box " + std::to_string(b) + " reads outside its input allocation");
+            if (!within_allocation(out_ptrs[b], B.out_offset,
+                                   B.out_offset + (nv - 1) * B.out_plane + (B.ny - 1) * B.out_pitch + B.nx - 1))
+                return fail(c, HF_EINVAL, "box " + std::to_string(b) + " writes outside its output allocation");
+        }
+    } else if (is_device_ptr(Uin) && !within_allocation(Uin, 0, in_extent - 1))
         return fail(c, HF_EINVAL, "a box reads outside the input allocation");
-    if (is_device_ptr(Uout)) {
+    if (!per_box && is_device_ptr(Uout)) {
         for (int b = 0; b < nbox; ++b)
             if (!within_allocation(Uout, boxes[b].out_offset, boxes[b].out_offset + (nv - 1) * boxes[b].out_plane +
                                                                    (boxes[b].ny - 1) * boxes[b].out_pitch +
@@ -737,6 +754,7 @@ int flux_stage_impl(hf_ctx* c, const hf_box* boxes, int nbox, const double* Uin,
                 return fail(c, HF_EINVAL, "box " + std::to_string(b) + " writes outside the output allocation");
     }
     const size_t flux_table = fluxes ? sizeof(hf_face_fluxes) * nbox : 0;
+    const size_t ptr_table = per_box ? 2 * sizeof(void*) * nbox : 0;
     char* box_dev;
     int32_t* tile_dev;
     unsigned long long* scal;
@@ -745,7 +763,8 @@ int flux_stage_impl(hf_ctx* c, const hf_box* boxes, int nbox, const double* Uin,
         box_dev = fixed + kNoAllocBoxOff;
         tile_dev = reinterpret_cast<int32_t*>(fixed + kNoAllocTileOff);
     } else {
-        if (!c->boxes.ensure(sizeof(hf_box) * nbox + flux_table) || !c->tiles.ensure(sizeof(int32_t) * tiles.size()) ||
+        if (!c->boxes.ensure(sizeof(hf_box) * nbox + flux_table + ptr_table) ||
+            !c->tiles.ensure(sizeof(int32_t) * tiles.size()) ||
             !c->scal.ensure(sizeof(unsigned long long) * 16))
             return fail(c, HF_ENOMEM, "device scratch");
         scal = c->scal.as<unsigned long long>();
@@ -755,8 +774,15 @@ int flux_stage_impl(hf_ctx* c, const hf_box* boxes, int nbox, const double* Uin,
     char* flux_dev = noalloc ? fixed + kNoAllocFluxOff : box_dev + sizeof(hf_box) * nbox;
     HF_HIP(c, hipMemcpyAsync(box_dev, boxes, sizeof(hf_box) * nbox, hipMemcpyHostToDevice, c->stream));
     if (fluxes) HF_HIP(c, hipMemcpyAsync(flux_dev, fluxes, flux_table, hipMemcpyHostToDevice, c->stream));
+    char* ptr_dev = box_dev + sizeof(hf_box) * nbox + flux_table;
+    if (per_box) {
+        // the call synchronises before it returns, so the caller's host tables outlive these copies
+        HF_HIP(c, hipMemcpyAsync(ptr_dev, in_ptrs, sizeof(void*) * nbox, hipMemcpyHostToDevice, c->stream));
+        HF_HIP(c, hipMemcpyAsync(ptr_dev + sizeof(void*) * nbox, out_ptrs, sizeof(void*) * nbox, hipMemcpyHostToDevice,
+                                 c->stream));
+    }
     HF_HIP(c, hipMemcpyAsync(tile_dev, tiles.data(), sizeof(int32_t) * tiles.size(), hipMemcpyHostToDevice, c->stream));
-    const bool dev_in = is_device_ptr(Uin), dev_out = is_device_ptr(Uout);
+    const bool dev_in = per_box || is_device_ptr(Uin), dev_out = per_box || is_device_ptr(Uout);
     const double* din = Uin;
     double* dout = Uout;
     if (!dev_in) {
@@ -787,6 +813,8 @@ int flux_stage_impl(hf_ctx* c, const hf_box* boxes, int nbox, const double* Uin,
     L.first_failed = scal;
     L.counters = scal + 1;
     L.fluxes = fluxes ? reinterpret_cast<const hf_face_fluxes*>(flux_dev) : nullptr;
+    L.in_ptrs = per_box ? reinterpret_cast<const double* const*>(ptr_dev) : nullptr;
+    L.out_ptrs = per_box ? reinterpret_cast<double* const*>(ptr_dev + sizeof(void*) * nbox) : nullptr;
     L.stream = c->stream;
     float ms = 0;
     L.kernel_ms = noalloc ? nullptr : &ms;
@@ -824,6 +852,11 @@ int hf_flux_stage_fluxes(hf_ctx* c, const hf_box* boxes, int nbox, const double*
                          const hf_face_fluxes* fluxes, hf_flux_stats* stats, int64_t* first_failed) {
     if (c && nbox > 0 && !fluxes) return fail(c, HF_EINVAL, "missing face-flux descriptors");
     return flux_stage_impl(c, boxes, nbox, Uin, Uout, first_order, fluxes, stats, first_failed);
+}
+
+int hf_flux_stage_boxes(hf_ctx* c, const hf_box* boxes, int nbox, const double* const* in, double* const* out,
+                        int first_order, const hf_face_fluxes* fluxes, hf_flux_stats* stats, int64_t* first_failed) {
+    return flux_stage_impl(c, boxes, nbox, nullptr, nullptr, first_order, fluxes, stats, first_failed, false, in, out);
 }
 
 int hf_flux_stage_noalloc(hf_ctx* c, const hf_box* boxes, int nbox, const double* Uin, double* Uout, int first_order,

@@ -46,7 +46,8 @@ __global__ void __launch_bounds__((TX + 1) * (TWO_D ? TY + 1 : TY))
 flux_stage_kernel(const MechData<T>* __restrict__ dm, const hf_box* __restrict__ boxes,
                   const int32_t* __restrict__ tiles, int ntiles, const double* __restrict__ Uin,
                   double* __restrict__ Uout, int first_order, unsigned long long* __restrict__ first_failed,
-                  unsigned long long* __restrict__ counters, const hf_face_fluxes* __restrict__ fout) {
+                  unsigned long long* __restrict__ counters, const hf_face_fluxes* __restrict__ fout,
+                  const double* const* __restrict__ in_ptrs, double* const* __restrict__ out_ptrs) {
     using FT = FluxTile<T, TX, TY, TWO_D>;
     constexpr int NS = T::NS, NV = NS + 3, E = NS + 2;
     constexpr int GX = FT::GX, GY = FT::GY, BX = FT::BX;
@@ -64,6 +65,9 @@ flux_stage_kernel(const MechData<T>* __restrict__ dm, const hf_box* __restrict__
     if (t >= ntiles) return;
     const int bi = tiles[3 * t], ti = tiles[3 * t + 1], tj = tiles[3 * t + 2];
     const hf_box B = boxes[bi];
+    // the box's own arrays when the caller passed a pointer per box
+    const double* __restrict__ src = in_ptrs ? in_ptrs[bi] : Uin;
+    double* __restrict__ dst = out_ptrs ? out_ptrs[bi] : Uout;
     const int i0 = ti * TX, j0 = tj * TY;
     const int lane = threadIdx.x;
     const int lx = lane % BX, ly = lane / BX;
@@ -84,7 +88,7 @@ flux_stage_kernel(const MechData<T>* __restrict__ dm, const hf_box* __restrict__
             const int ci = i0 - 2 + gi, cj = jlo + gj;
             v[k] = 0;
             if (e < NV * GX * GY && ci < B.nx + 2 && cj < B.ny + 2)
-                v[k] = Uin[B.in_offset + n * B.in_plane + (int64_t)(cj + 2) * B.in_pitch + (ci + 2)];
+                v[k] = src[B.in_offset + n * B.in_plane + (int64_t)(cj + 2) * B.in_pitch + (ci + 2)];
         }
         #pragma unroll
         for (int k = 0; k < SB; ++k)
@@ -203,7 +207,7 @@ flux_stage_kernel(const MechData<T>* __restrict__ dm, const hf_box* __restrict__
     if (owns_cell) {
         #pragma unroll
         for (int c = 0; c < NV; ++c)
-            Uout[B.out_offset + c * B.out_plane + (int64_t)cj * B.out_pitch + ci] = acc[c];
+            dst[B.out_offset + c * B.out_plane + (int64_t)cj * B.out_pitch + ci] = acc[c];
         Props p;
         if (!properties(d, acc, p, s_t[gyc * GX + lx + 2])) {
             fail = true;

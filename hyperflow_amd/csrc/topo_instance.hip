@@ -613,7 +613,7 @@ void launch_flux(const FluxLaunch& L, unsigned grid) {
     hipLaunchKernelGGL((flux_stage_kernel<Topo, TX, TY, TWO_D>), dim3(grid),
                        dim3(FluxTile<Topo, TX, TY, TWO_D>::THREADS), 0, L.stream, static_cast<const MD*>(L.mech),
                        L.boxes, L.tiles, L.ntiles, L.Uin, L.Uout, L.first_order, L.first_failed, L.counters,
-                       L.fluxes);
+                       L.fluxes, L.in_ptrs, L.out_ptrs);
 }
 
 int flux(const FluxLaunch& L) {

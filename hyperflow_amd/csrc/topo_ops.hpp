@@ -106,6 +106,9 @@ struct FluxLaunch {
     unsigned long long* first_failed;  // device scalar
     unsigned long long* counters;   // device: [faces, first-order sides]
     const hf_face_fluxes* fluxes;   // device copy of the per-box face-flux outputs, or null
+    // device tables of per-box base pointers (box b reads in_ptrs[b] + in_offset ...), or null for Uin/Uout
+    const double* const* in_ptrs;
+    double* const* out_ptrs;
     hipStream_t stream;
     float* kernel_ms;
 };

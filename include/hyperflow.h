@@ -204,6 +204,17 @@ typedef struct hf_face_fluxes {
 int hf_flux_stage_fluxes(hf_ctx *ctx, const hf_box *boxes, int nbox, const double *Uin, double *Uout, int first_order,
                          const hf_face_fluxes *fluxes, hf_flux_stats *stats, int64_t *first_failed);
 
+/* hf_flux_stage_fluxes over boxes that live in separate allocations (an
+ * AMReX MultiFab's FArrayBoxes, BlockAMR.cpp:210-243): box b reads
+ * in[b] + in_offset + ... and writes out[b] + out_offset + ..., with its
+ * hf_box offsets relative to its own pointers (typically in_offset 0 for the
+ * ghosted FAB and the valid box's corner for the output FAB). in and out are
+ * host arrays of nbox device pointers; each box must lie inside its own
+ * allocations (HF_EINVAL otherwise). fluxes may be NULL. The update is
+ * bit-identical to hf_flux_stage's on the same data. */
+int hf_flux_stage_boxes(hf_ctx *ctx, const hf_box *boxes, int nbox, const double *const *in, double *const *out,
+                        int first_order, const hf_face_fluxes *fluxes, hf_flux_stats *stats, int64_t *first_failed);
+
 /* hf_flux_stage_fluxes (fluxes may be NULL: then hf_flux_stage) that
  * allocates nothing. No counterpart in the reference: it is for callers that
  * cannot allocate device memory (code under HIP graph capture, tight serving

@@ -173,3 +173,20 @@ def test_chemistry_interval_argument_checks():
                 np.full(20, 1e-7)[::2]):
         with pytest.raises(ValueError):
             _interval(bad, 10)
+
+
+def test_cxx_consumer_builds_against_the_header(tmp_path):
+    """tests/abi_consumer.cpp (the integration guide's ChemistryBatch wrapper
+    and the AMReX-style per-box table) compiles with the host compiler
+    against include/hyperflow.h, links the library and runs its no-device
+    check: a missing mechanism throws std::runtime_error."""
+    import subprocess
+    from hyperflow_amd import build as B
+    exe = tmp_path / "abi_consumer"
+    cmd = ["g++", "-std=c++17", "-O1", "-Wall", "-Werror", "-D__HIP_PLATFORM_AMD__", "-I", str(B.INCLUDE), "-I",
+           "/opt/rocm/include", str(B.CONSUMER_SRC), "-o", str(exe), "-L", str(B.PKG), "-lhyperflow_hip", "-L",
+           "/opt/rocm/lib", "-lamdhip64", f"-Wl,-rpath,{B.PKG}:/opt/rocm/lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([str(exe), str(H2), "--compile-only"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and "PASS" in r.stdout and "FAIL" not in r.stdout, r.stdout + r.stderr
