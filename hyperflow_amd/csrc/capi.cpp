@@ -1207,6 +1207,7 @@ int hf_euler_stage(hf_ctx* c, double* Uin, double* Uout, const hf_box* box, cons
         F.one_d = one_d ? 1 : 0;
         F.first_failed = scal;
         F.counters = scal + 1;
+        F.skip_validation = 1;             // the repair and validation kernels below judge the update
         F.fluxes = nullptr;
         F.stream = c->stream;
         F.kernel_ms = nullptr;
@@ -1714,6 +1715,7 @@ int step_enqueue(hf_ctx* c, const StepPlan& P) {
         F.first_order = 0;
         F.one_d = one_d ? 1 : 0;
         F.first_failed = scal + 2;           // the raw update is judged by the repair and validation
+        F.skip_validation = 1;
         F.counters = scal + 3;
         F.stream = c->stream;
         if (c->ops->flux(F) != HF_OK) return HF_EDEVICE;

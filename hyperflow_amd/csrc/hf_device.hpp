@@ -97,6 +97,7 @@ template <class T>
 struct EnergyPoly {
     double lo[6], hi[6];   // e/R = c0 T + c1 T^2/2 + c2 T^3/3 + c3 T^4/4 + c4 T^5/5 + c5
     double tmid;
+    double rw;             // sum_k Y_k / W_k (1/Wbar)
 
     __device__ __forceinline__ void build(const MechData<T>& d, const double* Y) {
         #pragma unroll
@@ -117,6 +118,7 @@ struct EnergyPoly {
         lo[0] -= s;
         hi[0] -= s;
         tmid = d.tmid;
+        rw = s;
     }
     // e (J/kg) and cv (J/kg/K) at temperature t
     __device__ __forceinline__ void eval(double t, double& e, double& cv) const {
@@ -151,28 +153,44 @@ __device__ __forceinline__ void energy_species(const MechData<T>& d, const doubl
     cv *= kR;
 }
 
-// Energy -> temperature Newton. Converged when the applied step is below
-// 1e-9 T: Newton's quadratic convergence leaves the result at round-off level.
-// Returns false when the iteration does not converge or leaves T <= 0.
+// Energy -> temperature Newton on the mixture polynomial (every species
+// shares Tmid). Converged when the applied step is below 1e-9 T: Newton's
+// quadratic convergence leaves the result at round-off level. Returns false
+// when the iteration does not converge or leaves T <= 0.
 template <class T>
+__device__ __forceinline__ bool poly_newton(const MechData<T>& d, const double* Y, double e_target, double& t,
+                                            EnergyPoly<T>& P, int max_iter) {
+    P.build(d, Y);
+    for (int it = 0; it < max_iter; ++it) {
+        double e, cv;
+        P.eval(t, e, cv);
+        const double dT = (e - e_target) / cv;
+        double tn = t - dT;
+        tn = tn > 0 ? tn : 0.5 * t;
+        const bool done = fabs(dT) <= 1e-9 * t;
+        t = tn;
+        if (done) return finite(t) && cv > 0;
+    }
+    return false;
+}
+
+// Which thermodynamic form a kernel carries: the mixture polynomial (every
+// species shares Tmid), the per-species sums, or both behind a run-time test
+// of the mechanism. A kernel specialised on one form holds none of the
+// other's code, which keeps large kernels compact in the instruction cache.
+enum ThermoForm { kThermoAny = 0, kThermoPoly = 1, kThermoSpecies = 2 };
+
+// Energy -> temperature Newton (the per-species sums when the species switch
+// range at different temperatures). Same convergence test as poly_newton.
+template <class T, int TF = kThermoAny>
 __device__ __forceinline__ bool temperature_from_e(const MechData<T>& d, const double* Y, double e_target,
                                                    double& t, int max_iter = 60) {
     if (!(t > 0) || !finite(t)) t = 1000.0;
-    if (d.common_tmid) {
+    if (TF == kThermoPoly || (TF == kThermoAny && d.common_tmid)) {
         EnergyPoly<T> P;
-        P.build(d, Y);
-        for (int it = 0; it < max_iter; ++it) {
-            double e, cv;
-            P.eval(t, e, cv);
-            const double dT = (e - e_target) / cv;
-            double tn = t - dT;
-            tn = tn > 0 ? tn : 0.5 * t;
-            const bool done = fabs(dT) <= 1e-9 * t;
-            t = tn;
-            if (done) return finite(t) && cv > 0;
-        }
-        return false;
+        return poly_newton(d, Y, e_target, t, P, max_iter);
     }
+    if constexpr (TF == kThermoPoly) return false;
     for (int it = 0; it < max_iter; ++it) {
         double e, cv;
         energy_species(d, Y, t, e, cv);
@@ -210,7 +228,7 @@ struct Props {
 
 // properties(u) of physics.cpp:4-15 on nv = NS+3 conserved values.
 // Returns false for an inadmissible state (the reference throws domain_error).
-template <class T>
+template <class T, int TF = kThermoAny>
 __device__ __forceinline__ bool properties(const MechData<T>& d, const double* u, Props& out, double t_guess) {
     constexpr int NS = T::NS;
     double rho = 0;
@@ -229,9 +247,20 @@ __device__ __forceinline__ bool properties(const MechData<T>& d, const double* u
     const double ux = u[NS] * rr, vy = u[NS + 1] * rr;
     const double e = u[NS + 2] * rr - 0.5 * (ux * ux + vy * vy);
     double t = t_guess;
-    if (!temperature_from_e(d, Y, e, t)) return false;
     double cp, cv, rw;
-    cp_cv_rw(d, Y, t, cp, cv, rw);
+    if (TF == kThermoPoly || (TF == kThermoAny && d.common_tmid)) {
+        // cv at the converged T from the same polynomial; cp - cv = R / Wbar
+        if (!(t > 0) || !finite(t)) t = 1000.0;
+        EnergyPoly<T> P;
+        if (!poly_newton(d, Y, e, t, P, 60)) return false;
+        double ee;
+        P.eval(t, ee, cv);
+        rw = P.rw;
+        cp = cv + kR * rw;
+    } else if constexpr (TF != kThermoPoly) {
+        if (!temperature_from_e<T, kThermoSpecies>(d, Y, e, t)) return false;
+        cp_cv_rw(d, Y, t, cp, cv, rw);
+    }
     out.T = t;
     out.p = rho * kR * t * rw;
     out.a = sqrt(cp / cv * kR * t * rw);
@@ -241,7 +270,7 @@ __device__ __forceinline__ bool properties(const MechData<T>& d, const double* u
 // Temperature of a conserved state (the inversion inside properties()), or
 // `fallback` when the state is not admissible. Used to warm-start the
 // inversions of nearby states, as Cantera's phase object keeps the last T.
-template <class T>
+template <class T, int TF = kThermoAny>
 __device__ __forceinline__ double state_temperature(const MechData<T>& d, const double* u, double fallback) {
     constexpr int NS = T::NS;
     double rho = 0;
@@ -260,7 +289,7 @@ __device__ __forceinline__ double state_temperature(const MechData<T>& d, const 
     const double ux = u[NS] * rr, vy = u[NS + 1] * rr;
     const double e = u[NS + 2] * rr - 0.5 * (ux * ux + vy * vy);
     double t = fallback;
-    return temperature_from_e(d, Y, e, t) ? t : fallback;
+    return temperature_from_e<T, TF>(d, Y, e, t) ? t : fallback;
 }
 
 __device__ __forceinline__ double minmod(double x, double y) {

@@ -589,14 +589,18 @@ int chem(const ChemLaunch& L) {
 }
 
 // ------------------------------------------------------------------ flux stage
-// The tile and its face fluxes are staged in LDS: (TX+4)(TY+4) cells and
-// (TX+1)(TY+1) faces of NV doubles. 15 x 15 (one 256-lane workgroup) for small
-// mechanisms; 7 x 7 (one wave, 63 KB at NV = 42) for large ones; 1-D rows of
-// 255 or 63 cells.
+// The tile is staged in LDS: (TX+4)(TY+4) cells of NV doubles and their warm
+// start temperatures, plus the face fluxes that cross waves. 15 x 15 (one
+// 256-lane workgroup) for small mechanisms; 7 x 7 (one wave) for large ones;
+// 1-D rows of 255 or 63 cells.
 constexpr bool kLargeState = Topo::NS > 16;
 constexpr int TX2 = kLargeState ? 7 : 15, TY2 = TX2, TX1 = kLargeState ? 63 : 255;
-static_assert(sizeof(double) * (Topo::NS + 3) * ((TX2 + 4) * (TY2 + 4) + (TX2 + 1) * (TY2 + 1)) +
-                  sizeof(double) * (TX2 + 4) * (TY2 + 4) <= 65536,
+template <int TX, int TY, bool TWO_D>
+constexpr size_t flux_lds_bytes() {
+    using FT = FluxTile<Topo, TX, TY, TWO_D>;
+    return sizeof(double) * (FT::LDS_IN + FT::LDS_FLUX + FT::GX * FT::GY);
+}
+static_assert(flux_lds_bytes<TX2, TY2, true>() <= 65536 && flux_lds_bytes<TX1, 1, false>() <= 65536,
               "flux tile exceeds 64 KB of LDS");
 
 // Other 2-D tile shapes of 256 lanes for the tuner (small mechanisms only):
@@ -605,14 +609,15 @@ static_assert(sizeof(double) * (Topo::NS + 3) * ((TX2 + 4) * (TY2 + 4) + (TX2 + 
 // depend on the shape.
 constexpr int NSHAPE = kLargeState ? 1 : 3;
 constexpr int SHAPE_TX[3] = {TX2, 31, 7}, SHAPE_TY[3] = {TY2, 7, 31};
-static_assert(kLargeState || sizeof(double) * (Topo::NS + 3) * (35 * 11 + 32 * 8) + sizeof(double) * 35 * 11 <= 65536,
+static_assert(kLargeState || (flux_lds_bytes<31, 7, true>() <= 65536 && flux_lds_bytes<7, 31, true>() <= 65536),
               "flux tile exceeds 64 KB of LDS");
 
 template <int TX, int TY, bool TWO_D>
 void launch_flux(const FluxLaunch& L, unsigned grid) {
     hipLaunchKernelGGL((flux_stage_kernel<Topo, TX, TY, TWO_D>), dim3(grid),
                        dim3(FluxTile<Topo, TX, TY, TWO_D>::THREADS), 0, L.stream, static_cast<const MD*>(L.mech),
-                       L.boxes, L.tiles, L.ntiles, L.Uin, L.Uout, L.first_order, L.first_failed, L.counters,
+                       L.boxes, L.tiles, L.ntiles, L.Uin, L.Uout, L.first_order, !L.skip_validation,
+                       L.first_failed, L.counters,
                        L.fluxes, L.in_ptrs, L.out_ptrs);
 }
 

@@ -101,6 +101,10 @@ struct FluxLaunch {
     const double* Uin;
     double* Uout;
     int first_order;
+    // 1: the stage does not validate the updated cells (a caller that repairs
+    // and validates them with its own kernels); first_failed then reports only
+    // inadmissible cell values met by the face loop
+    int skip_validation;
     int one_d;                      // x-only stage: tiles are flux_tx1 x 1, else flux_tx2 x flux_ty2
     int shape;                      // 2-D tile shape index (flux_shape_tx/ty), 0 the default
     unsigned long long* first_failed;  // device scalar
