@@ -1028,18 +1028,39 @@ __global__ void __launch_bounds__(256) avgdown_kernel(const double* __restrict__
 
 // Roundoff repair (BlockAMR::repairRoundoff, BlockAMR.cpp:299-334) on one
 // level of equal cell volumes. scal layout: [0, NS) peak bits, [NS, 2NS)
-// donor index, [2NS, 3NS) deficit (double bits), 3NS repairs, 3NS+1 failure.
+// donor index, [2NS, 3NS) deficit in fixed point (units of 64 eps peak /
+// 2^40), 3NS repairs, 3NS+1 failure.
 __global__ void __launch_bounds__(256) repair_peak_kernel(const double* __restrict__ U, hf_box B,
                                                           unsigned long long* __restrict__ scal) {
     constexpr int NS = Topo::NS;
+    __shared__ unsigned long long part[4][NS];
     const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= (long)B.nx * B.ny) return;
-    const int i = (int)(c % B.nx), j = (int)(c / B.nx);
+    // peak starts at 0 (the reference's zero-initialised array): only positive
+    // values compete; positive doubles order like their bit patterns, so the
+    // workgroup's maximum is taken first and one atomic per species follows
+    unsigned long long pv[NS];
+    #pragma unroll
+    for (int n = 0; n < NS; ++n) pv[n] = 0;
+    if (c < (long)B.nx * B.ny) {
+        const int i = (int)(c % B.nx), j = (int)(c / B.nx);
+        #pragma unroll
+        for (int n = 0; n < NS; ++n) {
+            const double v = box_at(const_cast<double*>(U), B, n, i, j);
+            if (v > 0) pv[n] = (unsigned long long)__double_as_longlong(v);
+        }
+    }
     #pragma unroll
     for (int n = 0; n < NS; ++n) {
-        const double v = box_at(const_cast<double*>(U), B, n, i, j);
-        // peak starts at 0 (the reference's zero-initialised array): only positive values compete
-        if (v > 0) atomicMax(&scal[n], (unsigned long long)__double_as_longlong(v));
+        #pragma unroll
+        for (int off = 32; off > 0; off >>= 1) pv[n] = max(pv[n], (unsigned long long)__shfl_xor(pv[n], off, 64));
+        if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6][n] = pv[n];
+    }
+    __syncthreads();
+    if (threadIdx.x < NS) {
+        const int n = threadIdx.x;
+        unsigned long long m = 0;
+        for (int w = 0; w < (int)(blockDim.x + 63) / 64; ++w) m = max(m, part[w][n]);
+        if (m) atomicMax(&scal[n], m);
     }
 }
 // donor: the first cell in BoxIterator order (i fastest) holding the peak mass
@@ -1068,11 +1089,14 @@ __global__ void __launch_bounds__(256) repair_zero_kernel(double* __restrict__ U
         double& v = box_at(U, B, n, i, j);
         if (v >= 0) continue;
         const double peak = __longlong_as_double((long long)scal[n]);
-        if (!finite(v) || -v > 64.0 * 2.220446049250313e-16 * peak) {
+        const double limit = 64.0 * 2.220446049250313e-16 * peak;
+        if (!finite(v) || -v > limit) {
             atomicMin(first_failed, (unsigned long long)c);
             continue;
         }
-        atomicAdd(reinterpret_cast<double*>(&scal[2 * NS + n]), -v);
+        // the deficit in fixed point, in units of limit / 2^40 (|v| <= limit):
+        // integer sums do not depend on the order the cells add in
+        atomicAdd(&scal[2 * NS + n], (unsigned long long)llrint(-v / limit * 1099511627776.0));
         v = 0.0;
         ++repairs;
     }
@@ -1083,9 +1107,9 @@ __global__ void repair_donate_kernel(double* __restrict__ U, hf_box B, unsigned 
     constexpr int NS = Topo::NS;
     const int n = threadIdx.x;
     if (n >= NS) return;
-    const double deficit = __longlong_as_double((long long)scal[2 * NS + n]);
-    if (!(deficit > 0)) return;
     const double peak = __longlong_as_double((long long)scal[n]);
+    const double deficit = (double)scal[2 * NS + n] * (64.0 * 2.220446049250313e-16 * peak / 1099511627776.0);
+    if (!(deficit > 0)) return;
     const unsigned long long d = scal[NS + n];
     if (!(peak > deficit) || d == ~0ull) {
         atomicMin(first_failed, d == ~0ull ? 0ull : d);
@@ -1100,7 +1124,7 @@ __global__ void repair_init_kernel(unsigned long long* __restrict__ scal, unsign
     if (t < NS) {
         scal[t] = 0;                 // peak 0.0
         scal[NS + t] = ~0ull;        // no donor
-        scal[2 * NS + t] = 0;        // deficit 0.0
+        scal[2 * NS + t] = 0;        // deficit 0 (fixed point)
     }
     if (t == 0) {
         scal[3 * NS] = 0;
@@ -1186,7 +1210,7 @@ __global__ void __launch_bounds__(256) cells_props_kernel(const MD* __restrict__
 // [NS, 2NS) largest mass (value * volume) bits, [2NS, 3NS) donor cell id (the
 // first cell holding the largest mass), 3NS repairs, 3NS+1 donor failure.
 // The deficits are summed per workgroup in a fixed order (partials) and then
-// over the workgroups in order, so the result does not depend on timing.
+// over the workgroups in a fixed order, so the result does not depend on timing.
 __global__ void cells_repair_init_kernel(unsigned long long* __restrict__ scal,
                                          unsigned long long* __restrict__ first_failed) {
     constexpr int NS = Topo::NS;
@@ -1202,22 +1226,51 @@ __global__ void cells_repair_init_kernel(unsigned long long* __restrict__ scal,
         *first_failed = ~0ull;
     }
 }
+// Largest value and largest mass per species: reduced within the workgroup
+// first (positive doubles order like their bit patterns; a maximum does not
+// depend on the order it is taken in), so each workgroup issues one atomic
+// per species and quantity instead of one per cell.
 __global__ void __launch_bounds__(256) cells_repair_peak_kernel(CellMap S, long total,
                                                                 unsigned long long* __restrict__ scal) {
     constexpr int NS = Topo::NS;
+    __shared__ unsigned long long part[4][2 * NS];
     const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= total) return;
-    int b, i, j;
-    const hf_cells& B = S.find(c, b, i, j);
-    if (cell_kind(B, i, j) != 1) return;
-    const double vol = B.dy > 0 ? B.dx * B.dy : B.dx;
+    unsigned long long pv[NS], pm[NS];
+    #pragma unroll
+    for (int n = 0; n < NS; ++n) pv[n] = pm[n] = 0;
+    if (c < total) {
+        int b, i, j;
+        const hf_cells& B = S.find(c, b, i, j);
+        if (cell_kind(B, i, j) == 1) {
+            const double vol = B.dy > 0 ? B.dx * B.dy : B.dx;
+            #pragma unroll
+            for (int n = 0; n < NS; ++n) {
+                const double v = cell_at(B, n, i, j);
+                if (v > 0) {
+                    pv[n] = (unsigned long long)__double_as_longlong(v);
+                    pm[n] = (unsigned long long)__double_as_longlong(v * vol);
+                }
+            }
+        }
+    }
     #pragma unroll
     for (int n = 0; n < NS; ++n) {
-        const double v = cell_at(B, n, i, j);
-        if (v > 0) {
-            atomicMax(&scal[n], (unsigned long long)__double_as_longlong(v));
-            atomicMax(&scal[NS + n], (unsigned long long)__double_as_longlong(v * vol));
+        #pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            pv[n] = max(pv[n], (unsigned long long)__shfl_xor(pv[n], off, 64));
+            pm[n] = max(pm[n], (unsigned long long)__shfl_xor(pm[n], off, 64));
         }
+        if ((threadIdx.x & 63) == 0) {
+            part[threadIdx.x >> 6][n] = pv[n];
+            part[threadIdx.x >> 6][NS + n] = pm[n];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 2 * NS) {
+        const int q = threadIdx.x;
+        unsigned long long m = 0;
+        for (int w = 0; w < (int)(blockDim.x + 63) / 64; ++w) m = max(m, part[w][q]);
+        if (m) atomicMax(&scal[q], m);
     }
 }
 __global__ void __launch_bounds__(256) cells_repair_donor_kernel(CellMap S, long total,
@@ -1284,14 +1337,26 @@ __global__ void __launch_bounds__(256) cells_repair_zero_kernel(CellMap S, long 
         partials[(long)blockIdx.x * NS + n] = ((part[0][n] + part[1][n]) + part[2][n]) + part[3][n];
     }
 }
-__global__ void cells_repair_donate_kernel(CellMap S, int nparts, unsigned long long* __restrict__ scal,
-                                           const double* __restrict__ partials,
-                                           unsigned long long* __restrict__ first_failed) {
+// One workgroup per species: the deficit is the sum of the workgroup partials
+// in a fixed order (a strided pass per lane, then a fixed butterfly and the
+// four wave sums in order), so it does not depend on timing; lane 0 then
+// charges it to the donor.
+__global__ void __launch_bounds__(256) cells_repair_donate_kernel(CellMap S, int nparts,
+                                                                  unsigned long long* __restrict__ scal,
+                                                                  const double* __restrict__ partials,
+                                                                  unsigned long long* __restrict__ first_failed) {
     constexpr int NS = Topo::NS;
-    const int n = threadIdx.x;
-    if (n >= NS || *first_failed != ~0ull) return;
-    double deficit = 0.0;
-    for (int g = 0; g < nparts; ++g) deficit += partials[(long)g * NS + n];
+    __shared__ double wsum[4];
+    const int n = blockIdx.x;
+    if (*first_failed != ~0ull) return;
+    double x = 0.0;
+    for (int g = threadIdx.x; g < nparts; g += 256) x += partials[(long)g * NS + n];
+    #pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = x;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    const double deficit = ((wsum[0] + wsum[1]) + wsum[2]) + wsum[3];
     if (!(deficit > 0)) return;
     const double mass = __longlong_as_double((long long)scal[NS + n]);
     const unsigned long long d = scal[2 * NS + n];
@@ -1416,8 +1481,8 @@ int box_ops(const BoxLaunch& L) {
         hipLaunchKernelGGL(cells_repair_donor_kernel, dim3(g), dim3(256), 0, L.stream, S, (long)L.total, L.scal);
         hipLaunchKernelGGL(cells_repair_zero_kernel, dim3(g), dim3(256), 0, L.stream, S, (long)L.total, L.scal,
                            L.partials, L.first_failed);
-        hipLaunchKernelGGL(cells_repair_donate_kernel, dim3(1), dim3(64), 0, L.stream, S, (int)g, L.scal, L.partials,
-                           L.first_failed);
+        hipLaunchKernelGGL(cells_repair_donate_kernel, dim3(Topo::NS), dim3(256), 0, L.stream, S, (int)g, L.scal,
+                           L.partials, L.first_failed);
         break;
     }
     case BoxLaunch::REFLUX: {
