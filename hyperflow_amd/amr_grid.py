@@ -34,15 +34,18 @@ import numpy as np
 
 
 def grow_mask(m, n):
-    """Cells within n cells (Chebyshev distance) of a set cell."""
+    """Cells within n cells (Chebyshev distance) of a set cell: the square
+    neighbourhood is separable, a dilation along x then one along y."""
     if n <= 0:
         return m.copy()
     out = m.copy()
-    H, W = m.shape
-    for dj in range(-n, n + 1):
-        for di in range(-n, n + 1):
-            src = m[max(0, -dj):H - max(0, dj), max(0, -di):W - max(0, di)]
-            out[max(0, dj):H - max(0, -dj), max(0, di):W - max(0, -di)] |= src
+    for d in range(1, n + 1):
+        out[:, d:] |= m[:, :-d]
+        out[:, :-d] |= m[:, d:]
+    row = out.copy()
+    for d in range(1, n + 1):
+        out[d:, :] |= row[:-d, :]
+        out[:-d, :] |= row[d:, :]
     return out
 
 
@@ -57,9 +60,15 @@ def shrink_mask(m, n, keep_edges=True):
 
 
 def coarsen_mask(m, r):
-    """A flag per r x r block: set when any cell of the block is set."""
-    H, W = m.shape
-    return m.reshape(H // r, r, W // r, r).any(axis=(1, 3))
+    """A flag per r x r block: set when any cell of the block is set (an OR of
+    r strided slices along x, then along y)."""
+    a = m[:, 0::r].copy()
+    for k in range(1, r):
+        a |= m[:, k::r]
+    b = a[0::r, :].copy()
+    for k in range(1, r):
+        b |= a[k::r, :]
+    return b
 
 
 def boxes_mask(boxes, shape):

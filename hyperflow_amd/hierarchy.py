@@ -70,8 +70,12 @@ class _Level:
         return hf_patch(2 * self.gx + 2, self.gx, self.gx * self.gy)
 
     def cells(self, A):
-        return hf_cells(A.data_ptr(), 2 * self.gx + 2, self.gx, self.gx * self.gy, self.nx, self.ny,
-                        self.mask.data_ptr(), self.nx, self.dx, self.dy)
+        """One hf_cells block per box of the level (valid cells only, box
+        order, i fastest: the reference's MFIter / BoxIterator order), with
+        the level mask telling leaves from covered cells."""
+        base, mask = A.data_ptr(), self.mask.data_ptr()
+        return [hf_cells(base, (j0 + 2) * self.gx + i0 + 2, self.gx, self.gx * self.gy, i1 - i0 + 1, j1 - j0 + 1,
+                         mask + j0 * self.nx + i0, self.nx, self.dx, self.dy) for (i0, j0, i1, j1) in self.boxes]
 
     def interior(self, A):
         return A[:, 2:self.ny + 2, 2:self.nx + 2]
@@ -226,8 +230,9 @@ class AmrDriver:
                                                          byref(cp), C.mask.data_ptr(), C.nx, reg))
 
     def _cells(self, key):
-        n = self.finest + 1
-        return (hf_cells * n)(*[self.levels[lev].cells(self.levels[lev].A[key]) for lev in range(n)]), n
+        blocks = [b for lev in range(self.finest + 1) for b in self.levels[lev].cells(self.levels[lev].A[key])]
+        n = len(blocks)
+        return (hf_cells * n)(*blocks), n
 
     def _validate(self, key):
         blocks, n = self._cells(key)
@@ -365,7 +370,11 @@ class AmrDriver:
         self._fill("U", lev)
         out = np.zeros((L.ny, L.nx), dtype=bool)
         k = self.tag
-        for (i0, j0, i1, j1) in L.boxes:
+        # every box's tags go to one device buffer, read back once
+        sizes = [(i1 - i0 + 1) * (j1 - j0 + 1) for (i0, j0, i1, j1) in L.boxes]
+        tg_all = self.torch.zeros(sum(sizes), dtype=self.torch.uint8, device="cuda")
+        at = 0
+        for (i0, j0, i1, j1), size in zip(L.boxes, sizes):
             nx, ny = i1 - i0 + 1, j1 - j0 + 1
             box = hf_box(nx, ny, 0, 0, j0 * L.gx + i0, L.gx, L.gx * L.gy, 0, nx, nx * ny, 0.0, 0.0)
             box_ = k["box"]
@@ -374,12 +383,16 @@ class AmrDriver:
                               (c_double * 4)(*(box_ if box_ is not None else (0.0,) * 4)),
                               (c_double * 2)(*k["velocity"]), self.x0 + i0 * L.dx, self.y0 + j0 * L.dy, L.dx, L.dy,
                               self.time)
-            tg = self.torch.zeros((ny, nx), dtype=self.torch.uint8, device="cuda")
             n, ff = c_int64(0), c_int64(-1)
-            r = load().hf_tag_cells(self._h, L.A["U"].data_ptr(), byref(box), byref(P), tg.data_ptr(), byref(n),
-                                    byref(ff))
+            r = load().hf_tag_cells(self._h, L.A["U"].data_ptr(), byref(box), byref(P), tg_all[at:].data_ptr(),
+                                    byref(n), byref(ff))
             check(self._h, r, ff.value)
-            out[j0:j1 + 1, i0:i1 + 1] |= tg.cpu().numpy().astype(bool)
+            at += size
+        host = tg_all.cpu().numpy().astype(bool)
+        at = 0
+        for (i0, j0, i1, j1), size in zip(L.boxes, sizes):
+            out[j0:j1 + 1, i0:i1 + 1] |= host[at:at + size].reshape(j1 - j0 + 1, i1 - i0 + 1)
+            at += size
         return out
 
     def regrid(self):
