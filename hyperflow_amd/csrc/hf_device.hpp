@@ -61,6 +61,18 @@ struct MechData {
     int16_t inc_start[NS + 1];
     int16_t inc_rxn[2 * NR * T::NRMAX];
     int8_t inc_nu[2 * NR * T::NRMAX];
+    // The analytic Jacobian of the one-cell-per-wave kernel by concentration
+    // columns: column s takes the entries jc_start[s] .. jc_start[s+1), each
+    // adding jc_coef x (derivative jc_m of reaction jc_rxn) to row jc_k of
+    // d wdot / d C. Derivative m < 2 NRMAX is d q / d C through reaction slot
+    // m (reactant slots, then product slots); m = 2 NRMAX is d q / d[M]
+    // (for the listed third-body efficiencies; the default efficiency, common
+    // to every column, is added per row). Rows are species < NS - 1.
+    static constexpr int NJC = NR * 2 * T::NRMAX * (2 * T::NRMAX + (NEFF > 0 ? NEFF : 1));
+    int32_t jc_start[NS + 1];
+    int16_t jc_rxn[NJC];
+    int8_t jc_m[NJC], jc_k[NJC];
+    double jc_coef[NJC];
 };
 
 // ----------------------------------------------------------------- utilities
@@ -382,7 +394,8 @@ struct RateI {
                 const double A = d.troeA[I];
                 const double e3 = exp(-t * d.troeRT3[I]), e1 = exp(-t * d.troeRT1[I]);
                 double fc = (1.0 - A) * e3 + A * e1;
-                double dfc = -(1.0 - A) * d.troeRT3[I] * e3 - A * d.troeRT1[I] * e1;
+                // a zero T3 or T1 is an infinite rate whose term vanishes (0 x inf would be NaN)
+                double dfc = -(1.0 - A) * (e3 > 0 ? d.troeRT3[I] * e3 : 0.0) - A * (e1 > 0 ? d.troeRT1[I] * e1 : 0.0);
                 if constexpr (T::troe[I] == 4) {
                     const double e2 = exp(-d.troeT2[I] * it);
                     fc += e2;

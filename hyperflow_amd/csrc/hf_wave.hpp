@@ -11,8 +11,9 @@
 // The integrator is the same CVODE-equivalent BDF as hf_bdf.hpp (the
 // reference's CVODE call, HyperFlow_V3/src/chemistry.cpp:10-26,64-68):
 // Nordsieck form, orders 1-5, WRMS error test, Newton with CVODE's
-// convergence test and Jacobian reuse, difference-quotient Jacobian
-// (cvLsDenseDQJac, the reference's configuration). The Jacobian is kept in
+// convergence test and Jacobian reuse. The Jacobian is analytic (the
+// reference's CVODE forms it by difference quotients, cvLsDenseDQJac, at N
+// rate evaluations each; see WaveGas::jacobian). It is kept in
 // a per-wave global slot (row = lane, one entry per column); I - gamma J is
 // formed and factored in LDS: in FP32 by a blocked LU whose trailing updates
 // run on the matrix cores, in FP64 row by row against the broadcast pivot
@@ -61,7 +62,7 @@ template <class T>
 struct WaveGas {
     static constexpr int NS = T::NS, N = NS - 1, NR = T::NR;
     const MechData<T>& d;
-    double* lds;                 // per wave: C[NS], grt[NS], q[NR]
+    double* lds;                 // per wave: C[NS], grt[NS], q[NR], h/RT[NS]
     const int* inc;              // per wave (LDS): species incidences, reaction + nu * 65536
     int inc_b, inc_e;            // this lane's species: inc[inc_b .. inc_e)
     int lane;
@@ -237,6 +238,199 @@ struct WaveGas {
         return !__any(!finite(f));
     }
 
+    // ---------------------------------------------------------------- analytic Jacobian
+    // Derivatives of reaction i's net rate of progress (the runtime form of
+    // RateI<., ., true> in hf_device.hpp) into deriv[m * NR + i]: m < NRMAX
+    // d q / d C through reactant slot m (k_f times the product of the other
+    // reactant concentrations), NRMAX <= m < 2 NRMAX through product slot
+    // m - NRMAX (-k_r times the others), 2 NRMAX d q / d[M], 2 NRMAX + 1
+    // d q / d T at fixed concentrations. C, g/RT and h/RT come from LDS.
+    __device__ __forceinline__ void rate_derivs(int i, double logt, double it, double logc0, double ctot,
+                                                double* __restrict__ deriv) const {
+        constexpr int R = T::NRMAX;
+        const double* C = lds;
+        const double* grt = lds + NS;
+        const double* hrt = lds + 2 * NS + NR;
+        const int kind = d.rk_kind[i];
+        const double kinf = exp(d.lnA[i] + d.b[i] * logt - d.EaR[i] * it);
+        const double dlnkinf = (d.b[i] + d.EaR[i] * it) * it;
+        double Mc = 0;
+        if (kind != 0) {
+            Mc = d.defeff[i] * ctot;
+            #pragma unroll
+            for (int j = 0; j < (T::NEFF > 0 ? T::NEFF : 1); ++j) {
+                const int sp_ = d.rk_eff[i][j];
+                if (sp_ >= 0) Mc = fma(d.effd[i][j], C[sp_], Mc);
+            }
+        }
+        double kf = kinf, dlnkf = dlnkinf, dkf_dM = 0;
+        if (kind == 1) {
+            kf = kinf * Mc;
+            dkf_dM = kinf;
+        } else if (kind == 2) {
+            const double k0 = exp(d.lnA0[i] + d.b0[i] * logt - d.Ea0R[i] * it);
+            const double pr = Mc * k0 / (kinf + kSmall);
+            double F = 1.0, g_lpr = 0.0, g_lc = 0.0, dlc_dT = 0.0;
+            const int troe = d.rk_troe[i];
+            if (troe > 0) {
+                const double A = d.troeA[i];
+                const double e3 = exp(-t * d.troeRT3[i]), e1 = exp(-t * d.troeRT1[i]);
+                double fc = (1.0 - A) * e3 + A * e1;
+                // a zero T3 or T1 is an infinite rate whose term vanishes (0 x inf would be NaN)
+                double dfc = -(1.0 - A) * (e3 > 0 ? d.troeRT3[i] * e3 : 0.0) - A * (e1 > 0 ? d.troeRT1[i] * e1 : 0.0);
+                if (troe == 4) {
+                    const double e2 = exp(-d.troeT2[i] * it);
+                    fc += e2;
+                    dfc += d.troeT2[i] * it * it * e2;
+                }
+                const double fcc = fmax(fc, kSmall);
+                const double lc = log10(fcc);
+                const double lpr = log10(fmax(pr, kSmall));
+                const double c = -0.4 - 0.67 * lc, n = 0.75 - 1.27 * lc;
+                const double u = lpr + c;
+                const double den = n - 0.14 * u;
+                const double f1 = u / den;
+                const double qq = 1.0 / (1.0 + f1 * f1);
+                F = exp(kLn10 * (lc * qq));
+                const double df1_du = n / (den * den), df1_dn = -u / (den * den);
+                const double dlgf_df1 = -2.0 * lc * f1 * qq * qq;
+                g_lpr = pr > kSmall ? dlgf_df1 * df1_du : 0.0;
+                g_lc = qq + dlgf_df1 * (df1_du * (-0.67) + df1_dn * (-1.27));
+                dlc_dT = fc > kSmall ? dfc / (fcc * kLn10) : 0.0;
+            }
+            const double opr = 1.0 / (1.0 + pr);
+            kf = kinf * (pr / (1.0 + pr)) * F;
+            const double dlnpr = (d.b0[i] + d.Ea0R[i] * it) * it - dlnkinf;
+            dlnkf = dlnkinf + opr * dlnpr + g_lpr * dlnpr + kLn10 * g_lc * dlc_dT;
+            dkf_dM = k0 * F * opr * (opr + g_lpr);
+        }
+        double cr[R], cp[R];
+        double pf = 1.0, pb = 1.0, dg = 0.0, dh = 0.0;
+        int dn = 0;
+        #pragma unroll
+        for (int j = 0; j < R; ++j) {
+            const int r = d.rk_reac[i][j], p = d.rk_prod[i][j];
+            cr[j] = r >= 0 ? C[r] : 1.0;
+            cp[j] = p >= 0 ? C[p] : 1.0;
+            pf *= cr[j];
+            pb *= cp[j];
+            if (r >= 0) {
+                dg -= grt[r];
+                dh -= hrt[r];
+                --dn;
+            }
+            if (p >= 0) {
+                dg += grt[p];
+                dh += hrt[p];
+                ++dn;
+            }
+        }
+        const bool rev = d.rk_rev[i] != 0;
+        const double rkc = rev ? exp(dg - dn * logc0) : 0.0;
+        const double kr = kf * rkc;
+        const double dlnrkc = rev ? (dn - dh) * it : 0.0;
+        #pragma unroll
+        for (int j = 0; j < R; ++j) {
+            double a = kf, b = -kr;
+            #pragma unroll
+            for (int m = 0; m < R; ++m) {
+                if (m != j) {
+                    a *= cr[m];
+                    b *= cp[m];
+                }
+            }
+            deriv[j * NR + i] = d.rk_reac[i][j] >= 0 ? a : 0.0;
+            deriv[(R + j) * NR + i] = d.rk_prod[i][j] >= 0 ? b : 0.0;
+        }
+        deriv[2 * R * NR + i] = dkf_dM * (pf - rkc * pb);
+        deriv[(2 * R + 1) * NR + i] = kf * dlnkf * pf - kr * (dlnkf + dlnrkc) * pb;
+    }
+
+    // J = d f / d y at y (f_k = wdot_k W_k / rho, k < N; y the unknowns of
+    // lanes < N, the last species closing the mass; T from e at fixed e):
+    //   J_ks = W_k / W_s D_ks - W_k / W_L D_kL + W_k / rho dwdot_k/dT dT/dy_s,
+    //   dT/dy_s = -(e_s - e_L) / cv,
+    // with D = d wdot / d C. D is assembled by concentration columns, lane s
+    // walking its column's list (MechData::jc_*) into LDS, from the
+    // reactions' derivatives, which the reaction lanes write to this wave's
+    // global scratch `deriv` ((2 NRMAX + 2) x NR doubles); the part of d[M]/d
+    // C_s common to every column (the default efficiency) and dwdot/dT are
+    // gathered per row over the species' incidences. Writes J into the
+    // wave's slot jg (entry (row lane, column s) at jg[s * 64 + lane]); Dl:
+    // LDS room for NS x 64 entries (the Newton matrix's, re-formed after this).
+    // Returns the wave-uniform success flag.
+    template <class MT>
+    __device__ __forceinline__ bool jacobian(double y, MT* __restrict__ jg, MT* Dl, double* __restrict__ deriv) {
+        constexpr int R = T::NRMAX;
+        const double Yk = mass_fraction(y);
+        double tt = t;
+        if (!temperature(Yk, e, tt)) return false;
+        t = tt;
+        const double logt = log(t), it = 1.0 / t;
+        const double logc0 = kLnP0R - logt;
+        double C = 0, ek = 0, cvk = 0;
+        if (sp) {
+            const bool low = t <= tmid;
+            double a[7];
+            #pragma unroll
+            for (int i = 0; i < 7; ++i) a[i] = low ? lo[i] : hi[i];
+            const double hrt = fma(t, fma(t, fma(t, fma(t, a[4] * 0.2, a[3] * 0.25), a[2] * (1.0 / 3.0)), a[1] * 0.5),
+                                   a[0]) + a[5] * it;
+            const double sr = fma(a[0], logt, fma(t, fma(t, fma(t, fma(t, a[4] * 0.25, a[3] * (1.0 / 3.0)), a[2] * 0.5),
+                                                     a[1]), a[6]));
+            const double cpr = fma(t, fma(t, fma(t, fma(t, a[4], a[3]), a[2]), a[1]), a[0]);
+            C = rho * Yk * invW;
+            lds[lane] = C;
+            lds[NS + lane] = hrt - sr;
+            lds[2 * NS + NR + lane] = hrt;
+            ek = (hrt - 1.0) * kR * t * invW;
+            cvk = Yk * invW * (cpr - 1.0);
+        }
+        const double ctot = wave_sum(C);
+        const double cv = kR * wave_sum(cvk);
+        #pragma unroll 1
+        for (int s = 0; s < NS; ++s) Dl[s * 64 + lane] = MT(0);
+        __syncthreads();
+        for (int sl = lane; sl < NR; sl += 64) rate_derivs(d.rate_slot[sl], logt, it, logc0, ctot, deriv);
+        __syncthreads();
+        // column s of D on lane s
+        if (sp) {
+            const int b = d.jc_start[lane], en = d.jc_start[lane + 1];
+            MT* col = Dl + lane * 64;
+            for (int p = b; p < en; ++p) {
+                const double v = d.jc_coef[p] * deriv[d.jc_m[p] * NR + d.jc_rxn[p]];
+                col[d.jc_k[p]] += (MT)v;
+            }
+        }
+        // row factors: dwdot_k/dT and the default-efficiency part of d wdot_k / d C
+        double ak = 0, uk = 0;
+        if (lane < N) {
+            for (int p = inc_b; p < inc_e; ++p) {
+                const int v = inc[p];
+                const int r = v & 0xffff;
+                const double nu = (double)(v >> 16);
+                ak = fma(nu, deriv[(2 * R + 1) * NR + r], ak);
+                if (d.rk_kind[r] != 0) uk = fma(nu * d.defeff[r], deriv[2 * R * NR + r], uk);
+            }
+        }
+        __syncthreads();
+        const double rc = 1.0 / cv, eL = lane_of(ek, N);
+        const double dL = lane < N ? (double)Dl[N * 64 + lane] + uk : 0.0;
+        const double wl = lane < N ? W * d.invW[N] : 0.0, tk = lane < N ? W / rho * ak : 0.0;
+        bool ok = true;
+        #pragma unroll 2
+        for (int s = 0; s < N; ++s) {
+            const double dTdy = -(lane_of(ek, s) - eL) * rc;
+            double J = 0.0;
+            if (lane < N) {
+                J = W * d.invW[s] * ((double)Dl[s * 64 + lane] + uk) - wl * dL + tk * dTdy;
+                ok = ok && finite(J);
+            }
+            jg[s * 64 + lane] = (MT)J;
+        }
+        return !__any(!ok);
+    }
+
     // properties() on the cell state held one species per lane: rhoY_k on
     // lane k, momenta and energy (uniform). Returns admissibility; T warm-started from t.
     __device__ __forceinline__ bool properties(double rhoY, double mx, double my, double E, double& tt) const {
@@ -274,7 +468,9 @@ struct WaveBdf {
     static constexpr double UROUND = 2.220446049250313e-16;
     WaveGas<T> g;                // held by value: the whole integrator stays in registers
     MT* jg;                      // global, this wave's slot: Jacobian entry (lane, j) at jg[j * 64 + lane]
-    MT* lu;                      // LDS: factored Newton matrix, entry (lane, j) at lu[j * 64 + lane]
+    MT* lu;                      // LDS: factored Newton matrix, entry (lane, j) at lu[j * 64 + lane] (NS x 64
+                                 // entries: the Jacobian's assembly uses the extra column)
+    double* jd;                  // global, this wave's reaction-derivative scratch ((2 NRMAX + 2) x NR)
     double* coef;                // LDS: tau[QMAX + 2], l[QMAX + 1]
     int lane;
     bool own;                    // lane < N
@@ -288,8 +484,8 @@ struct WaveBdf {
     WaveCounters c;
 
     __device__ __forceinline__ WaveBdf(const MechData<T>& d, double* scratch, int* inc_lds, MT* jslot, MT* lds_lu,
-                                       double* cf, double rt, double at)
-        : g(d, scratch, inc_lds), jg(jslot), lu(lds_lu), coef(cf), rtol(rt), atol(at) {
+                                       double* deriv, double* cf, double rt, double at)
+        : g(d, scratch, inc_lds), jg(jslot), lu(lds_lu), jd(deriv), coef(cf), rtol(rt), atol(at) {
         lane = threadIdx.x & 63;
         own = lane < N;
     }
@@ -444,23 +640,6 @@ struct WaveBdf {
     }
 
     // ---------------------------------------------------------------- linear algebra
-    // Difference-quotient Jacobian (cvLsDenseDQJac) into the wave's global
-    // slot (read back only when the Newton matrix is re-formed); y, f: this
-    // lane's unknown and f(y).
-    __device__ __forceinline__ bool dq_jacobian(double y, double f) {
-        const double fnorm = wrms(f);
-        const double min_inc = fnorm != 0 ? 1000.0 * fabs(h) * UROUND * N * fnorm : 1.0;
-        const double srur = sqrt(UROUND);
-        for (int j = 0; j < N; ++j) {
-            const double yj = lane_of(y, j), ewj = lane_of(ewt, j);
-            const double inc = fmax(srur * fabs(yj), min_inc / ewj);
-            double fj;
-            ++c.rhs;
-            if (!g.rhs(lane == j ? yj + inc : y, fj)) return false;
-            jg[j * 64 + lane] = (MT)((fj - f) * (1.0 / inc));
-        }
-        return true;
-    }
     // lu = I - gamma J from the wave's global Jacobian slot; rows N..63 are
     // zero. (Loading 16 columns at a time measured slower than this rolled
     // loop: the wider batch spills.)
@@ -641,7 +820,7 @@ struct WaveBdf {
         ++c.lu;
         if (jbad) {
             ++c.jac;
-            if (!dq_jacobian(y, f)) return false;
+            if (!g.jacobian(y, jg, lu, jd)) return false;
             nstlj = nst;
             jfresh = true;
         } else {

@@ -93,6 +93,51 @@ void fill(const Mechanism& m, void* host) {
         }
     }
     d.inc_start[ns] = (int16_t)n;
+    // Jacobian columns: for each slot m of reaction i holding species s,
+    // d q_i / d C_s gets that slot's derivative, and every slot m2 of the
+    // reaction passes it to its species with the slot's sign (reactants -1,
+    // products +1; a species in two slots gets two entries). A listed
+    // third-body efficiency of s adds its correction times d q_i / d[M].
+    {
+        constexpr int R = Topo::NRMAX;
+        auto slot = [](int i, int m) { return m < R ? Topo::reac[i][m] : Topo::prod[i][m - R]; };
+        int q = 0;
+        for (int sp = 0; sp < ns; ++sp) {
+            d.jc_start[sp] = q;
+            for (int i = 0; i < nr; ++i) {
+                for (int m = 0; m < 2 * R; ++m) {
+                    if (slot(i, m) != sp) continue;
+                    for (int m2 = 0; m2 < 2 * R; ++m2) {
+                        const int k = slot(i, m2);
+                        if (k < 0 || k >= ns - 1) continue;
+                        d.jc_rxn[q] = (int16_t)i;
+                        d.jc_m[q] = (int8_t)m;
+                        d.jc_k[q] = (int8_t)k;
+                        d.jc_coef[q] = m2 < R ? -1.0 : 1.0;
+                        ++q;
+                    }
+                }
+                if (Topo::kind[i] != 0) {
+                    for (int j = 0; j < NEFF; ++j) {
+                        if (Topo::NEFF == 0 || Topo::effsp[i][j] != sp) continue;
+                        for (int m2 = 0; m2 < 2 * R; ++m2) {
+                            const int k = slot(i, m2);
+                            if (k < 0 || k >= ns - 1) continue;
+                            d.jc_rxn[q] = (int16_t)i;
+                            d.jc_m[q] = (int8_t)(2 * R);
+                            d.jc_k[q] = (int8_t)k;
+                            d.jc_coef[q] = (m2 < R ? -1.0 : 1.0) * d.effd[i][j];
+                            ++q;
+                        }
+                    }
+                }
+            }
+        }
+        // q <= NJC: a reaction's slots hold at most 2 NRMAX species
+        // occurrences, each passing to at most 2 NRMAX rows, and each listed
+        // efficiency to at most 2 NRMAX rows
+        d.jc_start[ns] = q;
+    }
 }
 
 // ------------------------------------------------------------------ chemistry
@@ -419,16 +464,20 @@ chem_wave_kernel(const MechData<TT>* __restrict__ dm, CellMap cm, long ncell,
                  int8_t* __restrict__ status, unsigned long long* __restrict__ first_failed, ChemParams p,
                  unsigned long long* __restrict__ counters, Workspace w) {
     constexpr int NS = TT::NS, N = NS - 1, NR = TT::NR;
-    __shared__ double scratch[2 * NS + NR];
+    __shared__ double scratch[3 * NS + NR];
     __shared__ double coef[13];
-    __shared__ __attribute__((aligned(16))) MT lu[N * 64];   // 16-byte aligned for the blocked LU
+    __shared__ __attribute__((aligned(16))) MT lu[NS * 64];  // 16-byte aligned for the blocked LU
     __shared__ int inc[2 * NR * TT::NRMAX];
     const int lane = threadIdx.x;
     // this wave's Jacobian slot: N x 64 entries inside the per-lane Jacobian
     // region, which holds N x N entries for each of at least 64 x (the waves
     // per CU of the launch, or 4) lanes per CU: room for every wave of the grid
     MT* jslot = static_cast<MT*>(w.jsave) + (long)blockIdx.x * N * 64;
-    WaveBdf<TT, MT> bdf(*dm, scratch, inc, jslot, lu, coef, p.rtol, p.atol);
+    // this wave's reaction-derivative scratch for the analytic Jacobian, in
+    // the LU-factor region (the wave form factors in LDS), which holds N x N
+    // doubles per lane: (2 NRMAX + 2) x NR doubles per wave fit many times
+    double* jd = static_cast<double*>(w.lu) + (long)blockIdx.x * (2 * TT::NRMAX + 2) * NR;
+    WaveBdf<TT, MT> bdf(*dm, scratch, inc, jslot, lu, jd, coef, p.rtol, p.atol);
     WaveGas<TT>& gas = bdf.g;
     unsigned long long tot[7] = {0, 0, 0, 0, 0, 0, 0};
     int max_steps_cell = 0;

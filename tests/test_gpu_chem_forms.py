@@ -380,3 +380,35 @@ def test_online_tuner_keeps_every_result():
         assert np.array_equal(out, first)
     print(f"[gpu]   {len(seen)} configurations tried, states {tuned}, final {cfg}", flush=True)
     assert len(seen) >= 6 and tuned[0] == 1 and tuned[-1] == 2
+
+
+@pytest.mark.parametrize("precision", [hf.HF_PREC_FP64, hf.HF_PREC_MIXED], ids=["fp64", "mixed"])
+def test_dme_analytic_jacobian_newton_behaviour(precision):
+    """The one-cell-per-wave DME form builds its Jacobian analytically; the
+    plain form uses CVODE's difference quotients (chemistry.cpp:15-25).
+    The Jacobian only steers the Newton iteration, so a correct one keeps
+    the step sequence of the difference-quotient form (both approximate the
+    same derivative: within a few per cent in steps, Newton iterations per
+    step and setups per cell) while making no rate evaluations of its own
+    (about one evaluation per step instead of two). A wrong entry (a sign, a
+    missing third-body or temperature term) shows as slower Newton
+    convergence: more iterations and failures per step."""
+    print(f"[gpu] DME analytic vs difference-quotient Jacobian, precision={precision}, 192 cells seed 4",
+          flush=True)
+    n = 192
+    U0, dt = hf.Mechanism(hf.DME_SK39).synth_cells(1, 4, n)
+    stats = {}
+    for name, mode, prec in (("plain", hf.HF_CHEM_PLAIN, hf.HF_PREC_FP64), ("wave", hf.HF_CHEM_BATCHED, precision)):
+        mech = hf.Mechanism(hf.DME_SK39, rtol=1e-8, atol=1e-20, precision=prec)
+        Ud = dev(U0)
+        stats[name] = hf.ChemistryBatch(mech, mode=mode).advance(Ud, dev(dt))
+    p, w = stats["plain"], stats["wave"]
+    print(f"[gpu]   plain: steps {p['steps']} rhs {p['rhs']} newton {p['newton']} lu {p['lu']} jac {p['jac']} "
+          f"ncf {p['conv_failures']}; wave: steps {w['steps']} rhs {w['rhs']} newton {w['newton']} lu {w['lu']} "
+          f"jac {w['jac']} ncf {w['conv_failures']}", flush=True)
+    assert abs(w["steps"] - p["steps"]) <= 0.05 * p["steps"]
+    assert w["newton"] / w["steps"] <= 1.05 * p["newton"] / p["steps"]
+    assert w["lu"] <= 1.05 * p["lu"]
+    assert w["conv_failures"] <= p["conv_failures"] + n // 64
+    # no rate evaluations for the Jacobian: about one per Newton iteration
+    assert w["rhs"] <= 1.02 * w["newton"] + w["steps"] * 0.2
