@@ -27,7 +27,7 @@ import numpy as np
 from ._lib import (HF_BC_INFLOW, HF_BC_KEEP, HF_BC_PERIODIC, HF_BC_REFLECTING, HF_BC_TRANSMISSIVE,  # noqa: F401
                    HF_BC_WEDGE,
                    HF_CHEM_BATCHED, HF_EDOMAIN, HF_OK,
-                   HyperflowError, check, hf_box, hf_flux_stats, load)
+                   HyperflowError, check, hf_box, hf_cells, hf_flux_stats, load)
 from .chemistry import ChemistryBatch
 
 
@@ -312,8 +312,9 @@ class TwoLevelDriver:
     over the active cells (the uncovered coarse cells and every fine cell),
     then average-down. stableDt: the largest CFL rate over both levels.
 
-    Roundoff repair runs per level (each species' integral is kept exactly on
-    each level); the reference picks one donor over both levels.
+    Roundoff repair runs over the leaf cells of both levels with one donor
+    per species, as the reference's does (the covered coarse cells are
+    average-downs of the fine box).
     fine_box = (ilo, jlo, ihi, jhi) fine cells. A side of the box either lies
     on a non-periodic domain side (its ghosts then take that side's physical
     rule on the fine level, its faces are walls where the domain's are, and it
@@ -381,6 +382,8 @@ class TwoLevelDriver:
         mask = torch.ones((c.ny, c.nx), dtype=torch.bool, device="cuda")
         mask[self.covered] = False
         self.active = mask.reshape(-1)
+        # the coarse level's cells as the repair sees them: 1 a leaf, 2 covered by the box
+        self.cmask = torch.where(mask, 1, 2).to(torch.uint8)
         touch = (FI0 == 0, FI1 == 2 * c.nx - 1, FJ0 == 0, FJ1 == 2 * c.ny - 1)
         self.fine_bc = (c_int32 * 4)(*[c.bc[k] if touch[k] else HF_BC_KEEP for k in range(4)])
         self.fine_walls = sum(1 << k for k in range(4)
@@ -462,13 +465,13 @@ class TwoLevelDriver:
         check(self._h, load().hf_reflux(self._h, dst_c.data_ptr(), byref(cp), byref(cf), byref(ff_), fb,
                                         self.reflux_sides, h, c.dx, c.dy))
         self._average(dst_c, dst_f)
+        # repairRoundoff over the leaf cells of both levels, one donor per species
+        # (BlockAMR.cpp:299-334): the covered coarse cells take the average-down
         rep, ff = c_int64(0), c_int64(-1)
-        repaired = 0
-        for A, box in ((dst_f, self._fbox()), (dst_c, c._box())):
-            r = load().hf_repair_roundoff(self._h, A.data_ptr(), byref(box), byref(rep), byref(ff))
-            if r != HF_OK:
-                return r, ff.value
-            repaired += rep.value
+        r = load().hf_repair_cells(self._h, self._leaf_blocks(dst_c, dst_f), 2, byref(rep), byref(ff))
+        if r != HF_OK:
+            return r, ff.value
+        repaired = rep.value
         self.repairs += repaired
         if repaired:
             self._average(dst_c, dst_f)
@@ -477,6 +480,17 @@ class TwoLevelDriver:
             if r != HF_OK:
                 return r, ff.value
         return HF_OK, -1
+
+    def _leaf_blocks(self, A_c, A_f):
+        """hf_cells of the coarse level (mask: leaves and covered cells) and of
+        the fine box (all leaves), in level order."""
+        c = self.coarse
+        gx, gy = c.nx + 4, c.ny + 4
+        gxf, gyf = self.nxf + 4, self.nyf + 4
+        return (hf_cells * 2)(hf_cells(A_c.data_ptr(), 2 * gx + 2, gx, gx * gy, c.nx, c.ny, self.cmask.data_ptr(),
+                                       c.nx, c.dx, c.dy),
+                              hf_cells(A_f.data_ptr(), 2 * gxf + 2, gxf, gxf * gyf, self.nxf, self.nyf, None, 0,
+                                       self.dxf, self.dyf))
 
     def euler(self, src_c, src_f, dst_c, dst_f, h):
         r, ff = self._composite_once(src_c, src_f, dst_c, dst_f, h, False)

@@ -223,7 +223,7 @@ class OracleTwoLevelDriver:
     one fixed interior fine box at ratio 2, no regrid) on the oracle's
     functions: the checker of hyperflow_amd.driver.TwoLevelDriver. Same
     composition: coarse fill, fine ghost ring by interpolation, both stages
-    with face fluxes, reflux, average-down, per-level roundoff repair,
+    with face fluxes, reflux, average-down, roundoff repair over the leaves of both levels,
     validation, first-order redo; Heun; one chemistry batch over the active
     cells; CFL over both levels. Parity status: pinned by the functions it
     composes and by composite conservation (tests/test_oracle_amr.py)."""
@@ -298,7 +298,7 @@ class OracleTwoLevelDriver:
                    self.c.dx, self.c.dy)
             self._average(oc, of)
             try:
-                n = repair_roundoff(of, self.ns) + repair_roundoff(oc, self.ns)
+                n = self._repair(oc, of)
                 if n:
                     self._average(oc, of)
                 self.c._validate(oc)
@@ -309,6 +309,45 @@ class OracleTwoLevelDriver:
                     raise
                 self.first_order_stages += 1
         raise DomainError("unreachable")
+
+    def _repair(self, oc, of):
+        """repairRoundoff (BlockAMR.cpp:299-334) over the leaf cells of both
+        levels: the uncovered coarse cells, then the fine box (level order,
+        i fastest); one donor per species, the first cell of largest mass
+        (value times cell volume); deficits summed over both levels."""
+        ns = self.ns
+        FI0, FJ0, FI1, FJ1 = self.fine_box
+        leaf_c = np.ones(oc.shape[1:], dtype=bool)
+        leaf_c[FJ0 // 2:FJ1 // 2 + 1, FI0 // 2:FI1 // 2 + 1] = False
+        levels = ((oc, leaf_c, self.c.dx * self.c.dy), (of, np.ones(of.shape[1:], dtype=bool), self.dxf * self.dyf))
+        peak, mass = np.zeros(ns), np.zeros(ns)
+        donor = [None] * ns
+        for lev, (Q, leaf, vol) in enumerate(levels):
+            for n in range(ns):
+                v = np.where(leaf, Q[n], 0.0)
+                peak[n] = max(peak[n], v.max())
+                k = int(np.argmax(v * vol))
+                if (v * vol).flat[k] > mass[n]:
+                    mass[n] = (v * vol).flat[k]
+                    donor[n] = (lev, k)
+        deficit = np.zeros(ns)
+        repairs = 0
+        for Q, leaf, vol in levels:
+            for n in range(ns):
+                for j, i in zip(*np.nonzero(leaf & ~(Q[n] >= 0))):
+                    v = Q[n, j, i]
+                    if not np.isfinite(v) or -v > 64 * EPS * peak[n]:
+                        raise DomainError("materially negative species after composite flow stage")
+                    deficit[n] += -v * vol
+                    Q[n, j, i] = 0.0
+                    repairs += 1
+        for n in range(ns):
+            if deficit[n] > 0:
+                if donor[n] is None or not (mass[n] > deficit[n]):
+                    raise DomainError("insufficient positive donor for roundoff redistribution")
+                Q, _, vol = levels[donor[n][0]]
+                Q[n].flat[donor[n][1]] -= deficit[n] / vol
+        return repairs
 
     def flow(self, h):
         oc, of = self.Qc.copy(), self.Qf.copy()

@@ -503,3 +503,33 @@ def test_two_level_wedge_box_matches_oracle(mech_h2, oracle_h2):
     assert not np.allclose(Fg, Fg[:, :1, :1])             # the wall turned the flow in the box
     _compare(Cg, o.Qc, 1e-8, mech_h2.ns)
     _compare(Fg, o.Qf, 1e-8, mech_h2.ns)
+
+
+def test_two_level_leaf_repair_matches_oracle(mech_h2, oracle_h2):
+    """The two-level driver's roundoff repair (hf_repair_cells over the
+    uncovered coarse cells and the fine box, BlockAMR.cpp:299-334) against
+    the oracle's: same zeroed cells, covered coarse cells untouched, the
+    deficit on the same donor (sums in another order: 1e-15 relative)."""
+    from ctypes import byref, c_int64
+    from hyperflow_amd._lib import load
+    from hyperflow_amd.driver import TwoLevelDriver
+    from oracle.driver import OracleTwoLevelDriver
+    from test_oracle_amr import repair_case
+    print("[gpu] two-level leaf repair vs oracle", flush=True)
+    oc, of, dx, box = repair_case(oracle_h2)
+    bc = (P, P, P, P)
+    g = TwoLevelDriver(mech_h2, oc.shape[2], oc.shape[1], dx, dx, (R_, R_, R_, R_), box, cfl=0.5)
+    o = OracleTwoLevelDriver(oracle_h2, oc.shape[2], oc.shape[1], dx, dx, bc, box, cfl=0.5)
+    c = g.coarse
+    c.U[:, 2:-2, 2:-2] = torch.from_numpy(oc).cuda()
+    g.Uf[:, 2:-2, 2:-2] = torch.from_numpy(of).cuda()
+    rep, ff = c_int64(0), c_int64(-1)
+    assert load().hf_repair_cells(g._h, g._leaf_blocks(c.U, g.Uf), 2, byref(rep), byref(ff)) == 0
+    C, F = oc.copy(), of.copy()
+    n = o._repair(C, F)
+    assert rep.value == n == 6
+    Cg = c.U[:, 2:-2, 2:-2].cpu().numpy()
+    Fg = g.Uf[:, 2:-2, 2:-2].cpu().numpy()
+    np.testing.assert_allclose(Cg, C, rtol=1e-15, atol=0)
+    np.testing.assert_allclose(Fg, F, rtol=1e-15, atol=0)
+    assert ((Cg != oc) == (C != oc)).all() and ((Fg != of) == (F != of)).all()

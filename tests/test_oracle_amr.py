@@ -424,3 +424,54 @@ def test_box_choice_matches_the_oracle_restatement():
         buf, blk = int(rng.integers(0, 4)), int(rng.choice([1, 2, 4, 8]))
         m = tuple(int(v) for v in rng.integers(0, 3, size=4))
         assert choose_fine_box(t, buf, blk, nx, ny, m) == fine_box_for_tags(t, buf, blk, nx, ny, m)
+
+
+def repair_case(o, seed=7):
+    """Two-level arrays with roundoff-size negatives: in leaf coarse cells,
+    in covered coarse cells (which the repair must leave to the
+    average-down) and in fine cells; species 2 only in fine cells, so its
+    deficit moves to the coarse level when the largest mass sits there."""
+    Q, dx, box = two_level_case(o)
+    rng = np.random.default_rng(seed)
+    ns = o.ns
+    oc = np.abs(Q) * (1 + 0.1 * rng.random(Q.shape))
+    FI0, FJ0, FI1, FJ1 = box
+    of = np.abs(rng.random((Q.shape[0], FJ1 - FJ0 + 1, FI1 - FI0 + 1))) * oc[:, 0:1, 0:1].mean()
+    eps = np.finfo(float).eps
+    for n, lev, j, i in ((1, 0, 0, 0), (1, 0, 3, 30), (2, 1, 4, 5), (2, 1, 9, 17), (5, 1, 0, 0), (5, 0, 23, 39)):
+        A = oc if lev == 0 else of
+        A[n, j, i] = -(2 + j % 5) * eps * max(oc[n].max(), of[n].max())
+    # a covered coarse cell with a negative value: not a leaf, not repaired
+    oc[3, FJ0 // 2 + 1, FI0 // 2 + 1] = -10 * eps * oc[3].max()
+    return oc, of, dx, box
+
+
+def test_two_level_repair_over_leaves(oracle_h2):
+    """repairRoundoff over the leaf cells of both levels (BlockAMR.cpp:
+    299-334): every negative leaf value is zeroed, the covered coarse cells
+    are left alone, each species' integral over the leaves (value times cell
+    volume) is unchanged to roundoff, and the deficit goes to the one cell
+    of largest mass over both levels."""
+    from oracle.driver import PERIODIC, OracleTwoLevelDriver
+    o = oracle_h2
+    oc, of, dx, box = repair_case(o)
+    d = OracleTwoLevelDriver(o, oc.shape[2], oc.shape[1], dx, dx, (PERIODIC,) * 4, box, cfl=0.5)
+    ns = o.ns
+    FI0, FJ0, FI1, FJ1 = box
+    leaf = np.ones(oc.shape[1:], dtype=bool)
+    leaf[FJ0 // 2:FJ1 // 2 + 1, FI0 // 2:FI1 // 2 + 1] = False
+    vc, vf = dx * dx, 0.25 * dx * dx
+    integral = lambda C, F: (np.where(leaf, C[:ns], 0.0).sum(axis=(1, 2)) * vc + F[:ns].sum(axis=(1, 2)) * vf)
+    before = integral(oc, of)
+    covered = oc[:, ~leaf].copy()
+    C, F = oc.copy(), of.copy()
+    n_rep = d._repair(C, F)
+    assert n_rep == 6
+    assert (np.where(leaf, C[:ns], 0.0) >= 0).all() and (F[:ns] >= 0).all()
+    np.testing.assert_array_equal(C[:, ~leaf], covered)
+    after = integral(C, F)
+    np.testing.assert_allclose(after, before, rtol=8 * np.finfo(float).eps, atol=0)
+    # species 2's deficit (fine cells only) went to the largest mass over both levels
+    changed_c = np.argwhere(C[2] != oc[2])
+    changed_f = np.argwhere((F[2] != of[2]) & (of[2] > 0))
+    assert len(changed_c) + len(changed_f) == 1
