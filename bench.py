@@ -955,7 +955,8 @@ def main():
                            "composite Heun stages on every level (masked ghost interpolation, flux stages over "
                            "all boxes, reflux, average-down, repair over the leaves, validation), one batched "
                            "FP64 chemistry launch over every leaf cell through the device block table (rtol 1e-8, "
-                           "atol 1e-12), CFL 0.5",
+                           "atol 1e-12), CFL 0.5; each Strang step one replayed HIP graph (hf_amr_strang_step, "
+                           "recaptured when a regrid changes the boxes; a step it hands back runs call by call)",
                "steps": args.amr_steps, "steps_per_s": args.amr_steps / awall,
                "ms_per_step": awall / args.amr_steps * 1e3,
                "Mleaf_cell_steps_per_s": leaf_steps / awall / 1e6,
@@ -963,7 +964,8 @@ def main():
                "leaf_cells_end": adrv.leaf_count(), "stored_cells_end": adrv.stored_count(),
                "regrids": adrv.regrids, "grid_changes": adrv.grid_changes,
                "max_remap_defect": adrv.max_remap_defect, "simulated_time_s": adrv.time,
-               "first_order_stages": adrv.first_order_stages, "retries": adrv.retries, "repairs": adrv.repairs}
+               "first_order_stages": adrv.first_order_stages, "retries": adrv.retries, "repairs": adrv.repairs,
+               "graph_steps": adrv.graph_steps}
         log(f"channel hierarchy: {amr['steps_per_s']:.4g} steps/s, {amr['Mleaf_cell_steps_per_s']:.4g} M leaf "
             f"cell-steps/s, boxes {amr['boxes_per_level_end']}")
 

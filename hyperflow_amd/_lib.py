@@ -59,6 +59,13 @@ class hf_cells(Structure):
                 ("ny", c_int32), ("mask", c_void_p), ("mask_pitch", c_int64), ("dx", c_double), ("dy", c_double)]
 
 
+class hf_level(Structure):
+    _fields_ = [("U", c_void_p), ("U1", c_void_p), ("U2", c_void_p), ("old", c_void_p), ("backup", c_void_p),
+                ("fx", c_void_p), ("fy", c_void_p), ("mask", c_void_p), ("nx", c_int32), ("ny", c_int32),
+                ("dx", c_double), ("dy", c_double), ("boxes", POINTER(c_int32)), ("nbox", c_int32),
+                ("ghosts", POINTER(c_int32)), ("nghost", c_int32), ("wedge_i", c_int32)]
+
+
 class hf_tag_params(Structure):
     _fields_ = [("pressure_tag", c_double), ("temperature_tag", c_double), ("fuel_rate_tag", c_double),
                 ("physics", c_int32), ("chemistry", c_int32), ("fuel", c_int32), ("has_box", c_int32),
@@ -130,6 +137,8 @@ _SIGNATURES = {
     "hf_box_strang_step": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, POINTER(hf_box),
                                    POINTER(c_int32), _dp, c_double, c_double, c_double, c_int, c_double, _dp,
                                    POINTER(c_int32), POINTER(c_int64)]),
+    "hf_amr_strang_step": (c_int, [c_void_p, POINTER(hf_level), c_int, POINTER(c_int32), _dp, c_double, c_int, c_int,
+                                   c_double, _dp, POINTER(c_int32), POINTER(c_int64)]),
     "hf_repair_roundoff": (c_int, [c_void_p, c_void_p, POINTER(hf_box), POINTER(c_int64), POINTER(c_int64)]),
     "hf_validate_box": (c_int, [c_void_p, c_void_p, POINTER(hf_box), c_double, c_double, _dp, POINTER(c_int64)]),
     "hf_lincomb": (c_int, [c_void_p, c_void_p, c_void_p, c_double, c_double, c_int64]),

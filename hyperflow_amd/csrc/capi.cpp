@@ -230,6 +230,9 @@ struct hf_ctx {
     // hf_box_strang_step: its captured graph and the tables and scalars it reads
     ChemGraph step_graph;
     DevBuf sbox, stiles, scells, sscal;
+    // hf_amr_strang_step: its captured graph and the device tables it reads
+    ChemGraph amr_graph;
+    DevBuf atab;
 };
 
 namespace {
@@ -1913,6 +1916,542 @@ int hf_box_strang_step(hf_ctx* c, double* U, double* U1, double* U2, double* old
         // step with host decisions
         const unsigned gcopy = (unsigned)std::min<int64_t>((P.n + 255) / 256, 8192);
         hipLaunchKernelGGL(step_copy, dim3(gcopy), dim3(256), 0, c->stream, U, backup, P.n);
+        HF_HIP(c, hipStreamSynchronize(c->stream));
+        return HF_OK;
+    }
+    if (taken) *taken = how == 0 ? 3 : how;
+    if (dt_taken) std::memcpy(dt_taken, &hs[SS_DT], sizeof(double));
+    if (repairs) *repairs = (int64_t)hs[SS_REPAIRS];
+    return HF_OK;
+}
+
+// ------------------------------------------------------------------ a hierarchy's Strang step as one graph
+namespace {
+
+// h/dx and h/dy of every flux box from the device h (hd: each box's level cell size)
+__global__ void amr_set_h(hf_box* boxes, const double2* hd, int nbox, const unsigned long long* s) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbox) return;
+    const double h = reinterpret_cast<const double*>(s)[SS_H];
+    boxes[b].dtdx = h / hd[b].x;
+    boxes[b].dtdy = h / hd[b].y;
+}
+
+enum { AK_U = 0, AK_U1 = 1, AK_U2 = 2, AK_OLD = 3, AK_BACKUP = 4 };
+
+struct AmrLevel {
+    double* A[5];                   // U, U1, U2, old, backup
+    double *fx, *fy;
+    const uint8_t* mask;
+    int nx, ny;
+    double dx, dy;
+    int wedge_i;
+    const int32_t* ghosts;          // host strips
+    int nghost;
+    const int32_t* ghosts_dev;
+    int64_t most;                   // the largest strip's parent count
+    int64_t n;                      // doubles per array
+    hf_patch patch() const { return hf_patch{2 * (int64_t)(nx + 4) + 2, nx + 4, (int64_t)(nx + 4) * (ny + 4)}; }
+    hf_face_fluxes faces() const {
+        return hf_face_fluxes{fx, 0, nx + 1, (int64_t)ny * (nx + 1), fy, 0, nx, (int64_t)(ny + 1) * nx};
+    }
+};
+
+struct AmrPlan {
+    std::vector<AmrLevel> lev;
+    int32_t bc[4];
+    double inlet[64];
+    int has_inlet;
+    double cfl;
+    int chemistry, chem_mode;
+    int nbox, ntiles, nblock;
+    int64_t total;                  // cells of every box of every level (the block tables' cell count)
+    int cfg[ChemTuner::NPARAM];     // chemistry refill group, waves per CU, bins, waits
+    // device tables
+    hf_box* boxes;
+    const hf_face_fluxes* fluxes;
+    const double2* hd;
+    const double* const* in_ptrs[2];  // the two stages of a flow: U -> U1, U1 -> U2
+    double* const* out_ptrs[2];
+    const int32_t* tiles;
+    const hf_cells* cells[3];       // leaf blocks of U, U1, U2
+    const int64_t* start;
+};
+
+// Enqueue one hierarchy Strang step on the context's stream: kernels only,
+// so the sequence can be captured. Every failure folds into the step status.
+int amr_enqueue(hf_ctx* c, const AmrPlan& P) {
+    const int ns = c->ops->ns;
+    const int nl = (int)P.lev.size();
+    auto* s = c->sscal.as<unsigned long long>();
+    unsigned long long* scal = c->scal.as<unsigned long long>();
+    unsigned long long *ff = scal, *rate = scal + 8;
+    BoxLaunch B0{};
+    B0.mech = c->d_mech;
+    B0.scal = scal + 8;
+    B0.first_failed = ff;
+    B0.stream = c->stream;
+    auto fold = [&](const unsigned long long* f, const unsigned long long* donor, const unsigned long long* rep,
+                    unsigned long long bit) {
+        hipLaunchKernelGGL(step_fold, dim3(1), dim3(64), 0, c->stream, s, f, donor, rep, bit);
+    };
+    auto copy = [&](int dst, int src) {
+        for (const AmrLevel& L : P.lev) {
+            const unsigned g = (unsigned)std::min<int64_t>((L.n + 255) / 256, 8192);
+            hipLaunchKernelGGL(step_copy, dim3(g), dim3(256), 0, c->stream, L.A[dst], L.A[src], L.n);
+        }
+    };
+    auto with_cells = [&](BoxLaunch& V, int key) {
+        V.blocks = P.cells[key];
+        V.block_start = P.start;
+        V.nblock = P.nblock;
+        V.total = P.total;
+    };
+    // validation of every valid cell, its CFL rate over the leaves
+    auto validate = [&](int key) {
+        hipLaunchKernelGGL(step_props_reset, dim3(1), dim3(64), 0, c->stream, ff, rate);
+        BoxLaunch V = B0;
+        with_cells(V, key);
+        V.op = BoxLaunch::CELLS_PROPS;
+        return c->ops->box_ops(V);
+    };
+    auto average = [&](int key) {
+        for (int l = nl - 1; l >= 1; --l) {
+            const AmrLevel &F = P.lev[l], &C = P.lev[l - 1];
+            BoxLaunch A = B0;
+            A.op = BoxLaunch::AVGDOWN;
+            A.src = F.A[key];
+            A.sp = F.patch();
+            A.dst = C.A[key];
+            A.dp = C.patch();
+            A.region[0] = A.region[1] = 0;
+            A.region[2] = C.nx - 1;
+            A.region[3] = C.ny - 1;
+            A.mask = C.mask;
+            A.mask_pitch = C.nx;
+            if (c->ops->box_ops(A) != HF_OK) return HF_EDEVICE;
+        }
+        return HF_OK;
+    };
+    // one composite Euler stage src -> dst (BlockAMR::euler over every level)
+    auto stage = [&](int src, int dst, int pass) {
+        for (int l = 0; l < nl; ++l) {
+            const AmrLevel& L = P.lev[l];
+            if (l > 0 && L.nghost > 0) {
+                const AmrLevel& C = P.lev[l - 1];
+                BoxLaunch I = B0;
+                I.op = BoxLaunch::INTERP;
+                I.src = C.A[src];
+                I.sp = C.patch();
+                I.dst = L.A[src];
+                I.dp = L.patch();
+                for (int k = 0; k < 4; ++k) I.region[k] = L.ghosts[k];
+                I.regions = L.ghosts_dev;
+                I.nregion = L.nghost;
+                I.total = L.most;
+                I.mask = L.mask;
+                I.mask_pitch = L.nx;
+                I.mask_nx = L.nx;
+                I.mask_ny = L.ny;
+                if (c->ops->box_ops(I) != HF_OK) return HF_EDEVICE;
+                fold(ff, nullptr, nullptr, 1ull);
+            }
+            BoxLaunch Fl = B0;
+            Fl.box = hf_box{L.nx, L.ny, 0, L.wedge_i, 0, L.nx + 4, (int64_t)(L.nx + 4) * (L.ny + 4), 0, L.nx,
+                            (int64_t)L.nx * L.ny, 0.0, 0.0};
+            set_fill(Fl, L.A[src], P.bc, P.has_inlet ? P.inlet : nullptr);
+            if (c->ops->box_ops(Fl) != HF_OK) return HF_EDEVICE;
+        }
+        FluxLaunch F{};
+        F.mech = c->d_mech;
+        F.boxes = P.boxes;
+        F.nbox = P.nbox;
+        F.tiles = P.tiles;
+        F.ntiles = P.ntiles;
+        F.first_order = 0;
+        F.one_d = 0;
+        F.skip_validation = 1;              // the repair and the validation below judge the update
+        F.first_failed = scal + 2;
+        F.counters = scal + 3;
+        F.fluxes = P.fluxes;
+        F.in_ptrs = P.in_ptrs[pass];
+        F.out_ptrs = P.out_ptrs[pass];
+        F.stream = c->stream;
+        if (c->ops->flux(F) != HF_OK) return HF_EDEVICE;
+        for (int l = nl - 1; l >= 1; --l) {
+            const AmrLevel &Fi = P.lev[l], &C = P.lev[l - 1];
+            BoxLaunch R = B0;
+            R.op = BoxLaunch::REFLUX_CELLS;
+            R.dst = C.A[dst];
+            R.dp = C.patch();
+            R.cflux = C.faces();
+            R.fflux = Fi.faces();
+            R.region[0] = R.region[1] = 0;
+            R.region[2] = C.nx - 1;
+            R.region[3] = C.ny - 1;
+            R.mask = C.mask;
+            R.mask_pitch = C.nx;
+            R.hp = reinterpret_cast<const double*>(s + SS_H);
+            R.dx = C.dx;
+            R.dy = C.dy;
+            if (c->ops->box_ops(R) != HF_OK) return HF_EDEVICE;
+        }
+        if (average(dst) != HF_OK) return HF_EDEVICE;
+        BoxLaunch Rp = B0;
+        with_cells(Rp, dst);
+        Rp.op = BoxLaunch::CELLS_REPAIR;
+        Rp.partials = c->aux[5].as<double>();
+        if (c->ops->box_ops(Rp) != HF_OK) return HF_EDEVICE;
+        fold(ff, scal + 8 + 3 * ns + 1, scal + 8 + 3 * ns, 1ull);
+        // the average-down again: a repaired leaf changes its coarse parent
+        // (with nothing repaired it rewrites the same values)
+        if (average(dst) != HF_OK || validate(dst) != HF_OK) return HF_EDEVICE;
+        fold(ff, nullptr, nullptr, 1ull);
+        return HF_OK;
+    };
+    // dt from the CFL rate of the leaves
+    if (validate(AK_U) != HF_OK) return HF_EDEVICE;
+    hipLaunchKernelGGL(step_dt, dim3(1), dim3(64), 0, c->stream, s, ff, rate, P.cfl);
+    hipLaunchKernelGGL(amr_set_h, dim3((unsigned)((P.nbox + 255) / 256)), dim3(256), 0, c->stream, P.boxes, P.hd,
+                       P.nbox, s);
+    copy(AK_BACKUP, AK_U);
+    for (int half = 0; half < 2; ++half) {
+        // flow(dt/2): U <- U/2 + E(E(U))/2 on every level, average-down, validation
+        copy(AK_OLD, AK_U);
+        if (stage(AK_U, AK_U1, 0) != HF_OK || stage(AK_U1, AK_U2, 1) != HF_OK) return HF_EDEVICE;
+        for (const AmrLevel& L : P.lev) {
+            const unsigned g = (unsigned)std::min<int64_t>((L.n + 255) / 256, 8192);
+            hipLaunchKernelGGL(step_lincomb, dim3(g), dim3(256), 0, c->stream, L.A[AK_U], L.A[AK_U2], L.A[AK_OLD], 0.5,
+                               0.5, L.n);
+        }
+        if (average(AK_U) != HF_OK || validate(AK_U) != HF_OK) return HF_EDEVICE;
+        fold(ff, nullptr, nullptr, 1ull);
+        if (half == 1 || !P.chemistry) continue;
+        // react(dt): one batch over every leaf cell, in place (BlockAMR::react)
+        ChemLaunch C{};
+        C.mech = c->d_mech;
+        C.ncell = P.total;
+        C.ld = P.total;
+        C.blocks = P.cells[AK_U];
+        C.block_start = P.start;
+        C.nblock = P.nblock;
+        C.dt = reinterpret_cast<const double*>(s + SS_DT);
+        C.dt_per_cell = 0;
+        C.out = c->out.as<double>();
+        C.ldo = P.total;
+        C.status = c->status.as<int8_t>();
+        C.first_failed = s + SS_CHEM;
+        C.counters = s + SS_CHEM + 1;
+        C.rtol = c->opt.rtol;
+        C.atol = c->opt.atol;
+        C.maxsteps = c->opt.max_steps;
+        C.mode = P.chem_mode;
+        C.precision = c->opt.precision;
+        C.num_cus = c->num_cus;
+        C.chunk = P.cfg[ChemTuner::CHUNK];
+        C.waves_per_cu = P.cfg[ChemTuner::WAVES];
+        C.nbins = P.cfg[ChemTuner::BINS];
+        C.jac_wait = P.cfg[ChemTuner::JWAIT];
+        C.setup_wait = P.cfg[ChemTuner::SWAIT];
+        C.workspace = c->work.p;
+        C.workspace_bytes = c->work.n;
+        C.stream = c->stream;
+        C.commit = P.lev[0].A[AK_U];        // non-null: commit through the block table when no cell failed
+        C.commit_ld = P.total;
+        if (c->ops->chem(C) != HF_OK) return HF_EDEVICE;
+        fold(s + SS_CHEM, nullptr, nullptr, 2ull);
+        if (average(AK_U) != HF_OK || validate(AK_U) != HF_OK) return HF_EDEVICE;
+        fold(ff, nullptr, nullptr, 1ull);
+    }
+    return hipGetLastError() == hipSuccess ? HF_OK : HF_EDEVICE;
+}
+
+// byte offset of the next table section, 256-byte aligned
+size_t amr_align(size_t n) { return (n + 255) & ~size_t(255); }
+
+}  // namespace
+
+int hf_amr_strang_step(hf_ctx* c, const hf_level* levels, int nlevel, const int32_t bc[4], const double* inlet,
+                       double cfl, int chemistry, int chem_mode, double time_left, double* dt_taken, int32_t* taken,
+                       int64_t* repairs) {
+    if (!c) return HF_EINVAL;
+    BusyGuard guard(c);
+    if (!guard.ok) return fail(c, HF_EBUSY, "concurrent call is unsupported");
+    if (taken) *taken = 0;
+    if (dt_taken) *dt_taken = 0;
+    if (repairs) *repairs = 0;
+    if (!levels || nlevel < 1 || nlevel > 16) return fail(c, HF_EINVAL, "invalid level list");
+    if (!(cfl > 0) || !std::isfinite(cfl) || (chem_mode != HF_CHEM_PLAIN && chem_mode != HF_CHEM_BATCHED))
+        return fail(c, HF_EINVAL, "invalid CFL number or chemistry form");
+    const int rb = check_bc(c, nullptr, bc, inlet);
+    if (rb != HF_OK) return rb;
+    if (bc[2] < 0 || bc[3] < 0) return fail(c, HF_EINVAL, "a hierarchy step is 2-D: y boundary codes are needed");
+    HF_DEVICE(c);
+    const int ns = c->mech.ns(), nv = ns + 3;
+    if (nv > 64) return fail(c, HF_EINVAL, "mechanism too large for the step graph's inlet state");
+    AmrPlan P{};
+    for (int k = 0; k < 4; ++k) P.bc[k] = bc[k];
+    if (inlet) {
+        P.has_inlet = 1;
+        for (int n = 0; n < nv; ++n) P.inlet[n] = inlet[n];
+    }
+    P.cfl = cfl;
+    P.chemistry = chemistry ? 1 : 0;
+    P.chem_mode = chem_mode;
+    // levels: extents, ratio 2, device arrays of the right size, boxes and strips inside
+    std::vector<hf_box> boxes;
+    std::vector<hf_face_fluxes> fluxes;
+    std::vector<double2> hd;
+    std::vector<const double*> ptrs[4];
+    std::vector<hf_cells> cells[3];
+    std::vector<int64_t> start(1, 0);
+    std::vector<int32_t> ghosts;
+    std::vector<size_t> ghost_at;
+    for (int l = 0; l < nlevel; ++l) {
+        const hf_level& H = levels[l];
+        const std::string name = "level " + std::to_string(l);
+        if (H.nx < 1 || H.ny < 1 || !(H.dx > 0) || !(H.dy > 0) || !std::isfinite(H.dx) || !std::isfinite(H.dy))
+            return fail(c, HF_EINVAL, name + " has an invalid extent or cell size");
+        if (l > 0 && (H.nx != 2 * levels[l - 1].nx || H.ny != 2 * levels[l - 1].ny))
+            return fail(c, HF_EINVAL, name + " is not a refinement by 2 of the level below");
+        if (H.nbox < 1 || !H.boxes || H.nghost < 0 || (H.nghost > 0 && !H.ghosts) || (l == 0 && H.nghost > 0))
+            return fail(c, HF_EINVAL, name + " has an invalid box or ghost-strip list");
+        AmrLevel L{};
+        double* arr[5] = {H.U, H.U1, H.U2, H.old, H.backup};
+        L.n = (int64_t)nv * (H.nx + 4) * (H.ny + 4);
+        for (int k = 0; k < 5; ++k) {
+            if (!arr[k] || !is_device_ptr(arr[k]) || !within_allocation(arr[k], 0, L.n - 1))
+                return fail(c, HF_EINVAL, name + ": every level array must be device memory of nv (nx+4)(ny+4) values");
+            L.A[k] = arr[k];
+        }
+        if (!H.fx || !H.fy || !is_device_ptr(H.fx) || !is_device_ptr(H.fy) || !H.mask || !is_device_ptr(H.mask) ||
+            !within_allocation(H.fx, 0, (int64_t)nv * H.ny * (H.nx + 1) - 1) ||
+            !within_allocation(H.fy, 0, (int64_t)nv * (H.ny + 1) * H.nx - 1))
+            return fail(c, HF_EINVAL, name + ": face fluxes and mask must be device memory of the level's size");
+        L.fx = H.fx;
+        L.fy = H.fy;
+        L.mask = H.mask;
+        L.nx = H.nx;
+        L.ny = H.ny;
+        L.dx = H.dx;
+        L.dy = H.dy;
+        L.wedge_i = H.wedge_i;
+        L.ghosts = H.ghosts;
+        L.nghost = H.nghost;
+        if (l == 0 && (H.nbox != 1 || H.boxes[0] != 0 || H.boxes[1] != 0 || H.boxes[2] != H.nx - 1 ||
+                       H.boxes[3] != H.ny - 1))
+            return fail(c, HF_EINVAL, "level 0 must be one box over the whole level");
+        const int64_t gx = H.nx + 4, plane = gx * (H.ny + 4);
+        for (int b = 0; b < H.nbox; ++b) {
+            const int32_t* g = H.boxes + 4 * b;
+            if (g[0] < 0 || g[1] < 0 || g[2] >= H.nx || g[3] >= H.ny || g[2] < g[0] || g[3] < g[1])
+                return fail(c, HF_EINVAL, name + " box " + std::to_string(b) + " lies outside the level");
+            int walls = 0;
+            const bool touch[4] = {g[0] == 0, g[2] == H.nx - 1, g[1] == 0, g[3] == H.ny - 1};
+            for (int k = 0; k < 4; ++k)
+                if (touch[k] && (bc[k] == HF_BC_REFLECTING || (k == 2 && bc[k] == HF_BC_WEDGE))) walls |= 1 << k;
+            const int32_t bnx = g[2] - g[0] + 1, bny = g[3] - g[1] + 1;
+            boxes.push_back(hf_box{bnx, bny, walls, H.wedge_i - g[0], g[1] * gx + g[0], gx, plane,
+                                   (g[1] + 2) * gx + g[0] + 2, gx, plane, 0.0, 0.0});
+            // box-local face (a, b) is the level's face (i0 + a, j0 + b)
+            fluxes.push_back(hf_face_fluxes{H.fx, (int64_t)g[1] * (H.nx + 1) + g[0], H.nx + 1,
+                                            (int64_t)H.ny * (H.nx + 1), H.fy, (int64_t)g[1] * H.nx + g[0], H.nx,
+                                            (int64_t)(H.ny + 1) * H.nx});
+            hd.push_back(make_double2(H.dx, H.dy));
+            ptrs[0].push_back(H.U);
+            ptrs[1].push_back(H.U1);
+            ptrs[2].push_back(H.U1);
+            ptrs[3].push_back(H.U2);
+            double* keyed[3] = {H.U, H.U1, H.U2};
+            for (int k = 0; k < 3; ++k)
+                cells[k].push_back(hf_cells{keyed[k], (g[1] + 2) * gx + g[0] + 2, gx, plane, bnx, bny,
+                                            H.mask + (int64_t)g[1] * H.nx + g[0], H.nx, H.dx, H.dy});
+            start.push_back(start.back() + (int64_t)bnx * bny);
+        }
+        ghost_at.push_back(ghosts.size());
+        if (l > 0) {
+            const AmrLevel& C = P.lev[l - 1];
+            const hf_patch cp = C.patch(), fp = L.patch();
+            for (int k = 0; k < H.nghost; ++k) {
+                const int32_t* g = H.ghosts + 4 * k;
+                if (g[2] < g[0] || g[3] < g[1]) return fail(c, HF_EINVAL, name + " has an empty ghost strip");
+                if (check_extents(c, BoxLaunch::INTERP, C.A[0], cp, L.A[0], fp, g) != HF_OK) return HF_EINVAL;
+                L.most = std::max<int64_t>(L.most, (int64_t)((g[2] >> 1) - (g[0] >> 1) + 1) *
+                                                       ((g[3] >> 1) - (g[1] >> 1) + 1));
+                for (int q = 0; q < 4; ++q) ghosts.push_back(g[q]);
+            }
+        }
+        P.lev.push_back(L);
+    }
+    P.nbox = (int)boxes.size();
+    P.nblock = P.nbox;
+    P.total = start.back();
+    const int TX = c->ops->flux_tx2, TY = c->ops->flux_ty2;
+    std::vector<int32_t> tiles;
+    for (int b = 0; b < P.nbox; ++b)
+        for (int tj = 0; tj < (boxes[b].ny + TY - 1) / TY; ++tj)
+            for (int ti = 0; ti < (boxes[b].nx + TX - 1) / TX; ++ti) {
+                tiles.push_back(b);
+                tiles.push_back(ti);
+                tiles.push_back(tj);
+            }
+    P.ntiles = (int)(tiles.size() / 3);
+    // the chemistry's tuning values: the options', else the tuner's result
+    // for this size class once it has one, else the defaults (no result
+    // depends on them)
+    const bool wave = c->ops->ns > 16;
+    const bool tune = !c->opt.chunk && !c->opt.waves_per_cu && !c->opt.nbins && !c->opt.jac_wait && !c->opt.setup_wait;
+    if (tune) {
+        const ChemTuner& T = c->tuner;
+        const bool same = T.done && T.wave == wave && P.total <= 2 * T.size && 2 * P.total >= T.size;
+        for (int p = 0; p < ChemTuner::NPARAM; ++p)
+            P.cfg[p] = same ? T.best[p] : (wave ? ChemTuner::kWave : ChemTuner::kLane)[p][0];
+    } else {
+        P.cfg[ChemTuner::CHUNK] = c->opt.chunk;
+        P.cfg[ChemTuner::WAVES] = c->opt.waves_per_cu;
+        P.cfg[ChemTuner::BINS] = c->opt.nbins;
+        P.cfg[ChemTuner::JWAIT] = c->opt.jac_wait;
+        P.cfg[ChemTuner::SWAIT] = c->opt.setup_wait;
+    }
+    // device tables, one buffer
+    const size_t nb = (size_t)P.nbox;
+    size_t at = 0;
+    const size_t o_box = at;
+    at = amr_align(at + sizeof(hf_box) * nb);
+    const size_t o_flux = at;
+    at = amr_align(at + sizeof(hf_face_fluxes) * nb);
+    const size_t o_hd = at;
+    at = amr_align(at + sizeof(double2) * nb);
+    const size_t o_ptr = at;
+    at = amr_align(at + 4 * sizeof(void*) * nb);
+    const size_t o_tile = at;
+    at = amr_align(at + sizeof(int32_t) * tiles.size());
+    const size_t o_ghost = at;
+    at = amr_align(at + sizeof(int32_t) * std::max<size_t>(ghosts.size(), 1));
+    const size_t o_cells = at;
+    at = amr_align(at + 3 * sizeof(hf_cells) * nb);
+    const size_t o_start = at;
+    at = amr_align(at + sizeof(int64_t) * start.size());
+    const size_t groups = (size_t)((P.total + 255) / 256);
+    const size_t wbytes =
+        P.chemistry ? c->ops->chem_workspace(P.total, P.cfg[ChemTuner::BINS], c->num_cus, P.cfg[ChemTuner::WAVES]) : 0;
+    if (!c->atab.ensure(at) || !c->sscal.ensure(8 * SS_WORDS) || !c->scal.ensure(sizeof(unsigned long long) * 256) ||
+        !c->aux[5].ensure(sizeof(double) * ns * groups) || (P.chemistry && (!c->out.ensure(sizeof(double) * ns * P.total) ||
+                                                                             !c->status.ensure(P.total))) ||
+        (wbytes && !c->work.ensure(wbytes)))
+        return fail(c, HF_ENOMEM, "device scratch");
+    char* T = c->atab.as<char>();
+    P.boxes = reinterpret_cast<hf_box*>(T + o_box);
+    P.fluxes = reinterpret_cast<const hf_face_fluxes*>(T + o_flux);
+    P.hd = reinterpret_cast<const double2*>(T + o_hd);
+    auto* pt = reinterpret_cast<double**>(T + o_ptr);
+    P.in_ptrs[0] = pt;
+    P.out_ptrs[0] = pt + nb;
+    P.in_ptrs[1] = pt + 2 * nb;
+    P.out_ptrs[1] = pt + 3 * nb;
+    P.tiles = reinterpret_cast<const int32_t*>(T + o_tile);
+    for (int k = 0; k < 3; ++k) P.cells[k] = reinterpret_cast<const hf_cells*>(T + o_cells) + k * nb;
+    P.start = reinterpret_cast<const int64_t*>(T + o_start);
+    for (int l = 1; l < nlevel; ++l)
+        P.lev[l].ghosts_dev = reinterpret_cast<const int32_t*>(T + o_ghost) + ghost_at[l];
+    // the key: every table, level, condition, option and context buffer the
+    // graph's kernels address (other calls may have grown the buffers)
+    std::vector<uint64_t> key;
+    auto bits = [](double x) {
+        uint64_t u;
+        std::memcpy(&u, &x, sizeof(u));
+        return u;
+    };
+    auto push_bytes = [&](const void* p, size_t n) {
+        const size_t w = key.size();
+        key.resize(w + (n + 7) / 8, 0);
+        std::memcpy(key.data() + w, p, n);
+    };
+    push_bytes(boxes.data(), sizeof(hf_box) * nb);
+    push_bytes(fluxes.data(), sizeof(hf_face_fluxes) * nb);
+    push_bytes(ghosts.data(), sizeof(int32_t) * ghosts.size());
+    for (const hf_cells& b : cells[0]) key.push_back((uint64_t)(uintptr_t)b.mask);
+    for (const AmrLevel& L : P.lev) {
+        for (double* a : L.A) key.push_back((uint64_t)(uintptr_t)a);
+        for (uint64_t v : {(uint64_t)L.nx, (uint64_t)L.ny, bits(L.dx), bits(L.dy), (uint64_t)(int64_t)L.nghost})
+            key.push_back(v);
+    }
+    for (int k = 0; k < 4; ++k) key.push_back((uint64_t)(int64_t)bc[k]);
+    key.push_back((uint64_t)P.has_inlet);
+    for (int n = 0; n < nv; ++n) key.push_back(bits(P.inlet[n]));
+    for (uint64_t v : {bits(cfl), (uint64_t)P.chemistry, (uint64_t)chem_mode, (uint64_t)nlevel, (uint64_t)TX,
+                       (uint64_t)TY, bits(c->opt.rtol), bits(c->opt.atol), (uint64_t)c->opt.max_steps,
+                       (uint64_t)c->opt.precision, (uint64_t)c->opt.eager})
+        key.push_back(v);
+    for (int p = 0; p < ChemTuner::NPARAM; ++p) key.push_back((uint64_t)(int64_t)P.cfg[p]);
+    for (const DevBuf* b : {&c->atab, &c->sscal, &c->scal, &c->aux[5], &c->out, &c->status, &c->work}) {
+        key.push_back((uint64_t)(uintptr_t)b->p);
+        key.push_back((uint64_t)b->n);
+    }
+    ChemGraph& g = c->amr_graph;
+    const bool fresh = !g.exec || g.key != key;
+    if (fresh) {
+        g.reset();
+        std::vector<const double*> pflat;
+        for (int k = 0; k < 4; ++k) pflat.insert(pflat.end(), ptrs[k].begin(), ptrs[k].end());
+        std::vector<hf_cells> cflat;
+        for (int k = 0; k < 3; ++k) cflat.insert(cflat.end(), cells[k].begin(), cells[k].end());
+        HF_HIP(c, hipMemcpyAsync(T + o_box, boxes.data(), sizeof(hf_box) * nb, hipMemcpyHostToDevice, c->stream));
+        HF_HIP(c, hipMemcpyAsync(T + o_flux, fluxes.data(), sizeof(hf_face_fluxes) * nb, hipMemcpyHostToDevice,
+                                 c->stream));
+        HF_HIP(c, hipMemcpyAsync(T + o_hd, hd.data(), sizeof(double2) * nb, hipMemcpyHostToDevice, c->stream));
+        HF_HIP(c, hipMemcpyAsync(T + o_ptr, pflat.data(), sizeof(void*) * pflat.size(), hipMemcpyHostToDevice,
+                                 c->stream));
+        HF_HIP(c, hipMemcpyAsync(T + o_tile, tiles.data(), sizeof(int32_t) * tiles.size(), hipMemcpyHostToDevice,
+                                 c->stream));
+        if (!ghosts.empty())
+            HF_HIP(c, hipMemcpyAsync(T + o_ghost, ghosts.data(), sizeof(int32_t) * ghosts.size(),
+                                     hipMemcpyHostToDevice, c->stream));
+        HF_HIP(c, hipMemcpyAsync(T + o_cells, cflat.data(), sizeof(hf_cells) * cflat.size(), hipMemcpyHostToDevice,
+                                 c->stream));
+        HF_HIP(c, hipMemcpyAsync(T + o_start, start.data(), sizeof(int64_t) * start.size(), hipMemcpyHostToDevice,
+                                 c->stream));
+        // the host tables above are locals: the copies finish before they go
+        HF_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    double left = time_left;
+    HF_HIP(c, hipMemcpyAsync(c->sscal.as<char>() + 8 * SS_LEFT, &left, 8, hipMemcpyHostToDevice, c->stream));
+    int how = 0;
+    if (c->opt.eager) {
+        if (amr_enqueue(c, P) != HF_OK) return fail(c, HF_EDEVICE, "step launch failed");
+    } else {
+        if (fresh) {
+            if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+                return fail(c, HF_EDEVICE, "graph capture failed");
+            const int rs = amr_enqueue(c, P);
+            hipGraph_t graph = nullptr;
+            const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+            if (rs != HF_OK || e != hipSuccess) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return fail(c, HF_EDEVICE, "step capture failed");
+            }
+            g.graph = graph;
+            if (hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0) != hipSuccess) {
+                g.reset();
+                return fail(c, HF_EDEVICE, "step graph instantiation failed");
+            }
+            g.key = key;
+            how = 2;
+        } else {
+            how = 1;
+        }
+        HF_HIP(c, hipGraphLaunch(g.exec, c->stream));
+    }
+    unsigned long long hs[SS_REPAIRS + 1];
+    HF_HIP(c, hipMemcpyAsync(hs, c->sscal.p, sizeof(hs), hipMemcpyDeviceToHost, c->stream));
+    HF_HIP(c, hipStreamSynchronize(c->stream));
+    if (hs[SS_STATUS] != 0) {
+        // a stage, a validation or the chemistry failed: every level goes
+        // back to the step's start for the caller's step with host decisions
+        for (const AmrLevel& L : P.lev) {
+            const unsigned gc = (unsigned)std::min<int64_t>((L.n + 255) / 256, 8192);
+            hipLaunchKernelGGL(step_copy, dim3(gc), dim3(256), 0, c->stream, L.A[AK_U], L.A[AK_BACKUP], L.n);
+        }
         HF_HIP(c, hipStreamSynchronize(c->stream));
         return HF_OK;
     }

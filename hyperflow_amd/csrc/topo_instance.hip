@@ -1429,9 +1429,11 @@ __global__ void __launch_bounds__(256) cells_repair_donate_kernel(CellMap S, int
 __global__ void __launch_bounds__(256) reflux_cells_kernel(double* __restrict__ Uc, hf_patch cp, hf_face_fluxes C,
                                                            hf_face_fluxes F, int4 region,
                                                            const uint8_t* __restrict__ cmask, long cmask_pitch,
-                                                           double h, double dxc, double dyc) {
+                                                           double h_arg, const double* __restrict__ hp, double dxc,
+                                                           double dyc) {
 #pragma clang fp contract(off)
     constexpr int NV = Topo::NS + 3;
+    const double h = hp ? *hp : h_arg;
     const int nx = region.z - region.x + 1;
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long)nx * (region.w - region.y + 1)) return;
@@ -1512,7 +1514,7 @@ int box_ops(const BoxLaunch& L) {
         const int4 r = make_int4(L.region[0], L.region[1], L.region[2], L.region[3]);
         const long t = (long)(r.z - r.x + 1) * (r.w - r.y + 1);
         hipLaunchKernelGGL(reflux_cells_kernel, dim3((unsigned)((t + 255) / 256)), dim3(256), 0, L.stream, L.dst, L.dp,
-                           L.cflux, L.fflux, r, L.mask, (long)L.mask_pitch, L.h, L.dx, L.dy);
+                           L.cflux, L.fflux, r, L.mask, (long)L.mask_pitch, L.h, L.hp, L.dx, L.dy);
         break;
     }
     case BoxLaunch::CELLS_PROPS: {

@@ -165,6 +165,7 @@ struct BoxLaunch {
     hf_face_fluxes cflux, fflux;    // REFLUX: coarse and fine face fluxes (level index spaces)
     int sides;                      // REFLUX: bit0 x-lo, bit1 x-hi, bit2 y-lo, bit3 y-hi
     double h;                       // REFLUX: stage time step (dx, dy: coarse cell size)
+    const double* hp;               // REFLUX_CELLS: device h, read in place of h when set
     hf_tag_params tag;              // TAG: criteria
     uint8_t* tags;                  // TAG: device output, nx*ny
     // CELLS_PROPS, CELLS_REPAIR: a device block table (CellMap's block form), total cells

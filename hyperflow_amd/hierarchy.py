@@ -46,8 +46,8 @@ import numpy as np
 
 from . import amr_grid
 from ._lib import (HF_BC_INFLOW, HF_BC_PERIODIC, HF_BC_REFLECTING, HF_BC_WEDGE, HF_CHEM_BATCHED, HF_EDOMAIN,  # noqa
-                   HF_OK, HyperflowError, check, hf_box, hf_cells, hf_face_fluxes, hf_flux_stats, hf_patch,
-                   hf_tag_params, load)
+                   HF_OK, HyperflowError, check, hf_box, hf_cells, hf_face_fluxes, hf_flux_stats, hf_level,
+                   hf_patch, hf_tag_params, load)
 
 
 class _Level:
@@ -93,11 +93,15 @@ class AmrDriver:
     rectangle (x0, x1, y0, y1) moving with velocity (u, v); mode "physics"
     (the criteria, plus the box if given) or "box" (the box alone);
     BlockAMR.cpp:170-200). chemistry False skips react and the fuel-rate
-    criterion (the configuration's "chemistry", BlockAMR.cpp:271-272)."""
+    criterion (the configuration's "chemistry", BlockAMR.cpp:271-272).
+    graph True takes each step as one replayed HIP graph
+    (hf_amr_strang_step, recaptured when the boxes change), bit-identical to
+    the call-by-call step, which serves a step the graph hands back (a
+    first-order redo or a halved dt) and graph False."""
 
     def __init__(self, mechanism, nx0, ny0, dx0, dy0, bc, max_level=2, cfl=0.5, chem_mode=HF_CHEM_BATCHED,
                  blocking=16, max_grid_size=64, grid_eff=0.7, n_error_buf=2, n_proper=2, regrid_interval=8,
-                 max_attempts=18, inlet=None, wedge_i=0, tag=None, x0=0.0, y0=0.0, chemistry=True):
+                 max_attempts=18, inlet=None, wedge_i=0, tag=None, x0=0.0, y0=0.0, chemistry=True, graph=True):
         import torch
         self.torch = torch
         self.mech = mechanism
@@ -114,6 +118,8 @@ class AmrDriver:
         self.wedge_i0 = int(wedge_i)
         self.x0, self.y0 = float(x0), float(y0)
         self.chemistry = bool(chemistry)
+        self.graph = bool(graph)
+        self.graph_steps = 0
         self.tag = dict(pressure_tag=0.08, temperature_tag=0.08, fuel_rate_tag=50.0, chemistry=self.chemistry,
                         box=None, velocity=(0.0, 0.0), mode="physics")
         self.tag.update(tag or {})
@@ -159,6 +165,9 @@ class AmrDriver:
         for L in self.levels[:self.finest + 1]:
             L.level_bc = (c_int32 * 4)(*self.bc)
             L.wedge_i = self.wedge_i0 << L.lev
+            L.box_table = (c_int32 * (4 * len(L.boxes)))(*[v for b in L.boxes for v in b])
+        self._leaves = int(sum((L.mask_host == 1).sum() for L in self.levels[:self.finest + 1]))
+        self._stored = int(sum((L.mask_host > 0).sum() for L in self.levels[:self.finest + 1]))
 
     def _box_table(self, L, h):
         """hf_box and face-flux descriptors of every box of level L."""
@@ -322,14 +331,51 @@ class AmrDriver:
         return self.cfl / rate
 
     def leaf_count(self):
-        return int(sum((L.mask_host == 1).sum() for L in self.levels[:self.finest + 1]))
+        return self._leaves
 
     def stored_count(self):
-        return int(sum((L.mask_host > 0).sum() for L in self.levels[:self.finest + 1]))
+        return self._stored
+
+    def _graph_step(self, target):
+        """The whole step as one HIP graph (hf_amr_strang_step). Returns dt,
+        or None when the graph handed the step back with U restored."""
+        levels = []
+        for L in self.levels[:self.finest + 1]:
+            A = L.A
+            levels.append(hf_level(A["U"].data_ptr(), A["U1"].data_ptr(), A["U2"].data_ptr(), A["old"].data_ptr(),
+                                   A["backup"].data_ptr(), L.fx.data_ptr(), L.fy.data_ptr(), L.mask.data_ptr(),
+                                   L.nx, L.ny, L.dx, L.dy, L.box_table, len(L.boxes),
+                                   L.regions if L.lev > 0 else None, L.nregion if L.lev > 0 else 0, L.wedge_i))
+        table = (hf_level * len(levels))(*levels)
+        dt, taken, rep = c_double(0.0), c_int32(0), c_int64(0)
+        left = target - self.time if target is not None else 0.0
+        r = load().hf_amr_strang_step(self._h, table, len(levels), self.levels[0].level_bc, self.inlet, self.cfl,
+                                      int(self.chemistry), self.chem_mode, left, byref(dt), byref(taken), byref(rep))
+        check(self._h, r)
+        self._rate = None
+        if not taken.value:
+            return None
+        self.graph_steps += 1
+        self.repairs += rep.value
+        if self.chemistry:
+            self.chem_cells += self._leaves
+        return dt.value
 
     def step(self, target=None):
         """One Strang step of the hierarchy, with the reference's halve-and-
         retry; a regrid follows every regrid_interval steps. Returns dt."""
+        dt = self._graph_step(target) if self.graph else None
+        if dt is None:
+            dt = self._host_step(target)
+        self.time = target if target is not None and dt >= target - self.time else self.time + dt
+        self.steps += 1
+        if self.max_level > 0 and self.regrid_interval > 0 and self.steps % self.regrid_interval == 0:
+            self.regrid()
+        return dt
+
+    def _host_step(self, target):
+        """The step call by call, with host decisions (the first-order redo
+        and the halving of dt)."""
         dt = self.stable_dt()
         if target is not None:
             dt = min(target - self.time, dt)
@@ -347,10 +393,6 @@ class AmrDriver:
                 dt *= 0.5
         else:
             raise RuntimeError("time advancement failed")
-        self.time = target if target is not None and dt >= target - self.time else self.time + dt
-        self.steps += 1
-        if self.max_level > 0 and self.regrid_interval > 0 and self.steps % self.regrid_interval == 0:
-            self.regrid()
         return dt
 
     # ------------------------------------------------------------ regrid

@@ -474,6 +474,56 @@ int hf_box_strang_step(hf_ctx *ctx, double *U, double *U1, double *U2, double *o
                        const hf_box *box, const int32_t bc[4], const double *inlet, double dx, double dy, double cfl,
                        int chem_mode, double time_left, double *dt_taken, int32_t *taken, int64_t *repairs);
 
+/* One level of a block-structured hierarchy at refinement ratio 2 (the
+ * levels of BlockAMR, BlockAMR.hpp). The level is one ghosted array over its
+ * whole index space: component n of cell (i, j) at
+ * n (nx + 4)(ny + 4) + (j + 2)(nx + 4) + i + 2, with five such arrays (the
+ * state U and the step's work arrays). fx, fy: the level's face fluxes,
+ * nv planes of ny x (nx + 1) and (ny + 1) x nx. mask: ny x nx bytes, 0 no
+ * box, 1 leaf, 2 covered by the next finer level. boxes: (i0, j0, i1, j1)
+ * per box, inclusive (level 0: one box over the level). ghosts: the fine
+ * ghost strips (i0, j0, i1, j1) interpolated from the coarser level where
+ * the mask is 0 (hf_interp_regions). wedge_i: the level's first wedge-wall
+ * column. Host tables; device arrays. */
+typedef struct hf_level {
+    double *U, *U1, *U2, *old, *backup;
+    double *fx, *fy;
+    const uint8_t *mask;
+    int32_t nx, ny;
+    double dx, dy;
+    const int32_t *boxes;
+    int32_t nbox;
+    const int32_t *ghosts;
+    int32_t nghost;
+    int32_t wedge_i;
+} hf_level;
+
+/* One whole Strang step of a hierarchy (BlockAMR::run's loop body,
+ * BlockAMR.cpp:400-425, over every level) as one HIP graph, captured on the
+ * first call and replayed while the levels, boxes, arrays, boundary,
+ * options and context buffers repeat: dt = cfl / (the CFL rate of the leaf
+ * cells) (capped at time_left when > 0), flow(dt/2), react(dt) over every
+ * leaf cell (chemistry != 0, chem_mode), flow(dt/2). flow is the Heun
+ * average of two composite Euler stages (BlockAMR::euler, BlockAMR.cpp:202-
+ * 298): per level, coarse to fine, the ghost strips interpolated from the
+ * coarser level and the physical ghost fill; one flux stage over every box
+ * of every level with face fluxes; reflux and average-down, finest to
+ * coarsest; roundoff repair over the leaves; average-down; validation.
+ * dt and h live in device memory, so no host value enters the graph, and
+ * the step makes the decisions of the stage-by-stage calls on the device
+ * except one: when any interpolation, repair, validation or chemistry batch
+ * fails (the reference then redoes a stage first-order or halves dt), every
+ * level's U is restored to the step's start and *taken is 0, for the
+ * caller's step with host decisions. Otherwise *taken is 1 (replayed), 2
+ * (captured and launched) or 3 (hf_options.eager), *dt_taken the step and
+ * *repairs the zeroed values. The results are bit-identical to the same
+ * step taken with hf_interp_regions, hf_fill_ghosts_flow,
+ * hf_flux_stage_fluxes, hf_reflux_masked, hf_average_down_masked,
+ * hf_repair_cells, hf_validate_cells and hf_chem_advance_cells. */
+int hf_amr_strang_step(hf_ctx *ctx, const hf_level *levels, int nlevel, const int32_t bc[4], const double *inlet,
+                       double cfl, int chemistry, int chem_mode, double time_left, double *dt_taken, int32_t *taken,
+                       int64_t *repairs);
+
 /* BlockAMR::repairRoundoff (BlockAMR.cpp:299-334) on the valid cells of one
  * box of equal cell volumes: negative species within 64 eps of the species'
  * peak are zeroed and their total is taken from the first cell holding the

@@ -190,3 +190,71 @@ def test_channel_workload_hierarchy():
     assert d.finest == 2 and d.regrids == 1 and len(g[1]) >= 2 and len(g[2]) >= 2
     assert all(16 <= w <= 64 and 16 <= h <= 64 for w, h in sizes)
     assert d.max_remap_defect < 1e-10
+
+
+def _graph_vs_host(mech, Q, dx, bc, steps, cfl, **kw):
+    """The same steps with each step one HIP graph (hf_amr_strang_step) and
+    call by call: (graph driver, host driver, dts of both)."""
+    runs = []
+    for graph in (True, False):
+        d = AmrDriver(mech, Q.shape[2], Q.shape[1], dx, dx, bc, cfl=cfl, graph=graph, **kw)
+        d.initial_grids(Q)
+        dts = [d.step() for _ in range(steps)]
+        dts.append(d.step(target=d.time + 0.25 * dts[-1]))
+        runs.append((d, dts))
+    (g, dg), (e, de) = runs
+    print(f"[gpu]   graph steps {g.graph_steps} of {g.steps}; host driver: first-order stages "
+          f"{e.first_order_stages}, retries {e.retries}, repairs {e.repairs}; grids {[len(b) for b in g.grids()]}",
+          flush=True)
+    return g, e, dg, de
+
+
+def _assert_same_bits(g, e):
+    assert g.grids() == e.grids()
+    assert g.repairs == e.repairs and g.regrids == e.regrids
+    for lev in range(g.finest + 1):
+        (a, ma), (b, mb) = g.level_state(lev), e.level_state(lev)
+        assert (ma == mb).all()
+        assert np.array_equal(a[:, ma > 0], b[:, mb > 0]), f"level {lev}"
+
+
+def test_graph_steps_equal_host_steps():
+    """Whole hierarchy steps as replayed HIP graphs (recaptured when a regrid
+    changes the boxes) give the same bits as the same steps call by call: the
+    adaptive channel with two refinement levels, regrids every 2 steps, and a
+    step to a target time."""
+    print("[gpu] hierarchy graph step vs host step, channel 128x32, 2 refinement levels, 7 steps", flush=True)
+    mech = hf.Mechanism(H2, **CONVERGED)
+    Q, dx = small_channel(mech)
+    bc = (hf.HF_BC_REFLECTING, hf.HF_BC_TRANSMISSIVE, hf.HF_BC_REFLECTING, hf.HF_BC_REFLECTING)
+    g, e, dg, de = _graph_vs_host(mech, Q, dx, bc, 6, 0.5, max_level=2, blocking=8, max_grid_size=32,
+                                  regrid_interval=2)
+    assert dg == de
+    _assert_same_bits(g, e)
+    assert e.graph_steps == 0
+    if e.first_order_stages == 0 and e.retries == 0:
+        assert g.graph_steps == g.steps == 7
+
+
+def test_graph_step_hands_back_failed_steps():
+    """A chemistry batch that fails inside the graph (max_steps half the
+    step count the first step needs): the graph restores every level and
+    the call-by-call step (dt halved until the batch converges) takes it,
+    with the same bits as the host driver throughout."""
+    print("[gpu] hierarchy graph step with a failing chemistry batch: hand-back to the host step", flush=True)
+    mech = hf.Mechanism(H2, **CONVERGED)
+    Q, dx = small_channel(mech)
+    bc = (hf.HF_BC_REFLECTING, hf.HF_BC_TRANSMISSIVE, hf.HF_BC_REFLECTING, hf.HF_BC_REFLECTING)
+    kw = dict(max_level=2, blocking=8, max_grid_size=32, regrid_interval=0)
+    probe = AmrDriver(mech, Q.shape[2], Q.shape[1], dx, dx, bc, graph=False, **kw)
+    probe.initial_grids(Q)
+    probe.step()
+    most = probe.last_chem_stats["max_cell_steps"]
+    print(f"[gpu]   first step: at most {most} integrator steps per cell; max_steps {max(most // 2, 8)}", flush=True)
+    tight = hf.Mechanism(H2, max_steps=max(most // 2, 8), **CONVERGED)
+    g, e, dg, de = _graph_vs_host(tight, Q, dx, bc, 2, 0.5, **kw)
+    assert dg == de
+    _assert_same_bits(g, e)
+    assert e.retries > 0
+    assert g.graph_steps < g.steps
+    assert g.first_order_stages == e.first_order_stages and g.retries == e.retries
