@@ -1939,6 +1939,29 @@ __global__ void amr_set_h(hf_box* boxes, const double2* hd, int nbox, const unsi
 
 enum { AK_U = 0, AK_U1 = 1, AK_U2 = 2, AK_OLD = 3, AK_BACKUP = 4 };
 
+// dst = fma(a, y, b x) (x null: dst = y) on every component of every cell
+// of a block table: the valid cells only, not the level arrays' empty space
+// (the tables share one geometry and differ in their arrays)
+__global__ void __launch_bounds__(256) step_cells_combine(const hf_cells* __restrict__ dst,
+                                                          const hf_cells* __restrict__ y,
+                                                          const hf_cells* __restrict__ x,
+                                                          const int64_t* __restrict__ start, int nblock,
+                                                          int64_t total, int nv, double a, double b) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= total) return;
+    const CellMap S{nullptr, 0, dst, start, nblock};
+    int bi, i, j;
+    const hf_cells& B = S.find(c, bi, i, j);
+    const int64_t at = B.offset + (int64_t)j * B.pitch + i;
+    double* __restrict__ out = dst[bi].U + at;
+    const double* __restrict__ py = y[bi].U + at;
+    const double* __restrict__ px = x ? x[bi].U + at : nullptr;
+    for (int n = 0; n < nv; ++n) {
+        const int64_t o = (int64_t)n * B.plane;
+        out[o] = px ? fma(a, py[o], b * px[o]) : py[o];
+    }
+}
+
 struct AmrLevel {
     double* A[5];                   // U, U1, U2, old, backup
     double *fx, *fy;
@@ -1974,7 +1997,7 @@ struct AmrPlan {
     const double* const* in_ptrs[2];  // the two stages of a flow: U -> U1, U1 -> U2
     double* const* out_ptrs[2];
     const int32_t* tiles;
-    const hf_cells* cells[3];       // leaf blocks of U, U1, U2
+    const hf_cells* cells[5];       // the boxes' blocks in U, U1, U2, old, backup
     const int64_t* start;
 };
 
@@ -1995,11 +2018,10 @@ int amr_enqueue(hf_ctx* c, const AmrPlan& P) {
                     unsigned long long bit) {
         hipLaunchKernelGGL(step_fold, dim3(1), dim3(64), 0, c->stream, s, f, donor, rep, bit);
     };
-    auto copy = [&](int dst, int src) {
-        for (const AmrLevel& L : P.lev) {
-            const unsigned g = (unsigned)std::min<int64_t>((L.n + 255) / 256, 8192);
-            hipLaunchKernelGGL(step_copy, dim3(g), dim3(256), 0, c->stream, L.A[dst], L.A[src], L.n);
-        }
+    const unsigned gcells = (unsigned)((P.total + 255) / 256);
+    auto combine = [&](int dst, int y, int x, double a, double b) {
+        hipLaunchKernelGGL(step_cells_combine, dim3(gcells), dim3(256), 0, c->stream, P.cells[dst], P.cells[y],
+                           x < 0 ? nullptr : P.cells[x], P.start, P.nblock, P.total, ns + 3, a, b);
     };
     auto with_cells = [&](BoxLaunch& V, int key) {
         V.blocks = P.cells[key];
@@ -2114,16 +2136,14 @@ int amr_enqueue(hf_ctx* c, const AmrPlan& P) {
     hipLaunchKernelGGL(step_dt, dim3(1), dim3(64), 0, c->stream, s, ff, rate, P.cfl);
     hipLaunchKernelGGL(amr_set_h, dim3((unsigned)((P.nbox + 255) / 256)), dim3(256), 0, c->stream, P.boxes, P.hd,
                        P.nbox, s);
-    copy(AK_BACKUP, AK_U);
+    // the step's start and the Heun average touch the valid cells only: the
+    // ghost cells are refilled before every read
+    combine(AK_BACKUP, AK_U, -1, 1.0, 0.0);
     for (int half = 0; half < 2; ++half) {
         // flow(dt/2): U <- U/2 + E(E(U))/2 on every level, average-down, validation
-        copy(AK_OLD, AK_U);
+        combine(AK_OLD, AK_U, -1, 1.0, 0.0);
         if (stage(AK_U, AK_U1, 0) != HF_OK || stage(AK_U1, AK_U2, 1) != HF_OK) return HF_EDEVICE;
-        for (const AmrLevel& L : P.lev) {
-            const unsigned g = (unsigned)std::min<int64_t>((L.n + 255) / 256, 8192);
-            hipLaunchKernelGGL(step_lincomb, dim3(g), dim3(256), 0, c->stream, L.A[AK_U], L.A[AK_U2], L.A[AK_OLD], 0.5,
-                               0.5, L.n);
-        }
+        combine(AK_U, AK_U2, AK_OLD, 0.5, 0.5);
         if (average(AK_U) != HF_OK || validate(AK_U) != HF_OK) return HF_EDEVICE;
         fold(ff, nullptr, nullptr, 1ull);
         if (half == 1 || !P.chemistry) continue;
@@ -2186,6 +2206,9 @@ int hf_amr_strang_step(hf_ctx* c, const hf_level* levels, int nlevel, const int3
     const int rb = check_bc(c, nullptr, bc, inlet);
     if (rb != HF_OK) return rb;
     if (bc[2] < 0 || bc[3] < 0) return fail(c, HF_EINVAL, "a hierarchy step is 2-D: y boundary codes are needed");
+    for (int k = 0; k < 4; ++k)
+        if (bc[k] == HF_BC_KEEP)
+            return fail(c, HF_EINVAL, "a hierarchy step refills every ghost cell: HF_BC_KEEP is not a domain side");
     HF_DEVICE(c);
     const int ns = c->mech.ns(), nv = ns + 3;
     if (nv > 64) return fail(c, HF_EINVAL, "mechanism too large for the step graph's inlet state");
@@ -2203,7 +2226,7 @@ int hf_amr_strang_step(hf_ctx* c, const hf_level* levels, int nlevel, const int3
     std::vector<hf_face_fluxes> fluxes;
     std::vector<double2> hd;
     std::vector<const double*> ptrs[4];
-    std::vector<hf_cells> cells[3];
+    std::vector<hf_cells> cells[5];
     std::vector<int64_t> start(1, 0);
     std::vector<int32_t> ghosts;
     std::vector<size_t> ghost_at;
@@ -2262,8 +2285,8 @@ int hf_amr_strang_step(hf_ctx* c, const hf_level* levels, int nlevel, const int3
             ptrs[1].push_back(H.U1);
             ptrs[2].push_back(H.U1);
             ptrs[3].push_back(H.U2);
-            double* keyed[3] = {H.U, H.U1, H.U2};
-            for (int k = 0; k < 3; ++k)
+            double* keyed[5] = {H.U, H.U1, H.U2, H.old, H.backup};
+            for (int k = 0; k < 5; ++k)
                 cells[k].push_back(hf_cells{keyed[k], (g[1] + 2) * gx + g[0] + 2, gx, plane, bnx, bny,
                                             H.mask + (int64_t)g[1] * H.nx + g[0], H.nx, H.dx, H.dy});
             start.push_back(start.back() + (int64_t)bnx * bny);
@@ -2329,7 +2352,7 @@ int hf_amr_strang_step(hf_ctx* c, const hf_level* levels, int nlevel, const int3
     const size_t o_ghost = at;
     at = amr_align(at + sizeof(int32_t) * std::max<size_t>(ghosts.size(), 1));
     const size_t o_cells = at;
-    at = amr_align(at + 3 * sizeof(hf_cells) * nb);
+    at = amr_align(at + 5 * sizeof(hf_cells) * nb);
     const size_t o_start = at;
     at = amr_align(at + sizeof(int64_t) * start.size());
     const size_t groups = (size_t)((P.total + 255) / 256);
@@ -2350,7 +2373,7 @@ int hf_amr_strang_step(hf_ctx* c, const hf_level* levels, int nlevel, const int3
     P.in_ptrs[1] = pt + 2 * nb;
     P.out_ptrs[1] = pt + 3 * nb;
     P.tiles = reinterpret_cast<const int32_t*>(T + o_tile);
-    for (int k = 0; k < 3; ++k) P.cells[k] = reinterpret_cast<const hf_cells*>(T + o_cells) + k * nb;
+    for (int k = 0; k < 5; ++k) P.cells[k] = reinterpret_cast<const hf_cells*>(T + o_cells) + k * nb;
     P.start = reinterpret_cast<const int64_t*>(T + o_start);
     for (int l = 1; l < nlevel; ++l)
         P.lev[l].ghosts_dev = reinterpret_cast<const int32_t*>(T + o_ghost) + ghost_at[l];
@@ -2395,7 +2418,7 @@ int hf_amr_strang_step(hf_ctx* c, const hf_level* levels, int nlevel, const int3
         std::vector<const double*> pflat;
         for (int k = 0; k < 4; ++k) pflat.insert(pflat.end(), ptrs[k].begin(), ptrs[k].end());
         std::vector<hf_cells> cflat;
-        for (int k = 0; k < 3; ++k) cflat.insert(cflat.end(), cells[k].begin(), cells[k].end());
+        for (int k = 0; k < 5; ++k) cflat.insert(cflat.end(), cells[k].begin(), cells[k].end());
         HF_HIP(c, hipMemcpyAsync(T + o_box, boxes.data(), sizeof(hf_box) * nb, hipMemcpyHostToDevice, c->stream));
         HF_HIP(c, hipMemcpyAsync(T + o_flux, fluxes.data(), sizeof(hf_face_fluxes) * nb, hipMemcpyHostToDevice,
                                  c->stream));
@@ -2448,10 +2471,8 @@ int hf_amr_strang_step(hf_ctx* c, const hf_level* levels, int nlevel, const int3
     if (hs[SS_STATUS] != 0) {
         // a stage, a validation or the chemistry failed: every level goes
         // back to the step's start for the caller's step with host decisions
-        for (const AmrLevel& L : P.lev) {
-            const unsigned gc = (unsigned)std::min<int64_t>((L.n + 255) / 256, 8192);
-            hipLaunchKernelGGL(step_copy, dim3(gc), dim3(256), 0, c->stream, L.A[AK_U], L.A[AK_BACKUP], L.n);
-        }
+        hipLaunchKernelGGL(step_cells_combine, dim3((unsigned)((P.total + 255) / 256)), dim3(256), 0, c->stream,
+                           P.cells[AK_U], P.cells[AK_BACKUP], nullptr, P.start, P.nblock, P.total, nv, 1.0, 0.0);
         HF_HIP(c, hipStreamSynchronize(c->stream));
         return HF_OK;
     }

@@ -45,7 +45,7 @@ from ctypes import byref, c_double, c_int32, c_int64
 import numpy as np
 
 from . import amr_grid
-from ._lib import (HF_BC_INFLOW, HF_BC_PERIODIC, HF_BC_REFLECTING, HF_BC_WEDGE, HF_CHEM_BATCHED, HF_EDOMAIN,  # noqa
+from ._lib import (HF_BC_INFLOW, HF_BC_KEEP, HF_BC_PERIODIC, HF_BC_REFLECTING, HF_BC_WEDGE, HF_CHEM_BATCHED, HF_EDOMAIN,  # noqa
                    HF_OK, HyperflowError, check, hf_box, hf_cells, hf_face_fluxes, hf_flux_stats, hf_level,
                    hf_patch, hf_tag_params, load)
 
@@ -118,7 +118,7 @@ class AmrDriver:
         self.wedge_i0 = int(wedge_i)
         self.x0, self.y0 = float(x0), float(y0)
         self.chemistry = bool(chemistry)
-        self.graph = bool(graph)
+        self.graph = bool(graph) and HF_BC_KEEP not in self.bc
         self.graph_steps = 0
         self.tag = dict(pressure_tag=0.08, temperature_tag=0.08, fuel_rate_tag=50.0, chemistry=self.chemistry,
                         box=None, velocity=(0.0, 0.0), mode="physics")

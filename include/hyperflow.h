@@ -516,8 +516,10 @@ typedef struct hf_level {
  * level's U is restored to the step's start and *taken is 0, for the
  * caller's step with host decisions. Otherwise *taken is 1 (replayed), 2
  * (captured and launched) or 3 (hf_options.eager), *dt_taken the step and
- * *repairs the zeroed values. The results are bit-identical to the same
- * step taken with hf_interp_regions, hf_fill_ghosts_flow,
+ * *repairs the zeroed values. The step's copies and Heun average touch the
+ * valid cells only (every ghost cell is refilled before it is read, so no
+ * side may be HF_BC_KEEP). The results are bit-identical to the same step
+ * taken with hf_interp_regions, hf_fill_ghosts_flow,
  * hf_flux_stage_fluxes, hf_reflux_masked, hf_average_down_masked,
  * hf_repair_cells, hf_validate_cells and hf_chem_advance_cells. */
 int hf_amr_strang_step(hf_ctx *ctx, const hf_level *levels, int nlevel, const int32_t bc[4], const double *inlet,
