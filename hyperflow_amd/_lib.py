@@ -102,6 +102,7 @@ _SIGNATURES = {
     "hf_chem_advance_cells": (c_int, [c_void_p, POINTER(hf_cells), c_int, c_void_p, c_int, POINTER(hf_chem_stats),
                                       POINTER(c_int64)]),
     "hf_validate_cells": (c_int, [c_void_p, POINTER(hf_cells), c_int, _dp, POINTER(c_int64)]),
+    "hf_totals_cells": (c_int, [c_void_p, POINTER(hf_cells), c_int, c_int, _dp]),
     "hf_repair_cells": (c_int, [c_void_p, POINTER(hf_cells), c_int, POINTER(c_int64), POINTER(c_int64)]),
     "hf_interp_regions": (c_int, [c_void_p, c_void_p, POINTER(hf_patch), c_void_p, POINTER(hf_patch),
                                   POINTER(c_int32), c_int, c_void_p, c_int64, POINTER(c_int32), POINTER(c_int64),

@@ -1476,6 +1476,85 @@ int hf_repair_cells(hf_ctx* c, const hf_cells* blocks, int nblock, int64_t* repa
     return HF_OK;
 }
 
+namespace {
+// Integrals over the leaf cells of a block table: per workgroup, a fixed-order
+// sum per component (butterfly within each wave, then the four waves in
+// order) into partials[group * nv + n]
+__global__ void __launch_bounds__(256) totals_partial_kernel(CellMap S, int64_t total, int nv, int absolute,
+                                                             double* __restrict__ partials) {
+    __shared__ double part[4][64];
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const double* q = nullptr;
+    int64_t plane = 0;
+    double vol = 0;
+    if (c < total) {
+        int b, i, j;
+        const hf_cells& B = S.find(c, b, i, j);
+        if (!B.mask || B.mask[(int64_t)j * B.mask_pitch + i] == 1) {
+            q = B.U + B.offset + (int64_t)j * B.pitch + i;
+            plane = B.plane;
+            vol = B.dy > 0 ? B.dx * B.dy : B.dx;
+        }
+    }
+    for (int n = 0; n < nv; ++n) {
+        double x = 0.0;
+        if (q) {
+            const double v = q[(int64_t)n * plane];
+            x = (absolute ? fabs(v) : v) * vol;
+        }
+        #pragma unroll
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+        if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6][n] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < nv) {
+        const int n = threadIdx.x;
+        partials[(int64_t)blockIdx.x * nv + n] = ((part[0][n] + part[1][n]) + part[2][n]) + part[3][n];
+    }
+}
+
+// one workgroup per component: the workgroup partials summed in a fixed order
+__global__ void __launch_bounds__(256) totals_final_kernel(const double* __restrict__ partials, int ngroups, int nv,
+                                                           double* __restrict__ out) {
+    __shared__ double wsum[4];
+    const int n = blockIdx.x;
+    double x = 0.0;
+    for (int g = threadIdx.x; g < ngroups; g += 256) x += partials[(int64_t)g * nv + n];
+    #pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) out[n] = ((wsum[0] + wsum[1]) + wsum[2]) + wsum[3];
+}
+}  // namespace
+
+int hf_totals_cells(hf_ctx* c, const hf_cells* blocks, int nblock, int absolute, double* totals) {
+    if (!c) return HF_EINVAL;
+    BusyGuard guard(c);
+    if (!guard.ok) return fail(c, HF_EBUSY, "concurrent call is unsupported");
+    if (!totals) return fail(c, HF_EINVAL, "missing output");
+    const int nv = c->mech.ns() + 3;
+    for (int n = 0; n < nv; ++n) totals[n] = 0.0;
+    if (nv > 64) return fail(c, HF_EINVAL, "mechanism too large for the totals reduction");
+    BoxLaunch L{};
+    std::vector<int64_t> start;
+    const int r = cells_common(c, L, blocks, nblock, start);
+    if (r != HF_OK) return r;
+    const int64_t groups = (L.total + 255) / 256;
+    if (groups > INT32_MAX || !c->aux[4].ensure(sizeof(double) * (nv * groups + 64)))
+        return fail(c, HF_ENOMEM, "device scratch");
+    double* part = c->aux[4].as<double>();
+    double* dout = part + nv * groups;
+    const CellMap S{nullptr, 0, L.blocks, L.block_start, L.nblock};
+    hipLaunchKernelGGL(totals_partial_kernel, dim3((unsigned)groups), dim3(256), 0, c->stream, S, L.total, nv,
+                       absolute ? 1 : 0, part);
+    hipLaunchKernelGGL(totals_final_kernel, dim3((unsigned)nv), dim3(256), 0, c->stream, part, (int)groups, nv, dout);
+    if (hipGetLastError() != hipSuccess) return fail(c, HF_EDEVICE, "totals launch failed");
+    HF_HIP(c, hipMemcpyAsync(totals, dout, sizeof(double) * nv, hipMemcpyDeviceToHost, c->stream));
+    HF_HIP(c, hipStreamSynchronize(c->stream));
+    return HF_OK;
+}
+
 int hf_interp_regions(hf_ctx* c, const double* coarse, const hf_patch* cp, double* fine, const hf_patch* fp,
                       const int32_t* regions, int nregion, const uint8_t* fine_mask, int64_t fine_mask_pitch,
                       const int32_t fine_extent[2], int64_t* limited, int64_t* first_failed) {

@@ -397,14 +397,12 @@ class AmrDriver:
 
     # ------------------------------------------------------------ regrid
     def totals(self, absolute=False):
-        """Conserved integrals over the leaf cells (BlockAMR::totals)."""
-        t = self.torch
-        s = t.zeros(self.nv, dtype=t.float64, device="cuda")
-        for L in self.levels[:self.finest + 1]:
-            Q = L.interior(L.A["U"])
-            Q = Q.abs() if absolute else Q
-            s += (Q * (L.mask == 1)).sum(dim=(1, 2)) * (L.dx * L.dy)
-        return s.cpu().numpy()
+        """Conserved integrals over the leaf cells (BlockAMR::totals, one
+        device reduction over every box: hf_totals_cells)."""
+        blocks, n = self._cells("U")
+        out = (c_double * self.nv)()
+        check(self._h, load().hf_totals_cells(self._h, blocks, n, int(bool(absolute)), out))
+        return np.array(out[:])
 
     def tags(self, lev):
         """ErrorEst's tags on level lev's valid cells (host bool array)."""
@@ -443,7 +441,8 @@ class AmrDriver:
         relative defect."""
         self._average("U")
         before, scale = self.totals(), self.totals(True)
-        tags = [self.tags(lev) for lev in range(self.finest + 1)]
+        # tags of the levels that may be refined (make_grids reads no others)
+        tags = [self.tags(lev) for lev in range(min(self.finest, self.max_level - 1) + 1)]
         valid = [L.mask_host > 0 for L in self.levels[:self.finest + 1]]
         new = amr_grid.make_grids(tags, valid, self.max_level, self.blocking, self.max_grid_size, self.grid_eff,
                                   self.n_error_buf, self.n_proper, self.margins)

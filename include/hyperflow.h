@@ -474,6 +474,14 @@ int hf_box_strang_step(hf_ctx *ctx, double *U, double *U1, double *U2, double *o
                        const hf_box *box, const int32_t bc[4], const double *inlet, double dx, double dy, double cfl,
                        int chem_mode, double time_left, double *dt_taken, int32_t *taken, int64_t *repairs);
 
+/* BlockAMR::totals (BlockAMR.cpp:347-355): the integral of every conserved
+ * component (|value| when absolute != 0) times the cell volume dx*dy (dx for
+ * a 1-D block) over the leaf cells (mask 1, or every cell without a mask) of
+ * a block table. The reference sums in long double cell by cell; here each
+ * workgroup sums its cells and the workgroups are summed in a fixed order, so
+ * the result does not depend on timing. totals: nv = ns + 3 values (host). */
+int hf_totals_cells(hf_ctx *ctx, const hf_cells *blocks, int nblock, int absolute, double *totals);
+
 /* One level of a block-structured hierarchy at refinement ratio 2 (the
  * levels of BlockAMR, BlockAMR.hpp). The level is one ghosted array over its
  * whole index space: component n of cell (i, j) at

@@ -258,3 +258,30 @@ def test_graph_step_hands_back_failed_steps():
     assert e.retries > 0
     assert g.graph_steps < g.steps
     assert g.first_order_stages == e.first_order_stages and g.retries == e.retries
+
+
+def test_totals_over_leaves():
+    """hf_totals_cells (BlockAMR::totals) over every box of three levels:
+    the leaf cells' values times the cell area, plain and absolute, against
+    numpy sums of the same cells to 1e-13."""
+    print("[gpu] hierarchy totals over the leaves, three levels", flush=True)
+    mech = hf.Mechanism(H2, **CONVERGED)
+    Q, dx = small_channel(mech)
+    bc = (hf.HF_BC_REFLECTING, hf.HF_BC_TRANSMISSIVE, hf.HF_BC_REFLECTING, hf.HF_BC_REFLECTING)
+    d = AmrDriver(mech, Q.shape[2], Q.shape[1], dx, dx, bc, max_level=2, blocking=8, max_grid_size=32)
+    d.initial_grids(Q)
+    d.step()
+    assert d.finest == 2
+    ref = {False: np.zeros(mech.nv), True: np.zeros(mech.nv)}
+    for lev in range(d.finest + 1):
+        A, m = d.level_state(lev)
+        L = d.levels[lev]
+        for absolute in (False, True):
+            v = np.abs(A) if absolute else A
+            ref[absolute] += v[:, m == 1].sum(axis=1) * (L.dx * L.dy)
+    for absolute in (False, True):
+        got = d.totals(absolute)
+        # relative to the integral of |value| (the momenta sum to nearly zero)
+        err = np.abs(got - ref[absolute]) / (ref[True] + 1e-300)
+        print(f"[gpu]   absolute={absolute}: worst relative difference {err.max():.2e}", flush=True)
+        assert err.max() < 1e-13
