@@ -226,53 +226,57 @@ def wedge_run(torch, hf, steps, warmup):
     return time.perf_counter() - t0, d, d.state()
 
 
-def wedge_amr_run(torch, hf, steps, warmup):
-    """The wedge case's 144 x 64 base level with one refined box regridded
-    every REGRID_INTERVAL steps from ErrorEst's tags (the case's own
-    pressure_tag 0.08, temperature_tag 0.06, fuel_rate_tag 50 /s;
-    cases/teng_case1), through TwoLevelDriver. Returns (wall, driver, states)."""
-    from hyperflow_amd.driver import TwoLevelDriver, teng_osw_base
-    mech = hf.Mechanism(hf.H2_LI, rtol=1e-8, atol=1e-12)
+# the wedge case's own refinement settings (cases/teng_case1): two levels, blocking 4, max_grid_size 32,
+# the case's tag criteria and tag box
+WEDGE_AMR = dict(max_level=2, blocking=4, max_grid_size=32, grid_eff=0.7, n_error_buf=2, n_proper=2,
+                 regrid_interval=8)
+WEDGE_TAG = dict(pressure_tag=0.08, temperature_tag=0.06, fuel_rate_tag=50.0, box=(-3.2e-5, 0.000128, 0.0, 6.4e-5))
+
+
+def wedge_amr_driver(hf, mech):
+    """The wedge case (cases/teng_case1) with its own hierarchy: 144 x 64
+    base cells, two refinement levels from ErrorEst's tags at the case's
+    criteria and tag box, inflow and wedge bottom on every level."""
+    from hyperflow_amd.driver import teng_osw_base
+    from hyperflow_amd.hierarchy import AmrDriver
     w = teng_osw_base(mech)
-    tag = dict(temperature_tag=0.06, x0=-0.000128)
+    d = AmrDriver(mech, 144, 64, w["dx"], w["dy"], w["bc"], cfl=0.6, chem_mode=hf.HF_CHEM_BATCHED,
+                  inlet=w["inlet"], wedge_i=w["wedge_i"], tag=WEDGE_TAG, x0=-0.000128, **WEDGE_AMR)
+    d.initial_grids(w["Q"])
+    return d, w
 
-    def fresh():
-        d = TwoLevelDriver(mech, 144, 64, w["dx"], w["dy"], w["bc"], (0, 0, 31, 15), cfl=0.6,
-                           chem_mode=hf.HF_CHEM_BATCHED, inlet=w["inlet"], wedge_i=w["wedge_i"])
-        d.set_state(w["Q"])
-        return d
 
-    def advance(d, n):
-        for _ in range(n):
-            d.step()
-            if d.steps % REGRID_INTERVAL == 0:
-                d.regrid(**tag)
-    d = fresh()
-    advance(d, warmup)
-    d = fresh()
+def wedge_amr_run(torch, hf, steps, warmup):
+    """Strang steps of the wedge case on its hierarchy. Returns (wall of the
+    timed steps, driver, leaf-cell steps done)."""
+    mech = hf.Mechanism(hf.H2_LI, rtol=1e-8, atol=1e-12)
+    d, _ = wedge_amr_driver(hf, mech)
+    for _ in range(warmup):
+        d.step()
+    d, _ = wedge_amr_driver(hf, mech)
+    leaf_steps = 0
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    advance(d, steps)
+    for _ in range(steps):
+        n = d.leaf_count()
+        d.step()
+        leaf_steps += n
     torch.cuda.synchronize()
-    return time.perf_counter() - t0, d, d.state()
+    return time.perf_counter() - t0, d, leaf_steps
 
 
-def wedge_amr_cpu(hf, target_s=12.0):
-    """The oracle's two-level loop on the same adaptive wedge case."""
-    from hyperflow_amd.driver import teng_osw_base
+def wedge_amr_cpu(torch, hf, target_s=12.0):
+    """The oracle's multi-level step loop (oracle/hierarchy.py) on the same
+    wedge hierarchy, in lockstep with the GPU driver, following the GPU's
+    boxes at every regrid. Returns (steps, oracle wall, leaf-cell steps,
+    per-level state differences GPU vs oracle)."""
     from oracle import Oracle
-    from oracle.driver import OracleTwoLevelDriver
-    w = teng_osw_base(hf.Mechanism(hf.H2_LI))
-    o = OracleTwoLevelDriver(Oracle(hf.H2_LI), 144, 64, w["dx"], w["dy"], w["bc"], (0, 0, 31, 15), cfl=0.6,
-                             rtol=1e-8, atol=1e-12, inlet=w["inlet"], wedge_i=w["wedge_i"])
-    o.set_state(w["Q"])
-    n, t0 = 0, time.perf_counter()
-    while n == 0 or time.perf_counter() - t0 < target_s:
-        o.step()
-        n += 1
-        if o.steps % REGRID_INTERVAL == 0:
-            o.regrid(temperature_tag=0.06, x0=-0.000128)
-    return n, time.perf_counter() - t0, (o.Qc.copy(), o.Qf.copy())
+    from oracle.hierarchy import OracleAmrDriver
+    mech = hf.Mechanism(hf.H2_LI, rtol=1e-8, atol=1e-12)
+    d, w = wedge_amr_driver(hf, mech)
+    o = OracleAmrDriver(Oracle(hf.H2_LI), 144, 64, w["dx"], w["dy"], w["bc"], max_level=WEDGE_AMR["max_level"],
+                        cfl=0.6, rtol=1e-8, atol=1e-12, inlet=w["inlet"], wedge_i=w["wedge_i"])
+    return hierarchy_lockstep(d, o, w["Q"], mech, target_s)
 
 
 def wedge_cpu(hf, target_s=12.0):
@@ -548,6 +552,14 @@ def amr_cpu(torch, hf, target_s=12.0, nx=256, ny=64):
     d, Q, dx, bc = amr_driver(hf, nx, ny, mech)
     o = OracleAmrDriver(Oracle(hf.H2_LI), nx, ny, dx, dx, bc, max_level=AMR["max_level"], cfl=0.5, rtol=1e-8,
                         atol=1e-12)
+    return hierarchy_lockstep(d, o, Q, mech, target_s)
+
+
+def hierarchy_lockstep(d, o, Q, mech, target_s):
+    """Whole oracle steps until about target_s of oracle time, the GPU
+    driver d stepping beside it and the oracle o taking d's boxes at every
+    regrid (its remap). Returns (steps, oracle wall, leaf-cell steps of the
+    oracle, per-level state differences GPU vs oracle)."""
     o.set_grids(d.grids())
     o.set_state(Q)
     n, wall, leaf_steps = 0, 0.0, 0
@@ -706,7 +718,7 @@ def main():
     ap.add_argument("--wedge-steps", type=int, default=200,
                     help="Strang steps of the wedge case's 144 x 64 base level (0 skips it)")
     ap.add_argument("--wedge-amr-steps", type=int, default=200,
-                    help="Strang steps of the wedge case with one adaptive refined box (0 skips it)")
+                    help="Strang steps of the wedge case on its own two-level hierarchy (0 skips it)")
     ap.add_argument("--transfer-level", type=int, default=2048,
                     help="coarse level width of the level-transfer leg (height half of it; 0 skips it)")
     ap.add_argument("--composite-stages", type=int, default=50,
@@ -859,18 +871,25 @@ def main():
 
     wamr = None
     if args.wedge_amr_steps:
-        log(f"wedge case with one adaptive refined box: {args.wedge_amr_steps} Strang steps")
-        wawall, wadrv, _ = wedge_amr_run(torch, hf, args.wedge_amr_steps, 1)
-        wamr = {"workload": "the wedge case (144 x 64 base level, cases/teng_case1) with one refined box at ratio 2, "
-                            "starting over the wedge tip and regridded every "
-                            f"{REGRID_INTERVAL} steps from ErrorEst's tags at the case's criteria (p jump 0.08, T "
-                            "jump 0.06, fuel rate 50 /s; 2-cell buffer, blocks of 4); inflow and wedge bottom on "
-                            "both levels; CFL 0.6; Strang chemistry (batched FP64 form, rtol 1e-8 atol 1e-12)",
+        log(f"wedge case hierarchy: 144 x 64 base, 2 refinement levels, {args.wedge_amr_steps} Strang steps")
+        wawall, wadrv, wleaf = wedge_amr_run(torch, hf, args.wedge_amr_steps, 1)
+        wg = wadrv.grids()
+        wamr = {"workload": "the wedge case (cases/teng_case1) with its own refinement settings: 144 x 64 base "
+                            "cells, two levels at ratio 2 from ErrorEst's tags at the case's criteria (p jump "
+                            "0.08, T jump 0.06, fuel rate 50 /s) and tag box, blocking 4, max_grid_size 32, "
+                            "n_error_buf 2, regridded every 8 steps; inflow left and top, wedge bottom, "
+                            "transmissive right on every level; CFL 0.6; Strang chemistry (batched FP64 form, "
+                            "rtol 1e-8 atol 1e-12) over every leaf; each Strang step one replayed HIP graph "
+                            "(hf_amr_strang_step)",
                 "steps": args.wedge_amr_steps, "steps_per_s": args.wedge_amr_steps / wawall,
-                "ms_per_step": wawall / args.wedge_amr_steps * 1e3, "simulated_time_s": wadrv.time,
-                "fine_box_end": list(wadrv.fine_box), "regrids": wadrv.regrids, "retries": wadrv.retries,
-                "first_order_stages": wadrv.first_order_stages}
-        log(f"adaptive wedge case: {wamr['steps_per_s']:.4g} steps/s, box {wadrv.fine_box}")
+                "ms_per_step": wawall / args.wedge_amr_steps * 1e3,
+                "Mleaf_cell_steps_per_s": wleaf / wawall / 1e6, "simulated_time_s": wadrv.time,
+                "boxes_per_level_end": [len(x) for x in wg][1:], "leaf_cells_end": wadrv.leaf_count(),
+                "regrids": wadrv.regrids, "grid_changes": wadrv.grid_changes,
+                "max_remap_defect": wadrv.max_remap_defect, "retries": wadrv.retries,
+                "first_order_stages": wadrv.first_order_stages, "repairs": wadrv.repairs,
+                "graph_steps": wadrv.graph_steps}
+        log(f"wedge case hierarchy: {wamr['steps_per_s']:.4g} steps/s, boxes {wamr['boxes_per_level_end']}")
 
     xfer = None
     if args.transfer_level:
@@ -1049,7 +1068,7 @@ def main():
     if wedge is not None:
         line["wedge_2d"] = wedge
     if wamr is not None:
-        line["wedge_2level"] = wamr
+        line["wedge_amr"] = wamr
     if xfer is not None:
         line["level_transfers"] = xfer
     if comp is not None:
@@ -1082,16 +1101,15 @@ def main():
             line["wedge_2d"]["parity_vs_oracle"] = {
                 "steps": nc, **state_difference(gstate, cstate, hf.Mechanism(hf.H2_LI).ns)}
         if wamr is not None:
-            log("adaptive wedge case: CPU oracle step loop")
-            nc, cw, (oc, of) = wedge_amr_cpu(hf)
-            _, _, (gc, gf) = wedge_amr_run(torch, hf, nc, 0)
-            nsx = hf.Mechanism(hf.H2_LI).ns
-            line["wedge_2level"]["cpu_baseline"] = {
-                "value": nc / cw, "unit": "Strang steps/s", "cores": len(os.sched_getaffinity(0)), "kind": "port",
-                "sample": f"the first {nc} steps of the same run, the oracle's two-level step loop, {cw:.1f} s"}
-            line["wedge_2level"]["parity_vs_oracle"] = {
-                "steps": nc, "coarse": state_difference(gc, oc, nsx),
-                "fine": state_difference(gf, of, nsx) if gf.shape == of.shape else "boxes differ"}
+            log("wedge case hierarchy: CPU oracle multi-level step loop")
+            nc, cw, cls, wdiffs = wedge_amr_cpu(torch, hf)
+            line["wedge_amr"]["cpu_baseline"] = {
+                "value": cls / cw, "unit": "leaf cell-steps/s", "cores": len(os.sched_getaffinity(0)),
+                "kind": "port",
+                "sample": f"the first {nc} Strang steps of the same run (the oracle's multi-level step loop "
+                          f"following the GPU's boxes), {cw:.1f} s; GPU / CPU leaf cell-steps/s = "
+                          f"{wamr['Mleaf_cell_steps_per_s'] * 1e6 / (cls / cw):.1f}"}
+            line["wedge_amr"]["parity_vs_oracle"] = {"steps": nc, **wdiffs}
         if two is not None:
             log("two-level channel: CPU oracle step loop")
             nc, cw, (oc, of) = two_level_cpu(hf)
