@@ -108,6 +108,13 @@ def flux_run(torch, hf, mech, nx, ny, steps, warmup):
     return wall, ms, st
 
 
+def step_mech(hf):
+    """The 2-D step loops' chemistry: the batched form with the FP32 Newton
+    matrix (FP64 state, residuals, rates and error control), rtol 1e-8, atol
+    1e-12."""
+    return hf.Mechanism(hf.H2_LI, rtol=1e-8, atol=1e-12, precision=hf.HF_PREC_MIXED)
+
+
 def shock_tube_setup(hf):
     """The 1-D plain workload: 2000 cells over 0.1 m of H2/air, x < 0.01 m at
     2000 K / 20 atm TP equilibrium, the rest at 300 K / 1 atm; reflecting left,
@@ -159,10 +166,11 @@ CHANNEL = dict(nx=1024, ny=256)
 
 def channel_run(torch, hf, steps, warmup):
     """Strang steps of the 2-D channel's base level through BoxDriver (every
-    stage on the GPU; batched FP64 chemistry over all cells each step).
+    stage on the GPU; batched chemistry, FP32 Newton matrix, over all cells
+    each step).
     Returns (wall of the timed steps, driver, state after them)."""
     from hyperflow_amd.driver import BoxDriver, channel_2d
-    mech = hf.Mechanism(hf.H2_LI, rtol=1e-8, atol=1e-12)
+    mech = step_mech(hf)
     Q, dx = channel_2d(mech, **CHANNEL)
     bc = (hf.HF_BC_REFLECTING, hf.HF_BC_TRANSMISSIVE, hf.HF_BC_REFLECTING, hf.HF_BC_REFLECTING)
     nx, ny = CHANNEL["nx"], CHANNEL["ny"]
@@ -189,7 +197,7 @@ def channel_cpu(hf, target_s=12.0):
     from hyperflow_amd.driver import channel_2d
     from oracle import Oracle
     from oracle.driver import OracleBoxDriver, REFLECTING, TRANSMISSIVE
-    mech = hf.Mechanism(hf.H2_LI, rtol=1e-8, atol=1e-12)
+    mech = step_mech(hf)
     Q, dx = channel_2d(mech, **CHANNEL)
     o = OracleBoxDriver(Oracle(hf.H2_LI), CHANNEL["nx"], CHANNEL["ny"], dx, dx,
                         (REFLECTING, TRANSMISSIVE, REFLECTING, REFLECTING), cfl=0.5, rtol=1e-8, atol=1e-12)
@@ -206,7 +214,7 @@ def wedge_run(torch, hf, steps, warmup):
     top, wedge bottom from x = 0, outflow right) through BoxDriver, batched
     FP64 chemistry. Returns (wall of the timed steps, driver, state after them)."""
     from hyperflow_amd.driver import BoxDriver, teng_osw_base
-    mech = hf.Mechanism(hf.H2_LI, rtol=1e-8, atol=1e-12)
+    mech = step_mech(hf)
     w = teng_osw_base(mech)
 
     def fresh():
@@ -249,7 +257,7 @@ def wedge_amr_driver(hf, mech):
 def wedge_amr_run(torch, hf, steps, warmup):
     """Strang steps of the wedge case on its hierarchy. Returns (wall of the
     timed steps, driver, leaf-cell steps done)."""
-    mech = hf.Mechanism(hf.H2_LI, rtol=1e-8, atol=1e-12)
+    mech = step_mech(hf)
     d, _ = wedge_amr_driver(hf, mech)
     for _ in range(warmup):
         d.step()
@@ -272,7 +280,7 @@ def wedge_amr_cpu(torch, hf, target_s=12.0):
     per-level state differences GPU vs oracle)."""
     from oracle import Oracle
     from oracle.hierarchy import OracleAmrDriver
-    mech = hf.Mechanism(hf.H2_LI, rtol=1e-8, atol=1e-12)
+    mech = step_mech(hf)
     d, w = wedge_amr_driver(hf, mech)
     o = OracleAmrDriver(Oracle(hf.H2_LI), 144, 64, w["dx"], w["dy"], w["bc"], max_level=WEDGE_AMR["max_level"],
                         cfl=0.6, rtol=1e-8, atol=1e-12, inlet=w["inlet"], wedge_i=w["wedge_i"])
@@ -447,13 +455,13 @@ REGRID_INTERVAL = 8                          # the reference's default regrid_in
 def two_level_run(torch, hf, steps, warmup):
     """Strang steps of the channel (1024 x 256, reflecting left, top and
     bottom, transmissive right) with one refined box through TwoLevelDriver,
-    batched FP64 chemistry over the active cells. The box starts at
+    batched chemistry (FP32 Newton matrix) over the active cells. The box starts at
     TWO_LEVEL_BOX, is re-chosen from the tags of the initial state, and is
     regridded every REGRID_INTERVAL steps (ErrorEst's default criteria, a
     2-cell buffer, blocks of 4 coarse cells). Returns (wall of the timed
     steps, driver, (coarse, fine))."""
     from hyperflow_amd.driver import TwoLevelDriver, channel_2d
-    mech = hf.Mechanism(hf.H2_LI, rtol=1e-8, atol=1e-12)
+    mech = step_mech(hf)
     Q, dx = channel_2d(mech, **CHANNEL)
     bc = (hf.HF_BC_REFLECTING, hf.HF_BC_TRANSMISSIVE, hf.HF_BC_REFLECTING, hf.HF_BC_REFLECTING)
 
@@ -485,7 +493,7 @@ def two_level_cpu(hf, target_s=12.0):
     from hyperflow_amd.driver import channel_2d
     from oracle import Oracle
     from oracle.driver import REFLECTING, TRANSMISSIVE, OracleTwoLevelDriver
-    Q, dx = channel_2d(hf.Mechanism(hf.H2_LI, rtol=1e-8, atol=1e-12), **CHANNEL)
+    Q, dx = channel_2d(step_mech(hf), **CHANNEL)
     o = OracleTwoLevelDriver(Oracle(hf.H2_LI), CHANNEL["nx"], CHANNEL["ny"], dx, dx,
                              (REFLECTING, TRANSMISSIVE, REFLECTING, REFLECTING), TWO_LEVEL_BOX, cfl=0.5,
                              rtol=1e-8, atol=1e-12)
@@ -523,7 +531,7 @@ def amr_run(torch, hf, steps, warmup, nx=1024, ny=256):
     cells (blocking 16, max_grid_size 64, grid_eff 0.7) from ErrorEst's tags
     at the reference defaults, regridded every REGRID_INTERVAL steps. Returns
     (wall of the timed steps, driver, leaf-cell steps done)."""
-    mech = hf.Mechanism(hf.H2_LI, rtol=1e-8, atol=1e-12)
+    mech = step_mech(hf)
     d, _, _, _ = amr_driver(hf, nx, ny, mech)
     for _ in range(warmup):
         d.step()
@@ -548,7 +556,7 @@ def amr_cpu(torch, hf, target_s=12.0, nx=256, ny=64):
     the oracle, per-level state differences GPU vs oracle)."""
     from oracle import Oracle
     from oracle.hierarchy import OracleAmrDriver
-    mech = hf.Mechanism(hf.H2_LI, rtol=1e-8, atol=1e-12)
+    mech = step_mech(hf)
     d, Q, dx, bc = amr_driver(hf, nx, ny, mech)
     o = OracleAmrDriver(Oracle(hf.H2_LI), nx, ny, dx, dx, bc, max_level=AMR["max_level"], cfl=0.5, rtol=1e-8,
                         atol=1e-12)
@@ -846,7 +854,7 @@ def main():
                             "(H2_Li), stoichiometric H2/air at 300 K / 1 atm with a 2500 K / 30 atm strip in "
                             "x < 0.01 m, +-1 % density blocks of 4 x 4 cells at fixed p (seed 3); reflecting left, top "
                             "and bottom, transmissive right; NND + HLLC, Heun, CFL 0.5, Strang chemistry (batched "
-                            "FP64 form, rtol 1e-8 atol 1e-12) over every cell",
+                            "form, FP32 Newton matrix, rtol 1e-8 atol 1e-12) over every cell",
                 "steps": args.channel_steps, "steps_per_s": args.channel_steps / cwall,
                 "Mcell_updates_per_s": ncell * 4 * args.channel_steps / cwall / 1e6,
                 "ms_per_step": cwall / args.channel_steps * 1e3, "simulated_time_s": cdrv.time,
@@ -862,7 +870,7 @@ def main():
                              "premixed H2/air (2:1:3.76) at 958.1 K / 1202900 Pa, 2737.7 m/s at 19 deg below x, in "
                              "the domain and through the inflow left and top; wedge bottom (outflow before x = 0, "
                              "slip wall after), transmissive right; NND + HLLC, Heun, CFL 0.6, Strang chemistry "
-                             "(batched FP64 form, rtol 1e-8 atol 1e-12) over every cell",
+                             "(batched form, FP32 Newton matrix, rtol 1e-8 atol 1e-12) over every cell",
                  "steps": args.wedge_steps, "steps_per_s": args.wedge_steps / wwall,
                  "Mcell_updates_per_s": 144 * 64 * 4 * args.wedge_steps / wwall / 1e6,
                  "ms_per_step": wwall / args.wedge_steps * 1e3, "simulated_time_s": wdrv.time,
@@ -878,8 +886,8 @@ def main():
                             "cells, two levels at ratio 2 from ErrorEst's tags at the case's criteria (p jump "
                             "0.08, T jump 0.06, fuel rate 50 /s) and tag box, blocking 4, max_grid_size 32, "
                             "n_error_buf 2, regridded every 8 steps; inflow left and top, wedge bottom, "
-                            "transmissive right on every level; CFL 0.6; Strang chemistry (batched FP64 form, "
-                            "rtol 1e-8 atol 1e-12) over every leaf; each Strang step one replayed HIP graph "
+                            "transmissive right on every level; CFL 0.6; Strang chemistry (batched form, FP32 "
+                            "Newton matrix, rtol 1e-8 atol 1e-12) over every leaf; each Strang step one replayed HIP graph "
                             "(hf_amr_strang_step)",
                 "steps": args.wedge_amr_steps, "steps_per_s": args.wedge_amr_steps / wawall,
                 "ms_per_step": wawall / args.wedge_amr_steps * 1e3,
@@ -949,8 +957,8 @@ def main():
                            "box at ratio 2 chosen from ErrorEst's tags at the reference defaults (p and T jumps "
                            "0.08, fuel rate 50 /s; 2-cell buffer, blocks of 4 coarse cells) and regridded every "
                            f"{REGRID_INTERVAL} steps with the remap defect check; two-level composite Heun stages "
-                           "(reflux, average-down, repair, validation), Strang chemistry (batched FP64 form, rtol "
-                           "1e-8 atol 1e-12) over the active cells, CFL 0.5 over both levels",
+                           "(reflux, average-down, repair, validation), Strang chemistry (batched form, FP32 "
+                           "Newton matrix, rtol 1e-8 atol 1e-12) over the active cells, CFL 0.5 over both levels",
                "fine_box_end": list(fb), "regrids": tdrv.regrids,
                "steps": args.two_level_steps, "steps_per_s": args.two_level_steps / twall,
                "ms_per_step": twall / args.two_level_steps * 1e3,
@@ -973,7 +981,8 @@ def main():
                            f"2, n_proper 2, regridded every {REGRID_INTERVAL} steps with the remap defect check; "
                            "composite Heun stages on every level (masked ghost interpolation, flux stages over "
                            "all boxes, reflux, average-down, repair over the leaves, validation), one batched "
-                           "FP64 chemistry launch over every leaf cell through the device block table (rtol 1e-8, "
+                           "chemistry launch (FP32 Newton matrix) over every leaf cell through the device block "
+                           "table (rtol 1e-8, "
                            "atol 1e-12), CFL 0.5; each Strang step one replayed HIP graph (hf_amr_strang_step, "
                            "recaptured when a regrid changes the boxes; a step it hands back runs call by call)",
                "steps": args.amr_steps, "steps_per_s": args.amr_steps / awall,
