@@ -27,6 +27,36 @@
 
 namespace hf {
 
+// x^n for n = 0..7 (by the bits of n).
+__device__ __forceinline__ double ipow(double x, int n) {
+    const double x2 = x * x, x4 = x2 * x2;
+    return ((n & 1) ? x : 1.0) * ((n & 2) ? x2 : 1.0) * ((n & 4) ? x4 : 1.0);
+}
+
+// x^(1/n) for n = 1..7, the root in CVODE's step-size ratios
+// eta = 1 / ((c dsm)^(1/n) + 1e-6). x = m 2^(n k) with k near log2(x) / n
+// (exact scalings), so m lies within a few powers of two of 1; a
+// single-precision estimate of m^(1/n) from the hardware log2 and exp2
+// (relative error below 1e-5) is refined by two double-precision Newton
+// steps on r^n = m, r <- r (1 + (m / r^n - 1) / n), whose fixed point does
+// not depend on the rounding of 1/n. The result is within 2 units in the
+// last place of the exact root (pow(x, 1.0 / n) is up to about 100 from it
+// at the ends of the double range, where the rounding of 1.0 / n shows), in
+// a quarter of pow's instructions; every trip of a batched wave evaluates
+// several of these. x = 0 gives 0, +inf gives +inf, a negative x or NaN
+// gives NaN, as pow does.
+__device__ __forceinline__ double root_n(double x, int n) {
+    if (!(x > 0) || x == __builtin_inf()) return x == 0 || x == __builtin_inf() ? x : __builtin_nan("");
+    const float rn = __builtin_amdgcn_rcpf((float)n);
+    const int k = (int)((float)__builtin_amdgcn_frexp_exp(x) * rn);
+    const double m = __builtin_ldexp(x, -n * k);
+    double r = (double)__builtin_amdgcn_exp2f(__builtin_amdgcn_logf((float)m) * rn);
+    const double c = (double)rn;
+    #pragma unroll
+    for (int it = 0; it < 2; ++it) r = fma(r * c, m / ipow(r, n) - 1.0, r);
+    return __builtin_ldexp(r, k);
+}
+
 struct BdfCounters {
     int steps = 0, rhs = 0, jac = 0, lu = 0, newton = 0, ncf = 0, netf = 0;
 };
@@ -533,7 +563,7 @@ struct Bdf {
             retry = true;
             phase = PH_START;
             if (nef <= 3) {
-                eta = 1.0 / (pow(6.0 * dsm, 1.0 / L) + 1e-6);
+                eta = 1.0 / (root_n(6.0 * dsm, L) + 1e-6);
                 eta = fmax(0.1, eta);
                 if (nef >= 2) eta = fmin(eta, 0.2);
                 rescale();
@@ -610,7 +640,7 @@ struct Bdf {
             n_hprime = h;
             n_eta = 1.0;
         } else {
-            const double etaq = 1.0 / (pow(6.0 * dsm, 1.0 / L) + 1e-6);
+            const double etaq = 1.0 / (root_n(6.0 * dsm, L) + 1e-6);
             double e = etaq;
             if (qwait == 0) {
                 n_qwait = 2;
@@ -620,15 +650,15 @@ struct Bdf {
                     #pragma unroll
                     for (int i = 0; i < N; ++i) tmp[i] = zrow(q, i);
                     const double ddn = wrms(tmp) * tq1;
-                    etaqm1 = 1.0 / (pow(6.0 * ddn, 1.0 / q) + 1e-6);
+                    etaqm1 = 1.0 / (root_n(6.0 * ddn, q) + 1e-6);
                 }
                 if (q != QMAX && saved_tq5 != 0) {
-                    const double cquot = (tq5 / saved_tq5) * pow(h / TAU(2), (double)L);
+                    const double cquot = (tq5 / saved_tq5) * ipow(h / TAU(2), L);
                     double tmp[N];
                     #pragma unroll
                     for (int i = 0; i < N; ++i) tmp[i] = acor[i] - cquot * Z(QMAX, i);
                     const double dup = wrms(tmp) * tq3;
-                    etaqp1 = 1.0 / (pow(10.0 * dup, 1.0 / (L + 1)) + 1e-6);
+                    etaqp1 = 1.0 / (root_n(10.0 * dup, L + 1) + 1e-6);
                 }
                 const double etam = fmax(etaqm1, fmax(etaq, etaqp1));
                 if (etam < 1.5) {

@@ -93,8 +93,9 @@ def jac_flops(ns, reactions):
 
 def step_flops(n, q=3):
     """BDF bookkeeping per step: prediction, coefficients, error test (one
-    WRMS norm), update, and the step-size ratio eta = (6 dsm)^(-1/L) (a pow,
-    i.e. a log and an exp; 1.5 per step on average with the order checks)."""
+    WRMS norm), update, and the step-size ratio eta = (6 dsm)^(-1/L) (a root,
+    counted as a log and an exp; 1.5 per step on average with the order
+    checks)."""
     pascal = n * q * (q + 1) // 2
     return 2 * pascal + 60 + 8 * n + 2 * n * (q + 1) + SQRT + 1.5 * (LOG + EXP + DIV)
 
