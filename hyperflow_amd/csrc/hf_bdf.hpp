@@ -67,14 +67,21 @@ template <int N, class Sys, class MT = double, int ZSTRIDE = 64, int ZAS = 3>
 struct Bdf {
     static constexpr int QMAX = 5;
     static constexpr double UROUND = 2.220446049250313e-16;
-    // doubles per lane in the LDS slab: Nordsieck history, tau, l
-    static constexpr int SLAB = (QMAX + 1) * N + (QMAX + 2) + (QMAX + 1);
+    // doubles per lane in the LDS slab: Nordsieck history, tau, l, and the
+    // saved correction for an order increase
+    static constexpr int SLAB = (QMAX + 1) * N + (QMAX + 2) + (QMAX + 1) + N;
 
     // Nordsieck history: element (j, i) at zs[(j*N + i) * ZSTRIDE]; the caller
     // points zs at this lane's slot of an array laid out [j][i][lane], so
     // consecutive lanes touch consecutive doubles. ZAS is its address space:
     // 3 (LDS) for small mechanisms, 1 (a per-wave tile in global memory) when
-    // the history outgrows LDS.
+    // the history outgrows LDS. Columns above the current order q are kept
+    // at zero, so the loops over the history (predict, restore, rescale, the
+    // corrector update and the interpolation at tout) run over every column
+    // without a per-lane test of q: the extra terms add or scale exact
+    // zeros. CVODE keeps the correction saved for an order increase in its
+    // column qmax; here it has a column of its own (ZS), so column qmax
+    // obeys the same rule.
     double* zs;
     double ewt[N], acor[N];
     double tq1, tq2, tq3, tq4, tq5;
@@ -116,6 +123,7 @@ struct Bdf {
     }
     __device__ __forceinline__ double& TAU(int i) const { return slab((QMAX + 1) * N + i); }
     __device__ __forceinline__ double& LC(int i) const { return slab((QMAX + 1) * N + QMAX + 2 + i); }
+    __device__ __forceinline__ double& ZS(int i) const { return slab((QMAX + 1) * N + 2 * QMAX + 3 + i); }
     __device__ __forceinline__ double wrms(const double* v) const {
         double s = 0;
         #pragma unroll
@@ -141,8 +149,7 @@ struct Bdf {
             double zz[QMAX + 1];
             zload(i, zz);
             #pragma unroll
-            for (int j = 1; j <= QMAX; ++j)
-                if (j <= q) zz[j] *= fac[j];
+            for (int j = 1; j <= QMAX; ++j) zz[j] *= fac[j];
             zstore(i, zz);
         }
         h = hscale * eta;
@@ -158,8 +165,7 @@ struct Bdf {
             #pragma unroll
             for (int k = 1; k <= QMAX; ++k)
                 #pragma unroll
-                for (int j = QMAX; j >= k; --j)
-                    if (k <= q && j <= q) zz[j - 1] += zz[j];
+                for (int j = QMAX; j >= k; --j) zz[j - 1] += zz[j];
             zstore(i, zz);
         }
     }
@@ -172,15 +178,19 @@ struct Bdf {
             #pragma unroll
             for (int k = 1; k <= QMAX; ++k)
                 #pragma unroll
-                for (int j = QMAX; j >= k; --j)
-                    if (k <= q && j <= q) zz[j - 1] -= zz[j];
+                for (int j = QMAX; j >= k; --j) zz[j - 1] -= zz[j];
             zstore(i, zz);
         }
     }
 
-    // Order q -> q-1: subtract the top column's contribution.
+    // Order q -> q-1: subtract the top column's contribution (for q > 2, as
+    // CVODE does) and clear it.
     __device__ __forceinline__ void decrease_order() {
-        if (q <= 2) return;
+        if (q <= 2) {
+            #pragma unroll
+            for (int i = 0; i < N; ++i) Z(2, i) = 0.0;
+            return;
+        }
         double lc[QMAX + 1];
         #pragma unroll
         for (int j = 0; j <= QMAX; ++j) lc[j] = 0;
@@ -203,12 +213,14 @@ struct Bdf {
             #pragma unroll
             for (int j = 0; j <= QMAX; ++j) top += j == q ? zz[j] : 0.0;
             #pragma unroll
-            for (int j = 2; j < QMAX; ++j)
+            for (int j = 2; j <= QMAX; ++j) {
                 if (j < q) zz[j] -= top * lc[j];
+                if (j == q) zz[j] = 0.0;
+            }
             zstore(i, zz);
         }
     }
-    // Order q -> q+1: new top column from the saved correction in zn[QMAX].
+    // Order q -> q+1: new top column from the saved correction.
     __device__ __forceinline__ void increase_order() {
         double lc[QMAX + 1];
         #pragma unroll
@@ -233,7 +245,7 @@ struct Bdf {
         for (int i = 0; i < N; ++i) {
             double zz[QMAX + 1];
             zload(i, zz);
-            const double nz = A1 * zz[QMAX];
+            const double nz = A1 * ZS(i);
             #pragma unroll
             for (int j = 2; j <= QMAX; ++j) {
                 if (j <= q) zz[j] += lc[j] * nz;
@@ -243,7 +255,6 @@ struct Bdf {
         }
     }
     __device__ __forceinline__ void adjust_order(int dq) {
-        if (q == 2 && dq != 1) return;
         if (dq == 1) increase_order();
         else if (dq == -1) decrease_order();
     }
@@ -603,11 +614,11 @@ struct Bdf {
         for (int i = 0; i < N; ++i) {
             double zz[QMAX + 1];
             zload(i, zz);
+            // l_j = 0 above q (set_bdf), so those columns stay zero
             #pragma unroll
-            for (int j = 0; j <= QMAX; ++j)
-                if (j <= q) zz[j] = fma(lcv[j], acor[i], zz[j]);
-            if (save_acor) zz[QMAX] = acor[i];
+            for (int j = 0; j <= QMAX; ++j) zz[j] = fma(lcv[j], acor[i], zz[j]);
             zstore(i, zz);
+            if (save_acor) ZS(i) = acor[i];
         }
         if (save_acor) saved_tq5 = tq5;
         prepare_next(dsm);
@@ -618,10 +629,11 @@ struct Bdf {
             const double s = (tout - tn) / h;
             #pragma unroll
             for (int i = 0; i < N; ++i) {
-                double v = zrow(q, i);
+                // Horner from column qmax: the zero columns above q leave
+                // v = Z(q) exactly where the sum of the order-q history starts
+                double v = Z(QMAX, i);
                 #pragma unroll
-                for (int j = QMAX - 1; j >= 0; --j)
-                    if (j < q) v = fma(s, v, Z(j, i));
+                for (int j = QMAX - 1; j >= 0; --j) v = fma(s, v, Z(j, i));
                 yout[i] = v;
             }
             return BDF_OK;
@@ -656,7 +668,7 @@ struct Bdf {
                     const double cquot = (tq5 / saved_tq5) * ipow(h / TAU(2), L);
                     double tmp[N];
                     #pragma unroll
-                    for (int i = 0; i < N; ++i) tmp[i] = acor[i] - cquot * Z(QMAX, i);
+                    for (int i = 0; i < N; ++i) tmp[i] = acor[i] - cquot * ZS(i);
                     const double dup = wrms(tmp) * tq3;
                     etaqp1 = 1.0 / (root_n(10.0 * dup, L + 1) + 1e-6);
                 }
@@ -672,7 +684,7 @@ struct Bdf {
                     e = etaqp1;
                     n_qprime = q + 1;
                     #pragma unroll
-                    for (int i = 0; i < N; ++i) Z(QMAX, i) = acor[i];
+                    for (int i = 0; i < N; ++i) ZS(i) = acor[i];
                 }
             }
             n_eta = e < 1.5 ? 1.0 : fmin(e, etamax);
@@ -694,7 +706,11 @@ struct Bdf {
         maxsteps = max_steps;
         c = BdfCounters{};
         #pragma unroll
-        for (int i = 0; i < N; ++i) Z(0, i) = y[i];
+        for (int i = 0; i < N; ++i) {
+            Z(0, i) = y[i];
+            #pragma unroll
+            for (int j = 2; j <= QMAX; ++j) Z(j, i) = 0.0;
+        }
         tn = 0;
         phase = PH_F0;
         return BDF_RUNNING;

@@ -168,10 +168,11 @@ struct Workspace {
 };
 
 // The per-lane BDF slab (Nordsieck history, step-size history, BDF
-// coefficients) of a wave: (6N + 13) doubles x 64 lanes. It lives in LDS
-// while that allows four waves per CU (31 KB per wave for H2), otherwise in a
-// per-wave tile of the workspace (123 KB per wave for DME, N = 38).
-constexpr int kSlab = 6 * (Topo::NS - 1) + 13;
+// coefficients, saved correction) of a wave: (7N + 13) doubles x 64 lanes.
+// It lives in LDS while that allows four waves per CU (35 KB per wave for
+// H2), otherwise in a per-wave tile of the workspace (140 KB per wave for
+// DME, N = 38).
+constexpr int kSlab = Bdf<Topo::NS - 1, ChemSys<Topo>>::SLAB;
 constexpr bool kSlabLds = kSlab * 64 * 8 <= 40 * 1024;
 
 inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
