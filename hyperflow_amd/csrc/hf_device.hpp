@@ -367,6 +367,26 @@ __device__ __forceinline__ void species_h_g(const MechData<T>& d, double t, doub
     }
 }
 
+// The equilibrium-constant factors of every species, exp(g_k/RT) and its
+// inverse, and the standard concentration c0 = p0 / (R T) and its inverse:
+// 1/Kc of a reaction is the product of its products' factors, its
+// reactants' inverse factors and c0^(-dn) (dn = products - reactants), the
+// exponential of sum nu g/RT - dn ln c0 that it equals. One exponential and
+// one division per species replace an exponential per reversible reaction.
+template <class T>
+struct GibbsFactors {
+    double eg[T::NS], ieg[T::NS], c0, ic0;
+    __device__ __forceinline__ GibbsFactors(const double* grt, double t, double it) {
+        #pragma unroll
+        for (int k = 0; k < T::NS; ++k) {
+            eg[k] = exp(grt[k]);
+            ieg[k] = 1.0 / eg[k];
+        }
+        c0 = (kP0 / kR) * it;
+        ic0 = (kR / kP0) * t;
+    }
+};
+
 // Rate constants of reaction I: forward kf (including [M] for three-body and
 // the falloff blend), and 1/Kc. Optional derivatives for the Jacobian:
 // dlnkf_dT (at fixed [M]), dkf_dM, dln_rKc_dT.
@@ -375,7 +395,8 @@ struct RateI {
     double kf, rkc, dlnkf_dT, dkf_dM, dlnrkc_dT;
 
     __device__ __forceinline__ RateI(const MechData<T>& d, double t, double logt, double it, double logc0,
-                                     const double* hrt, const double* grt, double Mc, double lnA_shift) {
+                                     const double* hrt, const double* grt, const GibbsFactors<T>& gf, double Mc,
+                                     double lnA_shift) {
         constexpr int kind = T::kind[I];
         double kinf = exp(d.lnA[I] + lnA_shift + d.b[I] * logt - d.EaR[I] * it);
         dlnkf_dT = 0;
@@ -430,16 +451,20 @@ struct RateI {
                 dkf_dM = k0 * F * opr * (opr + g_lpr);
             }
         }
-        // 1/Kc = exp(sum nu g/RT - dn ln(p0/RT))
+        // 1/Kc = exp(sum nu g/RT - dn ln(p0/RT)), as the product of the
+        // species' factors (products and reactants alternate, so the partial
+        // products stay near the result's magnitude) and c0^(-dn)
         if constexpr (T::rev[I]) {
-            double dg = 0, dh = 0;
+            double dh = 0, r_kc = 1.0;
             int dn = 0;
             static_for<0, T::NRMAX>([&](auto j) HF_AI {
                 constexpr int r = T::reac[I][j], p = T::prod[I][j];
-                if constexpr (r >= 0) { dg -= grt[r]; dn -= 1; if constexpr (DERIV) dh -= hrt[r]; }
-                if constexpr (p >= 0) { dg += grt[p]; dn += 1; if constexpr (DERIV) dh += hrt[p]; }
+                if constexpr (p >= 0) { r_kc *= gf.eg[p]; dn += 1; if constexpr (DERIV) dh += hrt[p]; }
+                if constexpr (r >= 0) { r_kc *= gf.ieg[r]; dn -= 1; if constexpr (DERIV) dh -= hrt[r]; }
             });
-            rkc = exp(dg - dn * logc0);
+            for (int m = 0; m < dn; ++m) r_kc *= gf.ic0;
+            for (int m = 0; m < -dn; ++m) r_kc *= gf.c0;
+            rkc = r_kc;
             if constexpr (DERIV) dlnrkc_dT = (dn - dh) * it;
         } else {
             rkc = 0;
@@ -468,6 +493,7 @@ __device__ __forceinline__ void production_rates(const MechData<T>& d, double t,
     const double logc0 = kLnP0R - logt;
     double hrt[NS], grt[NS];
     species_h_g(d, t, logt, hrt, grt);
+    const GibbsFactors<T> gf(grt, t, it);
     double Ctot = 0;
     #pragma unroll
     for (int k = 0; k < NS; ++k) { Ctot += C[k]; wdot[k] = 0; }
@@ -475,7 +501,7 @@ __device__ __forceinline__ void production_rates(const MechData<T>& d, double t,
         constexpr int I = decltype(ic)::value;
         double Mc = 0;
         if constexpr (T::kind[I] != 0) Mc = third_body<T, I>(d, C, Ctot);
-        const RateI<T, I, false> k(d, t, logt, it, logc0, hrt, grt, Mc, lnA_shift ? lnA_shift[I] : 0.0);
+        const RateI<T, I, false> k(d, t, logt, it, logc0, hrt, grt, gf, Mc, lnA_shift ? lnA_shift[I] : 0.0);
         double fwd = k.kf, bwd = k.kf * k.rkc;
         static_for<0, T::NRMAX>([&](auto j) HF_AI {
             constexpr int r = T::reac[I][j], p = T::prod[I][j];
@@ -512,6 +538,7 @@ __device__ __forceinline__ void chem_jacobian(const MechData<T>& d, double t, do
     const double logc0 = kLnP0R - logt;
     double hrt[NS], grt[NS], C[NS], wd[NS], dwdT[NS], dwdCL[N];
     species_h_g(d, t, logt, hrt, grt);
+    const GibbsFactors<T> gf(grt, t, it);
     double Ctot = 0;
     #pragma unroll
     for (int k = 0; k < NS; ++k) {
@@ -531,7 +558,7 @@ __device__ __forceinline__ void chem_jacobian(const MechData<T>& d, double t, do
         constexpr int I = decltype(ic)::value;
         double Mc = 0;
         if constexpr (T::kind[I] != 0) Mc = third_body<T, I>(d, C, Ctot);
-        const RateI<T, I, true> k(d, t, logt, it, logc0, hrt, grt, Mc, lnA_shift ? lnA_shift[I] : 0.0);
+        const RateI<T, I, true> k(d, t, logt, it, logc0, hrt, grt, gf, Mc, lnA_shift ? lnA_shift[I] : 0.0);
         const double kr = k.kf * k.rkc;
         double pf = 1.0, pb = 1.0;
         static_for<0, T::NRMAX>([&](auto j) HF_AI {
