@@ -86,6 +86,46 @@ __device__ __forceinline__ void static_for(F&& f) {
 
 __device__ __forceinline__ bool finite(double x) { return __builtin_isfinite(x); }
 
+// ----------------------------------------------------------------- logarithms
+// Double-precision ln x for the rate expressions (ln T, and log10 in the
+// Troe falloff), written for the FP64 pipe: no table lookups, two thirds of
+// the library routine's instructions, within 2 units in the last place.
+// ln 2 split for exact e ln 2 products (Cody and Waite): the high part has
+// 29 significant bits, so e * kLn2Hi is exact for |e| < 2^24.
+constexpr double kLn2Hi = 0x1.62e42ffp-1;
+constexpr double kLn2Lo = -4.2009150726810846e-11;   // ln 2 - kLn2Hi
+constexpr double kInvLn10 = 0.4342944819032518;
+
+// ln x: x = f 2^e with f in [sqrt(1/2), sqrt(2)); ln f = 2 atanh(s),
+// s = (f - 1) / (f + 1), |s| <= 0.1716, by its series 2 s sum s^(2j) / (2j + 1)
+// to j = 10 (remainder below 1e-18 relative); e ln 2 added in two parts.
+// x = 0 gives -inf, +inf gives +inf, x < 0 or NaN gives NaN.
+__device__ __forceinline__ double log_fast(double x) {
+    double f = __builtin_amdgcn_frexp_mant(x);
+    int e = __builtin_amdgcn_frexp_exp(x);
+    const bool low = f < 0.70710678118654752440;
+    f = low ? f + f : f;
+    e = low ? e - 1 : e;
+    const double s = (f - 1.0) / (f + 1.0);
+    const double z = s * s;
+    double p = 2.0 / 21.0;
+    p = fma(p, z, 2.0 / 19.0);
+    p = fma(p, z, 2.0 / 17.0);
+    p = fma(p, z, 2.0 / 15.0);
+    p = fma(p, z, 2.0 / 13.0);
+    p = fma(p, z, 2.0 / 11.0);
+    p = fma(p, z, 2.0 / 9.0);
+    p = fma(p, z, 2.0 / 7.0);
+    p = fma(p, z, 2.0 / 5.0);
+    p = fma(p, z, 2.0 / 3.0);
+    const double de = (double)e;
+    const double r = fma(de, kLn2Hi, fma(de, kLn2Lo, fma(s * z, p, s + s)));
+    if (x > 0 && x < __builtin_inf()) return r;
+    return x == 0 ? -__builtin_inf() : (x == __builtin_inf() ? x : __builtin_nan(""));
+}
+
+__device__ __forceinline__ double log10_fast(double x) { return log_fast(x) * kInvLn10; }
+
 // NASA7 coefficients of species k for temperature t (low range for t <= Tmid).
 // Both sets are read with uniform loads and the per-lane choice is a value
 // select, so the coefficients never go through a per-lane pointer.
@@ -423,8 +463,8 @@ struct RateI {
                     dfc += d.troeT2[I] * it * it * e2;
                 }
                 const double fcc = fmax(fc, kSmall);
-                const double lc = log10(fcc);
-                const double lpr = log10(fmax(pr, kSmall));
+                const double lc = log10_fast(fcc);
+                const double lpr = log10_fast(fmax(pr, kSmall));
                 const double c = -0.4 - 0.67 * lc, n = 0.75 - 1.27 * lc;
                 const double u = lpr + c;
                 const double den = n - 0.14 * u;
@@ -489,7 +529,7 @@ template <class T>
 __device__ __forceinline__ void production_rates(const MechData<T>& d, double t, const double* C, double* wdot,
                                                  const double* lnA_shift = nullptr, double* qout = nullptr) {
     constexpr int NS = T::NS;
-    const double logt = log(t), it = 1.0 / t;
+    const double logt = log_fast(t), it = 1.0 / t;
     const double logc0 = kLnP0R - logt;
     double hrt[NS], grt[NS];
     species_h_g(d, t, logt, hrt, grt);
@@ -534,7 +574,7 @@ __device__ __forceinline__ void chem_jacobian(const MechData<T>& d, double t, do
                                               MT* J, double* f, const double* lnA_shift = nullptr,
                                               double* fT = nullptr, double* qout = nullptr) {
     constexpr int NS = T::NS, N = NS - 1, L = NS - 1;
-    const double logt = log(t), it = 1.0 / t;
+    const double logt = log_fast(t), it = 1.0 / t;
     const double logc0 = kLnP0R - logt;
     double hrt[NS], grt[NS], C[NS], wd[NS], dwdT[NS], dwdCL[N];
     species_h_g(d, t, logt, hrt, grt);
