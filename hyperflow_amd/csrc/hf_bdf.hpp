@@ -25,37 +25,9 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "hf_device.hpp"
+
 namespace hf {
-
-// x^n for n = 0..7 (by the bits of n).
-__device__ __forceinline__ double ipow(double x, int n) {
-    const double x2 = x * x, x4 = x2 * x2;
-    return ((n & 1) ? x : 1.0) * ((n & 2) ? x2 : 1.0) * ((n & 4) ? x4 : 1.0);
-}
-
-// x^(1/n) for n = 1..7, the root in CVODE's step-size ratios
-// eta = 1 / ((c dsm)^(1/n) + 1e-6). x = m 2^(n k) with k near log2(x) / n
-// (exact scalings), so m lies within a few powers of two of 1; a
-// single-precision estimate of m^(1/n) from the hardware log2 and exp2
-// (relative error below 1e-5) is refined by two double-precision Newton
-// steps on r^n = m, r <- r (1 + (m / r^n - 1) / n), whose fixed point does
-// not depend on the rounding of 1/n. The result is within 2 units in the
-// last place of the exact root (pow(x, 1.0 / n) is up to about 100 from it
-// at the ends of the double range, where the rounding of 1.0 / n shows), in
-// a quarter of pow's instructions; every trip of a batched wave evaluates
-// several of these. x = 0 gives 0, +inf gives +inf, a negative x or NaN
-// gives NaN, as pow does.
-__device__ __forceinline__ double root_n(double x, int n) {
-    if (!(x > 0) || x == __builtin_inf()) return x == 0 || x == __builtin_inf() ? x : __builtin_nan("");
-    const float rn = __builtin_amdgcn_rcpf((float)n);
-    const int k = (int)((float)__builtin_amdgcn_frexp_exp(x) * rn);
-    const double m = __builtin_ldexp(x, -n * k);
-    double r = (double)__builtin_amdgcn_exp2f(__builtin_amdgcn_logf((float)m) * rn);
-    const double c = (double)rn;
-    #pragma unroll
-    for (int it = 0; it < 2; ++it) r = fma(r * c, m / ipow(r, n) - 1.0, r);
-    return __builtin_ldexp(r, k);
-}
 
 struct BdfCounters {
     int steps = 0, rhs = 0, jac = 0, lu = 0, newton = 0, ncf = 0, netf = 0;

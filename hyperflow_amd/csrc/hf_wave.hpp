@@ -153,8 +153,8 @@ struct WaveGas {
                 const double A = d.troeA[i];
                 double fc = (1.0 - A) * exp(-t * d.troeRT3[i]) + A * exp(-t * d.troeRT1[i]);
                 if (troe == 4) fc += exp(-d.troeT2[i] * it);
-                const double lc = log10(fmax(fc, kSmall));
-                const double lpr = log10(fmax(pr, kSmall));
+                const double lc = log10_fast(fmax(fc, kSmall));
+                const double lpr = log10_fast(fmax(pr, kSmall));
                 const double c = -0.4 - 0.67 * lc, n = 0.75 - 1.27 * lc;
                 const double u = lpr + c;
                 const double f1 = u / (n - 0.14 * u);
@@ -189,7 +189,7 @@ struct WaveGas {
         double tt = t;
         if (!temperature(Yk, e, tt)) return false;
         t = tt;
-        const double logt = log(t), it = 1.0 / t;
+        const double logt = log_fast(t), it = 1.0 / t;
         const double logc0 = kLnP0R - logt;
         double C = 0;
         if (sp) {
@@ -284,8 +284,8 @@ struct WaveGas {
                     dfc += d.troeT2[i] * it * it * e2;
                 }
                 const double fcc = fmax(fc, kSmall);
-                const double lc = log10(fcc);
-                const double lpr = log10(fmax(pr, kSmall));
+                const double lc = log10_fast(fcc);
+                const double lpr = log10_fast(fmax(pr, kSmall));
                 const double c = -0.4 - 0.67 * lc, n = 0.75 - 1.27 * lc;
                 const double u = lpr + c;
                 const double den = n - 0.14 * u;
@@ -366,7 +366,7 @@ struct WaveGas {
         double tt = t;
         if (!temperature(Yk, e, tt)) return false;
         t = tt;
-        const double logt = log(t), it = 1.0 / t;
+        const double logt = log_fast(t), it = 1.0 / t;
         const double logc0 = kLnP0R - logt;
         double C = 0, ek = 0, cvk = 0;
         if (sp) {
@@ -909,7 +909,7 @@ struct WaveBdf {
             if (nef >= 7) return false;
             etamax = 1;
             if (nef <= 3) {
-                eta = 1.0 / (pow(6.0 * dsm, 1.0 / L) + 1e-6);
+                eta = 1.0 / (root_n(6.0 * dsm, L) + 1e-6);
                 eta = fmax(0.1, eta);
                 if (nef >= 2) eta = fmin(eta, 0.2);
                 rescale();
@@ -964,19 +964,19 @@ struct WaveBdf {
             etamax = 10.0;
             return;
         }
-        const double etaq = 1.0 / (pow(6.0 * dsm, 1.0 / L) + 1e-6);
+        const double etaq = 1.0 / (root_n(6.0 * dsm, L) + 1e-6);
         double e = etaq;
         if (qwait == 0) {
             qwait = 2;
             double etaqm1 = 0, etaqp1 = 0;
             if (q > 1) {
                 const double ddn = wrms(zq(q)) * tq1;
-                etaqm1 = 1.0 / (pow(6.0 * ddn, 1.0 / q) + 1e-6);
+                etaqm1 = 1.0 / (root_n(6.0 * ddn, q) + 1e-6);
             }
             if (q != QMAX && saved_tq5 != 0) {
-                const double cquot = (tq5 / saved_tq5) * pow(h / TAU(2), (double)L);
+                const double cquot = (tq5 / saved_tq5) * ipow(h / TAU(2), L);
                 const double dup = wrms(acor - cquot * z[QMAX]) * tq3;
-                etaqp1 = 1.0 / (pow(10.0 * dup, 1.0 / (L + 1)) + 1e-6);
+                etaqp1 = 1.0 / (root_n(10.0 * dup, L + 1) + 1e-6);
             }
             const double etam = fmax(etaqm1, fmax(etaq, etaqp1));
             if (etam < 1.5) {

@@ -1,5 +1,5 @@
-"""The step-size root of the GPU BDF integrator (root_n in
-hyperflow_amd/csrc/hf_bdf.hpp), restated in numpy: x^(1/n) from a range
+"""The step-size root of the GPU BDF integrators (root_n in
+hyperflow_amd/csrc/hf_device.hpp), restated in numpy: x^(1/n) from a range
 reduction x = m 2^(n k), a single-precision estimate of m^(1/n) and two
 double-precision Newton steps on r^n = m. The device estimate comes from the
 hardware log2/exp2 (relative error below 1e-5); here the estimate is
