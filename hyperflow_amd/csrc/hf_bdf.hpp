@@ -61,9 +61,15 @@ struct Bdf {
     double rtol, atol;
     int q, qprime, L, qwait, nst, nstlp, nstlj;
     bool force_setup, jfresh;
-    // Saved Jacobian and factored Newton matrix: element k at [k * ZSTRIDE],
-    // the callers point them at this lane's column of a per-wave
-    // [element][64 lanes] tile, so the offsets are compile-time constants.
+    // Saved Jacobian and factored Newton matrix, in a per-wave tile of
+    // global memory: SV consecutive elements of a lane form one 16-byte
+    // vector, element k at [SI(k)] from the lane's base (the callers point
+    // jsave and lu at tile + lane * SV), so the offsets are compile-time
+    // constants and a row is read with 16-byte loads, each coalesced across
+    // the wave.
+    static constexpr int SV = 16 / sizeof(MT);
+    static __device__ __forceinline__ constexpr int SI(int k) { return (k / SV) * (ZSTRIDE * SV) + k % SV; }
+    static_assert((N * N) % SV == 0, "a lane's N x N slot must fill whole vectors (the tile holds N N ZSTRIDE)");
     MT* jsave = nullptr;
     MT* lu = nullptr;
     BdfCounters c;
@@ -285,11 +291,16 @@ struct Bdf {
     // hides a pointer's provenance from the optimiser so that every use
     // reloads from memory instead of keeping 64 matrix values (128 VGPRs in
     // FP64) live in registers across the rate evaluations around it.
+    // The slots are in global memory: addressing them in the global address
+    // space makes the accesses global (not flat) loads and stores.
+    typedef __attribute__((address_space(1))) MT GMT;
     template <class P>
     static __device__ __forceinline__ P* opaque(P* p) {
         asm volatile("" : "+v"(p));
-        return p;
+        return static_cast<P*>(__builtin_assume_aligned(p, 16));
     }
+    static __device__ __forceinline__ const GMT* gslot(const MT* p) { return (const GMT*)opaque(p); }
+    static __device__ __forceinline__ GMT* gslot(MT* p) { return (GMT*)opaque(p); }
 
     // Small systems are unrolled into registers; large ones (N > 16) run
     // rolled loops over per-lane arrays.
@@ -300,7 +311,7 @@ struct Bdf {
     // precision (double, or float for the mixed form, where the residuals and
     // the corrections stay double).
     __device__ __forceinline__ void lu_solve(double* bd) const {
-        const MT* lp = opaque(lu);
+        const GMT* lp = gslot(lu);
         MT b[N];
         #pragma unroll
         for (int i = 0; i < N; ++i) b[i] = (MT)bd[i];
@@ -308,28 +319,28 @@ struct Bdf {
             #pragma unroll
             for (int k = 0; k < N; ++k)
                 #pragma unroll
-                for (int i = k + 1; i < N; ++i) b[i] = fma(-lp[(i * N + k) * ZSTRIDE], b[k], b[i]);
+                for (int i = k + 1; i < N; ++i) b[i] = fma(-lp[SI(i * N + k)], b[k], b[i]);
             #pragma unroll
             for (int i = N - 1; i >= 0; --i) {
                 MT s = b[i];
                 #pragma unroll
-                for (int j = i + 1; j < N; ++j) s = fma(-lp[(i * N + j) * ZSTRIDE], b[j], s);
-                b[i] = s * lp[(i * N + i) * ZSTRIDE];
+                for (int j = i + 1; j < N; ++j) s = fma(-lp[SI(i * N + j)], b[j], s);
+                b[i] = s * lp[SI(i * N + i)];
             }
         } else {
             #pragma unroll 1
             for (int i = 1; i < N; ++i) {
                 MT s = b[i];
                 #pragma unroll 1
-                for (int k = 0; k < i; ++k) s = fma(-lp[(i * N + k) * ZSTRIDE], b[k], s);
+                for (int k = 0; k < i; ++k) s = fma(-lp[SI(i * N + k)], b[k], s);
                 b[i] = s;
             }
             #pragma unroll 1
             for (int i = N - 1; i >= 0; --i) {
                 MT s = b[i];
                 #pragma unroll 1
-                for (int j = i + 1; j < N; ++j) s = fma(-lp[(i * N + j) * ZSTRIDE], b[j], s);
-                b[i] = s * lp[(i * N + i) * ZSTRIDE];
+                for (int j = i + 1; j < N; ++j) s = fma(-lp[SI(i * N + j)], b[j], s);
+                b[i] = s * lp[SI(i * N + i)];
             }
         }
         #pragma unroll
@@ -338,24 +349,24 @@ struct Bdf {
 
     // Row i of I - gamma J eliminated against the finished rows 0..i-1 of
     // `lu` and stored; returns false on a zero or non-finite pivot.
-    __device__ __forceinline__ bool lu_row(int i, MT g, const MT* js) {
+    __device__ __forceinline__ bool lu_row(int i, MT g, const GMT* js) {
         MT row[N];
         #pragma unroll
-        for (int j = 0; j < N; ++j) row[j] = (i == j ? MT(1) : MT(0)) - g * js[(i * N + j) * ZSTRIDE];
-        const MT* up = opaque(lu);
+        for (int j = 0; j < N; ++j) row[j] = (i == j ? MT(1) : MT(0)) - g * js[SI(i * N + j)];
+        const GMT* up = gslot(lu);
         #pragma unroll 1
         for (int k = 0; k < i; ++k) {
-            const MT l = row[k] * up[(k * N + k) * ZSTRIDE];
+            const MT l = row[k] * up[SI(k * N + k)];
             row[k] = l;
             #pragma unroll 1
-            for (int j = k + 1; j < N; ++j) row[j] = fma(-l, up[(k * N + j) * ZSTRIDE], row[j]);
+            for (int j = k + 1; j < N; ++j) row[j] = fma(-l, up[SI(k * N + j)], row[j]);
         }
         const MT p = row[i];
         const bool ok = fabs(p) > __FLT_MIN__ && __builtin_isfinite(p);
         row[i] = MT(1) / p;
-        MT* wp = opaque(lu);
+        GMT* wp = gslot(lu);
         #pragma unroll
-        for (int j = 0; j < N; ++j) wp[(i * N + j) * ZSTRIDE] = row[j];
+        for (int j = 0; j < N; ++j) wp[SI(i * N + j)] = row[j];
         return ok;
     }
 
@@ -377,8 +388,9 @@ struct Bdf {
                 MT J[N * N];
                 double fj[N];
                 if (!sys.jac(y, J, fj)) return false;
+                GMT* jw = gslot(jsave);
                 #pragma unroll
-                for (int i = 0; i < N * N; ++i) jsave[i * ZSTRIDE] = J[i];
+                for (int i = 0; i < N * N; ++i) jw[SI(i)] = J[i];
             } else {
                 // cvLsDenseDQJac: column j from f(y + inc_j e_j) with
                 // inc_j = max(sqrt(uround) |y_j|, minInc / ewt_j),
@@ -398,9 +410,9 @@ struct Bdf {
                     if (!sys.rhs(yj, fj)) return false;
                     yj[j] = ysave;
                     const double rinc = 1.0 / inc;
-                    MT* js = opaque(jsave);
+                    GMT* js = gslot(jsave);
                     #pragma unroll
-                    for (int i = 0; i < N; ++i) js[(i * N + j) * ZSTRIDE] = (MT)((fj[i] - f[i]) * rinc);
+                    for (int i = 0; i < N; ++i) js[SI(i * N + j)] = (MT)((fj[i] - f[i]) * rinc);
                 }
             }
             nstlj = nst;
@@ -413,7 +425,7 @@ struct Bdf {
         crate = 1;
         nstlp = nst;
         const MT g = (MT)gamma;
-        const MT* js = opaque(jsave);
+        const GMT* js = gslot(jsave);
         bool ok = true;
         if constexpr (SMALL) {
             // The finished rows stay in registers for the rows below them
@@ -424,7 +436,7 @@ struct Bdf {
             for (int i = 0; i < N; ++i) {
                 MT row[N];
                 #pragma unroll
-                for (int j = 0; j < N; ++j) row[j] = (i == j ? MT(1) : MT(0)) - g * js[(i * N + j) * ZSTRIDE];
+                for (int j = 0; j < N; ++j) row[j] = (i == j ? MT(1) : MT(0)) - g * js[SI(i * N + j)];
                 #pragma unroll
                 for (int k = 0; k < i; ++k) {
                     const MT l = row[k] * fin[k * N + k];
@@ -435,11 +447,11 @@ struct Bdf {
                 const MT p = row[i];
                 ok = ok && fabs(p) > __FLT_MIN__ && __builtin_isfinite(p);
                 row[i] = MT(1) / p;
-                MT* wp = opaque(lu);
+                GMT* wp = gslot(lu);
                 #pragma unroll
                 for (int j = 0; j < N; ++j) {
                     fin[i * N + j] = row[j];
-                    wp[(i * N + j) * ZSTRIDE] = row[j];
+                    wp[SI(i * N + j)] = row[j];
                 }
             }
         } else {

@@ -82,8 +82,8 @@ struct CellRun {
     Bdf<N, ChemSys<T>, MT, 64, ZAS> bdf;
     double y[N];
 
-    // jsave/lu: this lane's saved-Jacobian and LU-factor slots (element k at
-    // [k * 64], a per-wave [element][lane] tile).
+    // jsave/lu: this lane's saved-Jacobian and LU-factor slots in a per-wave
+    // tile (element k at Bdf::SI(k) from tile + lane * Bdf::SV).
     __device__ __forceinline__ CellRun(const MechData<T>& d, double* zs, const ChemParams& p, MT* jsave, MT* lu)
         : sys{d, 1.0, 0.0, 1000.0, nullptr} {
         bdf.zs = zs;

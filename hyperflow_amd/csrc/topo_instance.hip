@@ -162,9 +162,10 @@ struct Workspace {
     unsigned* offs;
     unsigned* cursor;
     unsigned* head;
-    void* jsave;          // saved Jacobians: one [element][64 lanes] tile per wave
+    void* jsave;          // saved Jacobians: one N x N x 64 tile per wave, [element / SV][lane][SV]
+                          // (SV elements of a lane in 16 bytes; Bdf::SI)
     void* lu;             // factored Newton matrices, same layout
-    void* zs;             // Nordsieck slabs, same layout, when they do not fit in LDS
+    void* zs;             // Nordsieck slabs, [element][64 lanes] per wave, when they do not fit in LDS
 };
 
 // The per-lane BDF slab (Nordsieck history, step-size history, BDF
@@ -332,8 +333,9 @@ chem_lane_kernel(const MechData<TT>* __restrict__ dm, CellMap cm, long ncell,
     }
     const int lane = threadIdx.x;
     const long tile = (long)blockIdx.x * N * N * 64;
-    CellRun<TT, MT, kSlabLds ? 3 : 1> run(*dm, zbase + lane, p, static_cast<MT*>(w.jsave) + tile + lane,
-                                        static_cast<MT*>(w.lu) + tile + lane);
+    constexpr int sv = 16 / sizeof(MT);   // the slot's vector width (Bdf::SV)
+    CellRun<TT, MT, kSlabLds ? 3 : 1> run(*dm, zbase + lane, p, static_cast<MT*>(w.jsave) + tile + lane * sv,
+                                        static_cast<MT*>(w.lu) + tile + lane * sv);
     BdfCounters tot;
     int max_steps_cell = 0;
     long cell = -1;
