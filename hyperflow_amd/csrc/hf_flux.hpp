@@ -68,9 +68,12 @@ __device__ __forceinline__ void flux_tile(const MechData<T>& d, const hf_box* __
     constexpr int GX = FT::GX, GY = FT::GY, BX = FT::BX, RPW = FT::RPW;
     const int bi = tiles[3 * t], ti = tiles[3 * t + 1], tj = tiles[3 * t + 2];
     const hf_box B = boxes[bi];
-    // the box's own arrays when the caller passed a pointer per box
-    const double* __restrict__ src = in_ptrs ? in_ptrs[bi] : Uin;
-    double* __restrict__ dst = out_ptrs ? out_ptrs[bi] : Uout;
+    // the box's own arrays when the caller passed a pointer per box, addressed
+    // in the global address space (global, not flat, loads and stores)
+    typedef const __attribute__((address_space(1))) double GIn;
+    typedef __attribute__((address_space(1))) double GOut;
+    const GIn* __restrict__ src = (const GIn*)(in_ptrs ? in_ptrs[bi] : Uin);
+    GOut* __restrict__ dst = (GOut*)(out_ptrs ? out_ptrs[bi] : Uout);
     const int i0 = ti * TX, j0 = tj * TY;
     const int lane = threadIdx.x;
     const int lx = lane % BX, ly = lane / BX;
@@ -83,10 +86,17 @@ __device__ __forceinline__ void flux_tile(const MechData<T>& d, const hf_box* __
         const int gj = e / GX, gi = e - gj * GX;
         const int ci = i0 - 2 + gi, cj = jlo + gj;
         const bool in = ci < B.nx + 2 && cj < B.ny + 2;
-        const double* __restrict__ q = src + B.in_offset + (int64_t)(cj + 2) * B.in_pitch + (ci + 2);
+        // cells past the box's last ghost layer stage zeros; their loads read
+        // the last ghost cell instead (an address inside the array), so every
+        // lane issues the same unconditional loads
+        const int cil = min(ci, B.nx + 1), cjl = min(cj, B.ny + 1);
+        const GIn* __restrict__ q = src + B.in_offset + (int64_t)(cjl + 2) * B.in_pitch + (cil + 2);
         double v[NV];
         #pragma unroll
-        for (int n = 0; n < NV; ++n) v[n] = in ? q[n * B.in_plane] : 0.0;
+        for (int n = 0; n < NV; ++n) {
+            const double x = q[n * B.in_plane];
+            v[n] = in ? x : 0.0;
+        }
         #pragma unroll
         for (int n = 0; n < NV; ++n) s_in[n * GX * GY + e] = v[n];
     }
@@ -214,7 +224,7 @@ __device__ __forceinline__ void flux_tile(const MechData<T>& d, const hf_box* __
             // computed by both from the same staged cells: identical values)
             if (fout) {
                 const hf_face_fluxes& O = fout[bi];
-                double* fdst = dir == 0 ? O.fx : O.fy;
+                GOut* fdst = (GOut*)(dir == 0 ? O.fx : O.fy);
                 const int64_t off = dir == 0 ? O.fx_offset : O.fy_offset;
                 const int64_t pitch = dir == 0 ? O.fx_pitch : O.fy_pitch;
                 const int64_t plane = dir == 0 ? O.fx_plane : O.fy_plane;
