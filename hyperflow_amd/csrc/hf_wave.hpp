@@ -37,15 +37,40 @@ __device__ __forceinline__ double uniform(double x) {
     const int hi = __builtin_amdgcn_readfirstlane(__double2hiint(x));
     return __hiloint2double(hi, lo);
 }
+// Wave reductions by data-parallel primitives (DPP) on the vector pipe, not
+// by LDS permutes: pairs and quads by quad permutations, then 8 and 16 lanes
+// by half-row and row mirrors, then rows 0 + 1 into row 1 and rows 2 + 3
+// into row 3 (row broadcast of lane 15), and rows 0-1 into rows 2-3 (row
+// broadcast of lane 31): lane 63 holds the total, which every lane reads.
+// A disabled row of a broadcast step keeps `old`. All 64 lanes take part
+// (every call site is in wave-uniform control flow).
+template <int CTRL, int ROWS = 0xF>
+__device__ __forceinline__ double dpp(double x, double old) {
+    const int lo = __builtin_amdgcn_update_dpp(__double2loint(old), __double2loint(x), CTRL, ROWS, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(__double2hiint(old), __double2hiint(x), CTRL, ROWS, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
 __device__ __forceinline__ double wave_sum(double x) {
-    #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
-    return uniform(x);
+    x += dpp<0xB1>(x, 0.0);           // quad_perm [1, 0, 3, 2]
+    x += dpp<0x4E>(x, 0.0);           // quad_perm [2, 3, 0, 1]
+    x += dpp<0x141>(x, 0.0);          // row_half_mirror
+    x += dpp<0x140>(x, 0.0);          // row_mirror
+    x += dpp<0x142, 0xA>(x, 0.0);     // row_bcast:15 into rows 1 and 3
+    x += dpp<0x143, 0xC>(x, 0.0);     // row_bcast:31 into rows 2 and 3
+    const int lo = __builtin_amdgcn_readlane(__double2loint(x), 63);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(x), 63);
+    return __hiloint2double(hi, lo);
 }
 __device__ __forceinline__ double wave_max(double x) {
-    #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) x = fmax(x, __shfl_xor(x, off, 64));
-    return uniform(x);
+    x = fmax(x, dpp<0xB1>(x, x));
+    x = fmax(x, dpp<0x4E>(x, x));
+    x = fmax(x, dpp<0x141>(x, x));
+    x = fmax(x, dpp<0x140>(x, x));
+    x = fmax(x, dpp<0x142, 0xA>(x, x));
+    x = fmax(x, dpp<0x143, 0xC>(x, x));
+    const int lo = __builtin_amdgcn_readlane(__double2loint(x), 63);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(x), 63);
+    return __hiloint2double(hi, lo);
 }
 // value of x in lane l (l wave-uniform)
 __device__ __forceinline__ double lane_of(double x, int l) {
