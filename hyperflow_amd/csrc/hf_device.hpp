@@ -443,9 +443,10 @@ __device__ __forceinline__ void species_h_g(const MechData<T>& d, double t, doub
 // reactants' inverse factors and c0^(-dn) (dn = products - reactants), the
 // exponential of sum nu g/RT - dn ln c0 that it equals. One exponential and
 // one division per species replace an exponential per reversible reaction.
-// |g/RT| stays below about 300 above 100 K for these mechanisms' species, so
-// every factor and partial product is a finite double there (the NASA fits
-// hold from 200 or 300 K).
+// Above 200 K |g/RT| stays below 170 for H2_Li's species and below 325 for
+// dme_sk39's, so a product-reactant pair contributes at most e^650 and
+// every partial product is a finite double (the NASA fits hold from 200 or
+// 300 K).
 template <class T>
 struct GibbsFactors {
     double eg[T::NS], ieg[T::NS], c0, ic0;
