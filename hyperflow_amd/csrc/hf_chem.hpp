@@ -37,7 +37,11 @@ struct ChemSys {
         if (!temp(dd, y, Y)) return false;
         #pragma unroll
         for (int k = 0; k < NS; ++k) C[k] = rho * Y[k] * dd.invW[k];
-        production_rates(dd, t, C, w, shift);
+        // when every active lane of the wave is above every species' Tmid (as
+        // in ignition and burnt gas), the rates take the high-range NASA
+        // coefficients without a per-lane choice (a wave-uniform branch)
+        if (__all(t > dd.tmid) && dd.common_tmid) production_rates<T, true>(dd, t, C, w, shift);
+        else production_rates<T, false>(dd, t, C, w, shift);
         bool ok = true;
         const double rr = 1.0 / rho;
         #pragma unroll

@@ -422,13 +422,21 @@ __device__ __forceinline__ void hllc(const double* l, const double* r, int N, co
 }
 
 // ----------------------------------------------------------------- kinetics
-// Per-species h/RT and g/RT = h/RT - s/R at temperature t.
-template <class T>
+// Per-species h/RT and g/RT = h/RT - s/R at temperature t. HIGH: the
+// caller knows t is above every species' Tmid, so the high-range
+// coefficients are used without a per-lane choice.
+template <class T, bool HIGH = false>
 __device__ __forceinline__ void species_h_g(const MechData<T>& d, double t, double logt, double* hrt, double* grt) {
     const double it = 1.0 / t;
     #pragma unroll
     for (int k = 0; k < T::NS; ++k) {
-        const Nasa7 na = nasa_select(d, k, t);
+        Nasa7 na;
+        if constexpr (HIGH) {
+            #pragma unroll
+            for (int i = 0; i < 7; ++i) na.a[i] = d.hi[k][i];
+        } else {
+            na = nasa_select(d, k, t);
+        }
         const double* a = na.a;
         const double h = fma(t, fma(t, fma(t, fma(t, a[4] * 0.2, a[3] * 0.25), a[2] * (1.0 / 3.0)), a[1] * 0.5), a[0]) + a[5] * it;
         const double s = fma(a[0], logt, fma(t, fma(t, fma(t, fma(t, a[4] * 0.25, a[3] * (1.0 / 3.0)), a[2] * 0.5), a[1]), a[6]));
@@ -559,14 +567,14 @@ __device__ __forceinline__ double third_body(const MechData<T>& d, const double*
 
 // Net production rates wdot (kmol/m^3/s) at (t, C). lnA_shift (optional, per
 // reaction) multiplies both k and k0 of reaction i by exp(lnA_shift[i]).
-template <class T>
+template <class T, bool HIGH = false>
 __device__ __forceinline__ void production_rates(const MechData<T>& d, double t, const double* C, double* wdot,
                                                  const double* lnA_shift = nullptr, double* qout = nullptr) {
     constexpr int NS = T::NS;
     const double logt = log_fast(t), it = 1.0 / t;
     const double logc0 = kLnP0R - logt;
     double hrt[NS], grt[NS];
-    species_h_g(d, t, logt, hrt, grt);
+    species_h_g<T, HIGH>(d, t, logt, hrt, grt);
     const GibbsFactors<T> gf(grt, t, it);
     double Ctot = 0;
     #pragma unroll
