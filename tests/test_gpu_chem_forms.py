@@ -7,7 +7,7 @@ error control: the form the headline throughput is measured on); DME plain
 FP64, batched FP64 and batched mixed (one cell per wave).
 
 Parity is stated at two species floors: Y_k within tol * max(|Y_k|, floor)
-with floor 1e-6 and floor 1e-10 (the floor the baseline states). T within
+with floor 1e-6 and floor 1e-10 (the stated floor of the parity target). T within
 1e-6 relative, element masses within 1e-12.
 
 Reference tolerance (rtol 1e-10, atol 1e-14): the reference algorithm (the
