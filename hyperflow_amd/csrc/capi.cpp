@@ -116,7 +116,7 @@ struct ChemTuner {
     enum { CHUNK, WAVES, BINS, JWAIT, SWAIT, NPARAM };
     static constexpr int kCand = 3;
     // candidate values per parameter, the default first
-    static constexpr int kLane[NPARAM][kCand] = {{8, 4, 16}, {4, 2, 8}, {32, 16, 64}, {16, 8, 32}, {2, 1, 4}};
+    static constexpr int kLane[NPARAM][kCand] = {{8, 4, 16}, {4, 2, 8}, {64, 32, 128}, {16, 8, 32}, {2, 1, 4}};
     static constexpr int kWave[NPARAM][kCand] = {{8, 8, 8}, {8, 4, 16}, {32, 16, 64}, {16, 16, 16}, {2, 2, 2}};
     int64_t size = 0;
     bool wave = false;               // one cell per wave (large mechanisms): only waves and bins matter
