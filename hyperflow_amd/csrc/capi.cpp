@@ -167,9 +167,11 @@ struct ChemTuner {
 };
 
 // Online choice of the 2-D flux tile shape: the first calls of a size class
-// (cell counts within a factor of two) run each shape once and the fastest
-// per cell serves the following calls.
+// (cell counts within a factor of two) run each shape kRounds times in turn,
+// and the shape with the fastest of its runs per cell serves the following
+// calls (two runs each keep a single slow launch from deciding).
 struct FluxTuner {
+    static constexpr int kRounds = 2;
     int64_t size = 0;
     int calls = 0, best = 0, n = 1;
     double per_cell[3] = {};
@@ -180,15 +182,17 @@ struct FluxTuner {
             calls = 0;
             best = 0;
         }
-        return calls < n ? calls : best;
+        return calls < kRounds * n ? calls % n : best;
     }
     void record(double ms, int64_t cells) {
-        if (calls >= n || cells <= 0 || !(ms > 0)) return;
-        per_cell[calls] = ms / (double)cells;
-        if (++calls == n) {
+        if (calls >= kRounds * n || cells <= 0 || !(ms > 0)) return;
+        const int k = calls % n;
+        const double t = ms / (double)cells;
+        per_cell[k] = calls < n ? t : std::min(per_cell[k], t);
+        if (++calls == kRounds * n) {
             best = 0;
-            for (int k = 1; k < n; ++k)
-                if (per_cell[k] < per_cell[best]) best = k;
+            for (int j = 1; j < n; ++j)
+                if (per_cell[j] < per_cell[best]) best = j;
         }
     }
 };
