@@ -819,10 +819,7 @@ def main():
                          "steps_per_cell": st["steps"] / args.dme_cells, "rhs_per_step": st["rhs"] / st["steps"],
                          "jac_per_cell": st["jac"] / args.dme_cells, "lu_per_cell": st["lu"] / args.dme_cells,
                          "roofline": {"bound": "fp64-valu", "achieved": dach, "peak": FP64_PEAK_TFLOPS,
-                                      "unit": "TFLOP/s", "frac": dach / FP64_PEAK_TFLOPS, "flop_per_launch": dflop,
-                                      "note": "hyperflow_amd/flops.py over the kernel's counters (batched: analytic "
-                                              "Jacobian; plain: difference quotients, whose rate evaluations are "
-                                              "counted as RHS)"}}
+                                      "unit": "TFLOP/s", "frac": dach / FP64_PEAK_TFLOPS, "flop_per_launch": dflop}}
             if name == "batched":
                 dme_out = Dd.cpu().numpy()
             log(f"DME chemistry {name}: {dme[name]['value']:.4g} cell-steps/s, kernel {kms[0]:.1f} ms")
@@ -840,9 +837,7 @@ def main():
     if args.shock_tube_steps:
         log(f"1-D shock tube: 2000 cells, {args.shock_tube_steps} Strang steps")
         twall, tdrv, tstate = shock_tube_run(torch, hf, args.shock_tube_steps, 2)
-        tube = {"workload": "1-D reacting shock tube, 2000 cells over 0.1 m (H2_Li), 2000 K / 20 atm equilibrium "
-                            "driver in x < 0.01 m, 300 K / 1 atm stoichiometric H2/air elsewhere; NND + HLLC, Heun, "
-                            "CFL 0.5, Strang chemistry (plain FP64 form, rtol 1e-8 atol 1e-12)",
+        tube = {"workload": "1-D reacting shock tube, 2000 cells (H2_Li), Strang steps with plain FP64 chemistry",
                 "steps": args.shock_tube_steps, "steps_per_s": args.shock_tube_steps / twall,
                 "Mcell_updates_per_s": 2000 * 4 * args.shock_tube_steps / twall / 1e6,
                 "ms_per_step": twall / args.shock_tube_steps * 1e3, "simulated_time_s": tdrv.time,
@@ -854,11 +849,7 @@ def main():
         log(f"2-D channel: 1024 x 256 base level, {args.channel_steps} Strang steps")
         cwall, cdrv, _ = channel_run(torch, hf, args.channel_steps, 1)
         ncell = CHANNEL["nx"] * CHANNEL["ny"]
-        chan = {"workload": "2-D reacting channel, base level only (no refinement): 1024 x 256 cells over 0.2 x 0.05 m "
-                            "(H2_Li), stoichiometric H2/air at 300 K / 1 atm with a 2500 K / 30 atm strip in "
-                            "x < 0.01 m, +-1 % density blocks of 4 x 4 cells at fixed p (seed 3); reflecting left, top "
-                            "and bottom, transmissive right; NND + HLLC, Heun, CFL 0.5, Strang chemistry (batched "
-                            "form, FP32 Newton matrix, rtol 1e-8 atol 1e-12) over every cell",
+        chan = {"workload": "2-D channel, base level only: 1024 x 256 cells (H2_Li), batched chemistry over every cell",
                 "steps": args.channel_steps, "steps_per_s": args.channel_steps / cwall,
                 "Mcell_updates_per_s": ncell * 4 * args.channel_steps / cwall / 1e6,
                 "ms_per_step": cwall / args.channel_steps * 1e3, "simulated_time_s": cdrv.time,
@@ -869,12 +860,7 @@ def main():
     if args.wedge_steps:
         log(f"wedge case: 144 x 64 base level, {args.wedge_steps} Strang steps")
         wwall, wdrv, _ = wedge_run(torch, hf, args.wedge_steps, 1)
-        wedge = {"workload": "oblique-shock wedge case, base level only (no refinement): 144 x 64 cells over "
-                             "2.304 x 1.024 mm from x = -0.128 mm (H2_Li, the reference's cases/teng_case1); "
-                             "premixed H2/air (2:1:3.76) at 958.1 K / 1202900 Pa, 2737.7 m/s at 19 deg below x, in "
-                             "the domain and through the inflow left and top; wedge bottom (outflow before x = 0, "
-                             "slip wall after), transmissive right; NND + HLLC, Heun, CFL 0.6, Strang chemistry "
-                             "(batched form, FP32 Newton matrix, rtol 1e-8 atol 1e-12) over every cell",
+        wedge = {"workload": "wedge case (cases/teng_case1), base level only: 144 x 64 cells, batched chemistry",
                  "steps": args.wedge_steps, "steps_per_s": args.wedge_steps / wwall,
                  "Mcell_updates_per_s": 144 * 64 * 4 * args.wedge_steps / wwall / 1e6,
                  "ms_per_step": wwall / args.wedge_steps * 1e3, "simulated_time_s": wdrv.time,
@@ -886,13 +872,7 @@ def main():
         log(f"wedge case hierarchy: 144 x 64 base, 2 refinement levels, {args.wedge_amr_steps} Strang steps")
         wawall, wadrv, wleaf = wedge_amr_run(torch, hf, args.wedge_amr_steps, 1)
         wg = wadrv.grids()
-        wamr = {"workload": "the wedge case (cases/teng_case1) with its own refinement settings: 144 x 64 base "
-                            "cells, two levels at ratio 2 from ErrorEst's tags at the case's criteria (p jump "
-                            "0.08, T jump 0.06, fuel rate 50 /s) and tag box, blocking 4, max_grid_size 32, "
-                            "n_error_buf 2, regridded every 8 steps; inflow left and top, wedge bottom, "
-                            "transmissive right on every level; CFL 0.6; Strang chemistry (batched form, FP32 "
-                            "Newton matrix, rtol 1e-8 atol 1e-12) over every leaf; each Strang step one replayed HIP graph "
-                            "(hf_amr_strang_step)",
+        wamr = {"workload": "wedge case (cases/teng_case1) on two refinement levels, 144 x 64 base, graph Strang steps",
                 "steps": args.wedge_amr_steps, "steps_per_s": args.wedge_amr_steps / wawall,
                 "ms_per_step": wawall / args.wedge_amr_steps * 1e3,
                 "Mleaf_cell_steps_per_s": wleaf / wawall / 1e6, "simulated_time_s": wadrv.time,
@@ -912,9 +892,7 @@ def main():
         nvx = Cx.shape[0]
         b_i = 5 * nvx * 8 * tnx * tny / t_i / 1e9         # one coarse read + four children written per parent
         b_a = 5 * nvx * 8 * tnx * tny / t_a / 1e9         # four children read + one coarse written
-        xfer = {"workload": f"coarse-to-fine conservative interpolation (ratio 2) of a {tnx} x {tny} H2/air level "
-                            "(smooth: bilinear blend of a 33 x 17 lattice of admissible states, seed 6) and "
-                            "average-down of the result",
+        xfer = {"workload": f"interpolation (ratio 2) of a {tnx} x {tny} H2/air level, then average-down",
                 "interp": {"Mparents_per_s": tnx * tny / t_i / 1e6, "ms_per_call": t_i * 1e3,
                            "limited_parents": int(limited),
                            "hbm_algorithmic": {"achieved": b_i, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -937,15 +915,10 @@ def main():
         cw, ch, _, _, _, _, _ = composite_stages(torch, hf, args.composite_stages)
         tag_ms, tag_n, tagQ, tagdx = tag_run(torch, hf, 5)
         ncells = CHANNEL["nx"] * CHANNEL["ny"] + 512 * 256
-        comp = {"workload": "two-level Euler stages (the same h on both levels, reflux and average-down each "
-                            "stage): the 1024 x 256 channel (H2_Li, reflecting walls) as the coarse level and one "
-                            "interior box of 512 x 256 fine cells at ratio 2 (fine cells 128..639 x 128..383); "
-                            "coarse ghost fill, coarse stage with face fluxes, the fine ghost ring interpolated "
-                            "from the coarse level, fine stage with face fluxes, reflux, average-down",
+        comp = {"workload": "two-level Euler stages: 1024 x 256 coarse channel + one 512 x 256 fine box",
                 "stages": args.composite_stages, "ms_per_stage": cw * 1e3,
                 "Mcell_updates_per_s": ncells / cw / 1e6, "h_s": ch,
-                "tagging": {"workload": "ErrorEst criteria at the reference defaults (p and T jumps 0.08, fuel "
-                                        "rate 50 /s) over the 1024 x 256 coarse level",
+                "tagging": {"workload": "ErrorEst tags (reference defaults) over the 1024 x 256 coarse level",
                             "ms_per_call": tag_ms, "tagged": tag_n,
                             "Mcells_per_s": CHANNEL["nx"] * CHANNEL["ny"] / tag_ms / 1e3}}
         log(f"composite: {cw * 1e3:.3f} ms per two-level stage")
@@ -957,12 +930,7 @@ def main():
         fb = tdrv.fine_box
         nf = (fb[2] - fb[0] + 1) * (fb[3] - fb[1] + 1)
         leaves = CHANNEL["nx"] * CHANNEL["ny"] - nf // 4 + nf
-        two = {"workload": "the 2-D channel (1024 x 256 coarse cells over 0.2 x 0.05 m, H2_Li) with one refined "
-                           "box at ratio 2 chosen from ErrorEst's tags at the reference defaults (p and T jumps "
-                           "0.08, fuel rate 50 /s; 2-cell buffer, blocks of 4 coarse cells) and regridded every "
-                           f"{REGRID_INTERVAL} steps with the remap defect check; two-level composite Heun stages "
-                           "(reflux, average-down, repair, validation), Strang chemistry (batched form, FP32 "
-                           "Newton matrix, rtol 1e-8 atol 1e-12) over the active cells, CFL 0.5 over both levels",
+        two = {"workload": "channel 1024 x 256 with one adaptive refined box, two-level Strang steps",
                "fine_box_end": list(fb), "regrids": tdrv.regrids,
                "steps": args.two_level_steps, "steps_per_s": args.two_level_steps / twall,
                "ms_per_step": twall / args.two_level_steps * 1e3,
@@ -977,18 +945,7 @@ def main():
         awall, adrv, leaf_steps = amr_run(torch, hf, args.amr_steps, 1)
         g = adrv.grids()
         sizes = sorted({(b[2] - b[0] + 1, b[3] - b[1] + 1) for lev in range(1, len(g)) for b in g[lev]})
-        amr = {"workload": "config 3: the 2-D channel (1024 x 256 base cells over 0.2 x 0.05 m, H2_Li, hot strip "
-                           "2500 K / 30 atm along x < 0.01 m, +-1 % density blocks of 4 x 4 cells, reflecting "
-                           "left/top/bottom, transmissive right) with two refinement levels at ratio 2 from "
-                           "ErrorEst's tags at the reference defaults (p and T jumps 0.08, fuel rate 50 /s), "
-                           "Berger-Rigoutsos boxes with blocking 16, max_grid_size 64, grid_eff 0.7, n_error_buf "
-                           f"2, n_proper 2, regridded every {REGRID_INTERVAL} steps with the remap defect check; "
-                           "composite Heun stages on every level (masked ghost interpolation, flux stages over "
-                           "all boxes, reflux, average-down, repair over the leaves, validation), one batched "
-                           "chemistry launch (FP32 Newton matrix) over every leaf cell through the device block "
-                           "table (rtol 1e-8, "
-                           "atol 1e-12), CFL 0.5; each Strang step one replayed HIP graph (hf_amr_strang_step, "
-                           "recaptured when a regrid changes the boxes; a step it hands back runs call by call)",
+        amr = {"workload": "config 3: channel 1024 x 256 base + two refinement levels (ratio 2), graph Strang steps",
                "steps": args.amr_steps, "steps_per_s": args.amr_steps / awall,
                "ms_per_step": awall / args.amr_steps * 1e3,
                "Mleaf_cell_steps_per_s": leaf_steps / awall / 1e6,
@@ -1013,11 +970,8 @@ def main():
         torch.cuda.synchronize()
         awall = time.perf_counter() - t0
         nparam = ag_lnA.shape[1] + 1 + ag_Y0.shape[1]
-        adj = {"workload": f"discrete adjoint of fixed-step backward-Euler chemistry, {args.adjoint_cells} H2/air "
-                           f"cells (T0 [1000,1500] K, p [1,10] atm, phi [0.5,2], fresh mixture; seed 5), "
-                           f"{ADJ_STEPS} steps of {ADJ_H:g} s, checkpoints every {ADJ_CKPT}; J = T(t_end) + "
-                           f"integrated heat release and its gradient in {nparam} parameters "
-                           f"(ln A of {ag_lnA.shape[1]} reactions, T0, Y0)",
+        adj = {"workload": f"discrete adjoint, {args.adjoint_cells} H2/air cells, {ADJ_STEPS} backward-Euler "
+                           f"steps, {nparam} parameters",
                "cells_per_s": args.adjoint_cells / awall, "ms": awall * 1e3, "parameters": nparam}
         log(f"adjoint: {adj['cells_per_s']:.4g} cells/s ({awall * 1e3:.1f} ms)")
 
@@ -1042,13 +996,9 @@ def main():
                    "cells": args.cells, "form": "batched"},
         "roofline": {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": headline_traffic(args.cells),
-                     "note": "FP64 algorithmic operations of the chemistry kernel (hyperflow_amd/flops.py: "
-                             "the kernel's operation counters x per-operation counts, exp/log/log10/division "
-                             "at the FP64 arithmetic of their gfx950 code) over its event-timed duration; "
-                             "peak is the FP64 rate (the kernel runs on the FP64 vector pipe; MI355X's FP64 vector and "
-                             "matrix peaks are both 78.6 TFLOP/s); traffic is the kernel's HBM bytes per launch (FETCH_SIZE + "
-                             "WRITE_SIZE, copy-scaled) from the committed rocprofv3 PMC summary of this workload "
-                             "(profiles/hbm_counters_*), against 192 algorithmic bytes per cell"},
+                     "note": "algorithmic FP64 flops (hyperflow_amd/flops.py) / HIP-event time; traffic: "
+                             "copy-scaled FETCH_SIZE + WRITE_SIZE per launch (profiles/hbm_counters_*), "
+                             "192 algorithmic bytes per cell"},
     }
     line["config"]["precision"] = "FP64 state, residuals, rates and error control; FP32 Newton matrix and LU"
     for name in ("plain", "batched_fp64"):
@@ -1100,9 +1050,8 @@ def main():
             line["channel_amr"]["cpu_baseline"] = {
                 "value": cls / cw, "unit": "leaf cell-steps/s", "cores": len(os.sched_getaffinity(0)),
                 "kind": "port",
-                "sample": f"{nc} Strang steps of the same channel at the same cell size on a 256 x 64 base with the "
-                          f"same hierarchy rules (the oracle's multi-level step loop following the GPU's boxes), "
-                          f"{cw:.1f} s; GPU / CPU leaf cell-steps/s = {gpu_rate / (cls / cw):.1f}"}
+                "sample": f"{nc} Strang steps of the channel on a 256 x 64 base, same rules, oracle multi-level loop, "
+                          f"{cw:.1f} s; GPU/CPU {gpu_rate / (cls / cw):.1f}"}
             line["channel_amr"]["parity_vs_oracle"] = {"steps": nc, "case": "256 x 64 base, same rules", **diffs}
         if wedge is not None:
             log("wedge case: CPU oracle step loop")
@@ -1119,8 +1068,7 @@ def main():
             line["wedge_amr"]["cpu_baseline"] = {
                 "value": cls / cw, "unit": "leaf cell-steps/s", "cores": len(os.sched_getaffinity(0)),
                 "kind": "port",
-                "sample": f"the first {nc} Strang steps of the same run (the oracle's multi-level step loop "
-                          f"following the GPU's boxes), {cw:.1f} s; GPU / CPU leaf cell-steps/s = "
+                "sample": f"first {nc} Strang steps, oracle multi-level loop on the GPU's boxes, {cw:.1f} s; GPU/CPU "
                           f"{wamr['Mleaf_cell_steps_per_s'] * 1e6 / (cls / cw):.1f}"}
             line["wedge_amr"]["parity_vs_oracle"] = {"steps": nc, **wdiffs}
         if two is not None:
@@ -1208,8 +1156,8 @@ def main():
             _, _, gstate = shock_tube_run(torch, hf, nc, 0)
             line["shock_tube_1d"]["cpu_baseline"] = {
                 "value": nc / cwall, "unit": "Strang steps/s", "cores": len(os.sched_getaffinity(0)), "kind": "port",
-                "sample": f"the first {nc} steps of the same run, the oracle's step loop (C++ flux stage and BDF, "
-                          f"OpenMP chemistry), {cwall:.1f} s"}
+                "sample": f"first {nc} steps of the same run, oracle step loop (C++ flux and BDF, OpenMP), "
+                          f"{cwall:.1f} s"}
             line["shock_tube_1d"]["parity_vs_oracle"] = {
                 "steps": nc, **state_difference(gstate.reshape(cstate.shape), cstate, hf.Mechanism(hf.H2_LI).ns)}
         if chan is not None:
@@ -1245,10 +1193,9 @@ def main():
             fwd = n / t
             line["adjoint"]["cpu_baseline"] = {
                 "value": fwd, "unit": "cells/s (objective only)", "cores": len(os.sched_getaffinity(0)), "kind": "port",
-                "sample": f"first {n} cells, the oracle's fixed-step integration (OpenMP), {t:.1f} s; a central-"
-                          f"difference gradient in the same {adj['parameters']} parameters needs "
-                          f"{2 * adj['parameters'] + 1} such integrations per cell: {fwd / (2 * adj['parameters'] + 1):.4g} "
-                          "cells/s"}
+                "sample": f"first {n} cells, oracle fixed-step integration (OpenMP), {t:.1f} s; a central-difference "
+                          f"gradient ({2 * adj['parameters'] + 1} integrations): "
+                          f"{fwd / (2 * adj['parameters'] + 1):.4g} cells/s"}
         log("parity sample vs the converged oracle")
         line["parity_vs_oracle"] = {k: parity(U0, dt, v) for k, v in outs.items()}
         log("CPU baseline")
@@ -1259,7 +1206,30 @@ def main():
                 line["dme_sk39"]["parity_vs_oracle"] = parity(D0, ddt, dme_out, n=128, mech_path=hf.DME_SK39)
             line["dme_sk39"]["cpu_baseline"] = cpu_baseline(D0, ddt, atol=1e-20, mech_path=hf.DME_SK39,
                                                             label=f"{args.dme_cells}-cell DME")
-    print(json.dumps(line), flush=True)
+    print(json.dumps(compact(line), separators=(",", ":")), flush=True)
+
+
+CONTRACT_KEYS = ("metric", "value", "unit", "n_gpus", "steps", "warmup", "ms_per_step", "higher_is_better",
+                 "scaling", "vs_baseline", "dtype", "data", "config", "roofline", "cpu_baseline")
+
+
+def _round(o):
+    if isinstance(o, float):
+        return float(f"{o:.4g}")
+    if isinstance(o, dict):
+        return {k: _round(v) for k, v in o.items()}
+    if isinstance(o, (list, tuple)):
+        return [_round(v) for v in o]
+    return o
+
+
+def compact(line: dict) -> dict:
+    """The result line with the contract keys first, unchanged, and every other leg after them with its
+    figures at four significant digits, so that the whole line stays short enough to be read from the
+    tail of a log."""
+    out = {k: line[k] for k in CONTRACT_KEYS if k in line}
+    out.update({k: _round(v) for k, v in line.items() if k not in out})
+    return out
 
 
 if __name__ == "__main__":
